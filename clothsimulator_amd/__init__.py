@@ -1,0 +1,240 @@
+"""clothsimulator_amd — MI355X-native per-step cloth physics (ClothSimulator's --gpu step).
+
+The compute lives in libclothsim.so (hand-written gfx950 HIP kernels behind the
+C-ABI of include/clothsim.h).  This module is a thin ctypes mirror of that ABI
+used by tests, benchmarks and Python callers.  There is no CPU fallback: if
+the library is missing, or no gfx950 device is present, every call fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._cabi import (CS_E_NOT_CONVERGED, CS_MESH_CLOTH, CS_MESH_OBSTACLE, CS_OK, IMPACT_DTYPE, LIB_PATH, PROX_DTYPE,
+                    STATUS_NAMES, ImpactRec, MaterialIn, MeshArrays, MeshIn, Params, ProximityRec, Stats,
+                    default_params, fptr, iptr, material_struct)
+
+__all__ = ["load_library", "Simulator", "ClothError", "default_params", "material_struct", "MeshArrays"]
+
+_lib = None
+
+EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_last_error", "cs_set_params",
+            "cs_add_cloth", "cs_add_obstacle", "cs_add_handles", "cs_physics_step", "cs_collision_step", "cs_step",
+            "cs_run", "cs_get_counts", "cs_get_state", "cs_set_state", "cs_get_geometry", "cs_material_table",
+            "cs_lbvh_pairs", "cs_find_proximities", "cs_assemble", "cs_solve", "cs_find_impacts",
+            "cs_independent_impacts"]
+
+
+class ClothError(RuntimeError):
+    def __init__(self, status, what, msg=""):
+        self.status = status
+        super().__init__("%s failed: %s %s" % (what, STATUS_NAMES.get(status, status), msg))
+
+
+def load_library(path=LIB_PATH):
+    """Load libclothsim.so.  Raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError("%s is not built: run `python -m clothsimulator_amd.build`" % path)
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.cs_version.restype = C.c_char_p
+    L.cs_default_params.argtypes = [C.POINTER(Params)]
+    L.cs_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+    L.cs_destroy.argtypes = [vp]
+    L.cs_last_error.restype = C.c_char_p
+    L.cs_last_error.argtypes = [vp]
+    L.cs_set_params.argtypes = [vp, C.POINTER(Params)]
+    L.cs_add_cloth.argtypes = [vp, C.POINTER(MeshIn), C.POINTER(MaterialIn), C.POINTER(C.c_int)]
+    L.cs_add_obstacle.argtypes = [vp, C.POINTER(MeshIn), C.POINTER(C.c_int)]
+    L.cs_add_handles.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int]
+    for name in ("cs_physics_step", "cs_collision_step", "cs_step"):
+        getattr(L, name).argtypes = [vp, C.POINTER(Stats)]
+    L.cs_run.argtypes = [vp, C.c_int, C.POINTER(Stats)]
+    L.cs_get_counts.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    F = C.POINTER(C.c_float)
+    L.cs_get_state.argtypes = [vp, C.c_int, C.c_int, F, F, F]
+    L.cs_set_state.argtypes = [vp, C.c_int, C.c_int, F, F, F]
+    L.cs_get_geometry.argtypes = [vp, C.c_int, C.c_int, F, F, F]
+    L.cs_material_table.argtypes = [C.POINTER(MaterialIn), F, F, F]
+    L.cs_lbvh_pairs.argtypes = [vp, C.c_int, C.c_float, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
+    L.cs_find_proximities.argtypes = [vp, C.POINTER(ProximityRec), C.c_int, C.POINTER(C.c_int)]
+    L.cs_assemble.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), F, C.c_int, C.POINTER(C.c_int), F]
+    L.cs_solve.argtypes = [vp, C.c_int, F, C.POINTER(Stats)]
+    L.cs_find_impacts.argtypes = [vp, C.c_float, C.POINTER(ImpactRec), C.c_int, C.POINTER(C.c_int)]
+    L.cs_independent_impacts.argtypes = [vp, C.POINTER(ImpactRec), C.c_int, C.c_int, C.POINTER(ImpactRec),
+                                         C.POINTER(C.c_int)]
+    _lib = L
+    return L
+
+
+def material_table(mat):
+    L = load_library()
+    table = np.zeros(30 * 30 * 30 * 4, dtype=np.float32)
+    bend = np.zeros(15, dtype=np.float32)
+    dens = C.c_float()
+    st = L.cs_material_table(C.byref(mat), fptr(table), C.byref(dens), fptr(bend))
+    if st != CS_OK:
+        raise ClothError(st, "cs_material_table")
+    return table.reshape(30, 30, 30, 4), dens.value, bend.reshape(3, 5)
+
+
+class Simulator:
+    """One cs_ctx: cloths, static obstacles and handles on one gfx950 device."""
+
+    def __init__(self, spec=None, params=None):
+        self.L = load_library()
+        self.params = params if params is not None else spec.params
+        h = C.c_void_p()
+        st = self.L.cs_create(C.byref(self.params), C.byref(h))
+        if st != CS_OK:
+            raise ClothError(st, "cs_create")
+        self.h = h
+        self._keep = []
+        self.n_cloths = 0
+        if spec is not None:
+            for c in spec.cloths:
+                cid = self.add_cloth(c.mesh, c.material)
+                if c.handles:
+                    self.add_handles(cid, c.handles)
+            for o in spec.obstacles:
+                self.add_obstacle(o)
+
+    def _check(self, st, what, ok=(CS_OK,)):
+        if st not in ok:
+            msg = self.L.cs_last_error(self.h)
+            raise ClothError(st, what, msg.decode() if msg else "")
+        return st
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        self.params = p
+        self._check(self.L.cs_set_params(self.h, C.byref(p)), "cs_set_params")
+
+    def add_cloth(self, mesh, mat):
+        cid = C.c_int()
+        m = mesh.c_struct()
+        self._keep.append((mesh, mat))
+        self._check(self.L.cs_add_cloth(self.h, C.byref(m), C.byref(mat), C.byref(cid)), "cs_add_cloth")
+        self.n_cloths += 1
+        return cid.value
+
+    def add_obstacle(self, mesh):
+        oid = C.c_int()
+        m = mesh.c_struct()
+        self._keep.append(mesh)
+        self._check(self.L.cs_add_obstacle(self.h, C.byref(m), C.byref(oid)), "cs_add_obstacle")
+        return oid.value
+
+    def add_handles(self, cloth, nodes):
+        a = np.ascontiguousarray(nodes, dtype=np.int32)
+        self._check(self.L.cs_add_handles(self.h, cloth, iptr(a), len(a)), "cs_add_handles")
+
+    def physics_step(self):
+        st = Stats()
+        s = self._check(self.L.cs_physics_step(self.h, C.byref(st)), "cs_physics_step", (CS_OK, CS_E_NOT_CONVERGED))
+        return s, st
+
+    def collision_step(self):
+        st = Stats()
+        s = self.L.cs_collision_step(self.h, C.byref(st))
+        return s, st
+
+    def step(self):
+        st = Stats()
+        s = self.L.cs_step(self.h, C.byref(st))
+        return s, st
+
+    def run(self, steps):
+        st = Stats()
+        s = self.L.cs_run(self.h, steps, C.byref(st))
+        return s, st
+
+    def counts(self, kind=CS_MESH_CLOTH, idx=0):
+        nn, nf, ne = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.L.cs_get_counts(self.h, kind, idx, C.byref(nn), C.byref(nf), C.byref(ne)), "cs_get_counts")
+        return nn.value, nf.value, ne.value
+
+    def state(self, kind=CS_MESH_CLOTH, idx=0):
+        nn = self.counts(kind, idx)[0]
+        x, x0, v = (np.zeros((nn, 3), dtype=np.float32) for _ in range(3))
+        self._check(self.L.cs_get_state(self.h, kind, idx, fptr(x), fptr(x0), fptr(v)), "cs_get_state")
+        return x, x0, v
+
+    def set_state(self, kind, idx, x=None, x0=None, v=None):
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+        x, x0, v = conv(x), conv(x0), conv(v)
+        nul = C.POINTER(C.c_float)()
+        self._check(self.L.cs_set_state(self.h, kind, idx, fptr(x) if x is not None else nul,
+                                        fptr(x0) if x0 is not None else nul, fptr(v) if v is not None else nul),
+                    "cs_set_state")
+
+    def geometry(self, kind=CS_MESH_CLOTH, idx=0):
+        nn = self.counts(kind, idx)[0]
+        n = np.zeros((nn, 3), dtype=np.float32)
+        a = np.zeros(nn, dtype=np.float32)
+        m = np.zeros(nn, dtype=np.float32)
+        self._check(self.L.cs_get_geometry(self.h, kind, idx, fptr(n), fptr(a), fptr(m)), "cs_get_geometry")
+        return n, a, m
+
+    def lbvh_pairs(self, ccd, thickness):
+        n = C.c_int()
+        self._check(self.L.cs_lbvh_pairs(self.h, int(ccd), thickness, None, 0, C.byref(n)), "cs_lbvh_pairs")
+        pairs = np.zeros((max(n.value, 1), 2), dtype=np.int32)
+        self._check(self.L.cs_lbvh_pairs(self.h, int(ccd), thickness, iptr(pairs), n.value, C.byref(n)),
+                    "cs_lbvh_pairs")
+        return pairs[:n.value]
+
+    def proximities(self, cap=1 << 20):
+        out = np.zeros(cap, dtype=PROX_DTYPE)
+        n = C.c_int()
+        self._check(self.L.cs_find_proximities(self.h, out.ctypes.data_as(C.POINTER(ProximityRec)), cap, C.byref(n)),
+                    "cs_find_proximities")
+        return out[:n.value].copy()
+
+    def assemble(self, cloth=0):
+        nnzb = C.c_int()
+        self._check(self.L.cs_assemble(self.h, cloth, None, None, None, 0, C.byref(nnzb), None), "cs_assemble")
+        nn = self.counts(CS_MESH_CLOTH, cloth)[0]
+        rp = np.zeros(nn + 1, dtype=np.int32)
+        ci = np.zeros(nnzb.value, dtype=np.int32)
+        vals = np.zeros((nnzb.value, 3, 3), dtype=np.float32)
+        b = np.zeros(3 * nn, dtype=np.float32)
+        self._check(self.L.cs_assemble(self.h, cloth, iptr(rp), iptr(ci), fptr(vals), nnzb.value, C.byref(nnzb),
+                                       fptr(b)), "cs_assemble")
+        return rp, ci, vals, b
+
+    def solve(self, cloth=0):
+        nn = self.counts(CS_MESH_CLOTH, cloth)[0]
+        dv = np.zeros(3 * nn, dtype=np.float32)
+        st = Stats()
+        s = self._check(self.L.cs_solve(self.h, cloth, fptr(dv), C.byref(st)), "cs_solve", (CS_OK, CS_E_NOT_CONVERGED))
+        return dv, st
+
+    def impacts(self, thickness, cap=1 << 20):
+        out = np.zeros(cap, dtype=IMPACT_DTYPE)
+        n = C.c_int()
+        self._check(self.L.cs_find_impacts(self.h, thickness, out.ctypes.data_as(C.POINTER(ImpactRec)), cap,
+                                           C.byref(n)), "cs_find_impacts")
+        return out[:n.value].copy()
+
+    def independent_impacts(self, imps, deform):
+        a = np.ascontiguousarray(imps, dtype=IMPACT_DTYPE)
+        out = np.zeros(max(len(a), 1), dtype=IMPACT_DTYPE)
+        n = C.c_int()
+        self._check(self.L.cs_independent_impacts(self.h, a.ctypes.data_as(C.POINTER(ImpactRec)), len(a), deform,
+                                                  out.ctypes.data_as(C.POINTER(ImpactRec)), C.byref(n)),
+                    "cs_independent_impacts")
+        return out[:n.value].copy()

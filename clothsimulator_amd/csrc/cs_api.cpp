@@ -1,0 +1,575 @@
+// cs_api.cpp — the C-ABI of libclothsim.so (include/clothsim.h).
+//
+// Every entry point runs inside guard(): HIP failures and capacity/state
+// errors come back as cs_status with a message for cs_last_error, never as an
+// exception or exit().  Stage order follows Simulator::physicsStep /
+// collisionStep (src/Simulator.cu:571-684).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "cs_kernels.hpp"
+
+using namespace cs;
+
+namespace {
+
+template <class F> cs_status guard(cs_ctx* C, F&& f) {
+    try {
+        return static_cast<cs_status>(f());
+    } catch (const StatusError& e) {
+        if (C) C->err = e.what();
+        return static_cast<cs_status>(e.status);
+    } catch (const HipError& e) {
+        if (C) C->err = e.what();
+        return CS_E_HIP;
+    } catch (const std::bad_alloc&) {
+        if (C) C->err = "host allocation failed";
+        return CS_E_OOM;
+    } catch (const std::exception& e) {
+        if (C) C->err = e.what();
+        return CS_E_HIP;
+    }
+}
+
+bool valid_params(const cs_params* p) {
+    return p && p->dt > 0.0f && p->pcg_rel_tol > 0.0f && p->pcg_rel_tol < 1.0f && p->pcg_max_iters >= 0 &&
+           p->max_collision_iters >= 0 && p->repulsion_thickness > 0.0f && p->collision_thickness >= 0.0f &&
+           (p->precond == CS_PRECOND_BLOCK_JACOBI || p->precond == CS_PRECOND_LEARNED);
+}
+
+const MeshInfo* mesh_of(cs_ctx* C, int kind, int id) {
+    if (kind == CS_MESH_CLOTH) return (id >= 0 && id < C->nCloths) ? &C->meshes[id] : nullptr;
+    if (kind == CS_MESH_OBSTACLE) return (id >= 0 && id < C->nObstacles) ? &C->meshes[C->nCloths + id] : nullptr;
+    return nullptr;
+}
+
+struct StageTimer {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    float lap(hipStream_t s) {
+        CS_HIP(hipStreamSynchronize(s));
+        auto t1 = std::chrono::steady_clock::now();
+        float ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+        t0 = t1;
+        return ms;
+    }
+};
+
+int physics_step(cs_ctx* C, cs_stats* st) {
+    host_finalize(C);
+    hipStream_t s = C->stream;
+    StageTimer T;
+    cs_stats local{};
+    if (!(C->P.flags & CS_FLAG_NO_PROXIMITY)) find_proximities(C);
+    else C->nProx = 0;
+    local.n_proximities = C->nProx;
+    local.ms_proximity = T.lap(s);
+    int status = CS_OK;
+    local.pcg_converged = 1;
+    for (int c = 0; c < C->nCloths; c++) {
+        launch_element_records(C, c);
+        launch_assemble_rows(C, c);
+        local.n_contact_blocks += assemble_contacts(C, c);
+        launch_block_jacobi(C, c);
+        local.ms_assemble += T.lap(s);
+        float rel = 0.0f;
+        int conv = 0;
+        local.pcg_iters += pcg_solve(C, c, &rel, &conv);
+        local.pcg_rel_residual = std::max(local.pcg_rel_residual, rel);
+        if (!conv) { local.pcg_converged = 0; status = CS_E_NOT_CONVERGED; }
+        launch_apply_dv(C, c);
+        local.ms_solve += T.lap(s);
+    }
+    launch_update_positions(C, 0, C->N);
+    launch_face_geometry(C, 0, C->F);
+    launch_node_geometry(C, 0, C->N);
+    local.ms_update = T.lap(s);
+    if (st) {
+        st->pcg_iters = local.pcg_iters;
+        st->pcg_rel_residual = local.pcg_rel_residual;
+        st->pcg_converged = local.pcg_converged;
+        st->n_proximities = local.n_proximities;
+        st->n_contact_blocks = local.n_contact_blocks;
+        st->ms_proximity = local.ms_proximity;
+        st->ms_assemble = local.ms_assemble;
+        st->ms_solve = local.ms_solve;
+        st->ms_update = local.ms_update;
+    }
+    return status;
+}
+
+int collision_step(cs_ctx* C, cs_stats* st) {
+    host_finalize(C);
+    hipStream_t s = C->stream;
+    StageTimer T;
+    for (size_t m = 0; m < C->meshes.size(); m++) bvh_build(C, (int)m, true);
+    const float th = C->P.collision_thickness;
+    float obstacleMass = 1e3f;
+    bool success = false;
+    int deform = 0, iters = 0;
+    for (deform = 0; deform < 2; deform++) {
+        C->nImpAcc = 0;
+        for (int i = 0; i < C->P.max_collision_iters; i++) {
+            int n = find_impacts(C, th);
+            if (n == 0) { success = true; break; }
+            C->impAcc.grow_keep(sizeof(cs_impact_rec) * (size_t)(C->nImpAcc + n),
+                                sizeof(cs_impact_rec) * (size_t)C->nImpAcc, s);
+            int m = independent_impacts(C, C->impSorted.ptr<cs_impact_rec>(), n, deform,
+                                        C->impAcc.ptr<cs_impact_rec>() + C->nImpAcc);
+            C->nImpAcc += m;
+            collision_response(C, C->nImpAcc, deform, obstacleMass);
+            iters++;
+            for (int c = 0; c < C->nCloths; c++) bvh_refit(C, c);
+            if (deform == 1) {
+                for (int o = C->nCloths; o < (int)C->meshes.size(); o++) bvh_refit(C, o);
+                obstacleMass *= 0.5f;
+            }
+        }
+        if (success) break;
+    }
+    if (st) {
+        st->collision_iters = iters;
+        st->collision_deform = deform;
+        st->n_impacts = C->nImpAcc;
+    }
+    if (!success) {
+        C->err = "Collision step failed: impacts remain after max_collision_iters in both deform passes";
+        if (st) st->ms_collision = T.lap(s);
+        return CS_E_COLLISION_FAILED;
+    }
+    for (int c = 0; c < C->nCloths; c++) {
+        const MeshInfo& M = C->meshes[c];
+        launch_face_geometry(C, M.f0, M.nf);
+        launch_node_geometry(C, M.n0, M.nn);
+        launch_update_velocities(C, M.n0, M.nn);
+    }
+    if (deform == 1)
+        for (int o = C->nCloths; o < (int)C->meshes.size(); o++) {
+            const MeshInfo& M = C->meshes[o];
+            launch_face_geometry(C, M.f0, M.nf);
+            launch_node_geometry(C, M.n0, M.nn);
+            launch_update_velocities(C, M.n0, M.nn);
+        }
+    if (st) st->ms_collision = T.lap(s);
+    else CS_HIP(hipStreamSynchronize(s));
+    return CS_OK;
+}
+
+int step(cs_ctx* C, cs_stats* st) {
+    int a = physics_step(C, st);
+    if (a != CS_OK && a != CS_E_NOT_CONVERGED) return a;
+    if (C->P.flags & CS_FLAG_NO_COLLISION) return a;
+    int b = collision_step(C, st);
+    return b != CS_OK ? b : a;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cs_version(void) { return "clothsim-mi355x 0.1 (gfx950)"; }
+
+void cs_default_params(cs_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->dt = 0.005f;
+    p->gravity[2] = -9.8f;
+    p->wind_density = 1.0f;
+    p->handle_stiffness = 1e3f;
+    p->collision_stiffness = 1e9f;
+    p->repulsion_thickness = 1e-3f;
+    p->collision_thickness = 1e-4f;
+    p->friction = 0.6f;
+    p->obstacle_friction = 0.3f;
+    p->pcg_rel_tol = 1e-5f;
+    p->pcg_max_iters = 4000;
+    p->precond = CS_PRECOND_BLOCK_JACOBI;
+    p->max_collision_iters = 100;
+}
+
+cs_status cs_create(const cs_params* params, cs_ctx** ctx_out) {
+    if (!ctx_out) return CS_E_ARG;
+    *ctx_out = nullptr;
+    if (!valid_params(params)) return CS_E_ARG;
+    if (params->precond == CS_PRECOND_LEARNED) return CS_E_ARG;  // no trained preconditioner bundled in this build
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CS_E_NO_DEVICE;
+    cs_ctx* C = new cs_ctx();
+    cs_status st = guard(C, [&]() {
+        CS_HIP(hipGetDevice(&C->device));
+        hipDeviceProp_t prop;
+        CS_HIP(hipGetDeviceProperties(&prop, C->device));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            throw StatusError(CS_E_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", library built for gfx950");
+        CS_HIP(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+        CS_HIP(hipHostMalloc(reinterpret_cast<void**>(&C->hScal), sizeof(PcgScalars), hipHostMallocDefault));
+        CS_HIP(hipHostMalloc(reinterpret_cast<void**>(&C->hCount), 64, hipHostMallocDefault));
+        C->P = *params;
+        return (int)CS_OK;
+    });
+    if (st != CS_OK) {
+        cs_destroy(C);
+        return st;
+    }
+    *ctx_out = C;
+    return CS_OK;
+}
+
+void cs_destroy(cs_ctx* C) {
+    if (!C) return;
+    if (C->stream) (void)hipStreamSynchronize(C->stream);
+    if (C->hScal) (void)hipHostFree(C->hScal);
+    if (C->hCount) (void)hipHostFree(C->hCount);
+    cs_ctx* c = C;
+    // device buffers are released by the DBuf destructors
+    hipStream_t s = c->stream;
+    delete c;
+    if (s) (void)hipStreamDestroy(s);
+}
+
+const char* cs_last_error(const cs_ctx* C) { return C ? C->err.c_str() : "null context"; }
+
+cs_status cs_set_params(cs_ctx* C, const cs_params* p) {
+    if (!C || !valid_params(p)) return CS_E_ARG;
+    if (p->precond == CS_PRECOND_LEARNED) return CS_E_ARG;
+    C->P = *p;
+    return CS_OK;
+}
+
+cs_status cs_add_cloth(cs_ctx* C, const cs_mesh_in* mesh, const cs_material_in* material, int* cloth_id) {
+    if (!C || !mesh || !material) return CS_E_ARG;
+    if (C->finalized || C->nObstacles > 0) return CS_E_STATE;
+    return guard(C, [&]() {
+        HostMaterial m;
+        material_build(*material, m);
+        C->materials.push_back(m);
+        int st = host_add_mesh(C, *mesh, true, (int)C->materials.size() - 1);
+        if (st != CS_OK) { C->materials.pop_back(); return st; }
+        if (cloth_id) *cloth_id = C->nCloths - 1;
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_add_obstacle(cs_ctx* C, const cs_mesh_in* mesh, int* obstacle_id) {
+    if (!C || !mesh) return CS_E_ARG;
+    if (C->finalized) return CS_E_STATE;
+    return guard(C, [&]() {
+        int st = host_add_mesh(C, *mesh, false, -1);
+        if (st != CS_OK) return st;
+        if (obstacle_id) *obstacle_id = C->nObstacles - 1;
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_add_handles(cs_ctx* C, int cloth_id, const int* nodes, int n) {
+    if (!C || n < 0 || (n > 0 && !nodes)) return CS_E_ARG;
+    if (C->finalized) return CS_E_STATE;
+    if (cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
+    const MeshInfo& M = C->meshes[cloth_id];
+    for (int i = 0; i < n; i++)
+        if (nodes[i] < 0 || nodes[i] >= M.nn) return CS_E_ARG;
+    for (int i = 0; i < n; i++) {
+        cs_ctx::HandleH h;
+        h.cloth = cloth_id;
+        h.node = M.n0 + nodes[i];
+        for (int k = 0; k < 3; k++) h.p[k] = C->hx[3 * (size_t)h.node + k];
+        C->handles.push_back(h);
+    }
+    return CS_OK;
+}
+
+cs_status cs_physics_step(cs_ctx* C, cs_stats* st) {
+    if (!C) return CS_E_ARG;
+    if (C->nCloths == 0) return CS_E_STATE;
+    return guard(C, [&]() { return physics_step(C, st); });
+}
+
+cs_status cs_collision_step(cs_ctx* C, cs_stats* st) {
+    if (!C) return CS_E_ARG;
+    if (C->nCloths == 0) return CS_E_STATE;
+    return guard(C, [&]() { return collision_step(C, st); });
+}
+
+cs_status cs_step(cs_ctx* C, cs_stats* st) {
+    if (!C) return CS_E_ARG;
+    if (C->nCloths == 0) return CS_E_STATE;
+    return guard(C, [&]() { return step(C, st); });
+}
+
+cs_status cs_run(cs_ctx* C, int steps, cs_stats* acc) {
+    if (!C || steps < 0) return CS_E_ARG;
+    if (C->nCloths == 0) return CS_E_STATE;
+    return guard(C, [&]() {
+        int worst = CS_OK;
+        cs_stats sum{};
+        sum.pcg_converged = 1;
+        for (int i = 0; i < steps; i++) {
+            cs_stats one{};
+            int r = step(C, &one);
+            if (r != CS_OK && r != CS_E_NOT_CONVERGED) return r;
+            if (r != CS_OK) worst = r;
+            sum.pcg_iters += one.pcg_iters;
+            sum.pcg_rel_residual = std::max(sum.pcg_rel_residual, one.pcg_rel_residual);
+            sum.pcg_converged &= one.pcg_converged;
+            sum.n_proximities += one.n_proximities;
+            sum.n_contact_blocks += one.n_contact_blocks;
+            sum.collision_iters += one.collision_iters;
+            sum.collision_deform = std::max(sum.collision_deform, one.collision_deform);
+            sum.n_impacts += one.n_impacts;
+            sum.ms_proximity += one.ms_proximity;
+            sum.ms_assemble += one.ms_assemble;
+            sum.ms_solve += one.ms_solve;
+            sum.ms_update += one.ms_update;
+            sum.ms_collision += one.ms_collision;
+        }
+        if (acc) *acc = sum;
+        return worst;
+    });
+}
+
+cs_status cs_get_counts(cs_ctx* C, int kind, int id, int* nn, int* nf, int* ne) {
+    if (!C) return CS_E_ARG;
+    const MeshInfo* M = mesh_of(C, kind, id);
+    if (!M) return CS_E_ARG;
+    if (nn) *nn = M->nn;
+    if (nf) *nf = M->nf;
+    if (ne) *ne = M->ne;
+    return CS_OK;
+}
+
+cs_status cs_get_state(cs_ctx* C, int kind, int id, float* x, float* x0, float* v) {
+    if (!C) return CS_E_ARG;
+    const MeshInfo* M = mesh_of(C, kind, id);
+    if (!M) return CS_E_ARG;
+    return guard(C, [&]() {
+        host_finalize(C);
+        std::vector<float4> h(M->nn);
+        float* outs[3] = {x, x0, v};
+        DBuf* bufs[3] = {&C->x, &C->x0, &C->v};
+        for (int k = 0; k < 3; k++) {
+            if (!outs[k]) continue;
+            CS_HIP(hipMemcpyAsync(h.data(), bufs[k]->ptr<float4>() + M->n0, sizeof(float4) * M->nn,
+                                  hipMemcpyDeviceToHost, C->stream));
+            CS_HIP(hipStreamSynchronize(C->stream));
+            for (int i = 0; i < M->nn; i++) {
+                outs[k][3 * i] = h[i].x;
+                outs[k][3 * i + 1] = h[i].y;
+                outs[k][3 * i + 2] = h[i].z;
+            }
+        }
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_set_state(cs_ctx* C, int kind, int id, const float* x, const float* x0, const float* v) {
+    if (!C) return CS_E_ARG;
+    const MeshInfo* M = mesh_of(C, kind, id);
+    if (!M) return CS_E_ARG;
+    return guard(C, [&]() {
+        host_finalize(C);
+        std::vector<float4> h(M->nn);
+        const float* ins[3] = {x, x0, v};
+        DBuf* bufs[3] = {&C->x, &C->x0, &C->v};
+        for (int k = 0; k < 3; k++) {
+            if (!ins[k]) continue;
+            for (int i = 0; i < M->nn; i++) h[i] = make_float4(ins[k][3 * i], ins[k][3 * i + 1], ins[k][3 * i + 2], 0.0f);
+            CS_HIP(hipMemcpyAsync(bufs[k]->ptr<float4>() + M->n0, h.data(), sizeof(float4) * M->nn,
+                                  hipMemcpyHostToDevice, C->stream));
+            CS_HIP(hipStreamSynchronize(C->stream));
+        }
+        launch_face_geometry(C, M->f0, M->nf);
+        launch_node_geometry(C, M->n0, M->nn);
+        CS_HIP(hipStreamSynchronize(C->stream));
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_get_geometry(cs_ctx* C, int kind, int id, float* normals, float* area, float* mass) {
+    if (!C) return CS_E_ARG;
+    const MeshInfo* M = mesh_of(C, kind, id);
+    if (!M) return CS_E_ARG;
+    return guard(C, [&]() {
+        host_finalize(C);
+        hipStream_t s = C->stream;
+        if (normals) {
+            std::vector<float4> h(M->nn);
+            CS_HIP(hipMemcpyAsync(h.data(), C->nrm.ptr<float4>() + M->n0, sizeof(float4) * M->nn, hipMemcpyDeviceToHost, s));
+            CS_HIP(hipStreamSynchronize(s));
+            for (int i = 0; i < M->nn; i++) { normals[3 * i] = h[i].x; normals[3 * i + 1] = h[i].y; normals[3 * i + 2] = h[i].z; }
+        }
+        if (area) CS_HIP(hipMemcpyAsync(area, C->area.ptr<float>() + M->n0, sizeof(float) * M->nn, hipMemcpyDeviceToHost, s));
+        if (mass) CS_HIP(hipMemcpyAsync(mass, C->mass.ptr<float>() + M->n0, sizeof(float) * M->nn, hipMemcpyDeviceToHost, s));
+        CS_HIP(hipStreamSynchronize(s));
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_material_table(const cs_material_in* material, float* table, float* density, float* bending) {
+    if (!material) return CS_E_ARG;
+    try {
+        HostMaterial m;
+        material_build(*material, m);
+        if (table) std::memcpy(table, m.table.data(), sizeof(float) * m.table.size());
+        if (density) *density = m.density;
+        if (bending) std::memcpy(bending, m.bend, sizeof(m.bend));
+    } catch (...) {
+        return CS_E_OOM;
+    }
+    return CS_OK;
+}
+
+cs_status cs_lbvh_pairs(cs_ctx* C, int ccd, float thickness, int* pairs, int cap, int* n_out) {
+    if (!C || !n_out || cap < 0) return CS_E_ARG;
+    if (C->nCloths == 0) return CS_E_STATE;
+    return guard(C, [&]() {
+        host_finalize(C);
+        for (size_t m = 0; m < C->meshes.size(); m++) bvh_build(C, (int)m, ccd != 0);
+        long n = bvh_pairs_ordered(C, thickness, pairs, pairs ? cap : 0);
+        *n_out = (int)n;
+        return (pairs && n > cap) ? (int)CS_E_CAPACITY : (int)CS_OK;
+    });
+}
+
+cs_status cs_find_proximities(cs_ctx* C, cs_proximity_rec* out, int cap, int* n_out) {
+    if (!C || !n_out || cap < 0) return CS_E_ARG;
+    if (C->nCloths == 0) return CS_E_STATE;
+    return guard(C, [&]() {
+        host_finalize(C);
+        int n = find_proximities(C);
+        *n_out = n;
+        if (!out) return (int)CS_OK;
+        if (n > cap) return (int)CS_E_CAPACITY;
+        if (n > 0) {
+            CS_HIP(hipMemcpyAsync(out, C->prox.ptr<cs_proximity_rec>(), sizeof(cs_proximity_rec) * n,
+                                  hipMemcpyDeviceToHost, C->stream));
+            CS_HIP(hipStreamSynchronize(C->stream));
+        }
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_assemble(cs_ctx* C, int cloth_id, int* row_ptr, int* col_idx, float* vals, int cap_blocks, int* nnzb_out,
+                      float* b) {
+    if (!C || !nnzb_out) return CS_E_ARG;
+    if (cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
+    return guard(C, [&]() {
+        host_finalize(C);
+        hipStream_t s = C->stream;
+        if (!(C->P.flags & CS_FLAG_NO_PROXIMITY)) find_proximities(C);
+        else C->nProx = 0;
+        launch_element_records(C, cloth_id);
+        launch_assemble_rows(C, cloth_id);
+        assemble_contacts(C, cloth_id);
+        launch_block_jacobi(C, cloth_id);
+        ClothSys& S = C->sys[cloth_id];
+        const int nn = S.nn;
+        std::vector<float> sv((size_t)S.totalSlots * SLICE * 9), hb(3 * (size_t)nn), hbc(3 * (size_t)nn);
+        std::vector<int> crp(nn + 1, 0), ccol;
+        std::vector<float> cval;
+        CS_HIP(hipMemcpyAsync(sv.data(), S.vals.ptr<float>(), sizeof(float) * sv.size(), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipMemcpyAsync(hb.data(), S.b.ptr<float>(), sizeof(float) * hb.size(), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipMemcpyAsync(hbc.data(), S.bc.ptr<float>(), sizeof(float) * hbc.size(), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipMemcpyAsync(crp.data(), S.cRowPtr.ptr<int>(), sizeof(int) * (nn + 1), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipStreamSynchronize(s));
+        if (S.nContact > 0) {
+            ccol.resize(S.nContact);
+            cval.resize(9 * (size_t)S.nContact);
+            CS_HIP(hipMemcpyAsync(ccol.data(), S.cCol.ptr<int>(), sizeof(int) * ccol.size(), hipMemcpyDeviceToHost, s));
+            CS_HIP(hipMemcpyAsync(cval.data(), S.cVal.ptr<float>(), sizeof(float) * cval.size(), hipMemcpyDeviceToHost, s));
+            CS_HIP(hipStreamSynchronize(s));
+        }
+        // merge the static and contact parts of each row (columns ascending)
+        std::vector<int> rp(nn + 1, 0), cols;
+        std::vector<float> vv;
+        for (int r = 0; r < nn; r++) {
+            const int sl = r / SLICE, lane = r % SLICE;
+            std::vector<std::pair<int, std::array<float, 9>>> row;
+            for (int k = 0; k < S.hRowLen[r]; k++) {
+                std::array<float, 9> a;
+                for (int q = 0; q < 9; q++) a[q] = sv[(((size_t)S.hSliceOff[sl] + k) * 9 + q) * SLICE + lane];
+                row.push_back({S.hCol[S.hRowPtr[r] + k], a});
+            }
+            for (int k = crp[r]; k < crp[r + 1]; k++) {
+                std::array<float, 9> a;
+                for (int q = 0; q < 9; q++) a[q] = cval[9 * (size_t)k + q];
+                auto it = std::find_if(row.begin(), row.end(), [&](const std::pair<int, std::array<float, 9>>& e) { return e.first == ccol[k]; });
+                if (it != row.end()) for (int q = 0; q < 9; q++) it->second[q] += a[q];
+                else row.push_back({ccol[k], a});
+            }
+            std::sort(row.begin(), row.end(), [](const std::pair<int, std::array<float, 9>>& p, const std::pair<int, std::array<float, 9>>& q) { return p.first < q.first; });
+            for (auto& e : row) { cols.push_back(e.first); vv.insert(vv.end(), e.second.begin(), e.second.end()); }
+            rp[r + 1] = (int)cols.size();
+        }
+        *nnzb_out = (int)cols.size();
+        if (!row_ptr) return (int)CS_OK;
+        if ((int)cols.size() > cap_blocks) return (int)CS_E_CAPACITY;
+        std::copy(rp.begin(), rp.end(), row_ptr);
+        std::copy(cols.begin(), cols.end(), col_idx);
+        std::copy(vv.begin(), vv.end(), vals);
+        if (b)
+            for (size_t i = 0; i < hb.size(); i++) b[i] = hb[i] + hbc[i];
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_solve(cs_ctx* C, int cloth_id, float* dv, cs_stats* st) {
+    if (!C || !dv) return CS_E_ARG;
+    if (cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
+    if (!C->finalized) return CS_E_STATE;
+    return guard(C, [&]() {
+        float rel = 0.0f;
+        int conv = 0;
+        int it = pcg_solve(C, cloth_id, &rel, &conv);
+        ClothSys& S = C->sys[cloth_id];
+        CS_HIP(hipMemcpyAsync(dv, S.x.ptr<float>(), sizeof(float) * 3 * (size_t)S.nn, hipMemcpyDeviceToHost, C->stream));
+        CS_HIP(hipStreamSynchronize(C->stream));
+        if (st) {
+            st->pcg_iters = it;
+            st->pcg_rel_residual = rel;
+            st->pcg_converged = conv;
+        }
+        return conv ? (int)CS_OK : (int)CS_E_NOT_CONVERGED;
+    });
+}
+
+cs_status cs_find_impacts(cs_ctx* C, float thickness, cs_impact_rec* out, int cap, int* n_out) {
+    if (!C || !n_out || cap < 0) return CS_E_ARG;
+    if (C->nCloths == 0) return CS_E_STATE;
+    return guard(C, [&]() {
+        host_finalize(C);
+        for (size_t m = 0; m < C->meshes.size(); m++) bvh_build(C, (int)m, true);
+        int n = find_impacts(C, thickness);
+        *n_out = n;
+        if (!out) return (int)CS_OK;
+        if (n > cap) return (int)CS_E_CAPACITY;
+        if (n > 0) {
+            CS_HIP(hipMemcpyAsync(out, C->impSorted.ptr<cs_impact_rec>(), sizeof(cs_impact_rec) * n,
+                                  hipMemcpyDeviceToHost, C->stream));
+            CS_HIP(hipStreamSynchronize(C->stream));
+        }
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_independent_impacts(cs_ctx* C, const cs_impact_rec* in, int n, int deform, cs_impact_rec* out, int* n_out) {
+    if (!C || n < 0 || !n_out || (n > 0 && (!in || !out)) || (deform != 0 && deform != 1)) return CS_E_ARG;
+    return guard(C, [&]() {
+        host_finalize(C);
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < 4; k++)
+                if (in[i].nodes[k] < 0 || in[i].nodes[k] >= C->N) return (int)CS_E_ARG;
+        *n_out = 0;
+        if (n == 0) return (int)CS_OK;
+        hipStream_t s = C->stream;
+        cs_impact_rec* din = C->impSorted.as<cs_impact_rec>(n);
+        cs_impact_rec* dout = C->impacts.as<cs_impact_rec>(n);
+        CS_HIP(hipMemcpyAsync(din, in, sizeof(cs_impact_rec) * n, hipMemcpyHostToDevice, s));
+        int m = independent_impacts(C, din, n, deform, dout);
+        CS_HIP(hipMemcpyAsync(out, dout, sizeof(cs_impact_rec) * m, hipMemcpyDeviceToHost, s));
+        CS_HIP(hipStreamSynchronize(s));
+        *n_out = m;
+        return (int)CS_OK;
+    });
+}
+
+}  // extern "C"

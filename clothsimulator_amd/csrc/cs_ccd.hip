@@ -1,0 +1,392 @@
+// cs_ccd.hip — continuous collision detection in FP64 and the independent impact set.
+//
+// findImpacts (src/Simulator.cu:383-392) / checkImpactsGpu
+// (src/CollisionHelper.cu:137-162): for each candidate face pair of the CCD
+// LBVH (boxes over x0 and x), 6 vertex-face and 9 edge-edge tests, each with a
+// swept-box prefilter and the coplanarity cubic of checkImpact
+// (src/CollisionHelper.cu:62-109) solved in FP64.  An edge-edge pair is always
+// evaluated with the lower global edge id first, so a pair reached from
+// several face pairs yields one bit-identical result; the list is then
+// de-duplicated by (type, primitive ids) and sorted by (t, type, ids).
+//
+// independentImpacts GPU (src/Simulator.cu:422-461, kernels
+// src/CollisionHelper.cu:164-237): rounds in which a candidate is accepted when
+// it is the lowest-sorted live candidate on each of its eligible nodes; this
+// equals the sequential greedy pass in sorted order.
+#include "cs_kernels.hpp"
+#include "cs_math.hpp"
+
+namespace cs {
+
+namespace {
+
+__device__ double newton_root(double a, double b, double c, double d, double x0, int dir) {
+    if (dir != 0) {
+        double y0 = d + x0 * (c + x0 * (b + x0 * a));
+        double ddy0 = 2.0 * b + x0 * (6.0 * a);
+        x0 += dir * sqrt(tabs(2.0 * y0 / ddy0));
+    }
+    for (int iter = 0; iter < 100; iter++) {
+        double y = d + x0 * (c + x0 * (b + x0 * a));
+        double dy = c + x0 * (2.0 * b + 3.0 * x0 * a);
+        if (dy == 0) return x0;
+        double x1 = x0 - y / dy;
+        if (tabs(x0 - x1) < 1e-6) return x0;
+        x0 = x1;
+    }
+    return x0;
+}
+
+__device__ int solve_quadratic(double a, double b, double c, double x[2]) {
+    double d = b * b - 4.0 * a * c;
+    if (d < 0.0) {
+        x[0] = -b / (2.0 * a);
+        return 0;
+    }
+    double q = -(b + (b < 0.0 ? -1 : 1) * sqrt(d)) * 0.5;
+    int i = 0;
+    if (tabs(a) > 1e-12 * tabs(q)) x[i++] = q / a;
+    if (tabs(q) > 1e-12 * tabs(c)) x[i++] = c / q;
+    if (i == 2 && x[0] > x[1]) {
+        double t = x[0]; x[0] = x[1]; x[1] = t;
+    }
+    return i;
+}
+
+__device__ int solve_cubic(double a, double b, double c, double d, double x[3]) {
+    double xc[2] = {0.0, 0.0};
+    int n = solve_quadratic(3.0 * a, 2.0 * b, c, xc);
+    if (n == 0) {
+        x[0] = newton_root(a, b, c, d, xc[0], 0);
+        return 1;
+    }
+    if (n == 1) return solve_quadratic(b, c, d, x);
+    double yc0 = d + xc[0] * (c + xc[0] * (b + xc[0] * a));
+    double yc1 = d + xc[1] * (c + xc[1] * (b + xc[1] * a));
+    int i = 0;
+    if (yc0 * a >= 0.0) x[i++] = newton_root(a, b, c, d, xc[0], -1);
+    if (yc0 * yc1 <= 0.0) {
+        int closer = tabs(yc0) < tabs(yc1) ? 0 : 1;
+        x[i++] = newton_root(a, b, c, d, xc[closer], closer == 0 ? 1 : -1);
+    }
+    if (yc1 * a <= 0.0) x[i++] = newton_root(a, b, c, d, xc[1], 1);
+    return i;
+}
+
+// checkImpact (src/CollisionHelper.cu:62-109) in FP64
+__device__ bool check_impact(int type, const V3d X0[4], const V3d X1[4], cs_impact_rec& imp) {
+    V3d x0 = X0[0];
+    V3d v0 = X1[0] - x0;
+    V3d x1 = X0[1] - x0, x2 = X0[2] - x0, x3 = X0[3] - x0;
+    V3d v1 = (X1[1] - X0[1]) - v0, v2 = (X1[2] - X0[2]) - v0, v3 = (X1[3] - X0[3]) - v0;
+    double a0 = mixed(x1, x2, x3);
+    double a1 = mixed(v1, x2, x3) + mixed(x1, v2, x3) + mixed(x1, x2, v3);
+    double a2 = mixed(x1, v2, v3) + mixed(v1, x2, v3) + mixed(v1, v2, x3);
+    double a3 = mixed(v1, v2, v3);
+    double t[3];
+    int ns = solve_cubic(a3, a2, a1, a0, t);
+    for (int i = 0; i < ns; i++) {
+        if (t[i] < 0.0 || t[i] > 1.0) continue;
+        V3d p[4];
+        for (int k = 0; k < 4; k++) p[k] = X0[k] + t[i] * (X1[k] - X0[k]);
+        V3d n;
+        double w[4], d;
+        bool ok, inside;
+        if (type == 0) {
+            ok = signed_vf(p[0], p[1], p[2], p[3], n, w, d);
+            inside = ok && (tmin(-w[1], tmin(-w[2], -w[3])) >= -1e-6);
+        } else {
+            ok = signed_ee(p[0], p[1], p[2], p[3], n, w, d);
+            inside = ok && (tmin(tmin(w[0], w[1]), tmin(-w[2], -w[3])) >= -1e-6);
+        }
+        if (!ok) continue;
+        if (dot(n, w[1] * v1 + w[2] * v2 + w[3] * v3) > 0.0) n = -n;
+        if (tabs(d) < 1e-6 && inside) {
+            imp.t = t[i] + 0.0;  // a root of -0.0 is reported as +0.0 (t keys are sorted by their bits)
+            for (int k = 0; k < 4; k++) imp.w[k] = w[k];
+            imp.n[0] = n.x; imp.n[1] = n.y; imp.n[2] = n.z;
+            return true;
+        }
+    }
+    return false;
+}
+
+struct CcdArgs {
+    const int2* pairs;
+    long nPairs;
+    float th;
+    int keyBits;
+    const float4 *x, *x0;
+    const int4 *fnodes, *fedges;
+    const int2* enodes;
+    cs_impact_rec* out;
+    uint64_t* keys;
+    unsigned* count;
+    long cap;
+};
+
+__device__ __forceinline__ void swept_box(Box& b, const float4* x, const float4* x0, int n) {
+    box_add(b, ld3(x[n]));
+    box_add(b, ld3(x0[n]));
+}
+
+__device__ void emit(const CcdArgs& A, cs_impact_rec& imp, uint64_t key) {
+    unsigned slot = atomicAdd(A.count, 1u);
+    if ((long)slot < A.cap) {
+        A.out[slot] = imp;
+        A.keys[slot] = key;
+    }
+}
+
+__device__ void load4(const CcdArgs& A, const int nd[4], V3d X0[4], V3d X1[4]) {
+    for (int k = 0; k < 4; k++) {
+        X0[k] = tod(ld3(A.x0[nd[k]]));
+        X1[k] = tod(ld3(A.x[nd[k]]));
+    }
+}
+
+// checkVertexFaceImpact (src/CollisionHelper.cu:111-122)
+__device__ void vf_impact(const CcdArgs& A, int node, int f) {
+    int4 fn = A.fnodes[f];
+    if (node == fn.x || node == fn.y || node == fn.z) return;
+    Box nb = box_empty(), fb = box_empty();
+    swept_box(nb, A.x, A.x0, node);
+    swept_box(fb, A.x, A.x0, fn.x);
+    swept_box(fb, A.x, A.x0, fn.y);
+    swept_box(fb, A.x, A.x0, fn.z);
+    if (!box_overlap(nb, fb, A.th)) return;
+    cs_impact_rec imp;
+    imp.type = 0;
+    imp.nodes[0] = node; imp.nodes[1] = fn.x; imp.nodes[2] = fn.y; imp.nodes[3] = fn.z;
+    V3d X0[4], X1[4];
+    load4(A, imp.nodes, X0, X1);
+    if (check_impact(0, X0, X1, imp))
+        emit(A, imp, ((uint64_t)node << A.keyBits) | (uint64_t)f);
+}
+
+// checkEdgeEdgeImpact (src/CollisionHelper.cu:124-135), lower edge id first
+__device__ void ee_impact(const CcdArgs& A, int e0, int e1) {
+    if (e1 < e0) { int t = e0; e0 = e1; e1 = t; }
+    int2 a = A.enodes[e0], b = A.enodes[e1];
+    if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) return;
+    Box ba = box_empty(), bb = box_empty();
+    swept_box(ba, A.x, A.x0, a.x);
+    swept_box(ba, A.x, A.x0, a.y);
+    swept_box(bb, A.x, A.x0, b.x);
+    swept_box(bb, A.x, A.x0, b.y);
+    if (!box_overlap(ba, bb, A.th)) return;
+    cs_impact_rec imp;
+    imp.type = 1;
+    imp.nodes[0] = a.x; imp.nodes[1] = a.y; imp.nodes[2] = b.x; imp.nodes[3] = b.y;
+    V3d X0[4], X1[4];
+    load4(A, imp.nodes, X0, X1);
+    if (check_impact(1, X0, X1, imp))
+        emit(A, imp, (1ull << (2 * A.keyBits)) | ((uint64_t)e0 << A.keyBits) | (uint64_t)e1);
+}
+
+__global__ void __launch_bounds__(256) k_ccd_tests(CcdArgs A) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 15 * A.nPairs) return;
+    const long p = t / 15;
+    const int k = (int)(t - 15 * p);
+    const int2 pr = A.pairs[p];
+    if (k < 3) {
+        int4 fn = A.fnodes[pr.x];
+        vf_impact(A, k == 0 ? fn.x : (k == 1 ? fn.y : fn.z), pr.y);
+    } else if (k < 6) {
+        int4 fn = A.fnodes[pr.y];
+        vf_impact(A, k == 3 ? fn.x : (k == 4 ? fn.y : fn.z), pr.x);
+    } else {
+        const int j = (k - 6) / 3, l = (k - 6) % 3;
+        int4 e0 = A.fedges[pr.x], e1 = A.fedges[pr.y];
+        ee_impact(A, j == 0 ? e0.x : (j == 1 ? e0.y : e0.z), l == 0 ? e1.x : (l == 1 ? e1.y : e1.z));
+    }
+}
+
+__global__ void k_unique_heads(int n, const uint64_t* __restrict__ keys, uint32_t* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+// unique impacts, keyed by their t bits (t >= 0, so the bit order is the value order)
+__global__ void k_unique_compact(int n, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                 const uint32_t* __restrict__ idx, const cs_impact_rec* __restrict__ imps,
+                                 uint64_t* __restrict__ tkeys, uint32_t* __restrict__ tidx) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    uint32_t u = pos[i];
+    uint32_t j = idx[i];
+    tkeys[u] = (uint64_t)__double_as_longlong(imps[j].t);
+    tidx[u] = j;
+}
+
+__global__ void k_gather_impacts(int n, const uint32_t* __restrict__ idx, const cs_impact_rec* __restrict__ in,
+                                 cs_impact_rec* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = in[idx[i]];
+}
+
+// One workgroup: rounds of "minimum sorted index on every eligible node".
+// state: 0 live candidate, 1 accepted, 2 excluded.
+constexpr int IB = 1024;
+__global__ void __launch_bounds__(IB) k_independent(int n, const cs_impact_rec* __restrict__ imps, int deform, int NC,
+                                                    int* nodeMin, int* nodeRemoved, int* state,
+                                                    cs_impact_rec* __restrict__ out, int* nOut) {
+    __shared__ int live;
+    __shared__ int wsum[IB / 64];
+    auto eligible = [&](int nd) { return deform == 1 || nd < NC; };
+    for (int i = threadIdx.x; i < n; i += IB) {
+        state[i] = 0;
+        for (int k = 0; k < 4; k++) {
+            int nd = imps[i].nodes[k];
+            if (eligible(nd)) { __hip_atomic_store(&nodeRemoved[nd], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+        }
+    }
+    __syncthreads();
+    for (;;) {
+        for (int i = threadIdx.x; i < n; i += IB)
+            if (state[i] == 0)
+                for (int k = 0; k < 4; k++) {
+                    int nd = imps[i].nodes[k];
+                    if (eligible(nd)) __hip_atomic_store(&nodeMin[nd], 0x7fffffff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += IB)
+            if (state[i] == 0)
+                for (int k = 0; k < 4; k++) {
+                    int nd = imps[i].nodes[k];
+                    if (eligible(nd)) atomicMin(&nodeMin[nd], i);
+                }
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += IB)
+            if (state[i] == 0) {
+                bool ok = true;
+                for (int k = 0; k < 4; k++) {
+                    int nd = imps[i].nodes[k];
+                    if (eligible(nd) && __hip_atomic_load(&nodeMin[nd], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != i)
+                        ok = false;
+                }
+                if (ok) state[i] = 3;  // accepted this round
+            }
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += IB)
+            if (state[i] == 3) {
+                state[i] = 1;
+                for (int k = 0; k < 4; k++) {
+                    int nd = imps[i].nodes[k];
+                    if (eligible(nd)) __hip_atomic_store(&nodeRemoved[nd], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        if (threadIdx.x == 0) live = 0;
+        __syncthreads();
+        int mine = 0;
+        for (int i = threadIdx.x; i < n; i += IB)
+            if (state[i] == 0) {
+                bool rem = false;
+                for (int k = 0; k < 4; k++) {
+                    int nd = imps[i].nodes[k];
+                    if (eligible(nd) && __hip_atomic_load(&nodeRemoved[nd], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                        rem = true;
+                }
+                if (rem) state[i] = 2; else mine++;
+            }
+        if (mine) atomicAdd(&live, mine);
+        __syncthreads();
+        if (live == 0) break;
+        __syncthreads();
+    }
+    // ordered compaction of the accepted impacts
+    int carry = 0;
+    for (int base = 0; base < n; base += IB) {
+        int i = base + threadIdx.x;
+        int f = (i < n && state[i] == 1) ? 1 : 0;
+        int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        uint64_t bal = __ballot(f);
+        int before = __popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+        if (lane == 0) wsum[wave] = __popcll(bal);
+        __syncthreads();
+        int off = carry;
+        for (int w = 0; w < wave; w++) off += wsum[w];
+        int tot = 0;
+        for (int w = 0; w < IB / 64; w++) tot += wsum[w];
+        if (f) out[off + before] = imps[i];
+        __syncthreads();
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *nOut = carry;
+}
+
+}  // namespace
+
+int find_impacts(cs_ctx* C, float th) {
+    hipStream_t s = C->stream;
+    long nPairs = bvh_all_pairs(C, th);
+    if (nPairs == 0) return 0;
+    int keyBits = 1;
+    long mx = std::max(std::max((long)C->N, (long)C->F), (long)C->E);
+    while ((1L << keyBits) <= mx) keyBits++;
+    unsigned* count = C->impCount.as<unsigned>(2);
+    long cap = std::max<long>(4096, nPairs / 4);
+    for (int attempt = 0; attempt < 2; attempt++) {
+        cs_impact_rec* imps = C->impacts.as<cs_impact_rec>(cap);
+        uint64_t* keys = C->impKeys.as<uint64_t>(cap);
+        fill_u32(count, 0u, 1, s);
+        CcdArgs A;
+        A.pairs = C->pairs.ptr<int2>();
+        A.nPairs = nPairs;
+        A.th = th;
+        A.keyBits = keyBits;
+        A.x = C->x.ptr<float4>(); A.x0 = C->x0.ptr<float4>();
+        A.fnodes = C->fnodes.ptr<int4>(); A.fedges = C->fedges.ptr<int4>(); A.enodes = C->enodes.ptr<int2>();
+        A.out = imps; A.keys = keys; A.count = count; A.cap = cap;
+        hipLaunchKernelGGL(k_ccd_tests, dim3(cdiv(15 * nPairs, 256)), dim3(256), 0, s, A);
+        CS_LAUNCH_CHECK();
+        CS_HIP(hipMemcpyAsync(C->hCount, count, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipStreamSynchronize(s));
+        if ((long)C->hCount[0] <= cap) break;
+        cap = (long)C->hCount[0] + 1024;
+    }
+    const int n = (int)C->hCount[0];
+    if (n == 0) return 0;
+    cs_impact_rec* imps = C->impacts.ptr<cs_impact_rec>();
+    uint64_t* keys = C->impKeys.ptr<uint64_t>();
+    uint64_t* ktmp = C->impKeysTmp.as<uint64_t>(n);
+    uint32_t* idx = C->impIdx.as<uint32_t>(n);
+    uint32_t* itmp = C->impIdxTmp.as<uint32_t>(n);
+    iota_u32(idx, n, s);
+    radix_sort_pairs(keys, idx, ktmp, itmp, n, 0, 2 * keyBits + 1, C->ws, s);
+    uint32_t* flag = C->ws.a.as<uint32_t>(n + 1);
+    uint32_t* pos = C->ws.b.as<uint32_t>(n + 1);
+    hipLaunchKernelGGL(k_unique_heads, dim3(cdiv(n, 256)), dim3(256), 0, s, n, keys, flag);
+    CS_LAUNCH_CHECK();
+    scan_exclusive(flag, pos, n, C->ws, s);
+    CS_HIP(hipMemcpyAsync(C->hCount, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    CS_HIP(hipStreamSynchronize(s));
+    const int nu = (int)C->hCount[0];
+    // ktmp/itmp are free after the sort: reuse them for the t keys
+    hipLaunchKernelGGL(k_unique_compact, dim3(cdiv(n, 256)), dim3(256), 0, s, n, flag, pos, idx, imps, ktmp, itmp);
+    CS_LAUNCH_CHECK();
+    radix_sort_pairs(ktmp, itmp, keys, idx, nu, 0, 62, C->ws, s);
+    cs_impact_rec* sorted = C->impSorted.as<cs_impact_rec>(nu);
+    hipLaunchKernelGGL(k_gather_impacts, dim3(cdiv(nu, 256)), dim3(256), 0, s, nu, itmp, imps, sorted);
+    CS_LAUNCH_CHECK();
+    return nu;
+}
+
+int independent_impacts(cs_ctx* C, const cs_impact_rec* dIn, int n, int deform, cs_impact_rec* dOut) {
+    if (n <= 0) return 0;
+    hipStream_t s = C->stream;
+    int* nodeMin = C->nodeMin.as<int>(C->N);
+    int* removed = C->nodeRemoved.as<int>(C->N);
+    int* state = C->impState.as<int>(n + 1);
+    hipLaunchKernelGGL(k_independent, dim3(1), dim3(IB), 0, s, n, dIn, deform, C->NC, nodeMin, removed, state, dOut,
+                       state + n);
+    CS_LAUNCH_CHECK();
+    CS_HIP(hipMemcpyAsync(C->hCount, state + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    CS_HIP(hipStreamSynchronize(s));
+    return (int)C->hCount[0];
+}
+
+}  // namespace cs

@@ -1,0 +1,154 @@
+// cs_ctx.hpp — the context behind the opaque cs_ctx handle: scene topology,
+// device-resident state (HBM layout) and the per-stage working buffers.
+//
+// HBM layout (all meshes concatenated: cloths first, then obstacles):
+//   nodes  x, x0, x1, v, n         float4 each (16 B, one dwordx4 per gather)
+//          area, mass              float
+//          (free <=> global id < nClothNodes)
+//   faces  fnodes int4 (3 node ids + mesh id), fedges int4, fnrm float4,
+//          finv float4 (2x2 rest inverse), farea, fmass float
+//   edges  enodes int2 (min,max), eopp int2 (opposite node or -1), eadj int2 (face or -1)
+// Per cloth, the static implicit-Euler pattern is a sliced-ELL BSR: slices of
+// 64 rows (one wavefront), each slice padded to its longest row; within a slice
+// block slot k of value c for row r sits at vals[off + (k*9+c)*64 + r%64], so a
+// wave reads and writes 256 contiguous bytes per (k, c).  Padding slots hold
+// col = row and zero values.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../include/clothsim.h"
+#include "cs_common.hpp"
+#include "cs_prims.hpp"
+
+namespace cs {
+
+static const int MAT_N = 30;
+static const int SLICE = 64;
+
+struct HostMaterial {
+    float density = 0.0f;
+    std::vector<float> table;  // 30^3 x 4
+    float bend[15];
+};
+
+struct MeshInfo {
+    bool cloth;
+    int n0, nn, f0, nf, e0, ne, v0, nv;
+    int material;
+};
+
+// Per-mesh LBVH over faces.  Unified node array: internal i at i (i < nf-1),
+// leaf j at nf-1+j.  BNode is 32 B: lo[3], maxIndex, hi[3], face (-1 internal).
+struct BNode {
+    float lo[3];
+    int maxIndex;
+    float hi[3];
+    int face;
+};
+
+struct Bvh {
+    int mesh = -1;
+    int nf = 0;
+    bool ccd = false;
+    DBuf keys, keysTmp;    // uint64 (morton) keys
+    DBuf vals, valsTmp;    // uint32 local face ids
+    DBuf nodes;            // BNode[2nf-1]
+    DBuf child;            // int2[nf-1]
+    DBuf parent;           // int[2nf-1]
+    DBuf flags;            // uint32[nf-1]
+    DBuf fbox;             // float4 x2 per face (unsorted leaf boxes)
+    DBuf partial;          // block partial boxes
+    DBuf scene;            // float4 x2 (p, d)
+};
+
+struct ClothSys {
+    int mesh;
+    int nn, nslices, totalSlots, maxWidth;
+    // static sliced-ELL pattern
+    DBuf sliceOff;  // int[nslices+1] (in slots)
+    DBuf sliceW;    // int[nslices]
+    DBuf cols;      // int[totalSlots*64]
+    DBuf vals;      // float[totalSlots*64*9]
+    DBuf rowLen;    // int[nn]
+    DBuf diagSlot;  // int[nn]
+    DBuf incPtr;    // int[nn+1]
+    DBuf inc;       // int2 per incidence: x = element (face id, or edge id | 1<<31), y = a | slots<<2 (6 bits each)
+    DBuf hcount;    // float[nn]  handles on the node
+    DBuf hpos;      // float4[nn] sum of handle anchors
+    DBuf b;         // float[3nn] static right-hand side
+    DBuf bc;        // float[3nn] contact right-hand side
+    DBuf Minv;      // float[9nn]
+    // dynamic contact blocks (CSR over rows)
+    int nContact = 0;
+    DBuf cRowPtr, cCol, cVal, cDiag;
+    // PCG vectors
+    DBuf x, r, z, d, q;
+    // host copies used by cs_assemble
+    std::vector<int> hRowPtr, hCol;  // exact (unpadded) static pattern
+    std::vector<int> hSliceOff, hSliceW, hRowLen;
+    int nHandles = 0;
+};
+
+struct PcgScalars {
+    float rz, dq, alpha, beta, rr, bb, tol2bb, pad;
+    int iters, done, maxit, pad2;
+};
+
+}  // namespace cs
+
+struct cs_ctx {
+    cs_params P;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool finalized = false;
+    int device = 0;
+
+    // host staging (kept for topology and I/O)
+    std::vector<cs::MeshInfo> meshes;
+    int nCloths = 0, nObstacles = 0;
+    std::vector<float> hx, hv, huv;  // 3N, 3N, 2V
+    std::vector<int> hvnode;         // uv vertex -> node
+    std::vector<int> hfv, hfn;       // 3F uv-vertex ids, node ids
+    std::vector<int> hfe;            // 3F edge ids
+    std::vector<int> hen, heopp, headj;  // 2E each (eopp: uv vertex id)
+    std::vector<cs::HostMaterial> materials;
+    struct HandleH { int cloth, node; float p[3]; };
+    std::vector<HandleH> handles;
+
+    int N = 0, F = 0, E = 0, NC = 0, FC = 0, EC = 0;  // totals and cloth totals
+
+    // device state
+    cs::DBuf x, x0, x1, v, nrm, area, mass;  // nodes
+    cs::DBuf fnodes, fedges, fnrm, finv, farea, fmass;  // faces
+    cs::DBuf enodes, eopp, eadj, ebias;  // edges (ebias: float4 (id0, w0, id1, w1) per cloth edge)
+    cs::DBuf nfPtr, nfIdx;  // node -> incident faces (face*4 + local), faces ascending
+    cs::DBuf matTable, matBend;  // all materials concatenated
+    cs::DBuf fmat;  // int per cloth face: material index
+    cs::DBuf faceRec, edgeRec;   // per-step element records (cloth faces / edges)
+    std::vector<cs::ClothSys> sys;
+    std::vector<cs::Bvh> bvh;     // one per mesh (rebuilt per stage)
+
+    // proximity
+    cs::DBuf pairs, pairCount;           // int2 pairs, uint32 counter(s)
+    size_t pairCap = 0;
+    cs::DBuf bestNode, bestEdge, bestFace;  // uint64 keys
+    cs::DBuf proxFlag, proxPos, prox;     // uint32, uint32, cs_proximity_rec
+    int nProx = 0;
+    // contact assembly scratch
+    cs::DBuf cKeys, cKeysTmp, cIdx, cIdxTmp, cRec, cFlag, cPos;
+    // PCG
+    cs::DBuf pcgPartial, pcgScal, pcgCounter;
+    cs::PcgScalars* hScal = nullptr;  // pinned
+    // CCD / impacts
+    cs::DBuf impacts, impCount, impKeys, impKeysTmp, impIdx, impIdxTmp, impSorted;
+    cs::DBuf impAcc;  // accumulated impacts of the deform pass
+    int nImpAcc = 0;
+    cs::DBuf nodeMin, nodeRemoved, impState;
+    // response (optimization) scratch
+    cs::DBuf optKeys, optKeysTmp, optVals, optValsTmp, optFlag, optPos, optNodes, optStart, optSlot,
+        optX, optNext, optGrad, optLambda, optCoef, optScal;
+    uint32_t* hCount = nullptr;  // pinned small readback
+    cs::Scratch ws;
+    hipEvent_t ev[8];
+};

@@ -1,0 +1,45 @@
+// cs_kernels.hpp — host launchers of the gfx950 kernels, grouped by stage.
+#pragma once
+#include "cs_ctx.hpp"
+
+namespace cs {
+
+// host set-up (cs_host.cpp)
+void material_build(const cs_material_in& in, HostMaterial& m);
+int host_add_mesh(cs_ctx* C, const cs_mesh_in& in, bool cloth, int material);
+void host_finalize(cs_ctx* C);
+
+// geometry (cs_geometry.hip)
+void launch_face_geometry(cs_ctx* C, int f0, int nf);
+void launch_node_geometry(cs_ctx* C, int n0, int nn);
+void launch_update_positions(cs_ctx* C, int n0, int nn);
+void launch_update_velocities(cs_ctx* C, int n0, int nn);
+void launch_apply_dv(cs_ctx* C, int c);
+
+// assembly (cs_assemble.hip)
+void launch_element_records(cs_ctx* C, int c);
+void launch_assemble_rows(cs_ctx* C, int c);
+int assemble_contacts(cs_ctx* C, int c);  // returns number of contact blocks
+void launch_block_jacobi(cs_ctx* C, int c);
+
+// PCG (cs_pcg.hip)
+int pcg_solve(cs_ctx* C, int c, float* relres, int* converged);
+
+// LBVH (cs_lbvh.hip)
+void bvh_build(cs_ctx* C, int mesh, bool ccd);
+void bvh_refit(cs_ctx* C, int mesh);
+// Candidate face pairs of all traversals (cloth self, cloth x earlier cloth,
+// cloth x obstacle) appended to C->pairs; returns the count.
+long bvh_all_pairs(cs_ctx* C, float thickness);
+// Deterministic pair list (count / scan / emit), for cs_lbvh_pairs.
+long bvh_pairs_ordered(cs_ctx* C, float thickness, int* host_out, long cap);
+
+// proximities (cs_proximity.hip)
+int find_proximities(cs_ctx* C);
+
+// CCD and impact response (cs_ccd.hip, cs_response.hip)
+int find_impacts(cs_ctx* C, float thickness);  // sorted, unique impacts in C->impSorted (cs_impact_rec)
+int independent_impacts(cs_ctx* C, const cs_impact_rec* dIn, int n, int deform, cs_impact_rec* dOut);
+void collision_response(cs_ctx* C, int nImpacts, int deform, float obstacleMass);
+
+}  // namespace cs

@@ -1,0 +1,435 @@
+// cs_lbvh.hip — linear BVH over the faces of one mesh and leaf-vs-tree traversal.
+//
+// Build (reference: BVH::BVH GPU, src/BVH.cu:18-50; src/BVHHelper.cu):
+//   leaf boxes over x (and x0 for CCD), scene box, 30-bit Morton code of each
+//   box centre with the reference's FP32 operation order, a stable LSD radix
+//   sort of the codes carrying the face index (equivalent to sorting the
+//   reference's 64-bit key code<<32|face), Karras 2012 internal nodes, and
+//   bottom-up boxes / maxIndex (largest sorted leaf index in the subtree) where
+//   the second thread to reach a node computes it (no float atomics).
+// Traversal (src/BVHHelper.cu:151-307): one thread per query leaf, explicit
+//   stack in LDS; a leaf pair is reported when the query box overlaps the
+//   target leaf box dilated by `thickness`; for a mesh against itself only
+//   leaves with a larger sorted index are reported.
+#include "cs_kernels.hpp"
+#include "cs_math.hpp"
+
+namespace cs {
+
+namespace {
+
+constexpr int TB = 64;          // traversal block (one wavefront)
+constexpr int STACK = 64;       // traversal stack depth (the reference uses 64)
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__device__ __forceinline__ Box face_box(const float4* __restrict__ x, const float4* __restrict__ x0, int4 fn, bool ccd) {
+    Box b = box_empty();
+    const int ids[3] = {fn.x, fn.y, fn.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        box_add(b, ld3(x[ids[k]]));
+        if (ccd) box_add(b, ld3(x0[ids[k]]));
+    }
+    return b;
+}
+
+__global__ void __launch_bounds__(256) k_leaf_boxes(int f0, int nf, bool ccd, const float4* __restrict__ x,
+                                                    const float4* __restrict__ x0, const int4* __restrict__ fnodes,
+                                                    float4* __restrict__ fbox, float4* __restrict__ partial) {
+    __shared__ float red[6][256];
+    int f = blockIdx.x * 256 + threadIdx.x;
+    Box b = box_empty();
+    if (f < nf) {
+        b = face_box(x, x0, fnodes[f0 + f], ccd);
+        fbox[2 * f] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.0f);
+        fbox[2 * f + 1] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.0f);
+    }
+    for (int k = 0; k < 3; k++) { red[k][threadIdx.x] = b.lo[k]; red[3 + k][threadIdx.x] = b.hi[k]; }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int k = 0; k < 3; k++) {
+                red[k][threadIdx.x] = tmin(red[k][threadIdx.x], red[k][threadIdx.x + o]);
+                red[3 + k][threadIdx.x] = tmax(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + o]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = make_float4(red[0][0], red[1][0], red[2][0], 0.0f);
+        partial[2 * blockIdx.x + 1] = make_float4(red[3][0], red[4][0], red[5][0], 0.0f);
+    }
+}
+
+// scene box of the leaf boxes: min/max are exact, so the reduction order is free.
+__global__ void __launch_bounds__(256) k_scene_box(int nb, const float4* __restrict__ partial, float4* __restrict__ scene) {
+    __shared__ float red[6][256];
+    Box b = box_empty();
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        Box p;
+        float4 lo = partial[2 * i], hi = partial[2 * i + 1];
+        p.lo[0] = lo.x; p.lo[1] = lo.y; p.lo[2] = lo.z;
+        p.hi[0] = hi.x; p.hi[1] = hi.y; p.hi[2] = hi.z;
+        box_add(b, p);
+    }
+    for (int k = 0; k < 3; k++) { red[k][threadIdx.x] = b.lo[k]; red[3 + k][threadIdx.x] = b.hi[k]; }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int k = 0; k < 3; k++) {
+                red[k][threadIdx.x] = tmin(red[k][threadIdx.x], red[k][threadIdx.x + o]);
+                red[3 + k][threadIdx.x] = tmax(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + o]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float p0 = red[0][0], p1 = red[1][0], p2 = red[2][0];
+        scene[0] = make_float4(p0, p1, p2, 0.0f);
+        scene[1] = make_float4(red[3][0] - p0, red[4][0] - p1, red[5][0] - p2, 0.0f);
+    }
+}
+
+// mortonCode / initializeLeafNodes (src/BVHHelper.cu:18-42)
+__global__ void __launch_bounds__(256) k_morton(int nf, const float4* __restrict__ fbox, const float4* __restrict__ scene,
+                                                uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    int f = blockIdx.x * 256 + threadIdx.x;
+    if (f >= nf) return;
+    float4 lo = fbox[2 * f], hi = fbox[2 * f + 1];
+    float4 p = scene[0], d = scene[1];
+    const float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+    const float pp[3] = {p.x, p.y, p.z}, dd[3] = {d.x, d.y, d.z};
+    uint32_t e[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        float s = dd[i] == 0.0f ? 0.0f : (c[i] - pp[i]) / dd[i];
+        s = fminf(fmaxf(s * 1024.0f, 0.0f), 1023.0f);
+        e[i] = expand_bits((uint32_t)s);
+    }
+    keys[f] = (uint64_t)((e[0] << 2) + (e[1] << 1) + e[2]);
+    vals[f] = (uint32_t)f;
+}
+
+// leaves in sorted order; keys become the reference's 64-bit code<<32 | face
+__global__ void __launch_bounds__(256) k_fill_leaves(int nf, int f0, uint64_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ vals, const float4* __restrict__ fbox,
+                                                     BNode* __restrict__ nodes) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nf) return;
+    uint32_t f = vals[i];
+    keys[i] = (keys[i] << 32) | (uint64_t)f;
+    float4 lo = fbox[2 * f], hi = fbox[2 * f + 1];
+    BNode n;
+    n.lo[0] = lo.x; n.lo[1] = lo.y; n.lo[2] = lo.z;
+    n.hi[0] = hi.x; n.hi[1] = hi.y; n.hi[2] = hi.z;
+    n.maxIndex = i;
+    n.face = f0 + (int)f;
+    nodes[nf - 1 + i] = n;
+}
+
+__device__ __forceinline__ int clz64(uint64_t a) { return __clzll((long long)a); }
+
+// findRange / findSplit / initializeInternalNodes (src/BVHHelper.cu:48-103)
+__global__ void __launch_bounds__(256) k_karras(int ni, const uint64_t* __restrict__ k, int2* __restrict__ child,
+                                                int* __restrict__ parent) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ni) return;
+    int left, right;
+    if (i == 0) {
+        left = 0;
+        right = ni;
+        parent[0] = -1;
+    } else {
+        const uint64_t code = k[i];
+        const int dl = clz64(code ^ k[i - 1]);
+        const int dr = clz64(code ^ k[i + 1]);
+        if (dl > dr) {
+            int l = -1, r = i - 1;
+            while (r - l > 1) { int mid = (l + r) >> 1; if (clz64(code ^ k[mid]) > dr) r = mid; else l = mid; }
+            left = r; right = i;
+        } else {
+            int l = i + 1, r = ni + 1;
+            while (r - l > 1) { int mid = (l + r) >> 1; if (clz64(code ^ k[mid]) > dl) l = mid; else r = mid; }
+            left = i; right = l;
+        }
+    }
+    const uint64_t code = k[left];
+    const int dd = clz64(code ^ k[right]);
+    int l = left, r = right;
+    while (r - l > 1) { int mid = (l + r) >> 1; if (clz64(code ^ k[mid]) > dd) l = mid; else r = mid; }
+    const int mid = l;
+    const int L = (mid == left) ? ni + mid : mid;        // unified index: leaves start at ni = nf-1
+    const int R = (mid + 1 == right) ? ni + mid + 1 : mid + 1;
+    child[i] = make_int2(L, R);
+    parent[L] = i;
+    parent[R] = i;
+}
+
+// bottom-up boxes: the second thread to reach an internal node combines its children
+__global__ void __launch_bounds__(256) k_bottom_up(int nf, const int2* __restrict__ child, const int* __restrict__ parent,
+                                                   BNode* nodes, unsigned* flags) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nf) return;
+    int p = parent[nf - 1 + i];
+    while (p >= 0) {
+        __threadfence();
+        unsigned old = atomicAdd(&flags[p], 1u);
+        if (old == 0u) return;
+        __threadfence();
+        int2 ch = child[p];
+        BNode a = nodes[ch.x], b = nodes[ch.y];
+        BNode n;
+        for (int k = 0; k < 3; k++) { n.lo[k] = tmin(a.lo[k], b.lo[k]); n.hi[k] = tmax(a.hi[k], b.hi[k]); }
+        n.maxIndex = tmax(a.maxIndex, b.maxIndex);
+        n.face = -1;
+        nodes[p] = n;
+        p = parent[p];
+    }
+}
+
+// refit: recompute leaf boxes in place (sorted order, maxIndex unchanged)
+__global__ void __launch_bounds__(256) k_refit_leaves(int nf, bool ccd, const float4* __restrict__ x,
+                                                      const float4* __restrict__ x0, const int4* __restrict__ fnodes,
+                                                      BNode* __restrict__ nodes) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nf) return;
+    BNode& n = nodes[nf - 1 + i];
+    Box b = face_box(x, x0, fnodes[n.face], ccd);
+    for (int k = 0; k < 3; k++) { n.lo[k] = b.lo[k]; n.hi[k] = b.hi[k]; }
+}
+
+__device__ __forceinline__ Box node_box(const BNode& n) {
+    Box b;
+    for (int k = 0; k < 3; k++) { b.lo[k] = n.lo[k]; b.hi[k] = n.hi[k]; }
+    return b;
+}
+
+// Leaf-vs-tree traversal for query leaf qi.  emit(face) for each reported leaf.
+// Returns false if the stack overflowed (the query is then incomplete).
+template <bool SELF, class Emit>
+__device__ __forceinline__ bool traverse(const BNode* __restrict__ tn, const int2* __restrict__ tchild, int tnf,
+                                         const Box& qb, int qi, float th, int* stk, Emit&& emit) {
+    if (tnf == 1) {  // a one-face target has no internal node
+        if (!SELF && box_overlap(qb, node_box(tn[0]), th)) emit(tn[0].face);
+        return true;
+    }
+    const int ni = tnf - 1;
+    int top = 0;
+    int node = 0;
+    for (;;) {
+        const int2 ch = tchild[node];
+        const BNode L = tn[ch.x], R = tn[ch.y];
+        const bool lLeaf = ch.x >= ni, rLeaf = ch.y >= ni;
+        const bool ovL = (!SELF || qi < L.maxIndex) && box_overlap(qb, node_box(L), th);
+        const bool ovR = (!SELF || qi < R.maxIndex) && box_overlap(qb, node_box(R), th);
+        if (ovL && lLeaf) emit(L.face);
+        if (ovR && rLeaf) emit(R.face);
+        const bool tL = ovL && !lLeaf, tR = ovR && !rLeaf;
+        if (!tL && !tR) {
+            if (top == 0) break;
+            node = stk[(--top) * TB];
+        } else {
+            node = tL ? ch.x : ch.y;
+            if (tL && tR) {
+                if (top >= STACK) return false;
+                stk[(top++) * TB] = ch.y;
+            }
+        }
+    }
+    return true;
+}
+
+// mode 0: count into num[qi]; mode 1: emit at offs[qi]; mode 2: append with an atomic counter
+template <bool SELF, int MODE>
+__global__ void __launch_bounds__(TB) k_traverse(int qnf, const BNode* __restrict__ qn, const BNode* __restrict__ tn,
+                                                 const int2* __restrict__ tchild, int tnf, float th,
+                                                 uint32_t* __restrict__ num, const uint32_t* __restrict__ offs,
+                                                 int2* __restrict__ out, unsigned* __restrict__ counter, long cap,
+                                                 unsigned* __restrict__ overflow) {
+    __shared__ int stk[STACK * TB];
+    const int qi = blockIdx.x * TB + threadIdx.x;
+    if (qi >= qnf) return;
+    const BNode q = qn[qnf - 1 + qi];
+    const Box qb = node_box(q);
+    const int qface = q.face;
+    uint32_t cnt = 0;
+    uint32_t base = MODE == 1 ? offs[qi] : 0;
+    bool ok = traverse<SELF>(tn, tchild, tnf, qb, qi, th, stk + threadIdx.x, [&](int f) {
+        if (MODE == 1) out[base + cnt] = make_int2(qface, f);
+        if (MODE == 2) {
+            unsigned slot = atomicAdd(counter, 1u);
+            if ((long)slot < cap) out[slot] = make_int2(qface, f);
+        }
+        cnt++;
+    });
+    if (MODE == 0) num[qi] = cnt;
+    if (!ok) atomicOr(overflow, 1u);
+}
+
+void ensure_bvh(cs_ctx* C, Bvh& B, int nf) {
+    B.nf = nf;
+    B.keys.as<uint64_t>(nf);
+    B.keysTmp.as<uint64_t>(nf);
+    B.vals.as<uint32_t>(nf);
+    B.valsTmp.as<uint32_t>(nf);
+    B.nodes.as<BNode>(2L * nf - 1);
+    B.child.as<int2>(nf > 1 ? nf - 1 : 1);
+    B.parent.as<int>(2L * nf - 1);
+    B.flags.as<unsigned>(nf > 1 ? nf - 1 : 1);
+    B.fbox.as<float4>(2L * nf);
+    B.partial.as<float4>(2L * cdiv(nf, 256));
+    B.scene.as<float4>(2);
+}
+
+void bottom_up(cs_ctx* C, Bvh& B) {
+    if (B.nf <= 1) return;
+    hipStream_t s = C->stream;
+    fill_u32(B.flags.ptr<uint32_t>(), 0u, B.nf - 1, s);
+    hipLaunchKernelGGL(k_bottom_up, dim3(cdiv(B.nf, 256)), dim3(256), 0, s, B.nf, B.child.ptr<int2>(),
+                       B.parent.ptr<int>(), B.nodes.ptr<BNode>(), B.flags.ptr<unsigned>());
+    CS_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void bvh_build(cs_ctx* C, int mesh, bool ccd) {
+    const MeshInfo& M = C->meshes[mesh];
+    Bvh& B = C->bvh[mesh];
+    B.mesh = mesh;
+    B.ccd = ccd;
+    ensure_bvh(C, B, M.nf);
+    hipStream_t s = C->stream;
+    const int nb = cdiv(M.nf, 256);
+    hipLaunchKernelGGL(k_leaf_boxes, dim3(nb), dim3(256), 0, s, M.f0, M.nf, ccd, C->x.ptr<float4>(),
+                       C->x0.ptr<float4>(), C->fnodes.ptr<int4>(), B.fbox.ptr<float4>(), B.partial.ptr<float4>());
+    CS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scene_box, dim3(1), dim3(256), 0, s, nb, B.partial.ptr<float4>(), B.scene.ptr<float4>());
+    CS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_morton, dim3(nb), dim3(256), 0, s, M.nf, B.fbox.ptr<float4>(), B.scene.ptr<float4>(),
+                       B.keys.ptr<uint64_t>(), B.vals.ptr<uint32_t>());
+    CS_LAUNCH_CHECK();
+    radix_sort_pairs(B.keys.ptr<uint64_t>(), B.vals.ptr<uint32_t>(), B.keysTmp.ptr<uint64_t>(),
+                     B.valsTmp.ptr<uint32_t>(), M.nf, 0, 30, C->ws, s);
+    hipLaunchKernelGGL(k_fill_leaves, dim3(nb), dim3(256), 0, s, M.nf, M.f0, B.keys.ptr<uint64_t>(),
+                       B.vals.ptr<uint32_t>(), B.fbox.ptr<float4>(), B.nodes.ptr<BNode>());
+    CS_LAUNCH_CHECK();
+    if (M.nf > 1) {
+        hipLaunchKernelGGL(k_karras, dim3(cdiv(M.nf - 1, 256)), dim3(256), 0, s, M.nf - 1, B.keys.ptr<uint64_t>(),
+                           B.child.ptr<int2>(), B.parent.ptr<int>());
+        CS_LAUNCH_CHECK();
+        bottom_up(C, B);
+    }
+}
+
+void bvh_refit(cs_ctx* C, int mesh) {
+    Bvh& B = C->bvh[mesh];
+    hipLaunchKernelGGL(k_refit_leaves, dim3(cdiv(B.nf, 256)), dim3(256), 0, C->stream, B.nf, B.ccd,
+                       C->x.ptr<float4>(), C->x0.ptr<float4>(), C->fnodes.ptr<int4>(), B.nodes.ptr<BNode>());
+    CS_LAUNCH_CHECK();
+    bottom_up(C, B);
+}
+
+// Simulator::traverse order (src/Simulator.cu:179-196): for each cloth, itself,
+// then every earlier cloth, then every obstacle.
+template <class F> static void for_each_traversal(cs_ctx* C, F&& f) {
+    for (int c = 0; c < C->nCloths; c++) {
+        f(c, c, true);
+        for (int j = 0; j < c; j++) f(c, j, false);
+        for (int o = C->nCloths; o < (int)C->meshes.size(); o++) f(c, o, false);
+    }
+}
+
+long bvh_all_pairs(cs_ctx* C, float th) {
+    hipStream_t s = C->stream;
+    unsigned* counter = C->pairCount.as<unsigned>(4);
+    if (C->pairCap == 0) {
+        C->pairCap = std::max<size_t>(1 << 16, 16 * (size_t)C->FC);
+        C->pairs.as<int2>(C->pairCap);
+    }
+    for (int attempt = 0; attempt < 2; attempt++) {
+        fill_u32(counter, 0u, 2, s);
+        for_each_traversal(C, [&](int q, int t, bool self) {
+            Bvh& Q = C->bvh[q];
+            Bvh& T = C->bvh[t];
+            dim3 g(cdiv(Q.nf, TB));
+            if (self)
+                hipLaunchKernelGGL((k_traverse<true, 2>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
+                                   T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, nullptr,
+                                   C->pairs.ptr<int2>(), counter, (long)C->pairCap, counter + 1);
+            else
+                hipLaunchKernelGGL((k_traverse<false, 2>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
+                                   T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, nullptr,
+                                   C->pairs.ptr<int2>(), counter, (long)C->pairCap, counter + 1);
+            CS_LAUNCH_CHECK();
+        });
+        CS_HIP(hipMemcpyAsync(C->hCount, counter, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipStreamSynchronize(s));
+        if (C->hCount[1]) throw StatusError(CS_E_CAPACITY, "LBVH traversal stack overflow (tree deeper than 64)");
+        size_t n = C->hCount[0];
+        if (n <= C->pairCap) return (long)n;
+        C->pairCap = n + n / 2;
+        C->pairs.as<int2>(C->pairCap);
+    }
+    throw StatusError(CS_E_CAPACITY, "pair buffer growth failed");
+}
+
+long bvh_pairs_ordered(cs_ctx* C, float th, int* host_out, long cap) {
+    hipStream_t s = C->stream;
+    long total = 0;
+    std::vector<int> hbuf;
+    unsigned* flag = C->pairCount.as<unsigned>(4);
+    fill_u32(flag, 0u, 4, s);
+    for_each_traversal(C, [&](int q, int t, bool self) {
+        Bvh& Q = C->bvh[q];
+        Bvh& T = C->bvh[t];
+        dim3 g(cdiv(Q.nf, TB));
+        uint32_t* num = C->ws.a.as<uint32_t>(Q.nf + 1);
+        if (self)
+            hipLaunchKernelGGL((k_traverse<true, 0>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
+                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, num, nullptr, nullptr, nullptr, 0L,
+                               flag + 1);
+        else
+            hipLaunchKernelGGL((k_traverse<false, 0>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
+                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, num, nullptr, nullptr, nullptr, 0L,
+                               flag + 1);
+        CS_LAUNCH_CHECK();
+        uint32_t* offs = C->ws.b.as<uint32_t>(Q.nf + 1);
+        scan_exclusive(num, offs, Q.nf, C->ws, s);
+        CS_HIP(hipMemcpyAsync(C->hCount, offs + Q.nf, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipStreamSynchronize(s));
+        long n = C->hCount[0];
+        int2* out = C->pairs.as<int2>(std::max<long>(n, 1));
+        C->pairCap = 0;  // the append buffer was reused; let bvh_all_pairs size it again
+        if (self)
+            hipLaunchKernelGGL((k_traverse<true, 1>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
+                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, offs, out, nullptr, 0L,
+                               flag + 1);
+        else
+            hipLaunchKernelGGL((k_traverse<false, 1>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
+                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, offs, out, nullptr, 0L,
+                               flag + 1);
+        CS_LAUNCH_CHECK();
+        if (n > 0) {
+            size_t old = hbuf.size();
+            hbuf.resize(old + 2 * (size_t)n);
+            CS_HIP(hipMemcpyAsync(hbuf.data() + old, out, sizeof(int2) * n, hipMemcpyDeviceToHost, s));
+            CS_HIP(hipStreamSynchronize(s));
+        }
+        total += n;
+    });
+    CS_HIP(hipMemcpyAsync(C->hCount, flag + 1, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    CS_HIP(hipStreamSynchronize(s));
+    if (C->hCount[0]) throw StatusError(CS_E_CAPACITY, "LBVH traversal stack overflow (tree deeper than 64)");
+    if (host_out) {
+        long m = std::min(total, cap);
+        std::copy(hbuf.begin(), hbuf.begin() + 2 * m, host_out);
+    }
+    return total;
+}
+
+}  // namespace cs

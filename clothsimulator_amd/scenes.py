@@ -1,0 +1,179 @@
+"""Synthetic scenes of the benchmark configurations.
+
+Every generator is deterministic.  Meshes are returned as ``MeshArrays`` and
+scenes as ``SceneSpec`` (params, cloths, obstacles, handles) that can be fed to
+the product (``clothsimulator_amd.Simulator``) or to the CPU oracle alike.
+
+Workloads (cloth side n means an n x n node grid, quads split along the main
+diagonal, uv = (x, y) in metres as the reference assigns to OBJ faces without
+'/', src/Mesh.cu:334-338):
+
+* hang33    1 m x 1 m 33^2 grid at z = 0, the two corners of the y = 1 edge are
+            handles, dt = 5 ms, no obstacle.
+* drape129  129^2 grid, 1 m x 1 m centred 0.4 m above a fixed icosphere of
+            radius 0.3 (642 nodes / 1280 faces), dt = 5 ms.
+* crumple513 513^2 grid, z = 0.5 m + N(0, 1e-3 m) i.i.d. (splitmix64 +
+            Box-Muller, seed 0), over a ground plane of two triangles, dt = 2 ms.
+"""
+import math
+from dataclasses import dataclass, field
+from typing import List, Tuple
+
+import numpy as np
+
+from ._cabi import MeshArrays, default_params, material_struct
+
+
+def grid_mesh(n, size=1.0, origin=(0.0, 0.0, 0.0)):
+    """n x n node grid over [0, size]^2 (+origin); node (i, j) -> index j*n + i.
+    Quad (i, j)-(i+1, j+1) is split along its main diagonal into
+    (a, b, c) and (a, c, d) with a=(i,j), b=(i+1,j), c=(i+1,j+1), d=(i,j+1)."""
+    h = size / (n - 1)
+    jj, ii = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    x = np.zeros((n * n, 3), dtype=np.float64)
+    x[:, 0] = ii.ravel() * h + origin[0]
+    x[:, 1] = jj.ravel() * h + origin[1]
+    x[:, 2] = origin[2]
+    i = np.arange(n - 1)
+    I, J = np.meshgrid(i, i, indexing="xy")
+    a = (J * n + I).ravel()
+    b = a + 1
+    c = a + n + 1
+    d = a + n
+    faces = np.empty((2 * (n - 1) * (n - 1), 3), dtype=np.int32)
+    faces[0::2] = np.stack([a, b, c], axis=1)
+    faces[1::2] = np.stack([a, c, d], axis=1)
+    return x, faces
+
+
+def icosphere(level=3, radius=1.0):
+    """Subdivided icosahedron: level 3 gives 642 nodes and 1280 faces, the
+    counts of the reference's unit sphere asset."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+             (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache = {}
+        new_faces = []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for (a, b, c) in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            new_faces += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = new_faces
+    x = np.array(verts) * radius
+    return x, np.array(faces, dtype=np.int32)
+
+
+def splitmix64_normals(count, seed=0):
+    """i.i.d. N(0, 1) from splitmix64 and the Box-Muller transform in FP64
+    (a project-defined generator, reproducible across platforms)."""
+    mask = (1 << 64) - 1
+    state = seed & mask
+    out = np.empty(count, dtype=np.float64)
+    k = 0
+    while k < count:
+        us = []
+        for _ in range(2):
+            state = (state + 0x9E3779B97F4A7C15) & mask
+            z = state
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+            z = z ^ (z >> 31)
+            us.append(((z >> 11) + 0.5) / float(1 << 53))
+        r = math.sqrt(-2.0 * math.log(us[0]))
+        out[k] = r * math.cos(2.0 * math.pi * us[1])
+        if k + 1 < count:
+            out[k + 1] = r * math.sin(2.0 * math.pi * us[1])
+        k += 2
+    return out
+
+
+def splitmix64_normals_fast(count, seed=0):
+    """Vectorised splitmix64 + Box-Muller; identical output to splitmix64_normals."""
+    n_pairs = (count + 1) // 2
+    idx = np.arange(1, 2 * n_pairs + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) + idx * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = ((z >> np.uint64(11)).astype(np.float64) + 0.5) / float(1 << 53)
+    u1, u2 = u[0::2], u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    out = np.empty(2 * n_pairs, dtype=np.float64)
+    out[0::2] = r * np.cos(2.0 * np.pi * u2)
+    out[1::2] = r * np.sin(2.0 * np.pi * u2)
+    return out[:count]
+
+
+@dataclass
+class ClothSpec:
+    mesh: MeshArrays
+    material: object
+    handles: List[int] = field(default_factory=list)
+
+
+@dataclass
+class SceneSpec:
+    name: str
+    params: object
+    cloths: List[ClothSpec]
+    obstacles: List[MeshArrays]
+    steps: int
+
+    @property
+    def n_cloth_nodes(self):
+        return sum(c.mesh.n_nodes for c in self.cloths)
+
+
+def hang(n=33, steps=100):
+    x, f = grid_mesh(n, 1.0)
+    mesh = MeshArrays(x, f)
+    handles = [(n - 1) * n + 0, (n - 1) * n + (n - 1)]
+    return SceneSpec("hang%d" % n, default_params(dt=0.005), [ClothSpec(mesh, material_struct(), handles)], [], steps)
+
+
+def drape(n=129, steps=400, sphere_level=3):
+    x, f = grid_mesh(n, 1.0, origin=(-0.5, -0.5, 0.4))
+    cloth = MeshArrays(x, f)
+    sx, sf = icosphere(sphere_level, 0.3)
+    return SceneSpec("drape%d" % n, default_params(dt=0.005), [ClothSpec(cloth, material_struct())],
+                     [MeshArrays(sx, sf)], steps)
+
+
+def ground_plane(half=2.0, z=0.0):
+    x = np.array([[-half, -half, z], [half, -half, z], [half, half, z], [-half, half, z]], dtype=np.float64)
+    f = np.array([[0, 1, 2], [0, 2, 3]], dtype=np.int32)
+    return MeshArrays(x, f)
+
+
+def crumple(n=513, steps=300, seed=0, height=0.5, sigma=1e-3):
+    x, f = grid_mesh(n, 1.0, origin=(-0.5, -0.5, height))
+    x[:, 2] += sigma * splitmix64_normals_fast(n * n, seed)
+    # uv from the unperturbed (x, y): the perturbation is out of plane
+    cloth = MeshArrays(x, f)
+    return SceneSpec("crumple%d" % n, default_params(dt=0.002), [ClothSpec(cloth, material_struct())],
+                     [ground_plane()], steps)
+
+
+def by_name(name):
+    table = {"hang33": lambda: hang(33), "drape129": lambda: drape(129), "crumple513": lambda: crumple(513)}
+    if name in table:
+        return table[name]()
+    # parametrised small variants: hangN, drapeN, crumpleN
+    for prefix, fn in (("hang", hang), ("drape", drape), ("crumple", crumple)):
+        if name.startswith(prefix) and name[len(prefix):].isdigit():
+            return fn(int(name[len(prefix):]))
+    raise KeyError(name)

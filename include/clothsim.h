@@ -1,0 +1,186 @@
+/*
+ * clothsim.h — C-ABI of the MI355X-native cloth step (libclothsim.so).
+ *
+ * This is the drop-in boundary for the reference's `--gpu` per-step hot path.
+ * The reference has no FFI of its own: the hot path sits behind C++ methods
+ * selected by the global `gpu` flag.  Each entry point below names the
+ * reference interface it replaces (paths relative to the reference root).
+ *
+ * Conventions
+ *  - Plain C types only: pointers, sizes, POD structs.  No torch types.
+ *  - Every call returns cs_status; nothing calls exit()/abort().
+ *  - The caller owns every host buffer; the context owns every device buffer
+ *    and one HIP stream.  Calls on one context are serialised on that stream
+ *    and return after the stream completes.  A context is not thread-safe;
+ *    distinct contexts may be used from distinct threads.
+ *  - Indices are 0-based.  Global node/face/edge numbering is: all cloths in
+ *    creation order, then all obstacles in creation order (the order in which
+ *    the reference's MemoryPool hands out records, src/MemoryPool.cu:79-116),
+ *    and within a mesh: nodes and faces in OBJ order, edges sorted by their
+ *    (min node, max node) pair (src/Mesh.cu:405-447).
+ */
+#ifndef CLOTHSIM_H
+#define CLOTHSIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cs_status {
+    CS_OK = 0,
+    CS_E_ARG = 1,              /* invalid argument / size */
+    CS_E_HIP = 2,              /* HIP runtime error */
+    CS_E_OOM = 3,              /* device allocation failed */
+    CS_E_NOT_CONVERGED = 4,    /* PCG hit pcg_max_iters (the step still completes) */
+    CS_E_COLLISION_FAILED = 5, /* reference prints "Collision step failed!" and exit(1)s: src/Simulator.cu:668-671 */
+    CS_E_FORMAT = 6,           /* malformed mesh / material input */
+    CS_E_STATE = 7,            /* call not valid in the context's current state */
+    CS_E_CAPACITY = 8,         /* caller buffer or internal fixed limit too small */
+    CS_E_NO_DEVICE = 9         /* no usable gfx950 device */
+} cs_status;
+
+typedef struct cs_ctx cs_ctx; /* opaque context: device buffers + one HIP stream */
+
+enum { CS_PRECOND_BLOCK_JACOBI = 0, CS_PRECOND_LEARNED = 1 };
+enum { CS_MESH_CLOTH = 0, CS_MESH_OBSTACLE = 1 };
+
+/* cs_params.flags */
+enum {
+    CS_FLAG_NO_PROXIMITY = 1, /* skip repulsion/friction proximities (testing only; the reference always runs them) */
+    CS_FLAG_NO_COLLISION = 2  /* cs_step skips the collision step (testing only) */
+};
+
+/* Scene-level parameters.  Defaults (cs_default_params) are the reference's:
+ * Magic ctor src/Magic.cu:5-17, Wind ctor src/Wind.cu:3-7, Simulator ctor
+ * src/Simulator.cu:19-27, MAX_COLLISION_ITERATION src/CollisionHelper.cuh:21. */
+typedef struct cs_params {
+    float dt;
+    float gravity[3];
+    float wind_density, wind_drag, wind_velocity[3];
+    float handle_stiffness, collision_stiffness;
+    float repulsion_thickness, collision_thickness;
+    float friction, obstacle_friction;
+    float pcg_rel_tol;     /* stop when ||r|| <= pcg_rel_tol * ||b||   (default 1e-5) */
+    int pcg_max_iters;     /* default 4000 */
+    int precond;           /* CS_PRECOND_* */
+    int max_collision_iters; /* default 100 */
+    int flags;
+} cs_params;
+
+/* One triangle mesh, already placed in world space (the reference applies the
+ * JSON "transform" while reading the OBJ, src/Mesh.cu:320-338).
+ * x: 3*n_nodes, v: 3*n_nodes or NULL (zero), uv: 2*n_verts (material space, metres),
+ * face_nodes / face_verts: 3*n_faces. */
+typedef struct cs_mesh_in {
+    int n_nodes;
+    const float* x;
+    const float* v;
+    int n_verts;
+    const float* uv;
+    int n_faces;
+    const int* face_nodes;
+    const int* face_verts;
+} cs_mesh_in;
+
+/* Material JSON content plus the cloth's "materials" multipliers
+ * (src/Material.cu:3-41). */
+typedef struct cs_material_in {
+    float density;
+    float stretching[6][4];
+    float bending[3][5];
+    float thicken, density_mult, stretching_mult, bending_mult;
+} cs_material_in;
+
+/* Per-call statistics. */
+typedef struct cs_stats {
+    int pcg_iters;            /* summed over cloths */
+    float pcg_rel_residual;   /* max over cloths of ||r||/||b|| at exit (recursive residual) */
+    int pcg_converged;        /* 1 if every cloth reached pcg_rel_tol */
+    int n_proximities;
+    int n_contact_blocks;     /* dynamic 3x3 blocks merged into the system */
+    int collision_iters;      /* impact-response iterations of the last collision step */
+    int collision_deform;     /* 0 or 1: the deform pass that succeeded */
+    int n_impacts;            /* impacts resolved in the last collision step */
+    float ms_bvh, ms_proximity, ms_assemble, ms_solve, ms_update, ms_collision;
+} cs_stats;
+
+/* One repulsion/friction record (src/Proximity.cuh:10-21), nodes are global ids. */
+typedef struct cs_proximity_rec {
+    int nodes[4];
+    float n[3];
+    float mu, stiffness;
+    float w[4];
+} cs_proximity_rec;
+
+/* One CCD impact (src/Impact.cuh:7-15), computed in FP64.  type: 0 vertex-face
+ * (nodes[0] is the vertex), 1 edge-edge. */
+typedef struct cs_impact_rec {
+    int type;
+    int nodes[4];
+    double t;
+    double w[4];
+    double n[3];
+} cs_impact_rec;
+
+/* ---- context lifetime (replaces Simulator::Simulator / ~Simulator, src/Simulator.cu:3-135) ---- */
+const char* cs_version(void);
+void cs_default_params(cs_params* p);
+cs_status cs_create(const cs_params* params, cs_ctx** ctx_out);
+void cs_destroy(cs_ctx* ctx);
+const char* cs_last_error(const cs_ctx* ctx);
+cs_status cs_set_params(cs_ctx* ctx, const cs_params* params);
+
+/* ---- scene construction ---- */
+/* replaces Cloth::Cloth(json, pool) (src/Cloth.cu:3-23) + Mesh::load (src/Mesh.cu:306-452) */
+cs_status cs_add_cloth(cs_ctx* ctx, const cs_mesh_in* mesh, const cs_material_in* material, int* cloth_id);
+/* replaces Obstacle::Obstacle(json, transformations, pool) (src/Obstacle.cu:3-11); static obstacles only */
+cs_status cs_add_obstacle(cs_ctx* ctx, const cs_mesh_in* mesh, int* obstacle_id);
+/* replaces Cloth::addHandles(nodeIndices, motionIndex=-1, ...) (src/Cloth.cu:649-673); anchors = current x */
+cs_status cs_add_handles(cs_ctx* ctx, int cloth_id, const int* nodes, int n);
+
+/* ---- stepping ---- */
+/* replaces Simulator::physicsStep (src/Simulator.cu:571-600) incl. Cloth::physicsStep GPU (src/Cloth.cu:695-813) */
+cs_status cs_physics_step(cs_ctx* ctx, cs_stats* stats);
+/* replaces Simulator::collisionStep (src/Simulator.cu:602-684) */
+cs_status cs_collision_step(cs_ctx* ctx, cs_stats* stats);
+/* physics then collision: the per-step part of Simulator::simulateStep (src/Simulator.cu:838-897) */
+cs_status cs_step(cs_ctx* ctx, cs_stats* stats);
+/* run `steps` steps back to back with no host readback in between (benchmarks) */
+cs_status cs_run(cs_ctx* ctx, int steps, cs_stats* stats_accum);
+
+/* ---- state exchange (replaces Mesh::save/load node I/O, src/Mesh.cu:454-507) ---- */
+cs_status cs_get_counts(cs_ctx* ctx, int kind, int id, int* n_nodes, int* n_faces, int* n_edges);
+cs_status cs_get_state(cs_ctx* ctx, int kind, int id, float* x, float* x0, float* v);
+cs_status cs_set_state(cs_ctx* ctx, int kind, int id, const float* x, const float* x0, const float* v);
+/* node normal (3n), area (n), mass (n) as maintained by Mesh::updateNodeGeometries (src/Mesh.cu:182-214) */
+cs_status cs_get_geometry(cs_ctx* ctx, int kind, int id, float* normals, float* area, float* mass);
+
+/* ---- sub-stage entry points (parity tests run them on the context's current state) ---- */
+/* Material::Material stretching table (30^3 x 4) and bending table (3x5) (src/Material.cu:3-85) */
+cs_status cs_material_table(const cs_material_in* material, float* table, float* density, float* bending);
+/* BVH(mesh, ccd) + Simulator::traverse (src/BVH.cu:3-51, src/Simulator.cu:179-196): candidate face pairs
+ * (global face ids, first = query face) over cloth self, cloth x earlier cloth, cloth x obstacle. */
+cs_status cs_lbvh_pairs(cs_ctx* ctx, int ccd, float thickness, int* pairs, int cap, int* n_out);
+/* Simulator::findProximitiesGpu (src/Simulator.cu:300-349) */
+cs_status cs_find_proximities(cs_ctx* ctx, cs_proximity_rec* out, int cap, int* n_out);
+/* Assembled implicit-Euler system of one cloth after proximity detection (src/Cloth.cu:695-764):
+ * A as BSR (3x3 row-major blocks, rows sorted, columns sorted and unique), b (3n).
+ * Pass NULL buffers to query nnzb only. */
+cs_status cs_assemble(cs_ctx* ctx, int cloth_id, int* row_ptr, int* col_idx, float* vals, int cap_blocks,
+                      int* nnzb_out, float* b);
+/* Solve the last assembled system of one cloth (src/Cloth.cu:766-809): dv (3n). */
+cs_status cs_solve(cs_ctx* ctx, int cloth_id, float* dv, cs_stats* stats);
+/* Simulator::findImpacts (src/Simulator.cu:383-392), deduplicated and sorted by (t, type, nodes). */
+cs_status cs_find_impacts(cs_ctx* ctx, float thickness, cs_impact_rec* out, int cap, int* n_out);
+/* Simulator::independentImpacts GPU (src/Simulator.cu:422-461) on caller-provided impacts
+ * (sorted by t, as cs_find_impacts returns them); writes the selected subset in order. */
+cs_status cs_independent_impacts(cs_ctx* ctx, const cs_impact_rec* in, int n, int deform,
+                                 cs_impact_rec* out, int* n_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLOTHSIM_H */

@@ -1,0 +1,278 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from clothsimulator_amd._cabi import (IMPACT_DTYPE, PROX_DTYPE, ImpactRec, MaterialIn, MeshIn, Params,
+                                      ProximityRec, Stats, dptr, fptr, iptr)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.or_create.restype = vp
+        L.or_create.argtypes = [C.POINTER(Params), C.c_int]
+        L.or_destroy.argtypes = [vp]
+        L.or_set_params.argtypes = [vp, C.POINTER(Params)]
+        L.or_add_cloth.argtypes = [vp, C.POINTER(MeshIn), C.POINTER(MaterialIn), C.POINTER(C.c_int)]
+        L.or_add_obstacle.argtypes = [vp, C.POINTER(MeshIn), C.POINTER(C.c_int)]
+        L.or_add_handles.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int]
+        L.or_physics_step.argtypes = [vp, C.c_int, C.POINTER(Stats)]
+        L.or_collision_step.argtypes = [vp, C.POINTER(Stats)]
+        L.or_step.argtypes = [vp, C.c_int, C.POINTER(Stats)]
+        L.or_get_counts.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_get_state.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.or_get_state64.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_set_state.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.or_get_geometry.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.or_get_edges.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_material_table.argtypes = [C.POINTER(MaterialIn), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.or_stretching_stiffness.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.or_stretching_force.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_bending_force.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_lbvh_keys.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_uint64)]
+        L.or_lbvh_pairs.argtypes = [vp, C.c_int, C.c_float, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
+        L.or_find_proximities.argtypes = [vp, C.POINTER(ProximityRec), C.c_int, C.POINTER(C.c_int)]
+        L.or_assemble.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int,
+                                  C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.or_solve_bsr.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.or_find_impacts.argtypes = [vp, C.c_float, C.POINTER(ImpactRec), C.c_int, C.POINTER(C.c_int)]
+        L.or_independent_impacts.argtypes = [vp, C.POINTER(ImpactRec), C.c_int, C.c_int, C.POINTER(ImpactRec),
+                                             C.POINTER(C.c_int)]
+        L.or_solve_cubic.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _check(st, what):
+    if st != 0:
+        raise RuntimeError("oracle %s failed with status %d" % (what, st))
+
+
+def material_table(mat):
+    L = lib()
+    table = np.zeros(30 * 30 * 30 * 4, dtype=np.float32)
+    bend = np.zeros(15, dtype=np.float32)
+    dens = C.c_float()
+    _check(L.or_material_table(C.byref(mat), fptr(table), C.byref(dens), fptr(bend)), "material_table")
+    return table.reshape(30, 30, 30, 4), dens.value, bend.reshape(3, 5)
+
+
+def solve_bsr(row_ptr, col, vals, b, direct=False, fp64=True, tol=1e-5, max_iters=4000):
+    L = lib()
+    n = len(row_ptr) - 1
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    ci = np.ascontiguousarray(col, dtype=np.int32)
+    vv = np.ascontiguousarray(vals, dtype=np.float64)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(3 * n, dtype=np.float64)
+    it = C.c_int()
+    rr = C.c_double()
+    _check(L.or_solve_bsr(n, iptr(rp), iptr(ci), dptr(vv), dptr(bb), int(direct), int(fp64), tol, max_iters,
+                          dptr(x), C.byref(it), C.byref(rr)), "solve_bsr")
+    return x, it.value, rr.value
+
+
+def solve_cubic(a, b, c, d):
+    r = (C.c_double * 3)()
+    n = lib().or_solve_cubic(a, b, c, d, r)
+    return [r[i] for i in range(n)]
+
+
+class OracleScene:
+    """CPU oracle scene.  fp64=False restates the reference's FP32 path."""
+
+    def __init__(self, spec=None, fp64=False, params=None):
+        self.L = lib()
+        self.params = params if params is not None else (spec.params if spec is not None else None)
+        self.h = self.L.or_create(C.byref(self.params), int(fp64))
+        self.fp64 = fp64
+        self._keep = []
+        self.cloth_nodes = []
+        if spec is not None:
+            for c in spec.cloths:
+                cid = self.add_cloth(c.mesh, c.material)
+                if c.handles:
+                    self.add_handles(cid, c.handles)
+            for o in spec.obstacles:
+                self.add_obstacle(o)
+
+    def close(self):
+        if self.h:
+            self.L.or_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        self.params = p
+        _check(self.L.or_set_params(self.h, C.byref(p)), "set_params")
+
+    def add_cloth(self, mesh, mat):
+        cid = C.c_int()
+        m = mesh.c_struct()
+        self._keep.append((mesh, mat))
+        _check(self.L.or_add_cloth(self.h, C.byref(m), C.byref(mat), C.byref(cid)), "add_cloth")
+        self.cloth_nodes.append(mesh.n_nodes)
+        return cid.value
+
+    def add_obstacle(self, mesh):
+        oid = C.c_int()
+        m = mesh.c_struct()
+        self._keep.append(mesh)
+        _check(self.L.or_add_obstacle(self.h, C.byref(m), C.byref(oid)), "add_obstacle")
+        return oid.value
+
+    def add_handles(self, cloth, nodes):
+        a = np.ascontiguousarray(nodes, dtype=np.int32)
+        _check(self.L.or_add_handles(self.h, cloth, iptr(a), len(a)), "add_handles")
+
+    def physics_step(self, direct=False):
+        st = Stats()
+        s = self.L.or_physics_step(self.h, int(direct), C.byref(st))
+        return s, st
+
+    def collision_step(self):
+        st = Stats()
+        s = self.L.or_collision_step(self.h, C.byref(st))
+        return s, st
+
+    def step(self, direct=False):
+        st = Stats()
+        s = self.L.or_step(self.h, int(direct), C.byref(st))
+        return s, st
+
+    def counts(self, kind=0, idx=0):
+        nn, nf, ne = C.c_int(), C.c_int(), C.c_int()
+        _check(self.L.or_get_counts(self.h, kind, idx, C.byref(nn), C.byref(nf), C.byref(ne)), "get_counts")
+        return nn.value, nf.value, ne.value
+
+    def state(self, kind=0, idx=0, fp64=False):
+        nn = self.counts(kind, idx)[0]
+        if fp64:
+            x, x0, v = (np.zeros((nn, 3), dtype=np.float64) for _ in range(3))
+            _check(self.L.or_get_state64(self.h, kind, idx, dptr(x), dptr(x0), dptr(v)), "get_state64")
+        else:
+            x, x0, v = (np.zeros((nn, 3), dtype=np.float32) for _ in range(3))
+            _check(self.L.or_get_state(self.h, kind, idx, fptr(x), fptr(x0), fptr(v)), "get_state")
+        return x, x0, v
+
+    def set_state(self, kind, idx, x=None, x0=None, v=None):
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+        x, x0, v = conv(x), conv(x0), conv(v)
+        nul = C.POINTER(C.c_float)()
+        _check(self.L.or_set_state(self.h, kind, idx, fptr(x) if x is not None else nul,
+                                   fptr(x0) if x0 is not None else nul, fptr(v) if v is not None else nul),
+               "set_state")
+
+    def geometry(self, kind=0, idx=0):
+        nn = self.counts(kind, idx)[0]
+        n = np.zeros((nn, 3), dtype=np.float32)
+        a = np.zeros(nn, dtype=np.float32)
+        m = np.zeros(nn, dtype=np.float32)
+        _check(self.L.or_get_geometry(self.h, kind, idx, fptr(n), fptr(a), fptr(m)), "get_geometry")
+        return n, a, m
+
+    def edges(self, kind=0, idx=0):
+        ne = self.counts(kind, idx)[2]
+        nodes = np.zeros((ne, 2), dtype=np.int32)
+        opp = np.zeros((ne, 2), dtype=np.int32)
+        adj = np.zeros((ne, 2), dtype=np.int32)
+        _check(self.L.or_get_edges(self.h, kind, idx, iptr(nodes), iptr(opp), iptr(adj)), "get_edges")
+        return nodes, opp, adj
+
+    def stretching_stiffness(self, cloth, g00, g01, g11):
+        k = np.zeros(4, dtype=np.float64)
+        _check(self.L.or_stretching_stiffness(self.h, cloth, g00, g01, g11, dptr(k)), "stretching_stiffness")
+        return k
+
+    def stretching_force(self, face):
+        f = np.zeros(9)
+        J = np.zeros(81)
+        _check(self.L.or_stretching_force(self.h, face, dptr(f), dptr(J)), "stretching_force")
+        return f, J.reshape(9, 9)
+
+    def bending_force(self, edge):
+        f = np.zeros(12)
+        J = np.zeros(144)
+        ang = C.c_double()
+        _check(self.L.or_bending_force(self.h, edge, dptr(f), dptr(J), C.byref(ang)), "bending_force")
+        return f, J.reshape(12, 12), ang.value
+
+    def lbvh_keys(self, mesh, ccd=False):
+        nf = None
+        # mesh is a global mesh index; count faces through the cloth/obstacle accessor
+        ncl = len(self.cloth_nodes)
+        kind, idx = (0, mesh) if mesh < ncl else (1, mesh - ncl)
+        nf = self.counts(kind, idx)[1]
+        k = np.zeros(nf, dtype=np.uint64)
+        _check(self.L.or_lbvh_keys(self.h, mesh, int(ccd), k.ctypes.data_as(C.POINTER(C.c_uint64))), "lbvh_keys")
+        return k
+
+    def lbvh_pairs(self, ccd, thickness, brute=False):
+        n = C.c_int()
+        self.L.or_lbvh_pairs(self.h, int(ccd), thickness, int(brute), None, 0, C.byref(n))
+        pairs = np.zeros((max(n.value, 1), 2), dtype=np.int32)
+        _check(self.L.or_lbvh_pairs(self.h, int(ccd), thickness, int(brute), iptr(pairs), n.value, C.byref(n)),
+               "lbvh_pairs")
+        return pairs[:n.value]
+
+    def proximities(self, cap=1 << 20):
+        out = np.zeros(cap, dtype=PROX_DTYPE)
+        n = C.c_int()
+        _check(self.L.or_find_proximities(self.h, out.ctypes.data_as(C.POINTER(ProximityRec)), cap, C.byref(n)),
+               "find_proximities")
+        return out[:n.value].copy()
+
+    def assemble(self, cloth=0):
+        L = self.L
+        nnzb = C.c_int()
+        L.or_assemble(self.h, cloth, None, None, None, 0, C.byref(nnzb), None)
+        nn = self.counts(0, cloth)[0]
+        rp = np.zeros(nn + 1, dtype=np.int32)
+        ci = np.zeros(nnzb.value, dtype=np.int32)
+        vals = np.zeros((nnzb.value, 3, 3), dtype=np.float64)
+        b = np.zeros(3 * nn, dtype=np.float64)
+        _check(L.or_assemble(self.h, cloth, iptr(rp), iptr(ci), dptr(vals), nnzb.value, C.byref(nnzb), dptr(b)),
+               "assemble")
+        return rp, ci, vals, b
+
+    def impacts(self, thickness, cap=1 << 20):
+        out = np.zeros(cap, dtype=IMPACT_DTYPE)
+        n = C.c_int()
+        _check(self.L.or_find_impacts(self.h, thickness, out.ctypes.data_as(C.POINTER(ImpactRec)), cap, C.byref(n)),
+               "find_impacts")
+        return out[:n.value].copy()
+
+    def independent_impacts(self, imps, deform):
+        a = np.ascontiguousarray(imps, dtype=IMPACT_DTYPE)
+        out = np.zeros(max(len(a), 1), dtype=IMPACT_DTYPE)
+        n = C.c_int()
+        _check(self.L.or_independent_impacts(self.h, a.ctypes.data_as(C.POINTER(ImpactRec)), len(a), deform,
+                                             out.ctypes.data_as(C.POINTER(ImpactRec)), C.byref(n)),
+               "independent_impacts")
+        return out[:n.value].copy()
