@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X cloth step (one GPU).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload crumple513]
+
+A step is one full simulation step (physics: LBVH + proximities + implicit
+assembly + block-Jacobi PCG to 1e-5 relative residual + position/geometry
+update; collision: CCD LBVH + FP64 impacts + impact response + velocity
+update).  The default workload is the 513 x 513 crumple (263,169 nodes,
+524,288 faces, dt = 2 ms, ground plane), timed over its full 300 steps from the
+initial state with every input resident in HBM.  W warmup steps run first on
+the same context, which is then reset to the initial state.
+
+Prints one JSON line: node-steps/s, CG iterations per step, the roofline of the
+dominant kernel (the PCG SpMV, timed live with HIP events on the library's
+stream) and a CPU baseline (the oracle restatement on the host cores, on a
+bounded sample: the first step of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "node-steps/s + CG iters to 1e-5 rel residual: learned-precond vs Jacobi PCG"
+HBM_PEAK_GBS = 8000.0
+
+
+def static_nnzb(faces, n_nodes):
+    """Block count of the static pattern: node pairs sharing a face or a hinge stencil."""
+    f = np.asarray(faces, dtype=np.int64)
+    pairs = [f[:, [a, b]] for a in range(3) for b in range(3)]
+    e = np.sort(np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]]), axis=1)
+    opp = np.concatenate([f[:, 2], f[:, 0], f[:, 1]])
+    key = e[:, 0] * n_nodes + e[:, 1]
+    order = np.argsort(key, kind="stable")
+    ks, os_ = key[order], opp[order]
+    same = ks[1:] == ks[:-1]
+    flaps = np.stack([os_[:-1][same], os_[1:][same]], axis=1)
+    pairs.append(flaps)
+    pairs.append(flaps[:, ::-1])
+    allp = np.concatenate(pairs)
+    u = np.unique(allp[:, 0] * n_nodes + allp[:, 1])
+    return int(len(u))
+
+
+def read_traffic(workload):
+    """HBM bytes per SpMV launch measured with rocprofv3 PMC counters (committed under profiles/)."""
+    p = os.path.join(ROOT, "profiles", "pcg_spmv_%s_traffic.json" % workload)
+    if os.path.exists(p):
+        with open(p) as fh:
+            return json.load(fh).get("bytes_per_launch")
+    return None
+
+
+def cpu_baseline(spec, steps):
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    o = oracle.OracleScene(spec)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step(direct=False)
+    dt = time.perf_counter() - t0
+    n = spec.n_cloth_nodes
+    return {"value": n * steps / dt, "unit": "node-steps/s", "cores": threads, "kind": "port",
+            "sample": "first %d step(s) of %s from its initial state: oracle FP32 restatement (block-Jacobi PCG "
+                      "to 1e-5, FP64 CCD), OpenMP on %d threads, %.1f s" % (steps, spec.name, threads, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="crumple513")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (the cloth step has no "
+              "multi-GPU path here; see DESIGN.md for how it would shard)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from clothsimulator_amd import CS_MESH_CLOTH, CS_MESH_OBSTACLE, Simulator, scenes
+    from clothsimulator_amd._cabi import CS_FLAG_TIME_KERNELS, CS_KERNEL_PCG_SPMV
+
+    spec = scenes.by_name(a.workload)
+    steps = a.steps if a.steps is not None else spec.steps
+    spec.params.flags |= CS_FLAG_TIME_KERNELS
+    sim = Simulator(spec)
+    init = [sim.state(CS_MESH_CLOTH, i) for i in range(len(spec.cloths))]
+    obs = [sim.state(CS_MESH_OBSTACLE, i) for i in range(len(spec.obstacles))]
+    for _ in range(a.warmup):
+        sim.step()
+    for i, (x, x0, v) in enumerate(init):
+        sim.set_state(CS_MESH_CLOTH, i, x, x0, v)
+    for i, (x, x0, v) in enumerate(obs):
+        sim.set_state(CS_MESH_OBSTACLE, i, x, x0, v)
+    sim.kernel_time(CS_KERNEL_PCG_SPMV, reset=True)
+
+    iters, contacts, failed = [], [], 0
+    t0 = time.perf_counter()  # every library call returns after its stream has completed
+    for _ in range(steps):
+        s, st = sim.step()
+        if s not in (0, 4):
+            failed += 1
+        iters.append(st.pcg_iters)
+        contacts.append(st.n_contact_blocks)
+    elapsed = time.perf_counter() - t0
+
+    n_nodes = spec.n_cloth_nodes
+    value = n_nodes * steps / elapsed
+    spmv_ms, spmv_n = sim.kernel_time(CS_KERNEL_PCG_SPMV)
+    c0 = spec.cloths[0]
+    nnzb = static_nnzb(c0.mesh.faces, c0.mesh.n_nodes)
+    nnzb_dyn = nnzb + float(np.mean(contacts))
+    # algorithmic bytes of one SpMV launch: 36 B values + 4 B column index per block,
+    # d gathered once (12 B/node) and q written once (12 B/node)
+    bytes_per_launch = 40.0 * nnzb_dyn + 24.0 * n_nodes
+    avg_s = (spmv_ms / spmv_n) * 1e-3 if spmv_n else float("nan")
+    achieved = bytes_per_launch / avg_s / 1e9 if spmv_n else None
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": read_traffic(a.workload),
+                "kernel": "k_pcg_spmv", "avg_us": avg_s * 1e6, "launches_sampled": spmv_n,
+                "bytes_per_launch": bytes_per_launch}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "node-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": spec.name, "nodes": n_nodes, "faces": int(c0.mesh.n_faces), "dt": spec.params.dt,
+                   "precond": "block_jacobi", "rel_tol": 1e-5, "obstacles": len(spec.obstacles)},
+        "cg_iters_mean": float(np.mean(iters)),
+        "cg_iters_max": int(np.max(iters)),
+        "cg_iters_learned": None,
+        "failed_steps": failed,
+        "roofline": roofline,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scenes.by_name(a.workload), a.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -22,7 +22,7 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_add_cloth", "cs_add_obstacle", "cs_add_handles", "cs_physics_step", "cs_collision_step", "cs_step",
             "cs_run", "cs_get_counts", "cs_get_state", "cs_set_state", "cs_get_geometry", "cs_material_table",
             "cs_lbvh_pairs", "cs_find_proximities", "cs_assemble", "cs_solve", "cs_find_impacts",
-            "cs_independent_impacts"]
+            "cs_independent_impacts", "cs_kernel_time"]
 
 
 class ClothError(RuntimeError):
@@ -66,6 +66,7 @@ def load_library(path=LIB_PATH):
     L.cs_find_impacts.argtypes = [vp, C.c_float, C.POINTER(ImpactRec), C.c_int, C.POINTER(C.c_int)]
     L.cs_independent_impacts.argtypes = [vp, C.POINTER(ImpactRec), C.c_int, C.c_int, C.POINTER(ImpactRec),
                                          C.POINTER(C.c_int)]
+    L.cs_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]
     _lib = L
     return L
 
@@ -229,6 +230,13 @@ class Simulator:
         self._check(self.L.cs_find_impacts(self.h, thickness, out.ctypes.data_as(C.POINTER(ImpactRec)), cap,
                                            C.byref(n)), "cs_find_impacts")
         return out[:n.value].copy()
+
+    def kernel_time(self, kernel_id, reset=False):
+        """(summed ms, sampled launches) of a hot kernel; needs CS_FLAG_TIME_KERNELS in params.flags."""
+        tot = C.c_double()
+        cnt = C.c_int()
+        self._check(self.L.cs_kernel_time(self.h, kernel_id, C.byref(tot), C.byref(cnt), int(reset)), "cs_kernel_time")
+        return tot.value, cnt.value
 
     def independent_impacts(self, imps, deform):
         a = np.ascontiguousarray(imps, dtype=IMPACT_DTYPE)

@@ -31,6 +31,10 @@ CS_MESH_CLOTH = 0
 CS_MESH_OBSTACLE = 1
 CS_FLAG_NO_PROXIMITY = 1
 CS_FLAG_NO_COLLISION = 2
+CS_FLAG_TIME_KERNELS = 4
+CS_KERNEL_PCG_SPMV = 0
+CS_KERNEL_ASSEMBLE_ROWS = 1
+CS_KERNEL_FACE_RECORDS = 2
 
 
 class Params(C.Structure):

@@ -12,6 +12,41 @@
 
 using namespace cs;
 
+namespace cs {
+
+int ktimer_begin(cs_ctx* C, int kernel) {
+    if (!(C->P.flags & CS_FLAG_TIME_KERNELS)) return -1;
+    KernelTimer& K = C->kt;
+    int pair = (int)K.pending.size();
+    while ((int)K.events.size() < 2 * (pair + 1)) {
+        hipEvent_t e;
+        CS_HIP(hipEventCreate(&e));
+        K.events.push_back(e);
+    }
+    K.pending.push_back({kernel, pair});
+    CS_HIP(hipEventRecord(K.events[2 * pair], C->stream));
+    return pair;
+}
+
+void ktimer_end(cs_ctx* C, int token) {
+    if (token < 0) return;
+    CS_HIP(hipEventRecord(C->kt.events[2 * token + 1], C->stream));
+}
+
+void ktimer_collect(cs_ctx* C) {
+    KernelTimer& K = C->kt;
+    for (auto& p : K.pending) {
+        float ms = 0.0f;
+        CS_HIP(hipEventSynchronize(K.events[2 * p.second + 1]));
+        CS_HIP(hipEventElapsedTime(&ms, K.events[2 * p.second], K.events[2 * p.second + 1]));
+        K.total[p.first] += ms;
+        K.count[p.first] += 1;
+    }
+    K.pending.clear();
+}
+
+}  // namespace cs
+
 namespace {
 
 template <class F> cs_status guard(cs_ctx* C, F&& f) {
@@ -84,6 +119,7 @@ int physics_step(cs_ctx* C, cs_stats* st) {
     launch_face_geometry(C, 0, C->F);
     launch_node_geometry(C, 0, C->N);
     local.ms_update = T.lap(s);
+    ktimer_collect(C);
     if (st) {
         st->pcg_iters = local.pcg_iters;
         st->pcg_rel_residual = local.pcg_rel_residual;
@@ -220,6 +256,7 @@ void cs_destroy(cs_ctx* C) {
     if (C->stream) (void)hipStreamSynchronize(C->stream);
     if (C->hScal) (void)hipHostFree(C->hScal);
     if (C->hCount) (void)hipHostFree(C->hCount);
+    for (hipEvent_t e : C->kt.events) (void)hipEventDestroy(e);
     cs_ctx* c = C;
     // device buffers are released by the DBuf destructors
     hipStream_t s = c->stream;
@@ -568,6 +605,20 @@ cs_status cs_independent_impacts(cs_ctx* C, const cs_impact_rec* in, int n, int 
         CS_HIP(hipMemcpyAsync(out, dout, sizeof(cs_impact_rec) * m, hipMemcpyDeviceToHost, s));
         CS_HIP(hipStreamSynchronize(s));
         *n_out = m;
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_kernel_time(cs_ctx* C, int kernel_id, double* total_ms, int* count, int reset) {
+    if (!C || kernel_id < 0 || kernel_id >= CS_KERNEL_COUNT) return CS_E_ARG;
+    return guard(C, [&]() {
+        ktimer_collect(C);
+        if (total_ms) *total_ms = C->kt.total[kernel_id];
+        if (count) *count = C->kt.count[kernel_id];
+        if (reset) {
+            C->kt.total[kernel_id] = 0.0;
+            C->kt.count[kernel_id] = 0;
+        }
         return (int)CS_OK;
     });
 }

@@ -484,12 +484,14 @@ void launch_element_records(cs_ctx* C, int c) {
     const cs_params& P = C->P;
     hipStream_t s = C->stream;
     // records are indexed by global face / edge id (cloth faces and edges come first)
+    const int tok = ktimer_begin(C, CS_KERNEL_FACE_RECORDS);
     hipLaunchKernelGGL(k_face_records, dim3(cdiv(M.nf, 256)), dim3(256), 0, s, M.f0, M.nf, P.dt, P.wind_density,
                        P.wind_drag, P.wind_velocity[0], P.wind_velocity[1], P.wind_velocity[2], C->x.ptr<float4>(),
                        C->v.ptr<float4>(), C->fnodes.ptr<int4>(), C->finv.ptr<float4>(), C->farea.ptr<float>(),
                        C->fnrm.ptr<float4>(), C->fmat.ptr<int>(), C->matTable.ptr<float>(),
                        C->faceRec.ptr<float4>() + (size_t)M.f0 * FACE_REC);
     CS_LAUNCH_CHECK();
+    ktimer_end(C, tok);
     hipLaunchKernelGGL(k_edge_records, dim3(cdiv(M.ne, 256)), dim3(256), 0, s, M.e0, M.ne, P.dt, C->x.ptr<float4>(),
                        C->v.ptr<float4>(), C->enodes.ptr<int2>(), C->eopp.ptr<int2>(), C->eadj.ptr<int2>(),
                        C->farea.ptr<float>(), C->fnrm.ptr<float4>(), C->ebias.ptr<float4>(), C->fmat.ptr<int>(),
@@ -502,6 +504,7 @@ void launch_assemble_rows(cs_ctx* C, int c) {
     ClothSys& S = C->sys[c];
     const cs_params& P = C->P;
     size_t lds = (size_t)S.maxWidth * 9 * SLICE * sizeof(float);
+    const int tok = ktimer_begin(C, CS_KERNEL_ASSEMBLE_ROWS);
     hipLaunchKernelGGL(k_assemble_rows, dim3(S.nslices), dim3(SLICE), lds, C->stream, M.n0, M.nn, M.f0, M.e0, P.dt,
                        P.gravity[0], P.gravity[1], P.gravity[2], P.handle_stiffness, S.sliceOff.ptr<int>(),
                        S.sliceW.ptr<int>(), S.incPtr.ptr<int>(), S.inc.ptr<int2>(),
@@ -510,6 +513,7 @@ void launch_assemble_rows(cs_ctx* C, int c) {
                        S.hcount.ptr<float>(), S.hpos.ptr<float4>(), C->x.ptr<float4>(), C->v.ptr<float4>(),
                        S.vals.ptr<float>(), S.b.ptr<float>());
     CS_LAUNCH_CHECK();
+    ktimer_end(C, tok);
 }
 
 int assemble_contacts(cs_ctx* C, int c) {

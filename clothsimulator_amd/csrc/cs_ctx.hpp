@@ -95,6 +95,14 @@ struct PcgScalars {
     int iters, done, maxit, pad2;
 };
 
+// Sampled HIP-event timing of hot kernels (enabled by CS_FLAG_TIME_KERNELS).
+struct KernelTimer {
+    std::vector<hipEvent_t> events;              // pairs (start, stop)
+    std::vector<std::pair<int, int>> pending;    // (kernel id, pair index) awaiting collection
+    double total[CS_KERNEL_COUNT] = {0, 0, 0};
+    int count[CS_KERNEL_COUNT] = {0, 0, 0};
+};
+
 }  // namespace cs
 
 struct cs_ctx {
@@ -150,5 +158,5 @@ struct cs_ctx {
         optX, optNext, optGrad, optLambda, optCoef, optScal;
     uint32_t* hCount = nullptr;  // pinned small readback
     cs::Scratch ws;
-    hipEvent_t ev[8];
+    cs::KernelTimer kt;
 };

@@ -4,6 +4,11 @@
 
 namespace cs {
 
+// sampled kernel timing (cs_api.cpp): begin returns a token (-1 when disabled)
+int ktimer_begin(cs_ctx* C, int kernel);
+void ktimer_end(cs_ctx* C, int token);
+void ktimer_collect(cs_ctx* C);  // call after a stream synchronisation
+
 // host set-up (cs_host.cpp)
 void material_build(const cs_material_in& in, HostMaterial& m);
 int host_add_mesh(cs_ctx* C, const cs_mesh_in& in, bool cloth, int material);

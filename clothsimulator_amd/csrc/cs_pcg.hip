@@ -209,9 +209,13 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     int done = 0;
     for (int it = 0; it < maxit + chunk && !done; it += chunk) {
         for (int k = 0; k < chunk; k++) {
+            // the first SpMV of a chunk is sampled: the host saw `done == 0` after the previous chunk,
+            // so this launch always does the full product
+            const int tok = k == 0 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
             hipLaunchKernelGGL(k_pcg_spmv, dim3(nb), dim3(PB), 0, s, nn, S.sliceOff.ptr<int>(), S.sliceW.ptr<int>(),
                                S.cols.ptr<int>(), S.vals.ptr<float>(), cRowPtr, cCol, cVal, S.d.ptr<float>(),
                                S.q.ptr<float>(), partial, counter, sc);
+            ktimer_end(C, tok);
             hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(PB), 0, s, nn, S.Minv.ptr<float>(), S.x.ptr<float>(),
                                S.r.ptr<float>(), S.z.ptr<float>(), S.d.ptr<float>(), S.q.ptr<float>(), partial,
                                counter, sc);
@@ -222,6 +226,7 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
         CS_HIP(hipMemcpyAsync(C->hScal, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
         CS_HIP(hipStreamSynchronize(s));
         done = C->hScal->done;
+        ktimer_collect(C);
     }
     const PcgScalars& h = *C->hScal;
     if (relres) *relres = h.bb > 0.0f ? sqrtf(h.rr / h.bb) : 0.0f;

@@ -49,8 +49,12 @@ enum { CS_MESH_CLOTH = 0, CS_MESH_OBSTACLE = 1 };
 /* cs_params.flags */
 enum {
     CS_FLAG_NO_PROXIMITY = 1, /* skip repulsion/friction proximities (testing only; the reference always runs them) */
-    CS_FLAG_NO_COLLISION = 2  /* cs_step skips the collision step (testing only) */
+    CS_FLAG_NO_COLLISION = 2, /* cs_step skips the collision step (testing only) */
+    CS_FLAG_TIME_KERNELS = 4  /* sample HIP-event durations of the hot kernels (see cs_kernel_time) */
 };
+
+/* kernel ids for cs_kernel_time */
+enum { CS_KERNEL_PCG_SPMV = 0, CS_KERNEL_ASSEMBLE_ROWS = 1, CS_KERNEL_FACE_RECORDS = 2, CS_KERNEL_COUNT = 3 };
 
 /* Scene-level parameters.  Defaults (cs_default_params) are the reference's:
  * Magic ctor src/Magic.cu:5-17, Wind ctor src/Wind.cu:3-7, Simulator ctor
@@ -178,6 +182,13 @@ cs_status cs_find_impacts(cs_ctx* ctx, float thickness, cs_impact_rec* out, int 
  * (sorted by t, as cs_find_impacts returns them); writes the selected subset in order. */
 cs_status cs_independent_impacts(cs_ctx* ctx, const cs_impact_rec* in, int n, int deform,
                                  cs_impact_rec* out, int* n_out);
+
+/* ---- instrumentation (no counterpart in the reference, which only prints wall-clock stage
+ * times, src/Simulator.cu:845-896) ----
+ * With CS_FLAG_TIME_KERNELS set, the context brackets sampled launches of the hot kernels with
+ * HIP events on its own stream.  Returns the summed duration (ms) and the number of sampled
+ * launches of kernel `kernel_id` (CS_KERNEL_*) since the last reset, and optionally resets them. */
+cs_status cs_kernel_time(cs_ctx* ctx, int kernel_id, double* total_ms, int* count, int reset);
 
 #ifdef __cplusplus
 }

@@ -229,6 +229,33 @@ template <class T> void traverse_bruteforce(const Scene<T>& S, const vector<BVH<
     }
 }
 
+// All candidate pairs of traverse_all, in the same order, computed over leaf
+// chunks in parallel (each chunk's pairs are kept in leaf order and the
+// chunks concatenated in order).
+template <class T> vector<std::pair<int, int>> collect_pairs(const Scene<T>& S, const vector<BVH<T>>& bvhs, T th) {
+    vector<std::pair<int, int>> all;
+    auto run = [&](int c, int target, bool self) {
+        const BVH<T>& B = bvhs[c];
+        const BVH<T>& Tg = bvhs[target];
+        const int CH = 1024;
+        const int nch = (B.nLeaves + CH - 1) / CH;
+        vector<vector<std::pair<int, int>>> parts(nch);
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int ch = 0; ch < nch; ch++)
+            for (int i = ch * CH; i < std::min(B.nLeaves, (ch + 1) * CH); i++) {
+                int f0 = B.leafFace[i];
+                bvh_query(Tg, B.leafBox[i], i, self, th, [&](int f1) { parts[ch].push_back({f0, f1}); });
+            }
+        for (auto& p : parts) all.insert(all.end(), p.begin(), p.end());
+    };
+    for (int c = 0; c < S.nCloths; c++) {
+        run(c, c, true);
+        for (int j = 0; j < c; j++) run(c, j, false);
+        for (int o = S.nCloths; o < (int)S.meshes.size(); o++) run(c, o, false);
+    }
+    return all;
+}
+
 template <class T> void build_all(const Scene<T>& S, bool ccd, vector<BVH<T>>& bvhs) {
     bvhs.assign(S.meshes.size(), BVH<T>());
     for (int m = 0; m < (int)S.meshes.size(); m++) bvh_build(S, m, ccd, bvhs[m]);
@@ -293,10 +320,19 @@ template <class T> bool in_edge(const Scene<T>& S, T w, int e0, int e1) {
 
 template <class T> struct ProxCand {
     vector<Best<T>> node, edge, face;  // [2*count], key = 2*primitive + side
+    void offer(int cat, int slot, T d, int other) {
+        (cat == 0 ? node : (cat == 1 ? edge : face))[slot].offer(d, other);
+    }
+};
+// candidates of a chunk of pairs, merged into ProxCand afterwards
+template <class T> struct CandList {
+    struct C4 { int cat, slot; T d; int other; };
+    vector<C4> v;
+    void offer(int cat, int slot, T d, int other) { v.push_back({cat, slot, d, other}); }
 };
 
 // checkVertexFaceProximityGpu (src/PhysicsHelper.cu:20-49); d = |h| with -0 read as +0
-template <class T> void vf_proximity(const Scene<T>& S, int node, int f, ProxCand<T>& C) {
+template <class T, class Sink> void vf_proximity(const Scene<T>& S, int node, int f, Sink& C) {
     int n0 = S.fn[f][0], n1 = S.fn[f][1], n2 = S.fn[f][2];
     if (node == n0 || node == n1 || node == n2) return;
     V3<T> n; T w[4], h;
@@ -306,15 +342,15 @@ template <class T> void vf_proximity(const Scene<T>& S, int node, int f, ProxCan
     if (!inside) return;
     if (S.nodeFree(node)) {
         int side = n.dot(S.nrm[node]) >= T(0) ? 0 : 1;
-        C.node[2 * node + side].offer(d, f);
+        C.offer(0, 2 * node + side, d, f);
     }
     if (S.faceFree(f)) {
         int side = -n.dot(S.fnrm[f]) >= T(0) ? 0 : 1;
-        C.face[2 * f + side].offer(d, node);
+        C.offer(2, 2 * f + side, d, node);
     }
 }
 // checkEdgeEdgeProximityGpu (src/PhysicsHelper.cu:51-81)
-template <class T> void ee_proximity(const Scene<T>& S, int e0, int e1, ProxCand<T>& C) {
+template <class T, class Sink> void ee_proximity(const Scene<T>& S, int e0, int e1, Sink& C) {
     int a = S.en[e0][0], b = S.en[e0][1], c = S.en[e1][0], d4 = S.en[e1][1];
     if (a == c || a == d4 || b == c || b == d4) return;
     V3<T> n; T w[4], h;
@@ -324,11 +360,11 @@ template <class T> void ee_proximity(const Scene<T>& S, int e0, int e1, ProxCand
     if (!inside) return;
     if (S.edgeFree(e0)) {
         int side = n.dot(S.nrm[a] + S.nrm[b]) >= T(0) ? 0 : 1;
-        C.edge[2 * e0 + side].offer(d, e1);
+        C.offer(1, 2 * e0 + side, d, e1);
     }
     if (S.edgeFree(e1)) {
         int side = -n.dot(S.nrm[c] + S.nrm[d4]) >= T(0) ? 0 : 1;
-        C.edge[2 * e1 + side].offer(d, e0);
+        C.offer(1, 2 * e1 + side, d, e0);
     }
 }
 
@@ -363,12 +399,20 @@ template <class T> vector<Prox<T>> find_proximities(const Scene<T>& S) {
     C.node.assign(2 * S.x.size(), Best<T>());
     C.edge.assign(2 * S.en.size(), Best<T>());
     C.face.assign(2 * S.fn.size(), Best<T>());
-    traverse_all(S, bvhs, T(2) * thick, [&](int f0, int f1) {
-        for (int j = 0; j < 3; j++) vf_proximity(S, S.fn[f0][j], f1, C);
-        for (int j = 0; j < 3; j++) vf_proximity(S, S.fn[f1][j], f0, C);
-        for (int j = 0; j < 3; j++)
-            for (int k = 0; k < 3; k++) ee_proximity(S, S.fe[f0][j], S.fe[f1][k], C);
-    });
+    vector<std::pair<int, int>> pairs = collect_pairs(S, bvhs, T(2) * thick);
+    const long np = (long)pairs.size(), CH = 4096, nch = (np + CH - 1) / CH;
+    vector<CandList<T>> parts(nch);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long ch = 0; ch < nch; ch++)
+        for (long p = ch * CH; p < std::min(np, (ch + 1) * CH); p++) {
+            int f0 = pairs[p].first, f1 = pairs[p].second;
+            for (int j = 0; j < 3; j++) vf_proximity(S, S.fn[f0][j], f1, parts[ch]);
+            for (int j = 0; j < 3; j++) vf_proximity(S, S.fn[f1][j], f0, parts[ch]);
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++) ee_proximity(S, S.fe[f0][j], S.fe[f1][k], parts[ch]);
+        }
+    for (auto& part : parts)  // the minimum per slot does not depend on the merge order
+        for (auto& c4 : part.v) C.offer(c4.cat, c4.slot, c4.d, c4.other);
     vector<Prox<T>> out;
     T lim = T(2) * thick;
     for (size_t k = 0; k < C.node.size(); k++)
@@ -507,13 +551,20 @@ inline bool impact_less(const Impact& p, const Impact& q) { return p.t < q.t || 
 // Simulator::findImpacts (src/Simulator.cu:383-392) + checkImpactsGpu
 // (src/CollisionHelper.cu:137-162), deduplicated and sorted by (t, key).
 template <class T> vector<Impact> find_impacts_with(const Scene<T>& S, const vector<BVH<T>>& bvhs, T th) {
+    vector<std::pair<int, int>> pairs = collect_pairs(S, bvhs, th);
+    const long np = (long)pairs.size(), CH = 4096, nch = (np + CH - 1) / CH;
+    vector<vector<Impact>> parts(nch);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long ch = 0; ch < nch; ch++)
+        for (long p = ch * CH; p < std::min(np, (ch + 1) * CH); p++) {
+            int f0 = pairs[p].first, f1 = pairs[p].second;
+            for (int j = 0; j < 3; j++) vf_impact(S, S.fn[f0][j], f1, th, parts[ch]);
+            for (int j = 0; j < 3; j++) vf_impact(S, S.fn[f1][j], f0, th, parts[ch]);
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++) ee_impact(S, S.fe[f0][j], S.fe[f1][k], th, parts[ch]);
+        }
     vector<Impact> all;
-    traverse_all(S, bvhs, th, [&](int f0, int f1) {
-        for (int j = 0; j < 3; j++) vf_impact(S, S.fn[f0][j], f1, th, all);
-        for (int j = 0; j < 3; j++) vf_impact(S, S.fn[f1][j], f0, th, all);
-        for (int j = 0; j < 3; j++)
-            for (int k = 0; k < 3; k++) ee_impact(S, S.fe[f0][j], S.fe[f1][k], th, all);
-    });
+    for (auto& part : parts) all.insert(all.end(), part.begin(), part.end());
     std::sort(all.begin(), all.end(), [](const Impact& p, const Impact& q) { return p.key < q.key; });
     vector<Impact> uniq;
     for (size_t i = 0; i < all.size(); i++)

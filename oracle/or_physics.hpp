@@ -310,6 +310,22 @@ template <class T> void inverse3(const T a[9], T out[9]) {
 // iteration as the reference GPU loop (src/Cloth.cu:766-809), except the
 // preconditioner is the inverse 3x3 diagonal block and the stop test is
 // ||r|| <= tol*||b|| (instead of scalar Jacobi and ||r|| < 1e-6).
+// Dot products are summed per fixed 4096-entry chunk and the chunk sums in
+// chunk order, so the result does not depend on the OpenMP thread count.
+template <class T> T chunked_dot(const vector<T>& p, const vector<T>& q) {
+    const long m = (long)p.size(), CH = 4096, nc = (m + CH - 1) / CH;
+    vector<T> part(nc);
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nc; c++) {
+        T s = T(0);
+        for (long i = c * CH; i < std::min(m, (c + 1) * CH); i++) s += p[i] * q[i];
+        part[c] = s;
+    }
+    T s = T(0);
+    for (long c = 0; c < nc; c++) s += part[c];
+    return s;
+}
+
 template <class T> int pcg_block_jacobi(const vector<int>& rowPtr, const vector<int>& col, const vector<T>& val,
                                         const vector<T>& b, T tol, int maxIters, vector<T>& x, T* relres) {
     int n = (int)rowPtr.size() - 1;
@@ -322,6 +338,7 @@ template <class T> int pcg_block_jacobi(const vector<int>& rowPtr, const vector<
         inverse3(dg, &Minv[9 * i]);
     }
     auto applyM = [&](const vector<T>& in, vector<T>& out) {
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n; i++)
             for (int a = 0; a < 3; a++) {
                 T s = T(0);
@@ -329,15 +346,15 @@ template <class T> int pcg_block_jacobi(const vector<int>& rowPtr, const vector<
                 out[3 * i + a] = s;
             }
     };
-    auto dotp = [&](const vector<T>& p, const vector<T>& qq) { T s = T(0); for (int i = 0; i < m; i++) s += p[i] * qq[i]; return s; };
-    T bnorm = std::sqrt(dotp(b, b));
+    T bnorm = std::sqrt(chunked_dot(b, b));
     applyM(r, z);
     d = z;
-    T rz = dotp(r, z);
+    T rz = chunked_dot(r, z);
     int it = 0;
     for (; it < maxIters; it++) {
-        T rn = std::sqrt(dotp(r, r));
+        T rn = std::sqrt(chunked_dot(r, r));
         if (rn <= tol * bnorm) break;
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n; i++)
             for (int a = 0; a < 3; a++) {
                 T s = T(0);
@@ -345,15 +362,17 @@ template <class T> int pcg_block_jacobi(const vector<int>& rowPtr, const vector<
                     for (int c = 0; c < 3; c++) s += val[9 * k + 3 * a + c] * d[3 * col[k] + c];
                 q[3 * i + a] = s;
             }
-        T alpha = rz / dotp(d, q);
+        T alpha = rz / chunked_dot(d, q);
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < m; i++) { x[i] += alpha * d[i]; r[i] -= alpha * q[i]; }
         applyM(r, z);
-        T rzn = dotp(r, z);
+        T rzn = chunked_dot(r, z);
         T beta = rzn / rz;
         rz = rzn;
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < m; i++) d[i] = z[i] + beta * d[i];
     }
-    if (relres) *relres = bnorm > T(0) ? std::sqrt(dotp(r, r)) / bnorm : T(0);
+    if (relres) *relres = bnorm > T(0) ? std::sqrt(chunked_dot(r, r)) / bnorm : T(0);
     return it;
 }
 
