@@ -20,57 +20,65 @@ namespace cs {
 
 namespace {
 
-__device__ double newton_root(double a, double b, double c, double d, double x0, int dir) {
-    if (dir != 0) {
-        double y0 = d + x0 * (c + x0 * (b + x0 * a));
-        double ddy0 = 2.0 * b + x0 * (6.0 * a);
-        x0 += dir * sqrt(tabs(2.0 * y0 / ddy0));
-    }
-    for (int iter = 0; iter < 100; iter++) {
-        double y = d + x0 * (c + x0 * (b + x0 * a));
-        double dy = c + x0 * (2.0 * b + 3.0 * x0 * a);
-        if (dy == 0) return x0;
-        double x1 = x0 - y / dy;
-        if (tabs(x0 - x1) < 1e-6) return x0;
-        x0 = x1;
-    }
-    return x0;
+// Roots of a t^3 + b t^2 + c t + d on [0, 1], ascending, by isolation: [0, 1]
+// is cut at the critical points, each piece with a sign change holds one root
+// found by safeguarded Newton (bisection fallback) to |dt| < 1e-12, and exact
+// zeros at cut points are roots.  Only roots in [0, 1] matter to the caller, so
+// no iteration is spent on the reference's whole-line Newton search
+// (src/CollisionHelper.cu:3-60).
+__device__ __forceinline__ double cubic_eval(double a, double b, double c, double d, double t) {
+    return d + t * (c + t * (b + t * a));
 }
-
-__device__ int solve_quadratic(double a, double b, double c, double x[2]) {
-    double d = b * b - 4.0 * a * c;
-    if (d < 0.0) {
-        x[0] = -b / (2.0 * a);
-        return 0;
+__device__ int cubic_roots_unit(double a, double b, double c, double d, double out[4]) {
+    double cut[4];
+    int nc = 0;
+    cut[nc++] = 0.0;
+    const double qa = 3.0 * a, qb = 2.0 * b, qc = c;
+    double cr[2];
+    int ncr = 0;
+    if (qa != 0.0) {
+        double disc = qb * qb - 4.0 * qa * qc;
+        if (disc >= 0.0) {
+            double sq = sqrt(disc);
+            double q = -0.5 * (qb + (qb < 0.0 ? -sq : sq));
+            double r1 = q / qa;
+            double r2 = q != 0.0 ? qc / q : r1;
+            if (r2 < r1) { double tt = r1; r1 = r2; r2 = tt; }
+            cr[ncr++] = r1;
+            if (r2 != r1) cr[ncr++] = r2;
+        }
+    } else if (qb != 0.0) {
+        cr[ncr++] = -qc / qb;
     }
-    double q = -(b + (b < 0.0 ? -1 : 1) * sqrt(d)) * 0.5;
-    int i = 0;
-    if (tabs(a) > 1e-12 * tabs(q)) x[i++] = q / a;
-    if (tabs(q) > 1e-12 * tabs(c)) x[i++] = c / q;
-    if (i == 2 && x[0] > x[1]) {
-        double t = x[0]; x[0] = x[1]; x[1] = t;
+    for (int i = 0; i < ncr; i++)
+        if (cr[i] > 0.0 && cr[i] < 1.0) cut[nc++] = cr[i];
+    cut[nc++] = 1.0;
+    int n = 0;
+    for (int s = 0; s + 1 < nc; s++) {
+        double l = cut[s], r = cut[s + 1];
+        double fl = cubic_eval(a, b, c, d, l), fr = cubic_eval(a, b, c, d, r);
+        if (fl == 0.0) {
+            if (n == 0 || out[n - 1] != l) out[n++] = l;
+            continue;
+        }
+        if (fr == 0.0 || (fl < 0.0) == (fr < 0.0)) continue;
+        double t = 0.5 * (l + r);
+        for (int it = 0; it < 100; it++) {
+            double ft = cubic_eval(a, b, c, d, t);
+            if (ft == 0.0) break;
+            if ((ft < 0.0) == (fl < 0.0)) { l = t; fl = ft; } else { r = t; }
+            double df = c + t * (2.0 * b + 3.0 * t * a);
+            double tn = df != 0.0 ? t - ft / df : 0.5 * (l + r);
+            if (!(tn > l && tn < r)) tn = 0.5 * (l + r);
+            double step = tn - t;
+            t = tn;
+            if (tabs(step) < 1e-12 || r - l < 1e-12) break;
+        }
+        out[n++] = t;
     }
-    return i;
-}
-
-__device__ int solve_cubic(double a, double b, double c, double d, double x[3]) {
-    double xc[2] = {0.0, 0.0};
-    int n = solve_quadratic(3.0 * a, 2.0 * b, c, xc);
-    if (n == 0) {
-        x[0] = newton_root(a, b, c, d, xc[0], 0);
-        return 1;
-    }
-    if (n == 1) return solve_quadratic(b, c, d, x);
-    double yc0 = d + xc[0] * (c + xc[0] * (b + xc[0] * a));
-    double yc1 = d + xc[1] * (c + xc[1] * (b + xc[1] * a));
-    int i = 0;
-    if (yc0 * a >= 0.0) x[i++] = newton_root(a, b, c, d, xc[0], -1);
-    if (yc0 * yc1 <= 0.0) {
-        int closer = tabs(yc0) < tabs(yc1) ? 0 : 1;
-        x[i++] = newton_root(a, b, c, d, xc[closer], closer == 0 ? 1 : -1);
-    }
-    if (yc1 * a <= 0.0) x[i++] = newton_root(a, b, c, d, xc[1], 1);
-    return i;
+    double f1 = cubic_eval(a, b, c, d, 1.0);
+    if (f1 == 0.0 && (n == 0 || out[n - 1] != 1.0)) out[n++] = 1.0;
+    return n;
 }
 
 // checkImpact (src/CollisionHelper.cu:62-109) in FP64
@@ -83,8 +91,8 @@ __device__ bool check_impact(int type, const V3d X0[4], const V3d X1[4], cs_impa
     double a1 = mixed(v1, x2, x3) + mixed(x1, v2, x3) + mixed(x1, x2, v3);
     double a2 = mixed(x1, v2, v3) + mixed(v1, x2, v3) + mixed(v1, v2, x3);
     double a3 = mixed(v1, v2, v3);
-    double t[3];
-    int ns = solve_cubic(a3, a2, a1, a0, t);
+    double t[4];
+    int ns = cubic_roots_unit(a3, a2, a1, a0, t);
     for (int i = 0; i < ns; i++) {
         if (t[i] < 0.0 || t[i] > 1.0) continue;
         V3d p[4];
@@ -184,11 +192,14 @@ __device__ void ee_impact(const CcdArgs& A, int e0, int e1) {
         emit(A, imp, (1ull << (2 * A.keyBits)) | ((uint64_t)e0 << A.keyBits) | (uint64_t)e1);
 }
 
+// One thread per (test, pair), test-major: a wavefront runs one kind of test
+// (vertex-face or edge-edge) over 64 consecutive pairs, so the branches of
+// check_impact stay coherent and the pair loads are contiguous.
 __global__ void __launch_bounds__(256) k_ccd_tests(CcdArgs A) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 15 * A.nPairs) return;
-    const long p = t / 15;
-    const int k = (int)(t - 15 * p);
+    const int k = (int)(t / A.nPairs);
+    const long p = t - (long)k * A.nPairs;
     const int2 pr = A.pairs[p];
     if (k < 3) {
         int4 fn = A.fnodes[pr.x];

@@ -344,46 +344,15 @@ template <class F> static void for_each_traversal(cs_ctx* C, F&& f) {
     }
 }
 
+// All candidate pairs of the traversal order, written to C->pairs without any
+// atomic: per traversal a count pass, an exclusive scan of the per-leaf
+// counts, and an emit pass at the scanned offsets (so the order is
+// deterministic: traversal, then query leaf, then tree order).
 long bvh_all_pairs(cs_ctx* C, float th) {
     hipStream_t s = C->stream;
-    unsigned* counter = C->pairCount.as<unsigned>(4);
-    if (C->pairCap == 0) {
-        C->pairCap = std::max<size_t>(1 << 16, 16 * (size_t)C->FC);
-        C->pairs.as<int2>(C->pairCap);
-    }
-    for (int attempt = 0; attempt < 2; attempt++) {
-        fill_u32(counter, 0u, 2, s);
-        for_each_traversal(C, [&](int q, int t, bool self) {
-            Bvh& Q = C->bvh[q];
-            Bvh& T = C->bvh[t];
-            dim3 g(cdiv(Q.nf, TB));
-            if (self)
-                hipLaunchKernelGGL((k_traverse<true, 2>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
-                                   T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, nullptr,
-                                   C->pairs.ptr<int2>(), counter, (long)C->pairCap, counter + 1);
-            else
-                hipLaunchKernelGGL((k_traverse<false, 2>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
-                                   T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, nullptr,
-                                   C->pairs.ptr<int2>(), counter, (long)C->pairCap, counter + 1);
-            CS_LAUNCH_CHECK();
-        });
-        CS_HIP(hipMemcpyAsync(C->hCount, counter, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        CS_HIP(hipStreamSynchronize(s));
-        if (C->hCount[1]) throw StatusError(CS_E_CAPACITY, "LBVH traversal stack overflow (tree deeper than 64)");
-        size_t n = C->hCount[0];
-        if (n <= C->pairCap) return (long)n;
-        C->pairCap = n + n / 2;
-        C->pairs.as<int2>(C->pairCap);
-    }
-    throw StatusError(CS_E_CAPACITY, "pair buffer growth failed");
-}
-
-long bvh_pairs_ordered(cs_ctx* C, float th, int* host_out, long cap) {
-    hipStream_t s = C->stream;
-    long total = 0;
-    std::vector<int> hbuf;
     unsigned* flag = C->pairCount.as<unsigned>(4);
     fill_u32(flag, 0u, 4, s);
+    long total = 0;
     for_each_traversal(C, [&](int q, int t, bool self) {
         Bvh& Q = C->bvh[q];
         Bvh& T = C->bvh[t];
@@ -402,9 +371,11 @@ long bvh_pairs_ordered(cs_ctx* C, float th, int* host_out, long cap) {
         scan_exclusive(num, offs, Q.nf, C->ws, s);
         CS_HIP(hipMemcpyAsync(C->hCount, offs + Q.nf, sizeof(unsigned), hipMemcpyDeviceToHost, s));
         CS_HIP(hipStreamSynchronize(s));
-        long n = C->hCount[0];
-        int2* out = C->pairs.as<int2>(std::max<long>(n, 1));
-        C->pairCap = 0;  // the append buffer was reused; let bvh_all_pairs size it again
+        const long n = C->hCount[0];
+        if (n == 0) return;
+        C->pairs.grow_keep(sizeof(int2) * (size_t)(total + n), sizeof(int2) * (size_t)total, s);
+        C->pairCap = C->pairs.bytes / sizeof(int2);
+        int2* out = C->pairs.ptr<int2>() + total;
         if (self)
             hipLaunchKernelGGL((k_traverse<true, 1>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
                                T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, offs, out, nullptr, 0L,
@@ -414,20 +385,20 @@ long bvh_pairs_ordered(cs_ctx* C, float th, int* host_out, long cap) {
                                T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, offs, out, nullptr, 0L,
                                flag + 1);
         CS_LAUNCH_CHECK();
-        if (n > 0) {
-            size_t old = hbuf.size();
-            hbuf.resize(old + 2 * (size_t)n);
-            CS_HIP(hipMemcpyAsync(hbuf.data() + old, out, sizeof(int2) * n, hipMemcpyDeviceToHost, s));
-            CS_HIP(hipStreamSynchronize(s));
-        }
         total += n;
     });
     CS_HIP(hipMemcpyAsync(C->hCount, flag + 1, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     CS_HIP(hipStreamSynchronize(s));
     if (C->hCount[0]) throw StatusError(CS_E_CAPACITY, "LBVH traversal stack overflow (tree deeper than 64)");
-    if (host_out) {
+    return total;
+}
+
+long bvh_pairs_ordered(cs_ctx* C, float th, int* host_out, long cap) {
+    long total = bvh_all_pairs(C, th);
+    if (host_out && total > 0) {
         long m = std::min(total, cap);
-        std::copy(hbuf.begin(), hbuf.begin() + 2 * m, host_out);
+        CS_HIP(hipMemcpyAsync(host_out, C->pairs.ptr<int2>(), sizeof(int2) * m, hipMemcpyDeviceToHost, C->stream));
+        CS_HIP(hipStreamSynchronize(C->stream));
     }
     return total;
 }

@@ -6,11 +6,16 @@
 // Differences: M is the inverse 3x3 diagonal block (not the scalar diagonal)
 // and the loop stops when ||r|| <= tol ||b|| (not ||r|| < 1e-6 absolute).
 //
-// Three kernels per iteration; every scalar stays on the device.  Dot
-// products are reduced per 256-thread block and finished by the last block to
-// arrive (fixed summation order, so the result is deterministic).  A kernel
-// whose `done` flag is set returns immediately, so the host can queue a chunk
-// of iterations and poll the flag once per chunk.
+// Three kernels per iteration, every scalar on the device and no host
+// round trip inside a chunk of iterations.  Each kernel writes one partial sum
+// per block; the next kernel's blocks each reduce those partials themselves
+// (a few KB from L2) in a fixed order, so no grid-wide fence or atomic is
+// needed and the result is deterministic.  Kernels read `done` and return at
+// once after convergence, so the host queues a chunk (a captured hipGraph) and
+// polls the flag once per chunk.
+#include <algorithm>
+#include <cstring>
+
 #include "cs_kernels.hpp"
 #include "cs_math.hpp"
 
@@ -18,85 +23,82 @@ namespace cs {
 
 namespace {
 
-constexpr int PB = 256;
+constexpr int PB = 256;     // threads per block
+constexpr int PMAX = 1024;  // at most this many blocks (grid-stride over rows)
+
+// partial-sum slots (each PMAX floats): dq, rz[2] (ping-pong by iteration parity), rr, bb
+enum { P_DQ = 0, P_RZ0 = 1, P_RZ1 = 2, P_RR = 3, P_BB = 4, P_COUNT = 5 };
+
+struct PcgDev {
+    int iters;      // completed iterations (read by update/dir kernels)
+    int itersNext;  // written by the dir kernel, copied to iters by the next SpMV
+    int done;
+    int maxit;
+    float tol2bb, bb, rr, rz;
+};
 
 __device__ __forceinline__ float block_sum(float v, float* lds) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
     if (lane == 0) lds[wave] = v;
     __syncthreads();
-    float s = 0.0f;
-    if (threadIdx.x == 0) s = (lds[0] + lds[1]) + (lds[2] + lds[3]);
-    __syncthreads();
-    return s;
+    return (lds[0] + lds[1]) + (lds[2] + lds[3]);
 }
 
-// Last-block-done reduction of K partial sums per block.  Returns true in the
-// block that finishes; that block's thread 0 gets the totals in `tot`.
-template <int K>
-__device__ __forceinline__ bool finish_sums(const float (&part)[K], float* partial, unsigned* counter, float* tot,
-                                            float* lds) {
-    __shared__ int amLast;
-    float s[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) s[k] = block_sum(part[k], lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) partial[k * gridDim.x + blockIdx.x] = s[k];
-        __threadfence();
-        unsigned t = atomicAdd(counter, 1u);
-        amLast = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!amLast) return false;
-    __threadfence();
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        float a = 0.0f;
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += PB)
-            a += __hip_atomic_load(&partial[k * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tot[k] = block_sum(a, lds);
-    }
-    if (threadIdx.x == 0) *counter = 0u;
-    return true;
+// every block reduces the nb partials of slot k identically
+__device__ __forceinline__ float reduce_partials(const float* __restrict__ part, int k, int nb, float* lds) {
+    float a = 0.0f;
+    for (int b = threadIdx.x; b < nb; b += PB) a += part[k * PMAX + b];
+    return block_sum(a, lds);
 }
 
 __device__ __forceinline__ V3f mat3v(const float* __restrict__ m, V3f a) {
     return mk(m[0] * a.x + m[1] * a.y + m[2] * a.z, m[3] * a.x + m[4] * a.y + m[5] * a.z,
               m[6] * a.x + m[7] * a.y + m[8] * a.z);
 }
-
 __device__ __forceinline__ V3f ld3v(const float* p, int i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 __device__ __forceinline__ void st3v(float* p, int i, V3f a) { p[3 * i] = a.x; p[3 * i + 1] = a.y; p[3 * i + 2] = a.z; }
 
 __global__ void __launch_bounds__(PB) k_pcg_init(int nn, const float* __restrict__ b, const float* __restrict__ bc,
                                                  const float* __restrict__ Minv, float* __restrict__ x,
                                                  float* __restrict__ r, float* __restrict__ z, float* __restrict__ d,
-                                                 float* partial, unsigned* counter, PcgScalars* sc, float tol2,
-                                                 int maxit) {
+                                                 float* __restrict__ part) {
     __shared__ float lds[4];
-    int i = blockIdx.x * PB + threadIdx.x;
     float prz = 0.0f, pbb = 0.0f;
-    if (i < nn) {
+    for (int i = blockIdx.x * PB + threadIdx.x; i < nn; i += gridDim.x * PB) {
         V3f bi = ld3v(b, i) + ld3v(bc, i);
         V3f zi = mat3v(Minv + 9 * (size_t)i, bi);
         st3v(x, i, mk(0.0f, 0.0f, 0.0f));
         st3v(r, i, bi);
         st3v(z, i, zi);
         st3v(d, i, zi);
-        prz = dot(bi, zi);
-        pbb = dot(bi, bi);
+        prz += dot(bi, zi);
+        pbb += dot(bi, bi);
     }
-    float part[2] = {prz, pbb}, tot[2];
-    if (finish_sums<2>(part, partial, counter, tot, lds) && threadIdx.x == 0) {
-        sc->rz = tot[0];
-        sc->bb = tot[1];
-        sc->rr = tot[1];
-        sc->tol2bb = tol2 * tot[1];
+    float s0 = block_sum(prz, lds);
+    float s1 = block_sum(pbb, lds);
+    if (threadIdx.x == 0) {
+        part[P_RZ0 * PMAX + blockIdx.x] = s0;
+        part[P_BB * PMAX + blockIdx.x] = s1;
+    }
+}
+
+__global__ void __launch_bounds__(PB) k_pcg_setup(int nb, const float* __restrict__ part, PcgDev* sc, float tol2,
+                                                  int maxit) {
+    __shared__ float lds[4];
+    float bb = reduce_partials(part, P_BB, nb, lds);
+    float rz = reduce_partials(part, P_RZ0, nb, lds);
+    if (threadIdx.x == 0) {
         sc->iters = 0;
+        sc->itersNext = 0;
         sc->maxit = maxit;
-        sc->done = (tot[1] <= tol2 * tot[1]) || maxit <= 0;
+        sc->bb = bb;
+        sc->rr = bb;
+        sc->rz = rz;
+        sc->tol2bb = tol2 * bb;
+        sc->done = (bb <= tol2 * bb) || maxit <= 0;
     }
 }
 
@@ -105,14 +107,13 @@ __global__ void __launch_bounds__(PB) k_pcg_spmv(int nn, const int* __restrict__
                                                  const int* __restrict__ cols, const float* __restrict__ vals,
                                                  const int* __restrict__ cRowPtr, const int* __restrict__ cCol,
                                                  const float* __restrict__ cVal, const float* __restrict__ d,
-                                                 float* __restrict__ q, float* partial, unsigned* counter,
-                                                 PcgScalars* sc) {
+                                                 float* __restrict__ q, float* __restrict__ part, PcgDev* sc) {
     __shared__ float lds[4];
     if (sc->done) return;
-    const int i = blockIdx.x * PB + threadIdx.x;
-    const int s = i / SLICE, lane = i % SLICE;
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->iters = sc->itersNext;
     float pdq = 0.0f;
-    if (i < nn) {
+    for (int i = blockIdx.x * PB + threadIdx.x; i < nn; i += gridDim.x * PB) {
+        const int s = i / SLICE, lane = i % SLICE;
         const int W = sliceW[s];
         const size_t off = (size_t)sliceOff[s];
         const int* cp = cols + off * SLICE + lane;
@@ -138,25 +139,24 @@ __global__ void __launch_bounds__(PB) k_pcg_spmv(int nn, const int* __restrict__
         q[3 * i] = y0;
         q[3 * i + 1] = y1;
         q[3 * i + 2] = y2;
-        pdq = d[3 * i] * y0 + d[3 * i + 1] * y1 + d[3 * i + 2] * y2;
+        pdq += d[3 * i] * y0 + d[3 * i + 1] * y1 + d[3 * i + 2] * y2;
     }
-    float part[1] = {pdq}, tot[1];
-    if (finish_sums<1>(part, partial, counter, tot, lds) && threadIdx.x == 0) {
-        sc->dq = tot[0];
-        sc->alpha = sc->rz / tot[0];
-    }
+    float s = block_sum(pdq, lds);
+    if (threadIdx.x == 0) part[P_DQ * PMAX + blockIdx.x] = s;
 }
 
 __global__ void __launch_bounds__(PB) k_pcg_update(int nn, const float* __restrict__ Minv, float* __restrict__ x,
                                                    float* __restrict__ r, float* __restrict__ z,
                                                    const float* __restrict__ d, const float* __restrict__ q,
-                                                   float* partial, unsigned* counter, PcgScalars* sc) {
+                                                   float* __restrict__ part, const PcgDev* sc) {
     __shared__ float lds[4];
     if (sc->done) return;
-    const float alpha = sc->alpha;
-    const int i = blockIdx.x * PB + threadIdx.x;
+    const int par = sc->iters & 1;
+    const float dq = reduce_partials(part, P_DQ, gridDim.x, lds);
+    const float rz = reduce_partials(part, P_RZ0 + par, gridDim.x, lds);
+    const float alpha = rz / dq;
     float prz = 0.0f, prr = 0.0f;
-    if (i < nn) {
+    for (int i = blockIdx.x * PB + threadIdx.x; i < nn; i += gridDim.x * PB) {
         V3f di = ld3v(d, i), qi = ld3v(q, i);
         V3f xi = ld3v(x, i) + alpha * di;
         V3f ri = ld3v(r, i) - alpha * qi;
@@ -164,26 +164,36 @@ __global__ void __launch_bounds__(PB) k_pcg_update(int nn, const float* __restri
         st3v(x, i, xi);
         st3v(r, i, ri);
         st3v(z, i, zi);
-        prz = dot(ri, zi);
-        prr = dot(ri, ri);
+        prz += dot(ri, zi);
+        prr += dot(ri, ri);
     }
-    float part[2] = {prz, prr}, tot[2];
-    if (finish_sums<2>(part, partial, counter, tot, lds) && threadIdx.x == 0) {
-        sc->beta = tot[0] / sc->rz;
-        sc->rz = tot[0];
-        sc->rr = tot[1];
-        sc->iters += 1;
-        sc->done = (tot[1] <= sc->tol2bb) || (sc->iters >= sc->maxit);
+    float s0 = block_sum(prz, lds);
+    float s1 = block_sum(prr, lds);
+    if (threadIdx.x == 0) {
+        part[(P_RZ0 + (1 - par)) * PMAX + blockIdx.x] = s0;
+        part[P_RR * PMAX + blockIdx.x] = s1;
     }
 }
 
 __global__ void __launch_bounds__(PB) k_pcg_dir(int nn, const float* __restrict__ z, float* __restrict__ d,
-                                                const PcgScalars* sc) {
+                                                const float* __restrict__ part, PcgDev* sc) {
+    __shared__ float lds[4];
     if (sc->done) return;
-    const float beta = sc->beta;
-    const int i = blockIdx.x * PB + threadIdx.x;
-    if (i >= 3 * nn) return;
-    d[i] = z[i] + beta * d[i];
+    const int it = sc->iters;
+    const int par = it & 1;
+    const float rzOld = reduce_partials(part, P_RZ0 + par, gridDim.x, lds);
+    const float rzNew = reduce_partials(part, P_RZ0 + (1 - par), gridDim.x, lds);
+    const float rr = reduce_partials(part, P_RR, gridDim.x, lds);
+    const bool stop = (rr <= sc->tol2bb) || (it + 1 >= sc->maxit);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // `iters` itself is advanced by the next SpMV
+        sc->itersNext = it + 1;
+        sc->rr = rr;
+        sc->rz = rzNew;
+        if (stop) sc->done = 1;
+    }
+    if (stop) return;
+    const float beta = rzNew / rzOld;
+    for (int i = blockIdx.x * PB + threadIdx.x; i < 3 * nn; i += gridDim.x * PB) d[i] = z[i] + beta * d[i];
 }
 
 }  // namespace
@@ -192,10 +202,10 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     ClothSys& S = C->sys[c];
     hipStream_t s = C->stream;
     const int nn = S.nn;
-    const int nb = cdiv(nn, PB);
-    float* partial = C->pcgPartial.as<float>(2 * (size_t)nb + 8);
-    unsigned* counter = C->pcgCounter.ptr<unsigned>();
-    PcgScalars* sc = C->pcgScal.ptr<PcgScalars>();
+    const int nb = std::max(1, std::min(PMAX, cdiv(nn, PB)));
+    float* part = C->pcgPartial.as<float>((size_t)P_COUNT * PMAX);
+    PcgDev* sc = reinterpret_cast<PcgDev*>(C->pcgScal.as<PcgScalars>(1));
+    static_assert(sizeof(PcgDev) <= sizeof(PcgScalars), "scalar block");
     const float tol = C->P.pcg_rel_tol;
     const int maxit = C->P.pcg_max_iters;
     const int* cRowPtr = S.cRowPtr.ptr<int>();
@@ -203,10 +213,13 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     const float* cVal = S.nContact > 0 ? S.cVal.ptr<float>() : S.vals.ptr<float>();
     hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(), S.bc.ptr<float>(),
                        S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), S.z.ptr<float>(), S.d.ptr<float>(),
-                       partial, counter, sc, tol * tol, maxit);
+                       part);
+    CS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pcg_setup, dim3(1), dim3(PB), 0, s, nb, part, sc, tol * tol, maxit);
     CS_LAUNCH_CHECK();
     const int chunk = 16;
     int done = 0;
+    PcgDev h{};
     for (int it = 0; it < maxit + chunk && !done; it += chunk) {
         for (int k = 0; k < chunk; k++) {
             // the first SpMV of a chunk is sampled: the host saw `done == 0` after the previous chunk,
@@ -214,24 +227,22 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
             const int tok = k == 0 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
             hipLaunchKernelGGL(k_pcg_spmv, dim3(nb), dim3(PB), 0, s, nn, S.sliceOff.ptr<int>(), S.sliceW.ptr<int>(),
                                S.cols.ptr<int>(), S.vals.ptr<float>(), cRowPtr, cCol, cVal, S.d.ptr<float>(),
-                               S.q.ptr<float>(), partial, counter, sc);
+                               S.q.ptr<float>(), part, sc);
             ktimer_end(C, tok);
             hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(PB), 0, s, nn, S.Minv.ptr<float>(), S.x.ptr<float>(),
-                               S.r.ptr<float>(), S.z.ptr<float>(), S.d.ptr<float>(), S.q.ptr<float>(), partial,
-                               counter, sc);
-            hipLaunchKernelGGL(k_pcg_dir, dim3(cdiv(3L * nn, PB)), dim3(PB), 0, s, nn, S.z.ptr<float>(),
-                               S.d.ptr<float>(), sc);
+                               S.r.ptr<float>(), S.z.ptr<float>(), S.d.ptr<float>(), S.q.ptr<float>(), part, sc);
+            hipLaunchKernelGGL(k_pcg_dir, dim3(nb), dim3(PB), 0, s, nn, S.z.ptr<float>(), S.d.ptr<float>(), part, sc);
         }
         CS_LAUNCH_CHECK();
-        CS_HIP(hipMemcpyAsync(C->hScal, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipMemcpyAsync(C->hScal, sc, sizeof(PcgDev), hipMemcpyDeviceToHost, s));
         CS_HIP(hipStreamSynchronize(s));
-        done = C->hScal->done;
+        std::memcpy(&h, C->hScal, sizeof(PcgDev));
+        done = h.done;
         ktimer_collect(C);
     }
-    const PcgScalars& h = *C->hScal;
     if (relres) *relres = h.bb > 0.0f ? sqrtf(h.rr / h.bb) : 0.0f;
     if (converged) *converged = (h.rr <= h.tol2bb);
-    return h.iters;
+    return h.itersNext;
 }
 
 }  // namespace cs

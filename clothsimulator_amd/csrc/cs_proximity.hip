@@ -90,8 +90,8 @@ __device__ void ee_test(const ProxArgs& A, int e0, int e1) {
 __global__ void __launch_bounds__(256) k_prox_tests(ProxArgs A) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 15 * A.nPairs) return;
-    const long p = t / 15;
-    const int k = (int)(t - 15 * p);
+    const int k = (int)(t / A.nPairs);  // test-major: one kind of test per wavefront
+    const long p = t - (long)k * A.nPairs;
     const int2 pr = A.pairs[p];
     if (k < 3) {
         int4 fn = A.fnodes[pr.x];

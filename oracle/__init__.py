@@ -60,6 +60,7 @@ def lib():
         L.or_independent_impacts.argtypes = [vp, C.POINTER(ImpactRec), C.c_int, C.c_int, C.POINTER(ImpactRec),
                                              C.POINTER(C.c_int)]
         L.or_solve_cubic.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.or_cubic_roots_unit.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -94,8 +95,16 @@ def solve_bsr(row_ptr, col, vals, b, direct=False, fp64=True, tol=1e-5, max_iter
 
 
 def solve_cubic(a, b, c, d):
+    """The reference's whole-line solver (FP64 restatement)."""
     r = (C.c_double * 3)()
     n = lib().or_solve_cubic(a, b, c, d, r)
+    return [r[i] for i in range(n)]
+
+
+def cubic_roots_unit(a, b, c, d):
+    """Roots on [0, 1] of the isolated root finder used by the CCD."""
+    r = (C.c_double * 4)()
+    n = lib().or_cubic_roots_unit(a, b, c, d, r)
     return [r[i] for i in range(n)]
 
 

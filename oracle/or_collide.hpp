@@ -477,6 +477,67 @@ inline int solve_cubic(double a, double b, double c, double d, double x[3]) {
     return i;
 }
 
+// Roots of f(t) = a t^3 + b t^2 + c t + d on [0, 1] in ascending order, by
+// isolation: [0, 1] is cut at the critical points of f, and each piece whose
+// end values have opposite signs holds exactly one root, found by safeguarded
+// Newton (bisection when a Newton step leaves the bracket) to |dt| < 1e-12.
+// Exact zeros at the cut points are roots too.  This replaces the
+// reference's whole-line Newton search (src/CollisionHelper.cu:3-60), whose
+// roots outside [0, 1] are discarded by its caller anyway; see the design
+// notes for the cases where the two differ (grazing double roots).
+inline double cubic_eval(double a, double b, double c, double d, double t) { return d + t * (c + t * (b + t * a)); }
+inline int cubic_roots_unit(double a, double b, double c, double d, double out[3]) {
+    double cut[4];
+    int nc = 0;
+    cut[nc++] = 0.0;
+    const double qa = 3.0 * a, qb = 2.0 * b, qc = c;
+    double cr[2];
+    int ncr = 0;
+    if (qa != 0.0) {
+        double disc = qb * qb - 4.0 * qa * qc;
+        if (disc >= 0.0) {
+            double sq = std::sqrt(disc);
+            double q = -0.5 * (qb + (qb < 0.0 ? -sq : sq));
+            double r1 = q / qa;
+            double r2 = q != 0.0 ? qc / q : r1;
+            if (r2 < r1) std::swap(r1, r2);
+            cr[ncr++] = r1;
+            if (r2 != r1) cr[ncr++] = r2;
+        }
+    } else if (qb != 0.0) {
+        cr[ncr++] = -qc / qb;
+    }
+    for (int i = 0; i < ncr; i++)
+        if (cr[i] > 0.0 && cr[i] < 1.0) cut[nc++] = cr[i];
+    cut[nc++] = 1.0;
+    int n = 0;
+    for (int s = 0; s + 1 < nc; s++) {
+        double l = cut[s], r = cut[s + 1];
+        double fl = cubic_eval(a, b, c, d, l), fr = cubic_eval(a, b, c, d, r);
+        if (fl == 0.0) {
+            if (n == 0 || out[n - 1] != l) out[n++] = l;
+            continue;
+        }
+        if (fr == 0.0 || (fl < 0.0) == (fr < 0.0)) continue;
+        double t = 0.5 * (l + r);
+        for (int it = 0; it < 100; it++) {
+            double ft = cubic_eval(a, b, c, d, t);
+            if (ft == 0.0) break;
+            if ((ft < 0.0) == (fl < 0.0)) { l = t; fl = ft; } else { r = t; }
+            double df = c + t * (2.0 * b + 3.0 * t * a);
+            double tn = df != 0.0 ? t - ft / df : 0.5 * (l + r);
+            if (!(tn > l && tn < r)) tn = 0.5 * (l + r);
+            double step = tn - t;
+            t = tn;
+            if (tabs(step) < 1e-12 || r - l < 1e-12) break;
+        }
+        out[n++] = t;
+    }
+    double f1 = cubic_eval(a, b, c, d, 1.0);
+    if (f1 == 0.0 && (n == 0 || out[n - 1] != 1.0)) out[n++] = 1.0;
+    return n;
+}
+
 // checkImpact (src/CollisionHelper.cu:62-109), promoted to FP64.
 inline bool check_impact(int type, const V3<double> X0[4], const V3<double> X1[4], Impact& imp) {
     V3<double> x0 = X0[0];
@@ -487,8 +548,8 @@ inline bool check_impact(int type, const V3<double> X0[4], const V3<double> X1[4
     double a1 = mixed(v1, x2, x3) + mixed(x1, v2, x3) + mixed(x1, x2, v3);
     double a2 = mixed(x1, v2, v3) + mixed(v1, x2, v3) + mixed(v1, v2, x3);
     double a3 = mixed(v1, v2, v3);
-    double t[3];
-    int ns = solve_cubic(a3, a2, a1, a0, t);
+    double t[4];
+    int ns = cubic_roots_unit(a3, a2, a1, a0, t);
     for (int i = 0; i < ns; i++) {
         if (t[i] < 0.0 || t[i] > 1.0) continue;
         V3<double> p[4];

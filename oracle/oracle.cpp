@@ -385,7 +385,12 @@ int or_independent_impacts(void* h, const cs_impact_rec* in, int n, int deform, 
     });
 }
 
-// FP64 coplanarity-cubic root finder (exposed for closed-form checks).
+// The reference's whole-line cubic solver restated in FP64 (src/CollisionHelper.cu:3-60).
 int or_solve_cubic(double a, double b, double c, double d, double* roots) { return solve_cubic(a, b, c, d, roots); }
+
+// The isolated root finder on [0, 1] used by the CCD (roots must hold 4 entries).
+int or_cubic_roots_unit(double a, double b, double c, double d, double* roots) {
+    return cubic_roots_unit(a, b, c, d, roots);
+}
 
 }  // extern "C"
