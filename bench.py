@@ -72,6 +72,18 @@ def cpu_baseline(spec, steps):
                       "to 1e-5, FP64 CCD), OpenMP on %d threads, %.1f s" % (steps, spec.name, threads, dt)}
 
 
+def copy_probe(mib):
+    """A copy kernel of known size: reads and writes mib MiB per launch, 5 launches."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,11 +92,17 @@ def main():
     ap.add_argument("--workload", default="crumple513")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the cloth step has no "
               "multi-GPU path here; see DESIGN.md for how it would shard)" % a.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
 
     from clothsimulator_amd import CS_MESH_CLOTH, CS_MESH_OBSTACLE, Simulator, scenes
     from clothsimulator_amd._cabi import CS_FLAG_TIME_KERNELS, CS_KERNEL_PCG_SPMV
@@ -105,12 +123,15 @@ def main():
 
     iters, contacts, failed = [], [], 0
     t0 = time.perf_counter()  # every library call returns after its stream has completed
-    for _ in range(steps):
+    for k in range(steps):
         s, st = sim.step()
         if s not in (0, 4):
             failed += 1
         iters.append(st.pcg_iters)
         contacts.append(st.n_contact_blocks)
+        if (k + 1) % 50 == 0:
+            print("step %d/%d  %.1f s  cg %d" % (k + 1, steps, time.perf_counter() - t0, st.pcg_iters),
+                  file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
 
     n_nodes = spec.n_cloth_nodes
@@ -150,6 +171,7 @@ def main():
         "roofline": roofline,
     }
     if not a.no_cpu:
+        print("cpu baseline: %d step(s) of %s on the host" % (a.cpu_steps, a.workload), file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(scenes.by_name(a.workload), a.cpu_steps)
     print(json.dumps(out))
 
