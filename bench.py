@@ -140,9 +140,10 @@ def main():
     c0 = spec.cloths[0]
     nnzb = static_nnzb(c0.mesh.faces, c0.mesh.n_nodes)
     nnzb_dyn = nnzb + float(np.mean(contacts))
-    # algorithmic bytes of one SpMV launch: 36 B values + 4 B column index per block,
-    # d gathered once (12 B/node) and q written once (12 B/node)
-    bytes_per_launch = 40.0 * nnzb_dyn + 24.0 * n_nodes
+    # algorithmic bytes of one SpMV launch: 36 B values + 4 B column index per block; per node
+    # z and the previous direction read once (16 B each), the new direction (16 B) and q (12 B)
+    # written once
+    bytes_per_launch = 40.0 * nnzb_dyn + 60.0 * n_nodes
     avg_s = (spmv_ms / spmv_n) * 1e-3 if spmv_n else float("nan")
     achieved = bytes_per_launch / avg_s / 1e9 if spmv_n else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

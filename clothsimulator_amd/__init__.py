@@ -11,8 +11,8 @@ import os
 import numpy as np
 
 from ._cabi import (CS_E_NOT_CONVERGED, CS_MESH_CLOTH, CS_MESH_OBSTACLE, CS_OK, IMPACT_DTYPE, LIB_PATH, PROX_DTYPE,
-                    STATUS_NAMES, ImpactRec, MaterialIn, MeshArrays, MeshIn, Params, ProximityRec, Stats,
-                    default_params, fptr, iptr, material_struct)
+                    STATUS_NAMES, ImpactRec, MaterialIn, MeshArrays, MeshIn, Params, ProximityRec, SceneInfo,
+                    Stats, default_params, fptr, iptr, material_struct)
 
 __all__ = ["load_library", "Simulator", "ClothError", "default_params", "material_struct", "MeshArrays"]
 
@@ -22,7 +22,7 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_add_cloth", "cs_add_obstacle", "cs_add_handles", "cs_physics_step", "cs_collision_step", "cs_step",
             "cs_run", "cs_get_counts", "cs_get_state", "cs_set_state", "cs_get_geometry", "cs_material_table",
             "cs_lbvh_pairs", "cs_find_proximities", "cs_assemble", "cs_solve", "cs_find_impacts",
-            "cs_independent_impacts", "cs_kernel_time"]
+            "cs_independent_impacts", "cs_kernel_time", "cs_load_scene", "cs_scene_error"]
 
 
 class ClothError(RuntimeError):
@@ -67,8 +67,20 @@ def load_library(path=LIB_PATH):
     L.cs_independent_impacts.argtypes = [vp, C.POINTER(ImpactRec), C.c_int, C.c_int, C.POINTER(ImpactRec),
                                          C.POINTER(C.c_int)]
     L.cs_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]
+    L.cs_load_scene.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Params), C.POINTER(vp), C.POINTER(SceneInfo)]
+    L.cs_scene_error.restype = C.c_char_p
     _lib = L
     return L
+
+
+def scene_info(config_path, root_dir=None):
+    """Parse a reference configuration file (and its meshes and materials) without a device."""
+    L = load_library()
+    info = SceneInfo()
+    st = L.cs_load_scene(config_path.encode(), (root_dir or "").encode(), None, None, C.byref(info))
+    if st != CS_OK:
+        raise ClothError(st, "cs_load_scene", L.cs_scene_error().decode())
+    return info
 
 
 def material_table(mat):
@@ -102,6 +114,25 @@ class Simulator:
                     self.add_handles(cid, c.handles)
             for o in spec.obstacles:
                 self.add_obstacle(o)
+
+    @classmethod
+    def from_config(cls, config_path, root_dir=None, params=None):
+        """A simulator for a reference configuration file (cs_load_scene); `params` supplies
+        the PCG settings and flags the file does not hold."""
+        self = cls.__new__(cls)
+        self.L = load_library()
+        h = C.c_void_p()
+        info = SceneInfo()
+        st = self.L.cs_load_scene(config_path.encode(), (root_dir or "").encode(),
+                                  C.byref(params) if params is not None else None, C.byref(h), C.byref(info))
+        if st != CS_OK:
+            raise ClothError(st, "cs_load_scene", self.L.cs_scene_error().decode())
+        self.h = h
+        self.info = info
+        self.params = params
+        self._keep = []
+        self.n_cloths = info.n_cloths
+        return self
 
     def _check(self, st, what, ok=(CS_OK,)):
         if st not in ok:

@@ -125,6 +125,20 @@ class ImpactRec(C.Structure):
     ]
 
 
+class SceneInfo(C.Structure):
+    _fields_ = [
+        ("frame_time", C.c_float),
+        ("frame_steps", C.c_int),
+        ("end_time", C.c_float),
+        ("end_frame", C.c_int),
+        ("n_cloths", C.c_int),
+        ("n_obstacles", C.c_int),
+        ("n_handles", C.c_int),
+        ("n_cloth_nodes", C.c_int),
+        ("n_cloth_faces", C.c_int),
+    ]
+
+
 PROX_DTYPE = np.dtype([("nodes", "<i4", 4), ("n", "<f4", 3), ("mu", "<f4"), ("stiffness", "<f4"), ("w", "<f4", 4)])
 IMPACT_DTYPE = np.dtype({"names": ["type", "nodes", "t", "w", "n"],
                          "formats": ["<i4", ("<i4", 4), "<f8", ("<f8", 4), ("<f8", 3)],

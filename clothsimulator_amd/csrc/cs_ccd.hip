@@ -5,9 +5,11 @@
 // LBVH (boxes over x0 and x), 6 vertex-face and 9 edge-edge tests, each with a
 // swept-box prefilter and the coplanarity cubic of checkImpact
 // (src/CollisionHelper.cu:62-109) solved in FP64.  An edge-edge pair is always
-// evaluated with the lower global edge id first, so a pair reached from
-// several face pairs yields one bit-identical result; the list is then
-// de-duplicated by (type, primitive ids) and sorted by (t, type, ids).
+// evaluated with the lower global edge id first, so a test reached from
+// several face pairs yields one bit-identical result.  Two stages: a per-pair
+// filter keeps every distinct test once (filters + ownership, below), then a
+// per-test kernel runs the FP64 cubic; impacts are sorted by (type, ids),
+// de-duplicated, and sorted by (t, type, ids).
 //
 // independentImpacts GPU (src/Simulator.cu:422-461, kernels
 // src/CollisionHelper.cu:164-237): rounds in which a candidate is accepted when
@@ -119,17 +121,33 @@ __device__ bool check_impact(int type, const V3d X0[4], const V3d X1[4], cs_impa
     return false;
 }
 
-struct CcdArgs {
+// ---- stage 1: per candidate face pair, the cheap filters and ownership ----
+//
+// A vertex-face test (n, f) or an edge-edge test (e0, e1) can be reached from
+// several face pairs of the list (every pair of faces holding its primitives).
+// Its filters (no shared node; swept node/edge box overlapping the dilated
+// face/edge box) and its FP64 evaluation depend on the primitives only, so all
+// those evaluations are bit-identical.  The filter stage keeps each test once:
+// from the first pair, in ascending face order, that the list holds.  Whether
+// the list holds a face pair is decided from the pair predicate itself (the
+// query face's box against the target face's box dilated by `thickness`, the
+// query being the lower sorted leaf of a mesh against itself, the cloth face
+// against an obstacle, the later cloth against an earlier one), so the kept set
+// is exactly the set of distinct tests the list reaches.
+
+struct FilterArgs {
     const int2* pairs;
     long nPairs;
     float th;
     int keyBits;
+    int nCloths;
     const float4 *x, *x0;
     const int4 *fnodes, *fedges;
-    const int2* enodes;
-    cs_impact_rec* out;
-    uint64_t* keys;
-    unsigned* count;
+    const int2 *enodes, *eadj;
+    const int *nfPtr, *nfIdx;
+    const float4* fbox;  // per face: (leaf box lo, mesh bits), (leaf box hi, sorted index bits)
+    uint64_t *candVF, *candEE;
+    unsigned* count;  // [0] vertex-face, [1] edge-edge
     long cap;
 };
 
@@ -137,80 +155,203 @@ __device__ __forceinline__ void swept_box(Box& b, const float4* x, const float4*
     box_add(b, ld3(x[n]));
     box_add(b, ld3(x0[n]));
 }
+__device__ __forceinline__ Box node_sbox(const FilterArgs& A, int n) {
+    Box b = box_empty();
+    swept_box(b, A.x, A.x0, n);
+    return b;
+}
+// A face's leaf box (the swept box of its three nodes), mesh and sorted leaf index.
+struct FaceRec {
+    Box b;
+    int mesh, rank;
+};
+__device__ __forceinline__ FaceRec face_rec(const FilterArgs& A, int f) {
+    const float4 lo = A.fbox[2 * (size_t)f], hi = A.fbox[2 * (size_t)f + 1];
+    FaceRec r;
+    r.b.lo[0] = lo.x; r.b.lo[1] = lo.y; r.b.lo[2] = lo.z;
+    r.b.hi[0] = hi.x; r.b.hi[1] = hi.y; r.b.hi[2] = hi.z;
+    r.mesh = __float_as_int(lo.w);
+    r.rank = __float_as_int(hi.w);
+    return r;
+}
 
-__device__ void emit(const CcdArgs& A, cs_impact_rec& imp, uint64_t key) {
-    unsigned slot = atomicAdd(A.count, 1u);
+// Does the candidate list hold the face pair {g, f} (g != f)?
+__device__ bool in_list(const FilterArgs& A, const FaceRec& g, const FaceRec& f) {
+    const bool og = g.mesh >= A.nCloths, of = f.mesh >= A.nCloths;
+    if (og && of) return false;  // obstacles are never traversed against each other
+    bool gQuery;
+    if (g.mesh == f.mesh) gQuery = g.rank < f.rank;
+    else if (og != of) gQuery = !og;  // the cloth face queries the obstacle's tree
+    else gQuery = g.mesh > f.mesh;    // the later cloth queries the earlier one
+    return gQuery ? box_overlap(g.b, f.b, A.th) : box_overlap(f.b, g.b, A.th);
+}
+
+// vertex-face test (n, f) reached from the pair (p, f), n in p
+__device__ bool vf_owner(const FilterArgs& A, int n, int p, const FaceRec& f) {
+    for (int k = A.nfPtr[n], e = A.nfPtr[n + 1]; k < e; k++) {
+        const int g = A.nfIdx[k] >> 2;  // ascending
+        if (g == p) return true;
+        if (in_list(A, face_rec(A, g), f)) return false;
+    }
+    return true;
+}
+
+// edge-edge test (e0, e1), e0 < e1, reached from the pair (p0, p1), e0 in p0, e1 in p1
+__device__ bool ee_owner(const FilterArgs& A, int e0, int e1, int p0, int p1) {
+    const int2 a = A.eadj[e0], b = A.eadj[e1];
+    int g0[2], g1[2], n0 = 0, n1 = 0;
+    if (a.x >= 0) g0[n0++] = a.x;
+    if (a.y >= 0) g0[n0++] = a.y;
+    if (b.x >= 0) g1[n1++] = b.x;
+    if (b.y >= 0) g1[n1++] = b.y;
+    if (n0 == 2 && g0[1] < g0[0]) { int t = g0[0]; g0[0] = g0[1]; g0[1] = t; }
+    if (n1 == 2 && g1[1] < g1[0]) { int t = g1[0]; g1[0] = g1[1]; g1[1] = t; }
+    for (int i = 0; i < n0; i++) {
+        if (g0[i] == p0 && g1[0] == p1) return true;
+        const FaceRec r0 = face_rec(A, g0[i]);
+        for (int j = 0; j < n1; j++) {
+            if (g0[i] == p0 && g1[j] == p1) return true;
+            if (g0[i] == g1[j]) continue;
+            if (in_list(A, r0, face_rec(A, g1[j]))) return false;
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ int pick3(int4 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// The 15 tests of a pair: t < 3 node t of fa against fb, t < 6 node t-3 of fb
+// against fa, else edge (t-6)/3 of fa against edge (t-6)%3 of fb.
+__device__ bool filter_test(const FilterArgs& A, int t, int fa, int4 na, const FaceRec& ra, int fb, int4 nb,
+                            const FaceRec& rb, uint64_t& key) {
+    if (t < 6) {
+        const bool fromA = t < 3;
+        const int n = fromA ? pick3(na, t) : pick3(nb, t - 3);
+        const int p = fromA ? fa : fb, f = fromA ? fb : fa;
+        const int4 ff = fromA ? nb : na;
+        const FaceRec& rf = fromA ? rb : ra;
+        if (n == ff.x || n == ff.y || n == ff.z) return false;
+        if (!box_overlap(node_sbox(A, n), rf.b, A.th)) return false;
+        if (!vf_owner(A, n, p, rf)) return false;
+        key = ((uint64_t)n << A.keyBits) | (uint64_t)f;
+        return true;
+    }
+    int e0 = pick3(A.fedges[fa], (t - 6) / 3), e1 = pick3(A.fedges[fb], (t - 6) % 3);
+    int p0 = fa, p1 = fb;
+    if (e1 < e0) { int s = e0; e0 = e1; e1 = s; s = p0; p0 = p1; p1 = s; }
+    const int2 a = A.enodes[e0], b = A.enodes[e1];
+    if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) return false;
+    Box b0 = box_empty(), b1 = box_empty();
+    swept_box(b0, A.x, A.x0, a.x);
+    swept_box(b0, A.x, A.x0, a.y);
+    swept_box(b1, A.x, A.x0, b.x);
+    swept_box(b1, A.x, A.x0, b.y);
+    if (!box_overlap(b0, b1, A.th)) return false;
+    if (!ee_owner(A, e0, e1, p0, p1)) return false;
+    key = (1ull << (2 * A.keyBits)) | ((uint64_t)e0 << A.keyBits) | (uint64_t)e1;
+    return true;
+}
+
+// wave-aggregated append of c items per lane; returns this lane's first slot
+__device__ __forceinline__ unsigned wave_append(unsigned* counter, int c) {
+    const int lane = threadIdx.x & 63;
+    int incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63, 64);
+    unsigned base = 0;
+    if (lane == 63 && total > 0) base = atomicAdd(counter, (unsigned)total);
+    base = __shfl(base, 63, 64);
+    return base + (unsigned)(incl - c);
+}
+
+// One thread per face pair.  Survivors are kept as a 15-bit mask and their
+// keys recomputed at write time (no dynamically indexed private array).
+__global__ void __launch_bounds__(256) k_ccd_filter(FilterArgs A) {
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    unsigned mvf = 0, mee = 0;
+    int fa = 0, fb = 0;
+    int4 na = make_int4(0, 0, 0, 0), nb = na;
+    if (p < A.nPairs) {
+        const int2 pr = A.pairs[p];
+        fa = pr.x;
+        fb = pr.y;
+        na = A.fnodes[fa];
+        nb = A.fnodes[fb];
+        const FaceRec ra = face_rec(A, fa), rb = face_rec(A, fb);
+        uint64_t key;
+        for (int t = 0; t < 6; t++)
+            if (filter_test(A, t, fa, na, ra, fb, nb, rb, key)) mvf |= 1u << t;
+        for (int t = 6; t < 15; t++)
+            if (filter_test(A, t, fa, na, ra, fb, nb, rb, key)) mee |= 1u << t;
+    }
+    const unsigned ovf = wave_append(&A.count[0], __popc(mvf));
+    const unsigned oee = wave_append(&A.count[1], __popc(mee));
+    unsigned k = 0;
+    for (int t = 0; t < 6; t++)
+        if (mvf >> t & 1u) {
+            const int n = t < 3 ? pick3(na, t) : pick3(nb, t - 3);
+            const int f = t < 3 ? fb : fa;
+            if ((long)(ovf + k) < A.cap) A.candVF[ovf + k] = ((uint64_t)n << A.keyBits) | (uint64_t)f;
+            k++;
+        }
+    k = 0;
+    for (int t = 6; t < 15; t++)
+        if (mee >> t & 1u) {
+            int e0 = pick3(A.fedges[fa], (t - 6) / 3), e1 = pick3(A.fedges[fb], (t - 6) % 3);
+            if (e1 < e0) { int s = e0; e0 = e1; e1 = s; }
+            if ((long)(oee + k) < A.cap)
+                A.candEE[oee + k] = (1ull << (2 * A.keyBits)) | ((uint64_t)e0 << A.keyBits) | (uint64_t)e1;
+            k++;
+        }
+}
+
+// ---- stage 2: the FP64 test of every distinct candidate ----
+
+struct CheckArgs {
+    const uint64_t* cand;
+    long n;
+    int keyBits;
+    const float4 *x, *x0;
+    const int4* fnodes;
+    const int2* enodes;
+    cs_impact_rec* out;
+    uint64_t* keys;
+    unsigned* count;
+    long cap;
+};
+
+// checkVertexFaceImpact / checkEdgeEdgeImpact (src/CollisionHelper.cu:111-135)
+// after their filters: the cubic, its roots and the proximity test at each root.
+template <int TYPE>
+__global__ void __launch_bounds__(256) k_ccd_check(CheckArgs A) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= A.n) return;
+    const uint64_t key = A.cand[i];
+    const uint64_t mask = (1ull << A.keyBits) - 1;
+    const int a = (int)((key >> A.keyBits) & mask), b = (int)(key & mask);
+    cs_impact_rec imp;
+    imp.type = TYPE;
+    if (TYPE == 0) {
+        const int4 fn = A.fnodes[b];
+        imp.nodes[0] = a; imp.nodes[1] = fn.x; imp.nodes[2] = fn.y; imp.nodes[3] = fn.z;
+    } else {
+        const int2 ea = A.enodes[a], eb = A.enodes[b];
+        imp.nodes[0] = ea.x; imp.nodes[1] = ea.y; imp.nodes[2] = eb.x; imp.nodes[3] = eb.y;
+    }
+    V3d X0[4], X1[4];
+    for (int k = 0; k < 4; k++) {
+        X0[k] = tod(ld3(A.x0[imp.nodes[k]]));
+        X1[k] = tod(ld3(A.x[imp.nodes[k]]));
+    }
+    if (!check_impact(TYPE, X0, X1, imp)) return;
+    const unsigned slot = atomicAdd(A.count, 1u);
     if ((long)slot < A.cap) {
         A.out[slot] = imp;
         A.keys[slot] = key;
-    }
-}
-
-__device__ void load4(const CcdArgs& A, const int nd[4], V3d X0[4], V3d X1[4]) {
-    for (int k = 0; k < 4; k++) {
-        X0[k] = tod(ld3(A.x0[nd[k]]));
-        X1[k] = tod(ld3(A.x[nd[k]]));
-    }
-}
-
-// checkVertexFaceImpact (src/CollisionHelper.cu:111-122)
-__device__ void vf_impact(const CcdArgs& A, int node, int f) {
-    int4 fn = A.fnodes[f];
-    if (node == fn.x || node == fn.y || node == fn.z) return;
-    Box nb = box_empty(), fb = box_empty();
-    swept_box(nb, A.x, A.x0, node);
-    swept_box(fb, A.x, A.x0, fn.x);
-    swept_box(fb, A.x, A.x0, fn.y);
-    swept_box(fb, A.x, A.x0, fn.z);
-    if (!box_overlap(nb, fb, A.th)) return;
-    cs_impact_rec imp;
-    imp.type = 0;
-    imp.nodes[0] = node; imp.nodes[1] = fn.x; imp.nodes[2] = fn.y; imp.nodes[3] = fn.z;
-    V3d X0[4], X1[4];
-    load4(A, imp.nodes, X0, X1);
-    if (check_impact(0, X0, X1, imp))
-        emit(A, imp, ((uint64_t)node << A.keyBits) | (uint64_t)f);
-}
-
-// checkEdgeEdgeImpact (src/CollisionHelper.cu:124-135), lower edge id first
-__device__ void ee_impact(const CcdArgs& A, int e0, int e1) {
-    if (e1 < e0) { int t = e0; e0 = e1; e1 = t; }
-    int2 a = A.enodes[e0], b = A.enodes[e1];
-    if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) return;
-    Box ba = box_empty(), bb = box_empty();
-    swept_box(ba, A.x, A.x0, a.x);
-    swept_box(ba, A.x, A.x0, a.y);
-    swept_box(bb, A.x, A.x0, b.x);
-    swept_box(bb, A.x, A.x0, b.y);
-    if (!box_overlap(ba, bb, A.th)) return;
-    cs_impact_rec imp;
-    imp.type = 1;
-    imp.nodes[0] = a.x; imp.nodes[1] = a.y; imp.nodes[2] = b.x; imp.nodes[3] = b.y;
-    V3d X0[4], X1[4];
-    load4(A, imp.nodes, X0, X1);
-    if (check_impact(1, X0, X1, imp))
-        emit(A, imp, (1ull << (2 * A.keyBits)) | ((uint64_t)e0 << A.keyBits) | (uint64_t)e1);
-}
-
-// One thread per (test, pair), test-major: a wavefront runs one kind of test
-// (vertex-face or edge-edge) over 64 consecutive pairs, so the branches of
-// check_impact stay coherent and the pair loads are contiguous.
-__global__ void __launch_bounds__(256) k_ccd_tests(CcdArgs A) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 15 * A.nPairs) return;
-    const int k = (int)(t / A.nPairs);
-    const long p = t - (long)k * A.nPairs;
-    const int2 pr = A.pairs[p];
-    if (k < 3) {
-        int4 fn = A.fnodes[pr.x];
-        vf_impact(A, k == 0 ? fn.x : (k == 1 ? fn.y : fn.z), pr.y);
-    } else if (k < 6) {
-        int4 fn = A.fnodes[pr.y];
-        vf_impact(A, k == 3 ? fn.x : (k == 4 ? fn.y : fn.z), pr.x);
-    } else {
-        const int j = (k - 6) / 3, l = (k - 6) % 3;
-        int4 e0 = A.fedges[pr.x], e1 = A.fedges[pr.y];
-        ee_impact(A, j == 0 ? e0.x : (j == 1 ? e0.y : e0.z), l == 0 ? e1.x : (l == 1 ? e1.y : e1.z));
     }
 }
 
@@ -338,23 +479,59 @@ int find_impacts(cs_ctx* C, float th) {
     int keyBits = 1;
     long mx = std::max(std::max((long)C->N, (long)C->F), (long)C->E);
     while ((1L << keyBits) <= mx) keyBits++;
-    unsigned* count = C->impCount.as<unsigned>(2);
-    long cap = std::max<long>(4096, nPairs / 4);
+    unsigned* count = C->impCount.as<unsigned>(4);
+    // stage 1: distinct candidate tests (vertex-face and edge-edge lists)
+    long ccap = std::max<long>(4096, 2 * nPairs);
+    long nVF = 0, nEE = 0;
     for (int attempt = 0; attempt < 2; attempt++) {
-        cs_impact_rec* imps = C->impacts.as<cs_impact_rec>(cap);
-        uint64_t* keys = C->impKeys.as<uint64_t>(cap);
-        fill_u32(count, 0u, 1, s);
-        CcdArgs A;
+        uint64_t* cand = C->ccdCand.as<uint64_t>(2 * ccap);
+        fill_u32(count, 0u, 2, s);
+        FilterArgs A;
         A.pairs = C->pairs.ptr<int2>();
         A.nPairs = nPairs;
         A.th = th;
         A.keyBits = keyBits;
+        A.nCloths = C->nCloths;
         A.x = C->x.ptr<float4>(); A.x0 = C->x0.ptr<float4>();
-        A.fnodes = C->fnodes.ptr<int4>(); A.fedges = C->fedges.ptr<int4>(); A.enodes = C->enodes.ptr<int2>();
-        A.out = imps; A.keys = keys; A.count = count; A.cap = cap;
-        hipLaunchKernelGGL(k_ccd_tests, dim3(cdiv(15 * nPairs, 256)), dim3(256), 0, s, A);
+        A.fnodes = C->fnodes.ptr<int4>(); A.fedges = C->fedges.ptr<int4>();
+        A.enodes = C->enodes.ptr<int2>(); A.eadj = C->eadj.ptr<int2>();
+        A.nfPtr = C->nfPtr.ptr<int>(); A.nfIdx = C->nfIdx.ptr<int>(); A.fbox = C->faceBox.ptr<float4>();
+        A.candVF = cand; A.candEE = cand + ccap;
+        A.count = count; A.cap = ccap;
+        hipLaunchKernelGGL(k_ccd_filter, dim3(cdiv(nPairs, 256)), dim3(256), 0, s, A);
         CS_LAUNCH_CHECK();
-        CS_HIP(hipMemcpyAsync(C->hCount, count, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipMemcpyAsync(C->hCount, count, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipStreamSynchronize(s));
+        nVF = C->hCount[0];
+        nEE = C->hCount[1];
+        if (nVF <= ccap && nEE <= ccap) break;
+        ccap = std::max(nVF, nEE) + 4096;
+    }
+    // stage 2: FP64 tests; impacts are appended with their keys
+    long cap = std::max<long>(4096, (nVF + nEE) / 8);
+    for (int attempt = 0; attempt < 2; attempt++) {
+        cs_impact_rec* imps = C->impacts.as<cs_impact_rec>(cap);
+        uint64_t* keys = C->impKeys.as<uint64_t>(cap);
+        fill_u32(count + 2, 0u, 1, s);
+        CheckArgs A;
+        A.keyBits = keyBits;
+        A.x = C->x.ptr<float4>(); A.x0 = C->x0.ptr<float4>();
+        A.fnodes = C->fnodes.ptr<int4>(); A.enodes = C->enodes.ptr<int2>();
+        A.out = imps; A.keys = keys; A.count = count + 2; A.cap = cap;
+        const uint64_t* cand = C->ccdCand.ptr<uint64_t>();
+        if (nVF > 0) {
+            A.cand = cand;
+            A.n = nVF;
+            hipLaunchKernelGGL(k_ccd_check<0>, dim3(cdiv(nVF, 256)), dim3(256), 0, s, A);
+            CS_LAUNCH_CHECK();
+        }
+        if (nEE > 0) {
+            A.cand = cand + ccap;
+            A.n = nEE;
+            hipLaunchKernelGGL(k_ccd_check<1>, dim3(cdiv(nEE, 256)), dim3(256), 0, s, A);
+            CS_LAUNCH_CHECK();
+        }
+        CS_HIP(hipMemcpyAsync(C->hCount, count + 2, sizeof(unsigned), hipMemcpyDeviceToHost, s));
         CS_HIP(hipStreamSynchronize(s));
         if ((long)C->hCount[0] <= cap) break;
         cap = (long)C->hCount[0] + 1024;

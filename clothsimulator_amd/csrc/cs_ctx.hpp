@@ -25,7 +25,6 @@ namespace cs {
 
 static const int MAT_N = 30;
 static const int SLICE = 64;
-
 struct HostMaterial {
     float density = 0.0f;
     std::vector<float> table;  // 30^3 x 4
@@ -55,8 +54,9 @@ struct Bvh {
     DBuf vals, valsTmp;    // uint32 local face ids
     DBuf nodes;            // BNode[2nf-1]
     DBuf child;            // int2[nf-1]
-    DBuf parent;           // int[2nf-1]
-    DBuf flags;            // uint32[nf-1]
+    DBuf range;            // int2[nf-1]: first and last sorted leaf under each internal node
+    DBuf leafTab;          // LBox[5][nf]: union of sorted leaves [j, j + 2^k), k = 1..5
+    DBuf blockTab;         // LBox[levels][nblk]: union of 64-leaf blocks [i, i + 2^k)
     DBuf fbox;             // float4 x2 per face (unsorted leaf boxes)
     DBuf partial;          // block partial boxes
     DBuf scene;            // float4 x2 (p, d)
@@ -82,7 +82,7 @@ struct ClothSys {
     // dynamic contact blocks (CSR over rows)
     int nContact = 0;
     DBuf cRowPtr, cCol, cVal, cDiag;
-    // PCG vectors
+    // PCG vectors: x, r, q float[3nn]; z float4[nn]; d float4[2nn] (two generations)
     DBuf x, r, z, d, q;
     // host copies used by cs_assemble
     std::vector<int> hRowPtr, hCol;  // exact (unpadded) static pattern
@@ -150,6 +150,10 @@ struct cs_ctx {
     cs::PcgScalars* hScal = nullptr;  // pinned
     // CCD / impacts
     cs::DBuf impacts, impCount, impKeys, impKeysTmp, impIdx, impIdxTmp, impSorted;
+    // per face, from its mesh's last-built LBVH: float4 (leaf box lo, mesh id bits),
+    // float4 (leaf box hi, sorted leaf index bits)
+    cs::DBuf faceBox;
+    cs::DBuf ccdCand;   // uint64 candidate keys of the CCD prefilter (one per distinct test)
     cs::DBuf impAcc;  // accumulated impacts of the deform pass
     int nImpAcc = 0;
     cs::DBuf nodeMin, nodeRemoved, impState;

@@ -299,10 +299,13 @@ static void build_cloth_system(cs_ctx* C, int c, ClothSys& S) {
     upload(S.hcount, hcount, st);
     upload(S.hpos, hpos, st);
     S.vals.as<float>((size_t)S.totalSlots * SLICE * 9);
+    fill_f32(S.vals.ptr<float>(), 0.0f, (long)S.totalSlots * SLICE * 9, st);  // rows past nn stay zero
     S.b.as<float>(3 * (size_t)M.nn);
     S.bc.as<float>(3 * (size_t)M.nn);
     S.Minv.as<float>(9 * (size_t)M.nn);
-    for (DBuf* v : {&S.x, &S.r, &S.z, &S.d, &S.q}) v->as<float>(3 * (size_t)M.nn);
+    for (DBuf* v : {&S.x, &S.r, &S.q}) v->as<float>(3 * (size_t)M.nn);
+    S.z.as<float4>((size_t)M.nn);      // gathered by the SpMV: one 16-byte load per node
+    S.d.as<float4>(2 * (size_t)M.nn);  // search direction, two generations (ping-pong)
     S.cRowPtr.as<int>(M.nn + 1);
     S.cDiag.as<int>(M.nn);
     fill_u32(S.cRowPtr.ptr<uint32_t>(), 0u, M.nn + 1, st);

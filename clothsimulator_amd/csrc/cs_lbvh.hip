@@ -4,9 +4,14 @@
 //   leaf boxes over x (and x0 for CCD), scene box, 30-bit Morton code of each
 //   box centre with the reference's FP32 operation order, a stable LSD radix
 //   sort of the codes carrying the face index (equivalent to sorting the
-//   reference's 64-bit key code<<32|face), Karras 2012 internal nodes, and
-//   bottom-up boxes / maxIndex (largest sorted leaf index in the subtree) where
-//   the second thread to reach a node computes it (no float atomics).
+//   reference's 64-bit key code<<32|face), Karras 2012 internal nodes.  An
+//   internal node covers a contiguous range [first, last] of sorted leaves, so
+//   its box is the union of that range and its maxIndex (largest sorted leaf
+//   index in the subtree) is `last`.  Boxes come from range-union tables
+//   (unions of 2^k consecutive leaves for k <= 5, and of 2^k consecutive
+//   64-leaf blocks), at most six table reads per node: no thread waits for
+//   another and no grid-wide fence is needed.  min/max are exact, so the boxes
+//   equal the reference's bottom-up unions bit for bit.
 // Traversal (src/BVHHelper.cu:151-307): one thread per query leaf, explicit
 //   stack in LDS; a leaf pair is reported when the query box overlaps the
 //   target leaf box dilated by `thickness`; for a mesh against itself only
@@ -116,14 +121,16 @@ __global__ void __launch_bounds__(256) k_morton(int nf, const float4* __restrict
 }
 
 // leaves in sorted order; keys become the reference's 64-bit code<<32 | face
-__global__ void __launch_bounds__(256) k_fill_leaves(int nf, int f0, uint64_t* __restrict__ keys,
+__global__ void __launch_bounds__(256) k_fill_leaves(int nf, int f0, int mesh, uint64_t* __restrict__ keys,
                                                      const uint32_t* __restrict__ vals, const float4* __restrict__ fbox,
-                                                     BNode* __restrict__ nodes) {
+                                                     BNode* __restrict__ nodes, float4* __restrict__ faceBox) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= nf) return;
     uint32_t f = vals[i];
     keys[i] = (keys[i] << 32) | (uint64_t)f;
     float4 lo = fbox[2 * f], hi = fbox[2 * f + 1];
+    faceBox[2 * (f0 + (int)f)] = make_float4(lo.x, lo.y, lo.z, __int_as_float(mesh));
+    faceBox[2 * (f0 + (int)f) + 1] = make_float4(hi.x, hi.y, hi.z, __int_as_float(i));
     BNode n;
     n.lo[0] = lo.x; n.lo[1] = lo.y; n.lo[2] = lo.z;
     n.hi[0] = hi.x; n.hi[1] = hi.y; n.hi[2] = hi.z;
@@ -136,14 +143,13 @@ __device__ __forceinline__ int clz64(uint64_t a) { return __clzll((long long)a);
 
 // findRange / findSplit / initializeInternalNodes (src/BVHHelper.cu:48-103)
 __global__ void __launch_bounds__(256) k_karras(int ni, const uint64_t* __restrict__ k, int2* __restrict__ child,
-                                                int* __restrict__ parent) {
+                                                int2* __restrict__ range) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= ni) return;
     int left, right;
     if (i == 0) {
         left = 0;
         right = ni;
-        parent[0] = -1;
     } else {
         const uint64_t code = k[i];
         const int dl = clz64(code ^ k[i - 1]);
@@ -166,41 +172,121 @@ __global__ void __launch_bounds__(256) k_karras(int ni, const uint64_t* __restri
     const int L = (mid == left) ? ni + mid : mid;        // unified index: leaves start at ni = nf-1
     const int R = (mid + 1 == right) ? ni + mid + 1 : mid + 1;
     child[i] = make_int2(L, R);
-    parent[L] = i;
-    parent[R] = i;
+    range[i] = make_int2(left, right);
 }
 
-// bottom-up boxes: the second thread to reach an internal node combines its children
-__global__ void __launch_bounds__(256) k_bottom_up(int nf, const int2* __restrict__ child, const int* __restrict__ parent,
-                                                   BNode* nodes, unsigned* flags) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nf) return;
-    int p = parent[nf - 1 + i];
-    while (p >= 0) {
-        __threadfence();
-        unsigned old = atomicAdd(&flags[p], 1u);
-        if (old == 0u) return;
-        __threadfence();
-        int2 ch = child[p];
-        BNode a = nodes[ch.x], b = nodes[ch.y];
-        BNode n;
-        for (int k = 0; k < 3; k++) { n.lo[k] = tmin(a.lo[k], b.lo[k]); n.hi[k] = tmax(a.hi[k], b.hi[k]); }
-        n.maxIndex = tmax(a.maxIndex, b.maxIndex);
-        n.face = -1;
-        nodes[p] = n;
-        p = parent[p];
+// A box as six floats (24 B) in the range-union tables.
+struct LBox {
+    float lo[3], hi[3];
+};
+__device__ __forceinline__ LBox lbox_empty() {
+    LBox b;
+    const Box e = box_empty();
+    for (int k = 0; k < 3; k++) { b.lo[k] = e.lo[k]; b.hi[k] = e.hi[k]; }
+    return b;
+}
+__device__ __forceinline__ LBox lbox_union(const LBox& a, const LBox& b) {
+    LBox c;
+    for (int k = 0; k < 3; k++) { c.lo[k] = tmin(a.lo[k], b.lo[k]); c.hi[k] = tmax(a.hi[k], b.hi[k]); }
+    return c;
+}
+__device__ __forceinline__ LBox leaf_lbox(const BNode* nodes, int nf, int j) {
+    LBox b;
+    if (j >= nf) return lbox_empty();
+    const BNode& n = nodes[nf - 1 + j];
+    for (int k = 0; k < 3; k++) { b.lo[k] = n.lo[k]; b.hi[k] = n.hi[k]; }
+    return b;
+}
+
+constexpr int LEAF_LEVELS = 5;  // leaf table levels 1..5 (runs of 2..32 leaves)
+
+// One 128-thread workgroup per 64-leaf block: leaves [64b, 64b + 128) in LDS,
+// levels 1..5 of the leaf table for the block's 64 leaves, and the block's own
+// box (level 0 of the block table).
+__global__ void __launch_bounds__(128) k_leaf_table(int nf, const BNode* __restrict__ nodes, LBox* __restrict__ leafTab,
+                                                    LBox* __restrict__ blockTab) {
+    __shared__ LBox buf[2][128];
+    const int t = threadIdx.x;
+    const int j0 = blockIdx.x * 64;
+    buf[0][t] = leaf_lbox(nodes, nf, j0 + t);
+    __syncthreads();
+    int cur = 0;
+    for (int k = 1; k <= LEAF_LEVELS + 1; k++) {
+        const int h = 1 << (k - 1);
+        const LBox v = t + h < 128 ? lbox_union(buf[cur][t], buf[cur][t + h]) : buf[cur][t];
+        buf[cur ^ 1][t] = v;
+        __syncthreads();
+        cur ^= 1;
+        if (k <= LEAF_LEVELS) {
+            if (t < 64 && j0 + t < nf) leafTab[(size_t)(k - 1) * nf + j0 + t] = v;
+        } else if (t == 0) {
+            blockTab[blockIdx.x] = v;  // 64 leaves
+        }
     }
+}
+
+// Level k of the block table from level k - 1.
+__global__ void __launch_bounds__(256) k_block_table(int nblk, int k, LBox* __restrict__ blockTab) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i + (1 << k) > nblk) return;
+    const LBox* prev = blockTab + (size_t)(k - 1) * nblk;
+    blockTab[(size_t)k * nblk + i] = lbox_union(prev[i], prev[i + (1 << (k - 1))]);
+}
+
+__device__ __forceinline__ int floor_log2(int v) { return 31 - __clz(v); }
+
+// union of sorted leaves [a, b], b - a < 63 (two overlapping runs of 2^k leaves, k <= 5)
+__device__ __forceinline__ LBox short_run(const BNode* nodes, const LBox* leafTab, int nf, int a, int b) {
+    const int len = b - a + 1;
+    const int k = floor_log2(len);
+    if (k == 0) return leaf_lbox(nodes, nf, a);
+    const LBox* tab = leafTab + (size_t)(k - 1) * nf;
+    return lbox_union(tab[a], tab[b - (1 << k) + 1]);
+}
+
+// internal node boxes from their leaf ranges
+__global__ void __launch_bounds__(256) k_internal_boxes(int nf, int nblk, const int2* __restrict__ range,
+                                                        const LBox* __restrict__ leafTab,
+                                                        const LBox* __restrict__ blockTab, BNode* __restrict__ nodes) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nf - 1) return;
+    const int2 rg = range[i];
+    const int a = rg.x, b = rg.y;
+    LBox box;
+    if (b - a < 63) {
+        box = short_run(nodes, leafTab, nf, a, b);
+    } else {
+        const int A = (a + 63) & ~63;  // first block boundary at or after a
+        const int B = (b + 1) & ~63;   // last block boundary at or before b + 1
+        box = lbox_empty();
+        if (A > a) box = lbox_union(box, short_run(nodes, leafTab, nf, a, A - 1));
+        if (b >= B) box = lbox_union(box, short_run(nodes, leafTab, nf, B, b));
+        const int nb = (B - A) >> 6;  // whole blocks A/64 .. B/64 - 1
+        if (nb > 0) {
+            const int k = floor_log2(nb);
+            const LBox* tab = blockTab + (size_t)k * nblk;
+            box = lbox_union(box, lbox_union(tab[A >> 6], tab[(B >> 6) - (1 << k)]));
+        }
+    }
+    BNode n;
+    for (int k = 0; k < 3; k++) { n.lo[k] = box.lo[k]; n.hi[k] = box.hi[k]; }
+    n.maxIndex = b;
+    n.face = -1;
+    nodes[i] = n;
 }
 
 // refit: recompute leaf boxes in place (sorted order, maxIndex unchanged)
 __global__ void __launch_bounds__(256) k_refit_leaves(int nf, bool ccd, const float4* __restrict__ x,
                                                       const float4* __restrict__ x0, const int4* __restrict__ fnodes,
-                                                      BNode* __restrict__ nodes) {
+                                                      BNode* __restrict__ nodes, float4* __restrict__ faceBox) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= nf) return;
     BNode& n = nodes[nf - 1 + i];
     Box b = face_box(x, x0, fnodes[n.face], ccd);
     for (int k = 0; k < 3; k++) { n.lo[k] = b.lo[k]; n.hi[k] = b.hi[k]; }
+    float4* r = faceBox + 2 * (size_t)n.face;  // mesh and sorted index (.w) are unchanged
+    r[0] = make_float4(b.lo[0], b.lo[1], b.lo[2], r[0].w);
+    r[1] = make_float4(b.hi[0], b.hi[1], b.hi[2], r[1].w);
 }
 
 __device__ __forceinline__ Box node_box(const BNode& n) {
@@ -271,6 +357,12 @@ __global__ void __launch_bounds__(TB) k_traverse(int qnf, const BNode* __restric
     if (!ok) atomicOr(overflow, 1u);
 }
 
+static int floor_log2_host(int v) {
+    int k = 0;
+    while ((2L << k) <= v) k++;
+    return k;
+}
+
 void ensure_bvh(cs_ctx* C, Bvh& B, int nf) {
     B.nf = nf;
     B.keys.as<uint64_t>(nf);
@@ -279,19 +371,31 @@ void ensure_bvh(cs_ctx* C, Bvh& B, int nf) {
     B.valsTmp.as<uint32_t>(nf);
     B.nodes.as<BNode>(2L * nf - 1);
     B.child.as<int2>(nf > 1 ? nf - 1 : 1);
-    B.parent.as<int>(2L * nf - 1);
-    B.flags.as<unsigned>(nf > 1 ? nf - 1 : 1);
+    B.range.as<int2>(nf > 1 ? nf - 1 : 1);
+    B.leafTab.as<LBox>((size_t)LEAF_LEVELS * nf);
+    const int nblk = cdiv(nf, 64);
+    B.blockTab.as<LBox>((size_t)(floor_log2_host(nblk) + 1) * nblk);
     B.fbox.as<float4>(2L * nf);
     B.partial.as<float4>(2L * cdiv(nf, 256));
     B.scene.as<float4>(2);
 }
 
-void bottom_up(cs_ctx* C, Bvh& B) {
+// internal boxes and maxIndex of the built tree from its current leaf boxes
+void internal_boxes(cs_ctx* C, Bvh& B) {
     if (B.nf <= 1) return;
     hipStream_t s = C->stream;
-    fill_u32(B.flags.ptr<uint32_t>(), 0u, B.nf - 1, s);
-    hipLaunchKernelGGL(k_bottom_up, dim3(cdiv(B.nf, 256)), dim3(256), 0, s, B.nf, B.child.ptr<int2>(),
-                       B.parent.ptr<int>(), B.nodes.ptr<BNode>(), B.flags.ptr<unsigned>());
+    const int nblk = cdiv(B.nf, 64);
+    const int levels = floor_log2_host(nblk) + 1;
+    hipLaunchKernelGGL(k_leaf_table, dim3(nblk), dim3(128), 0, s, B.nf, B.nodes.ptr<BNode>(), B.leafTab.ptr<LBox>(),
+                       B.blockTab.ptr<LBox>());
+    CS_LAUNCH_CHECK();
+    for (int k = 1; k < levels; k++) {
+        hipLaunchKernelGGL(k_block_table, dim3(cdiv(nblk - (1 << k) + 1, 256)), dim3(256), 0, s, nblk, k,
+                           B.blockTab.ptr<LBox>());
+        CS_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_internal_boxes, dim3(cdiv(B.nf - 1, 256)), dim3(256), 0, s, B.nf, nblk, B.range.ptr<int2>(),
+                       B.leafTab.ptr<LBox>(), B.blockTab.ptr<LBox>(), B.nodes.ptr<BNode>());
     CS_LAUNCH_CHECK();
 }
 
@@ -315,23 +419,25 @@ void bvh_build(cs_ctx* C, int mesh, bool ccd) {
     CS_LAUNCH_CHECK();
     radix_sort_pairs(B.keys.ptr<uint64_t>(), B.vals.ptr<uint32_t>(), B.keysTmp.ptr<uint64_t>(),
                      B.valsTmp.ptr<uint32_t>(), M.nf, 0, 30, C->ws, s);
-    hipLaunchKernelGGL(k_fill_leaves, dim3(nb), dim3(256), 0, s, M.nf, M.f0, B.keys.ptr<uint64_t>(),
-                       B.vals.ptr<uint32_t>(), B.fbox.ptr<float4>(), B.nodes.ptr<BNode>());
+    float4* faceBox = C->faceBox.as<float4>(2 * (size_t)C->F);  // one array over all meshes: fixed size
+    hipLaunchKernelGGL(k_fill_leaves, dim3(nb), dim3(256), 0, s, M.nf, M.f0, mesh, B.keys.ptr<uint64_t>(),
+                       B.vals.ptr<uint32_t>(), B.fbox.ptr<float4>(), B.nodes.ptr<BNode>(), faceBox);
     CS_LAUNCH_CHECK();
     if (M.nf > 1) {
         hipLaunchKernelGGL(k_karras, dim3(cdiv(M.nf - 1, 256)), dim3(256), 0, s, M.nf - 1, B.keys.ptr<uint64_t>(),
-                           B.child.ptr<int2>(), B.parent.ptr<int>());
+                           B.child.ptr<int2>(), B.range.ptr<int2>());
         CS_LAUNCH_CHECK();
-        bottom_up(C, B);
+        internal_boxes(C, B);
     }
 }
 
 void bvh_refit(cs_ctx* C, int mesh) {
     Bvh& B = C->bvh[mesh];
     hipLaunchKernelGGL(k_refit_leaves, dim3(cdiv(B.nf, 256)), dim3(256), 0, C->stream, B.nf, B.ccd,
-                       C->x.ptr<float4>(), C->x0.ptr<float4>(), C->fnodes.ptr<int4>(), B.nodes.ptr<BNode>());
+                       C->x.ptr<float4>(), C->x0.ptr<float4>(), C->fnodes.ptr<int4>(), B.nodes.ptr<BNode>(),
+                       C->faceBox.ptr<float4>());
     CS_LAUNCH_CHECK();
-    bottom_up(C, B);
+    internal_boxes(C, B);
 }
 
 // Simulator::traverse order (src/Simulator.cu:179-196): for each cloth, itself,

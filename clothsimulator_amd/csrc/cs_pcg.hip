@@ -6,13 +6,18 @@
 // Differences: M is the inverse 3x3 diagonal block (not the scalar diagonal)
 // and the loop stops when ||r|| <= tol ||b|| (not ||r|| < 1e-6 absolute).
 //
-// Three kernels per iteration, every scalar on the device and no host
-// round trip inside a chunk of iterations.  Each kernel writes one partial sum
-// per block; the next kernel's blocks each reduce those partials themselves
-// (a few KB from L2) in a fixed order, so no grid-wide fence or atomic is
-// needed and the result is deterministic.  Kernels read `done` and return at
-// once after convergence, so the host queues a chunk (a captured hipGraph) and
-// polls the flag once per chunk.
+// Two kernels per iteration, every scalar on the device and no host round
+// trip inside a chunk of iterations:
+//   SpMV    s = A z and t = A d_prev in one sweep over the matrix, then beta
+//           from the r.z partials, q = s + beta t (= A d exactly: no
+//           recurrence), d = z + beta d_prev for the block's own rows, partial
+//           d.q; it also takes the stop decision from the r.r partials;
+//   update  alpha, x += alpha d, r -= alpha q, z = M^-1 r, partials r.z, r.r.
+// Each kernel writes one partial sum per block; the next kernel's blocks each
+// reduce those partials themselves in a fixed order, so no grid-wide fence or
+// atomic is needed and the result is deterministic.  The partials are loaded
+// at kernel entry and reduced after the kernel's memory sweep (SpMV) or after
+// its row loads (update), so their latency hides behind the sweep.
 #include <algorithm>
 #include <cstring>
 
@@ -23,36 +28,84 @@ namespace cs {
 
 namespace {
 
-constexpr int PB = 256;     // threads per block
-constexpr int PMAX = 1024;  // at most this many blocks (grid-stride over rows)
+constexpr int PB = 256;      // threads per block
+constexpr int PPT = 8;       // partials held per thread by the fast reduction path
+constexpr int PMAX = 16384;  // at most this many blocks (grid-stride over rows beyond), a multiple of 8
 
 // partial-sum slots (each PMAX floats): dq, rz[2] (ping-pong by iteration parity), rr, bb
 enum { P_DQ = 0, P_RZ0 = 1, P_RZ1 = 2, P_RR = 3, P_BB = 4, P_COUNT = 5 };
 
 struct PcgDev {
-    int iters;      // completed iterations (read by update/dir kernels)
-    int itersNext;  // written by the dir kernel, copied to iters by the next SpMV
+    int iters;  // iterations completed when the loop stopped
     int done;
     int maxit;
+    int pad;
     float tol2bb, bb, rr, rz;
 };
 
-__device__ __forceinline__ float block_sum(float v, float* lds) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    __syncthreads();
-    if (lane == 0) lds[wave] = v;
-    __syncthreads();
-    return (lds[0] + lds[1]) + (lds[2] + lds[3]);
+// Workgroups are dealt to the 8 XCDs round-robin.  Logical block L gives each
+// XCD one contiguous range of rows, so the column gathers of neighbouring rows
+// and the next kernel's reads of the same rows hit that XCD's L2.
+// gridDim.x is a multiple of 8.
+__device__ __forceinline__ int logical_block() {
+    const int per = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
-// every block reduces the nb partials of slot k identically
-__device__ __forceinline__ float reduce_partials(const float* __restrict__ part, int k, int nb, float* lds) {
-    float a = 0.0f;
-    for (int b = threadIdx.x; b < nb; b += PB) a += part[k * PMAX + b];
-    return block_sum(a, lds);
+// Sum of NS values over the block (every thread gets the sums).
+template <int NS>
+__device__ __forceinline__ void block_sums(float (&v)[NS], float* lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[s] += __shfl_xor(v[s], o, 64);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int s = 0; s < NS; s++) lds[4 * s + wave] = v[s];
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NS; s++) v[s] = (lds[4 * s] + lds[4 * s + 1]) + (lds[4 * s + 2] + lds[4 * s + 3]);
 }
+
+__device__ __forceinline__ float block_sum(float v, float* lds) {
+    float a[1] = {v};
+    block_sums<1>(a, lds);
+    return a[0];
+}
+
+// Partials of NS slots: issue() loads this thread's share (nb <= PB * PPT),
+// sums() reduces them over the block in a fixed order.  Larger grids take the
+// looped path.
+template <int NS>
+struct Partials {
+    float raw[NS][PPT];
+    // unconditional loads (clamped index, masked in sums) so that nothing waits for them here
+    __device__ __forceinline__ void issue(const float* __restrict__ part, const int (&slot)[NS], int nb) {
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int u = 0; u < PPT; u++) raw[s][u] = part[slot[s] * PMAX + min((int)threadIdx.x + u * PB, nb - 1)];
+    }
+    __device__ __forceinline__ void sums(const float* __restrict__ part, const int (&slot)[NS], int nb, float* lds,
+                                         float (&out)[NS]) {
+        // keeps the compiler from summing (and so waiting for) the loads at issue()
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int u = 0; u < PPT; u++) asm volatile("" : "+v"(raw[s][u]));
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            float a = 0.0f;
+#pragma unroll
+            for (int u = 0; u < PPT; u++) a += (int)threadIdx.x + u * PB < nb ? raw[s][u] : 0.0f;
+            for (int b = threadIdx.x + PPT * PB; b < nb; b += PB) a += part[slot[s] * PMAX + b];
+            out[s] = a;
+        }
+        block_sums<NS>(out, lds);
+    }
+};
 
 __device__ __forceinline__ V3f mat3v(const float* __restrict__ m, V3f a) {
     return mk(m[0] * a.x + m[1] * a.y + m[2] * a.z, m[3] * a.x + m[4] * a.y + m[5] * a.z,
@@ -60,140 +113,203 @@ __device__ __forceinline__ V3f mat3v(const float* __restrict__ m, V3f a) {
 }
 __device__ __forceinline__ V3f ld3v(const float* p, int i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 __device__ __forceinline__ void st3v(float* p, int i, V3f a) { p[3 * i] = a.x; p[3 * i + 1] = a.y; p[3 * i + 2] = a.z; }
+__device__ __forceinline__ V3f v3(float4 a) { return mk(a.x, a.y, a.z); }
+__device__ __forceinline__ float4 f4(V3f a) { return make_float4(a.x, a.y, a.z, 0.0f); }
+
+__device__ __forceinline__ void acc3(float& y0, float& y1, float& y2, const float (&a)[9], V3f v) {
+    y0 += a[0] * v.x + a[1] * v.y + a[2] * v.z;
+    y1 += a[3] * v.x + a[4] * v.y + a[5] * v.z;
+    y2 += a[6] * v.x + a[7] * v.y + a[8] * v.z;
+}
 
 __global__ void __launch_bounds__(PB) k_pcg_init(int nn, const float* __restrict__ b, const float* __restrict__ bc,
                                                  const float* __restrict__ Minv, float* __restrict__ x,
-                                                 float* __restrict__ r, float* __restrict__ z, float* __restrict__ d,
-                                                 float* __restrict__ part) {
-    __shared__ float lds[4];
-    float prz = 0.0f, pbb = 0.0f;
-    for (int i = blockIdx.x * PB + threadIdx.x; i < nn; i += gridDim.x * PB) {
+                                                 float* __restrict__ r, float4* __restrict__ z,
+                                                 float4* __restrict__ d, float* __restrict__ part) {
+    __shared__ float lds[8];
+    const int L = logical_block();
+    float acc[2] = {0.0f, 0.0f};
+    for (int i = L * PB + threadIdx.x; i < nn; i += gridDim.x * PB) {
         V3f bi = ld3v(b, i) + ld3v(bc, i);
         V3f zi = mat3v(Minv + 9 * (size_t)i, bi);
         st3v(x, i, mk(0.0f, 0.0f, 0.0f));
         st3v(r, i, bi);
-        st3v(z, i, zi);
-        st3v(d, i, zi);
-        prz += dot(bi, zi);
-        pbb += dot(bi, bi);
+        z[i] = f4(zi);
+        d[i] = f4(zi);
+        acc[0] += dot(bi, zi);
+        acc[1] += dot(bi, bi);
     }
-    float s0 = block_sum(prz, lds);
-    float s1 = block_sum(pbb, lds);
+    block_sums<2>(acc, lds);
     if (threadIdx.x == 0) {
-        part[P_RZ0 * PMAX + blockIdx.x] = s0;
-        part[P_BB * PMAX + blockIdx.x] = s1;
+        part[P_RZ0 * PMAX + L] = acc[0];
+        part[P_BB * PMAX + L] = acc[1];
     }
 }
 
 __global__ void __launch_bounds__(PB) k_pcg_setup(int nb, const float* __restrict__ part, PcgDev* sc, float tol2,
                                                   int maxit) {
-    __shared__ float lds[4];
-    float bb = reduce_partials(part, P_BB, nb, lds);
-    float rz = reduce_partials(part, P_RZ0, nb, lds);
+    __shared__ float lds[8];
+    float acc[2] = {0.0f, 0.0f};
+    for (int b = threadIdx.x; b < nb; b += PB) {
+        acc[0] += part[P_BB * PMAX + b];
+        acc[1] += part[P_RZ0 * PMAX + b];
+    }
+    block_sums<2>(acc, lds);
     if (threadIdx.x == 0) {
+        const float bb = acc[0];
         sc->iters = 0;
-        sc->itersNext = 0;
         sc->maxit = maxit;
         sc->bb = bb;
         sc->rr = bb;
-        sc->rz = rz;
+        sc->rz = acc[1];
         sc->tol2bb = tol2 * bb;
         sc->done = (bb <= tol2 * bb) || maxit <= 0;
     }
 }
 
-// q = A d over the sliced-ELL static part plus the contact CSR; partial d.q.
-__global__ void __launch_bounds__(PB) k_pcg_spmv(int nn, const int* __restrict__ sliceOff, const int* __restrict__ sliceW,
-                                                 const int* __restrict__ cols, const float* __restrict__ vals,
-                                                 const int* __restrict__ cRowPtr, const int* __restrict__ cCol,
-                                                 const float* __restrict__ cVal, const float* __restrict__ d,
-                                                 float* __restrict__ q, float* __restrict__ part, PcgDev* sc) {
-    __shared__ float lds[4];
+// Iteration `it` over the sliced-ELL static part plus the contact CSR.  For
+// it == 0, d = z and q = A d.  For it > 0 one sweep accumulates s = A z and
+// t = A d_prev; then the stop test on r.r of the current residual, beta, and
+// q = s + beta t, d = z + beta d_prev (stored to dcur for the block's rows).
+// The grid covers every row in one pass (no second round for a few blocks).
+template <bool FIRST>
+__global__ void __launch_bounds__(PB) k_pcg_spmv(int it, int nn, const int* __restrict__ sliceOff,
+                                                 const int* __restrict__ sliceW, const int* __restrict__ cols,
+                                                 const float* __restrict__ vals, const int* __restrict__ cRowPtr,
+                                                 const int* __restrict__ cCol, const float* __restrict__ cVal,
+                                                 const float4* __restrict__ z, const float4* __restrict__ dprev,
+                                                 float4* __restrict__ dcur, float* __restrict__ q,
+                                                 float* __restrict__ part, PcgDev* sc) {
+    __shared__ float lds[16];
     if (sc->done) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc->iters = sc->itersNext;
+    const int L = logical_block();
+    const int nb = gridDim.x;
+    const int p = it & 1;
+    const int slot[3] = {P_RR, P_RZ0 + p, P_RZ0 + (1 - p)};
+    Partials<3> pr;
+    if (!FIRST) pr.issue(part, slot, nb);
+    // FIRST: (s0,s1,s2) = A d.  Otherwise (s0,s1,s2) = A z and (t0,t1,t2) = A d_prev.
+    const float4* __restrict__ g0 = FIRST ? dprev : z;
+    const int i0 = L * PB + threadIdx.x;
+    const int stride = gridDim.x * PB;
     float pdq = 0.0f;
-    for (int i = blockIdx.x * PB + threadIdx.x; i < nn; i += gridDim.x * PB) {
-        const int s = i / SLICE, lane = i % SLICE;
-        const int W = sliceW[s];
-        const size_t off = (size_t)sliceOff[s];
-        const int* cp = cols + off * SLICE + lane;
-        const float* vp = vals + off * 9 * SLICE + lane;
-        float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
-        for (int k = 0; k < W; k++) {
-            const int j = cp[k * SLICE];
-            const float d0 = d[3 * j], d1 = d[3 * j + 1], d2 = d[3 * j + 2];
-            const float* a = vp + (size_t)k * 9 * SLICE;
-            y0 += a[0 * SLICE] * d0 + a[1 * SLICE] * d1 + a[2 * SLICE] * d2;
-            y1 += a[3 * SLICE] * d0 + a[4 * SLICE] * d1 + a[5 * SLICE] * d2;
-            y2 += a[6 * SLICE] * d0 + a[7 * SLICE] * d1 + a[8 * SLICE] * d2;
+    float beta = 0.0f;
+    bool first_row = true;
+    for (int i = i0; first_row || i < nn; i += stride) {
+        const bool live = i < nn;
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+        if (live) {
+            // One row per lane, slots in order; the compiler's own scheduling of this
+            // plain loop keeps the column, value and gather loads of several slots in
+            // flight (measured faster on gfx950 than hand-grouped variants).
+            const int s = i / SLICE, lane = i % SLICE;
+            const int W = sliceW[s];
+            const size_t off = (size_t)sliceOff[s];
+            const int* cp = cols + off * SLICE + lane;
+            const float* vp = vals + off * 9 * SLICE + lane;
+            for (int k = 0; k < W; k++) {
+                const int j = cp[k * SLICE];
+                const float* ap = vp + (size_t)k * 9 * SLICE;
+                float a[9];
+#pragma unroll
+                for (int c = 0; c < 9; c++) a[c] = ap[c * SLICE];
+                acc3(s0, s1, s2, a, v3(g0[j]));
+                if (!FIRST) acc3(t0, t1, t2, a, v3(dprev[j]));
+            }
         }
-        const int ce = cRowPtr[i + 1];
-        for (int k = cRowPtr[i]; k < ce; k++) {
-            const int j = cCol[k];
-            const float d0 = d[3 * j], d1 = d[3 * j + 1], d2 = d[3 * j + 2];
-            const float* a = cVal + (size_t)k * 9;
-            y0 += a[0] * d0 + a[1] * d1 + a[2] * d2;
-            y1 += a[3] * d0 + a[4] * d1 + a[5] * d2;
-            y2 += a[6] * d0 + a[7] * d1 + a[8] * d2;
+        if (live) {
+            const int ce = cRowPtr[i + 1];
+            for (int kc = cRowPtr[i]; kc < ce; kc++) {
+                const int jj = cCol[kc];
+                float a[9];
+#pragma unroll
+                for (int c = 0; c < 9; c++) a[c] = cVal[(size_t)kc * 9 + c];
+                acc3(s0, s1, s2, a, v3(g0[jj]));
+                if (!FIRST) acc3(t0, t1, t2, a, v3(dprev[jj]));
+            }
         }
-        q[3 * i] = y0;
-        q[3 * i + 1] = y1;
-        q[3 * i + 2] = y2;
-        pdq += d[3 * i] * y0 + d[3 * i + 1] * y1 + d[3 * i + 2] * y2;
+        if (!FIRST && first_row) {
+            float tot[3];
+            pr.sums(part, slot, nb, lds, tot);
+            if (tot[0] <= sc->tol2bb || it >= sc->maxit) {  // the same decision in every block
+                if (L == 0 && threadIdx.x == 0) {
+                    sc->iters = it;
+                    sc->rr = tot[0];
+                    sc->done = 1;
+                }
+                return;
+            }
+            beta = tot[1] / tot[2];
+        }
+        first_row = false;
+        if (!live) continue;
+        V3f di, qi;
+        if (FIRST) {
+            di = v3(dprev[i]);
+            qi = mk(s0, s1, s2);
+        } else {
+            const float4 zi = z[i], dp = dprev[i];
+            di = mk(zi.x + beta * dp.x, zi.y + beta * dp.y, zi.z + beta * dp.z);
+            qi = mk(s0 + beta * t0, s1 + beta * t1, s2 + beta * t2);
+            dcur[i] = f4(di);
+        }
+        st3v(q, i, qi);
+        pdq += dot(di, qi);
     }
-    float s = block_sum(pdq, lds);
-    if (threadIdx.x == 0) part[P_DQ * PMAX + blockIdx.x] = s;
+    pdq = block_sum(pdq, lds);
+    if (threadIdx.x == 0) part[P_DQ * PMAX + L] = pdq;
 }
 
-__global__ void __launch_bounds__(PB) k_pcg_update(int nn, const float* __restrict__ Minv, float* __restrict__ x,
-                                                   float* __restrict__ r, float* __restrict__ z,
-                                                   const float* __restrict__ d, const float* __restrict__ q,
-                                                   float* __restrict__ part, const PcgDev* sc) {
-    __shared__ float lds[4];
+__global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* __restrict__ Minv,
+                                                   float* __restrict__ x, float* __restrict__ r,
+                                                   float4* __restrict__ z, const float4* __restrict__ d,
+                                                   const float* __restrict__ q, float* __restrict__ part,
+                                                   const PcgDev* sc) {
+    __shared__ float lds[8];
     if (sc->done) return;
-    const int par = sc->iters & 1;
-    const float dq = reduce_partials(part, P_DQ, gridDim.x, lds);
-    const float rz = reduce_partials(part, P_RZ0 + par, gridDim.x, lds);
-    const float alpha = rz / dq;
-    float prz = 0.0f, prr = 0.0f;
-    for (int i = blockIdx.x * PB + threadIdx.x; i < nn; i += gridDim.x * PB) {
-        V3f di = ld3v(d, i), qi = ld3v(q, i);
-        V3f xi = ld3v(x, i) + alpha * di;
-        V3f ri = ld3v(r, i) - alpha * qi;
-        V3f zi = mat3v(Minv + 9 * (size_t)i, ri);
+    const int L = logical_block();
+    const int nb = gridDim.x;
+    const int p = it & 1;
+    const int slot[2] = {P_DQ, P_RZ0 + p};
+    Partials<2> pr;
+    pr.issue(part, slot, nb);
+    float acc[2] = {0.0f, 0.0f};
+    float alpha = 0.0f;
+    bool first_row = true;
+    for (int i = L * PB + threadIdx.x; first_row || i < nn; i += gridDim.x * PB) {
+        const bool live = i < nn;
+        V3f di, qi, xi, ri;
+        float m[9];
+        if (live) {  // row loads before the scalar reduction
+            di = v3(d[i]);
+            qi = ld3v(q, i);
+            xi = ld3v(x, i);
+            ri = ld3v(r, i);
+#pragma unroll
+            for (int c = 0; c < 9; c++) m[c] = Minv[9 * (size_t)i + c];
+        }
+        if (first_row) {
+            float tot[2];
+            pr.sums(part, slot, nb, lds, tot);
+            alpha = tot[1] / tot[0];
+            first_row = false;
+        }
+        if (!live) continue;
+        xi = xi + alpha * di;
+        ri = ri - alpha * qi;
+        const V3f zi = mk(m[0] * ri.x + m[1] * ri.y + m[2] * ri.z, m[3] * ri.x + m[4] * ri.y + m[5] * ri.z,
+                          m[6] * ri.x + m[7] * ri.y + m[8] * ri.z);
         st3v(x, i, xi);
         st3v(r, i, ri);
-        st3v(z, i, zi);
-        prz += dot(ri, zi);
-        prr += dot(ri, ri);
+        z[i] = f4(zi);
+        acc[0] += dot(ri, zi);
+        acc[1] += dot(ri, ri);
     }
-    float s0 = block_sum(prz, lds);
-    float s1 = block_sum(prr, lds);
+    block_sums<2>(acc, lds);
     if (threadIdx.x == 0) {
-        part[(P_RZ0 + (1 - par)) * PMAX + blockIdx.x] = s0;
-        part[P_RR * PMAX + blockIdx.x] = s1;
+        part[(P_RZ0 + (1 - p)) * PMAX + L] = acc[0];
+        part[P_RR * PMAX + L] = acc[1];
     }
-}
-
-__global__ void __launch_bounds__(PB) k_pcg_dir(int nn, const float* __restrict__ z, float* __restrict__ d,
-                                                const float* __restrict__ part, PcgDev* sc) {
-    __shared__ float lds[4];
-    if (sc->done) return;
-    const int it = sc->iters;
-    const int par = it & 1;
-    const float rzOld = reduce_partials(part, P_RZ0 + par, gridDim.x, lds);
-    const float rzNew = reduce_partials(part, P_RZ0 + (1 - par), gridDim.x, lds);
-    const float rr = reduce_partials(part, P_RR, gridDim.x, lds);
-    const bool stop = (rr <= sc->tol2bb) || (it + 1 >= sc->maxit);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {  // `iters` itself is advanced by the next SpMV
-        sc->itersNext = it + 1;
-        sc->rr = rr;
-        sc->rz = rzNew;
-        if (stop) sc->done = 1;
-    }
-    if (stop) return;
-    const float beta = rzNew / rzOld;
-    for (int i = blockIdx.x * PB + threadIdx.x; i < 3 * nn; i += gridDim.x * PB) d[i] = z[i] + beta * d[i];
 }
 
 }  // namespace
@@ -202,7 +318,7 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     ClothSys& S = C->sys[c];
     hipStream_t s = C->stream;
     const int nn = S.nn;
-    const int nb = std::max(1, std::min(PMAX, cdiv(nn, PB)));
+    const int nb = std::min(PMAX, (cdiv(nn, PB) + 7) & ~7);
     float* part = C->pcgPartial.as<float>((size_t)P_COUNT * PMAX);
     PcgDev* sc = reinterpret_cast<PcgDev*>(C->pcgScal.as<PcgScalars>(1));
     static_assert(sizeof(PcgDev) <= sizeof(PcgScalars), "scalar block");
@@ -211,27 +327,33 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     const int* cRowPtr = S.cRowPtr.ptr<int>();
     const int* cCol = S.nContact > 0 ? S.cCol.ptr<int>() : cRowPtr;
     const float* cVal = S.nContact > 0 ? S.cVal.ptr<float>() : S.vals.ptr<float>();
+    float4* z = S.z.ptr<float4>();
+    float4* dgen[2] = {S.d.ptr<float4>(), S.d.ptr<float4>() + nn};
     hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(), S.bc.ptr<float>(),
-                       S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), S.z.ptr<float>(), S.d.ptr<float>(),
-                       part);
+                       S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, dgen[0], part);
     CS_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_pcg_setup, dim3(1), dim3(PB), 0, s, nb, part, sc, tol * tol, maxit);
     CS_LAUNCH_CHECK();
     const int chunk = 16;
     int done = 0;
     PcgDev h{};
-    for (int it = 0; it < maxit + chunk && !done; it += chunk) {
-        for (int k = 0; k < chunk; k++) {
-            // the first SpMV of a chunk is sampled: the host saw `done == 0` after the previous chunk,
-            // so this launch always does the full product
-            const int tok = k == 0 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
-            hipLaunchKernelGGL(k_pcg_spmv, dim3(nb), dim3(PB), 0, s, nn, S.sliceOff.ptr<int>(), S.sliceW.ptr<int>(),
-                               S.cols.ptr<int>(), S.vals.ptr<float>(), cRowPtr, cCol, cVal, S.d.ptr<float>(),
-                               S.q.ptr<float>(), part, sc);
+    int it = 0;
+    while (!done && it <= maxit + 2 * chunk) {
+        for (int k = 0; k < chunk; k++, it++) {
+            // the second SpMV of a chunk is sampled (the first follows a host wait)
+            const int tok = k == 1 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
+            float4* dcur = it == 0 ? dgen[0] : dgen[it & 1];
+            if (it == 0)
+                hipLaunchKernelGGL(k_pcg_spmv<true>, dim3(nb), dim3(PB), 0, s, it, nn, S.sliceOff.ptr<int>(),
+                                   S.sliceW.ptr<int>(), S.cols.ptr<int>(), S.vals.ptr<float>(), cRowPtr, cCol, cVal,
+                                   z, dgen[0], dgen[0], S.q.ptr<float>(), part, sc);
+            else
+                hipLaunchKernelGGL(k_pcg_spmv<false>, dim3(nb), dim3(PB), 0, s, it, nn, S.sliceOff.ptr<int>(),
+                                   S.sliceW.ptr<int>(), S.cols.ptr<int>(), S.vals.ptr<float>(), cRowPtr, cCol, cVal,
+                                   z, dgen[1 - (it & 1)], dcur, S.q.ptr<float>(), part, sc);
             ktimer_end(C, tok);
-            hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(PB), 0, s, nn, S.Minv.ptr<float>(), S.x.ptr<float>(),
-                               S.r.ptr<float>(), S.z.ptr<float>(), S.d.ptr<float>(), S.q.ptr<float>(), part, sc);
-            hipLaunchKernelGGL(k_pcg_dir, dim3(nb), dim3(PB), 0, s, nn, S.z.ptr<float>(), S.d.ptr<float>(), part, sc);
+            hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(), S.x.ptr<float>(),
+                               S.r.ptr<float>(), z, dcur, S.q.ptr<float>(), part, sc);
         }
         CS_LAUNCH_CHECK();
         CS_HIP(hipMemcpyAsync(C->hScal, sc, sizeof(PcgDev), hipMemcpyDeviceToHost, s));
@@ -240,9 +362,10 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
         done = h.done;
         ktimer_collect(C);
     }
+    if (!done) throw StatusError(CS_E_HIP, "PCG: the device never reported its stop");
     if (relres) *relres = h.bb > 0.0f ? sqrtf(h.rr / h.bb) : 0.0f;
     if (converged) *converged = (h.rr <= h.tol2bb);
-    return h.itersNext;
+    return h.iters;
 }
 
 }  // namespace cs

@@ -190,6 +190,29 @@ cs_status cs_independent_impacts(cs_ctx* ctx, const cs_impact_rec* in, int n, in
  * launches of kernel `kernel_id` (CS_KERNEL_*) since the last reset, and optionally resets them. */
 cs_status cs_kernel_time(cs_ctx* ctx, int kernel_id, double* total_ms, int* count, int reset);
 
+/* ---- the reference's scene files (replaces Simulator::Simulator reading the configuration,
+ * src/Simulator.cu:3-135, with Cloth ctor src/Cloth.cu:3-40, Obstacle ctor src/Obstacle.cu:3-10,
+ * Mesh::load src/Mesh.cu:305-347, Material ctor src/Material.cu:3-41, Magic ctor
+ * src/Magic.cu:5-17, Wind ctor src/Wind.cu:3-7) ---- */
+typedef struct cs_scene_info {
+    float frame_time;      /* "frame_time"; dt = frame_time / frame_steps */
+    int frame_steps;       /* "frame_steps" */
+    float end_time;        /* "end_time", +inf when absent */
+    int end_frame;         /* "end_frame", INT_MAX when absent */
+    int n_cloths, n_obstacles, n_handles;
+    int n_cloth_nodes, n_cloth_faces;
+} cs_scene_info;
+
+/* Reads the configuration JSON at config_path; relative mesh and material paths are resolved
+ * against root_dir (the reference resolves them against its working directory).  base (or the
+ * defaults when NULL) supplies what the file does not hold (PCG settings, flags).  With
+ * ctx_out == NULL only parses and fills info.  Scenes with obstacle or handle motions, or the
+ * "obstacles": "<pattern>" sequence form, give CS_E_ARG; unreadable or malformed files give
+ * CS_E_ARG / CS_E_FORMAT.  The message of the last failure on this thread: cs_scene_error(). */
+cs_status cs_load_scene(const char* config_path, const char* root_dir, const cs_params* base, cs_ctx** ctx_out,
+                        cs_scene_info* info);
+const char* cs_scene_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,192 @@
+"""Scene files of the reference (configuration JSON + OBJ + material JSON) read
+through cs_load_scene.  CPU tests parse only (no device); the GPU test checks
+the placed meshes against an explicit construction of the same scene."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import clothsimulator_amd as cs
+from clothsimulator_amd import CS_MESH_CLOTH, CS_MESH_OBSTACLE, ClothError
+from clothsimulator_amd._cabi import MeshArrays, default_params, material_struct
+from clothsimulator_amd.scenes import ClothSpec, SceneSpec
+
+REF = "/root/reference"
+
+MATERIAL = """{
+    // a material file in the reference's format (comments allowed)
+    "density": 0.2,
+    "stretching": [[10, 1, 20, 5], [11, 1, 21, 5], [12, 2, 22, 6], [13, 2, 23, 6], [14, 3, 24, 7], [15, 3, 25, 7]],
+    "bending": [[1e-6, 2e-6, 3e-6, 4e-6, 5e-6], [1e-6, 2e-6, 3e-6, 4e-6, 5e-6], [1e-6, 2e-6, 3e-6, 4e-6, 5e-6]]
+}
+"""
+
+# a 3x2-quad sheet written with quads (triangulated by the loader), velocities and no 'vt'
+SHEET = """# sheet
+v 0 0 0
+v 0.5 0 0
+v 1 0 0
+v 0 0.5 0
+v 0.5 0.5 0
+v 1 0.5 0
+nv 0 0 1
+nv 0 0 1
+nv 0 0 1
+nv 0 0 1
+nv 0 0 1
+nv 0 0 1
+f 1 2 5 4
+f 2 3 6 5
+"""
+
+TRI = """v -1 -1 0
+v 1 -1 0
+v 0 1 0
+vt 0 0
+vt 1 0
+vt 0 1
+f 1/1 2/2 3/3
+"""
+
+CONFIG = """// test scene
+{
+    "frame_time": 0.04,
+    "frame_steps": 8,
+    "end_time": 1.5,
+    "cloths": [{
+        "mesh": "sheet.obj",
+        "transform": {"translate": [0.1, -0.2, 0.5], "rotate": [90, 0, 0, 1], "scale": 2},
+        "materials": {"data": "mat.json", "thicken": 2, "bending_mult": 0.5},
+        "remeshing": {"refine_angle": 0.3}
+    }],
+    "obstacles": [{"mesh": "tri.obj", "transform": {"scale": 0.5}}],
+    "handles": [{"nodes": [0, 2]}],
+    "gravity": [0, 0, -9.8],
+    "wind": {"velocity": [1, 0, 0]},
+    "magic": {"repulsion_thickness": 5e-3, "collision_stiffness": 1e6},
+    "disable": ["remeshing"]
+}
+"""
+
+
+def write_scene(d, config=CONFIG):
+    for name, text in (("mat.json", MATERIAL), ("sheet.obj", SHEET), ("tri.obj", TRI), ("scene.json", config)):
+        with open(os.path.join(d, name), "w") as f:
+            f.write(text)
+    return os.path.join(d, "scene.json")
+
+
+def test_scene_info_of_written_scene(tmp_path):
+    path = write_scene(str(tmp_path))
+    info = cs.scene_info(path, str(tmp_path))
+    assert (info.n_cloths, info.n_obstacles, info.n_handles) == (1, 1, 1)
+    assert info.n_cloth_nodes == 6 and info.n_cloth_faces == 4  # two quads -> four triangles
+    assert info.frame_steps == 8 and info.frame_time == pytest.approx(0.04)
+    assert info.end_time == pytest.approx(1.5) and info.end_frame == 2 ** 31 - 1
+
+
+@pytest.mark.parametrize("bad, status", [
+    ('"obstacles": [{"mesh": "tri.obj", "motion": 0}]', "CS_E_ARG"),
+    ('"obstacles": "obs_%d.obj"', "CS_E_ARG"),
+    ('"handles": [{"nodes": [99]}]', "CS_E_FORMAT"),
+    ('"handles": [{"nodes": [0], "motion": 1}]', "CS_E_ARG"),
+])
+def test_unsupported_or_malformed_scenes(tmp_path, bad, status):
+    cfg = CONFIG.replace('"obstacles": [{"mesh": "tri.obj", "transform": {"scale": 0.5}}]', '"obstacles": []')
+    cfg = cfg.replace('"handles": [{"nodes": [0, 2]}]', '"handles": []')
+    key = bad.split(":")[0]
+    cfg = cfg.replace('%s: []' % key, bad)
+    path = write_scene(str(tmp_path), cfg)
+    with pytest.raises(ClothError, match=status):
+        cs.scene_info(path, str(tmp_path))
+
+
+def test_json_syntax_error_is_reported(tmp_path):
+    path = write_scene(str(tmp_path), CONFIG.replace('"frame_steps": 8,', '"frame_steps": 8'))
+    with pytest.raises(ClothError, match="CS_E_FORMAT.*line"):
+        cs.scene_info(path, str(tmp_path))
+
+
+def test_missing_mesh_file(tmp_path):
+    path = write_scene(str(tmp_path), CONFIG.replace("sheet.obj", "nothere.obj"))
+    with pytest.raises(ClothError, match="CS_E_ARG.*nothere.obj"):
+        cs.scene_info(path, str(tmp_path))
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference tree not present")
+def test_reference_configurations_parse():
+    """Every configuration of the reference either loads or is refused for a named
+    unsupported feature (motions / obstacle sequences), never for a parse error."""
+    loaded = 0
+    for name in sorted(os.listdir(os.path.join(REF, "conf"))):
+        try:
+            info = cs.scene_info(os.path.join(REF, "conf", name), REF)
+            assert info.n_cloths >= 1 and info.n_cloth_faces > 0
+            loaded += 1
+        except ClothError as e:
+            assert "CS_E_ARG" in str(e) and ("motion" in str(e) or "sequences" in str(e)), (name, str(e))
+    assert loaded >= 3
+
+
+def expected_scene():
+    """The written scene built explicitly: transform = scale * rotate + translate,
+    quads fanned from the corner with the largest minimum angle (corner 0 on ties)."""
+    raw = np.array([[0, 0, 0], [0.5, 0, 0], [1, 0, 0], [0, 0.5, 0], [0.5, 0.5, 0], [1, 0.5, 0]], dtype=np.float64)
+    rot = np.array([[0, -1, 0], [1, 0, 0], [0, 0, 1]], dtype=np.float64)  # 90 degrees about z
+    x = 2.0 * raw @ rot.T + np.array([0.1, -0.2, 0.5])
+    return x, rot
+
+
+@pytest.mark.gpu
+def test_loaded_scene_places_meshes(tmp_path):
+    path = write_scene(str(tmp_path))
+    sim = cs.Simulator.from_config(path, str(tmp_path))
+    x, x0, v = sim.state(CS_MESH_CLOTH, 0)
+    xe, rot = expected_scene()
+    assert np.allclose(x, xe, atol=1e-6)
+    assert np.allclose(v, np.tile([0, 0, 1.0], (6, 1)), atol=1e-7)  # rotation about z keeps +z
+    xo, _, _ = sim.state(CS_MESH_OBSTACLE, 0)
+    assert np.allclose(xo, 0.5 * np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]]), atol=1e-7)
+    n, area, mass = sim.geometry(CS_MESH_CLOTH, 0)
+    # uv = scale * raw (x, y): rest area 2^2 * 0.5 m^2, density 0.2 * thicken 2
+    assert area.sum() == pytest.approx(4 * 0.5, rel=1e-5)
+    assert mass.sum() == pytest.approx(0.2 * 2 * 4 * 0.5, rel=1e-5)
+    s, st = sim.step()
+    assert s in (0, 4)
+
+
+def build_host_mirror(out_dir):
+    """Compile tests/cpp/host_mirror.cpp (include/clothsim.hpp over libclothsim.so) with g++."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lib_dir = os.path.join(root, "clothsimulator_amd")
+    exe = os.path.join(out_dir, "host_mirror")
+    subprocess.check_call(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(root, "include"),
+                           os.path.join(root, "tests", "cpp", "host_mirror.cpp"), "-o", exe, "-L", lib_dir,
+                           "-lclothsim", "-Wl,-rpath," + lib_dir])
+    return exe
+
+
+def test_cpp_mirror_compiles(tmp_path):
+    cs.load_library()
+    assert os.path.exists(build_host_mirror(str(tmp_path)))
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_runs_frames(tmp_path):
+    import subprocess
+    exe = build_host_mirror(str(tmp_path))
+    path = write_scene(str(tmp_path))
+    out = tmp_path / "frames"
+    out.mkdir()
+    r = subprocess.run([exe, path, str(tmp_path), str(out), "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    files = sorted(os.listdir(out))
+    assert files == ["frame00000_00.obj", "frame00001_00.obj", "frame00002_00.obj"]
+    v = [l for l in open(out / "frame00000_00.obj") if l.startswith("v ")]
+    x0 = np.array([[float(t) for t in l.split()[1:]] for l in v])
+    assert np.allclose(x0, expected_scene()[0], atol=1e-6)
+    v2 = [l for l in open(out / "frame00002_00.obj") if l.startswith("v ")]
+    x2 = np.array([[float(t) for t in l.split()[1:]] for l in v2])
+    assert np.all(np.isfinite(x2)) and not np.allclose(x2, x0)  # 16 steps of gravity moved it
