@@ -35,6 +35,7 @@ CS_FLAG_TIME_KERNELS = 4
 CS_KERNEL_PCG_SPMV = 0
 CS_KERNEL_ASSEMBLE_ROWS = 1
 CS_KERNEL_FACE_RECORDS = 2
+CS_KERNEL_EVENT_PAIR = 3
 
 
 class Params(C.Structure):

@@ -99,8 +99,8 @@ struct PcgScalars {
 struct KernelTimer {
     std::vector<hipEvent_t> events;              // pairs (start, stop)
     std::vector<std::pair<int, int>> pending;    // (kernel id, pair index) awaiting collection
-    double total[CS_KERNEL_COUNT] = {0, 0, 0};
-    int count[CS_KERNEL_COUNT] = {0, 0, 0};
+    double total[CS_KERNEL_COUNT] = {};
+    int count[CS_KERNEL_COUNT] = {};
 };
 
 }  // namespace cs

@@ -181,7 +181,11 @@ __global__ void __launch_bounds__(PB) k_pcg_spmv(int it, int nn, const int* __re
                                                  float4* __restrict__ dcur, float* __restrict__ q,
                                                  float* __restrict__ part, PcgDev* sc) {
     __shared__ float lds[16];
-    if (sc->done) return;
+    // Only the first iteration reads `done` up front (an all-zero right-hand side).  Later
+    // iterations start their sweep at once and take the stop decision from the r.r partials,
+    // which no kernel changes after convergence: a launch past the stop sweeps and returns
+    // without writing, which costs less than every launch waiting for a scalar load.
+    if (FIRST && sc->done) return;
     const int L = logical_block();
     const int nb = gridDim.x;
     const int p = it & 1;
@@ -232,7 +236,7 @@ __global__ void __launch_bounds__(PB) k_pcg_spmv(int it, int nn, const int* __re
             float tot[3];
             pr.sums(part, slot, nb, lds, tot);
             if (tot[0] <= sc->tol2bb || it >= sc->maxit) {  // the same decision in every block
-                if (L == 0 && threadIdx.x == 0) {
+                if (L == 0 && threadIdx.x == 0 && !sc->done) {
                     sc->iters = it;
                     sc->rr = tot[0];
                     sc->done = 1;
@@ -266,7 +270,6 @@ __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* 
                                                    const float* __restrict__ q, float* __restrict__ part,
                                                    const PcgDev* sc) {
     __shared__ float lds[8];
-    if (sc->done) return;
     const int L = logical_block();
     const int nb = gridDim.x;
     const int p = it & 1;
@@ -289,6 +292,7 @@ __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* 
             for (int c = 0; c < 9; c++) m[c] = Minv[9 * (size_t)i + c];
         }
         if (first_row) {
+            if (sc->done) return;  // read after the row loads are in flight; the same in every thread
             float tot[2];
             pr.sums(part, slot, nb, lds, tot);
             alpha = tot[1] / tot[0];
@@ -340,7 +344,9 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     int it = 0;
     while (!done && it <= maxit + 2 * chunk) {
         for (int k = 0; k < chunk; k++, it++) {
-            // the second SpMV of a chunk is sampled (the first follows a host wait)
+            // the second SpMV of a chunk is sampled (the first follows a host wait), after an
+            // empty pair that measures the markers' own latency
+            if (k == 1) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
             const int tok = k == 1 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
             float4* dcur = it == 0 ? dgen[0] : dgen[it & 1];
             if (it == 0)

@@ -53,8 +53,16 @@ enum {
     CS_FLAG_TIME_KERNELS = 4  /* sample HIP-event durations of the hot kernels (see cs_kernel_time) */
 };
 
-/* kernel ids for cs_kernel_time */
-enum { CS_KERNEL_PCG_SPMV = 0, CS_KERNEL_ASSEMBLE_ROWS = 1, CS_KERNEL_FACE_RECORDS = 2, CS_KERNEL_COUNT = 3 };
+/* kernel ids for cs_kernel_time.  CS_KERNEL_EVENT_PAIR times an empty event pair recorded on
+ * the stream just before each sampled SpMV: the marker latency included in every sample, which
+ * a caller subtracts to get the kernel's own duration. */
+enum {
+    CS_KERNEL_PCG_SPMV = 0,
+    CS_KERNEL_ASSEMBLE_ROWS = 1,
+    CS_KERNEL_FACE_RECORDS = 2,
+    CS_KERNEL_EVENT_PAIR = 3,
+    CS_KERNEL_COUNT = 4
+};
 
 /* Scene-level parameters.  Defaults (cs_default_params) are the reference's:
  * Magic ctor src/Magic.cu:5-17, Wind ctor src/Wind.cu:3-7, Simulator ctor
