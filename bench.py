@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--workload", default="crumple513")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precond", choices=["amg", "jacobi"], default="amg",
+                    help="PCG preconditioner: aggregation multigrid (default) or block-Jacobi")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
     a = ap.parse_args()
@@ -105,11 +107,13 @@ def main():
         return
 
     from clothsimulator_amd import CS_MESH_CLOTH, CS_MESH_OBSTACLE, Simulator, scenes
-    from clothsimulator_amd._cabi import CS_FLAG_TIME_KERNELS, CS_KERNEL_EVENT_PAIR, CS_KERNEL_PCG_SPMV
+    from clothsimulator_amd._cabi import (CS_FLAG_TIME_KERNELS, CS_KERNEL_EVENT_PAIR, CS_KERNEL_PCG_SPMV,
+                                          CS_PRECOND_AMG, CS_PRECOND_BLOCK_JACOBI)
 
     spec = scenes.by_name(a.workload)
     steps = a.steps if a.steps is not None else spec.steps
     spec.params.flags |= CS_FLAG_TIME_KERNELS
+    spec.params.precond = CS_PRECOND_AMG if a.precond == "amg" else CS_PRECOND_BLOCK_JACOBI
     sim = Simulator(spec)
     init = [sim.state(CS_MESH_CLOTH, i) for i in range(len(spec.cloths))]
     obs = [sim.state(CS_MESH_OBSTACLE, i) for i in range(len(spec.obstacles))]
@@ -169,7 +173,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": spec.name, "nodes": n_nodes, "faces": int(c0.mesh.n_faces), "dt": spec.params.dt,
-                   "precond": "block_jacobi", "rel_tol": 1e-5, "obstacles": len(spec.obstacles)},
+                   "precond": "amg_v11" if a.precond == "amg" else "block_jacobi", "rel_tol": 1e-5, "obstacles": len(spec.obstacles)},
         "cg_iters_mean": float(np.mean(iters)),
         "cg_iters_max": int(np.max(iters)),
         "cg_iters_learned": None,

@@ -22,7 +22,7 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_add_cloth", "cs_add_obstacle", "cs_add_handles", "cs_physics_step", "cs_collision_step", "cs_step",
             "cs_run", "cs_get_counts", "cs_get_state", "cs_set_state", "cs_get_geometry", "cs_material_table",
             "cs_lbvh_pairs", "cs_find_proximities", "cs_assemble", "cs_solve", "cs_find_impacts",
-            "cs_independent_impacts", "cs_kernel_time", "cs_load_scene", "cs_scene_error"]
+            "cs_independent_impacts", "cs_kernel_time", "cs_load_scene", "cs_scene_error", "cs_precondition"]
 
 
 class ClothError(RuntimeError):
@@ -69,6 +69,7 @@ def load_library(path=LIB_PATH):
     L.cs_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]
     L.cs_load_scene.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Params), C.POINTER(vp), C.POINTER(SceneInfo)]
     L.cs_scene_error.restype = C.c_char_p
+    L.cs_precondition.argtypes = [vp, C.c_int, F, F, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
     _lib = L
     return L
 
@@ -261,6 +262,19 @@ class Simulator:
         self._check(self.L.cs_find_impacts(self.h, thickness, out.ctypes.data_as(C.POINTER(ImpactRec)), cap,
                                            C.byref(n)), "cs_find_impacts")
         return out[:n.value].copy()
+
+    def precondition(self, cloth=0, r=None):
+        """Level sizes of the multigrid hierarchy and, with r, z = one V-cycle applied to r."""
+        sizes = (C.c_int * 64)()
+        nl = C.c_int()
+        if r is None:
+            self._check(self.L.cs_precondition(self.h, cloth, None, None, sizes, 64, C.byref(nl)), "cs_precondition")
+            return [sizes[i] for i in range(nl.value)], None
+        rr = np.ascontiguousarray(r, dtype=np.float32).ravel()
+        z = np.zeros_like(rr)
+        self._check(self.L.cs_precondition(self.h, cloth, fptr(rr), fptr(z), sizes, 64, C.byref(nl)),
+                    "cs_precondition")
+        return [sizes[i] for i in range(nl.value)], z.reshape(-1, 3)
 
     def kernel_time(self, kernel_id, reset=False):
         """(summed ms, sampled launches) of a hot kernel; needs CS_FLAG_TIME_KERNELS in params.flags."""

@@ -1,0 +1,688 @@
+// cs_amg.hip — aggregation multigrid preconditioner of the PCG (CS_PRECOND_AMG).
+//
+// The reference preconditions its CG with the Jacobi diagonal
+// (src/Cloth.cu:766-809); at 513^2 nodes that takes thousands of iterations,
+// because the iteration count of a one-level preconditioner grows with the
+// mesh size.  This preconditioner is one V(1,1) cycle of aggregation
+// multigrid; it changes the iteration count, not the system solved
+// (||r|| <= tol ||b|| on the same A and b).
+//
+//   aggregation graph of level 0: cloth nodes adjacent through a face;
+//   aggregation (host, once per cloth; the topology never changes): in
+//     ascending order a node whose neighbours are all unaggregated founds an
+//     aggregate with all of them; then each remaining node joins the aggregate
+//     of its first aggregated neighbour (ascending) or founds a singleton;
+//     the coarse graph links aggregates holding adjacent nodes; levels are
+//     added until one has <= 64 nodes;
+//   prolongation: piecewise constant, a 3x3 identity block per node;
+//   operators: level 0 is the full system (sliced ELL + contact blocks);
+//     level l+1 = P^T A_l P of the static system (no contact blocks), summed
+//     per step on the device from precomputed contributor lists, in a fixed
+//     order;
+//   cycle: x = w D^-1 b; t = b - A x; coarse correction e from P^T t;
+//     result = x' + w D^-1 (b - A x') with x' = x + P e; w = 0.6; D the 3x3
+//     diagonal blocks (level 0: the block-Jacobi Minv of the full system);
+//     the coarsest level is solved with its dense inverse (Gauss-Jordan in
+//     FP64, one workgroup, once per solve).
+// The cycle is symmetric (same smoother before and after, exact coarse solve),
+// so the preconditioner is SPD and CG applies.
+#include <algorithm>
+#include <cstring>
+#include <map>
+
+#include "cs_kernels.hpp"
+#include "cs_math.hpp"
+
+namespace cs {
+
+namespace {
+
+constexpr int AMG_COARSEST = 64;
+constexpr float AMG_OMEGA = 0.6f;
+constexpr int AB = 256;
+
+// ---------------------------------------------------------------- host topology
+
+void aggregate(const std::vector<int>& gp, const std::vector<int>& gi, std::vector<int>& agg, int& na) {
+    const int n = (int)gp.size() - 1;
+    agg.assign(n, -1);
+    na = 0;
+    for (int i = 0; i < n; i++) {
+        if (agg[i] >= 0) continue;
+        bool free = true;
+        for (int k = gp[i]; k < gp[i + 1]; k++) free = free && agg[gi[k]] < 0;
+        if (!free) continue;
+        agg[i] = na;
+        for (int k = gp[i]; k < gp[i + 1]; k++) agg[gi[k]] = na;
+        na++;
+    }
+    for (int i = 0; i < n; i++) {
+        if (agg[i] >= 0) continue;
+        int a = -1;
+        for (int k = gp[i]; k < gp[i + 1] && a < 0; k++) a = agg[gi[k]];
+        agg[i] = a >= 0 ? a : na++;
+    }
+}
+
+void coarse_graph(const std::vector<int>& gp, const std::vector<int>& gi, const std::vector<int>& agg, int na,
+                  std::vector<int>& cp, std::vector<int>& cidx) {
+    std::vector<std::vector<int>> adj(na);
+    for (int i = 0; i + 1 < (int)gp.size(); i++)
+        for (int k = gp[i]; k < gp[i + 1]; k++)
+            if (agg[i] != agg[gi[k]]) adj[agg[i]].push_back(agg[gi[k]]);
+    cp.assign(na + 1, 0);
+    cidx.clear();
+    for (int a = 0; a < na; a++) {
+        std::sort(adj[a].begin(), adj[a].end());
+        adj[a].erase(std::unique(adj[a].begin(), adj[a].end()), adj[a].end());
+        cidx.insert(cidx.end(), adj[a].begin(), adj[a].end());
+        cp[a + 1] = (int)cidx.size();
+    }
+}
+
+template <class V> void upload_vec(DBuf& d, const std::vector<V>& h, hipStream_t s) {
+    V* p = d.as<V>(std::max<size_t>(1, h.size()));
+    if (!h.empty()) CS_HIP(hipMemcpyAsync(p, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice, s));
+}
+
+// ---------------------------------------------------------------- numeric setup kernels
+
+__device__ __forceinline__ void inv3f(const float a[9], float o[9]) {
+    float c00 = a[4] * a[8] - a[5] * a[7];
+    float c01 = a[5] * a[6] - a[3] * a[8];
+    float c02 = a[3] * a[7] - a[4] * a[6];
+    float det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    float id = 1.0f / det;
+    o[0] = c00 * id;
+    o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    o[3] = c01 * id;
+    o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    o[6] = c02 * id;
+    o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+
+// coarse block k = sum of its contributors; FROM_ELL: contributor ids are value
+// offsets in the level-0 sliced ELL (component c at id + 64 c), else block ids of
+// the finer block CSR (component c at 9 id + c)
+template <bool FROM_ELL>
+__global__ void __launch_bounds__(AB) k_amg_galerkin(int nnzb, const int* __restrict__ cptr,
+                                                     const int* __restrict__ cidx, const float* __restrict__ fine,
+                                                     float* __restrict__ out) {
+    const int k = blockIdx.x * AB + threadIdx.x;
+    if (k >= nnzb) return;
+    float s[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) s[c] = 0.0f;
+    for (int t = cptr[k]; t < cptr[k + 1]; t++) {
+        const size_t id = (size_t)cidx[t];
+#pragma unroll
+        for (int c = 0; c < 9; c++) s[c] += FROM_ELL ? fine[id + (size_t)c * SLICE] : fine[id * 9 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 9; c++) out[(size_t)k * 9 + c] = s[c];
+}
+
+__global__ void __launch_bounds__(AB) k_amg_dinv(int n, const int* __restrict__ diag, const float* __restrict__ val,
+                                                 float* __restrict__ Dinv) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    float a[9], o[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) a[c] = val[(size_t)diag[i] * 9 + c];
+    inv3f(a, o);
+#pragma unroll
+    for (int c = 0; c < 9; c++) Dinv[(size_t)i * 9 + c] = o[c];
+}
+
+// Dense inverse of the coarsest operator (m = 3n <= 192): Gauss-Jordan with
+// partial pivoting (ties: lowest row) in FP64 in one workgroup; A and the
+// identity live in `work` (2 m^2 doubles, L2-resident).
+constexpr int CB = 1024;
+__global__ void __launch_bounds__(CB) k_amg_coarse_inverse(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                           const float* __restrict__ val, double* __restrict__ work,
+                                                           float* __restrict__ inv) {
+    __shared__ double sv[CB];
+    __shared__ int si[CB];
+    __shared__ double fac[3 * AMG_COARSEST];
+    const int m = 3 * n, tid = threadIdx.x;
+    double* A = work;
+    double* I = work + (size_t)m * m;
+    for (int e = tid; e < m * m; e += CB) {
+        A[e] = 0.0;
+        I[e] = (e / m == e % m) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += CB)
+        for (int k = rp[i]; k < rp[i + 1]; k++)
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) A[(size_t)(3 * i + a) * m + 3 * ci[k] + b] = val[(size_t)k * 9 + 3 * a + b];
+    __syncthreads();
+    for (int c = 0; c < m; c++) {
+        double best = -1.0;
+        int br = m;
+        for (int r = c + tid; r < m; r += CB) {
+            const double v = fabs(A[(size_t)r * m + c]);
+            if (v > best) { best = v; br = r; }
+        }
+        sv[tid] = best;
+        si[tid] = br;
+        __syncthreads();
+        for (int s = CB / 2; s > 0; s >>= 1) {
+            if (tid < s && (sv[tid + s] > sv[tid] || (sv[tid + s] == sv[tid] && si[tid + s] < si[tid]))) {
+                sv[tid] = sv[tid + s];
+                si[tid] = si[tid + s];
+            }
+            __syncthreads();
+        }
+        const int p = si[0];
+        if (p != c)
+            for (int k = tid; k < m; k += CB) {
+                double t = A[(size_t)p * m + k]; A[(size_t)p * m + k] = A[(size_t)c * m + k]; A[(size_t)c * m + k] = t;
+                t = I[(size_t)p * m + k]; I[(size_t)p * m + k] = I[(size_t)c * m + k]; I[(size_t)c * m + k] = t;
+            }
+        __syncthreads();
+        const double d = 1.0 / A[(size_t)c * m + c];
+        __syncthreads();
+        for (int k = tid; k < m; k += CB) {
+            A[(size_t)c * m + k] *= d;
+            I[(size_t)c * m + k] *= d;
+        }
+        __syncthreads();
+        for (int r = tid; r < m; r += CB) fac[r] = r == c ? 0.0 : A[(size_t)r * m + c];
+        __syncthreads();
+        for (int e = tid; e < m * m; e += CB) {
+            const double f = fac[e / m];
+            if (f != 0.0) {
+                const int k = e % m;
+                A[e] -= f * A[(size_t)c * m + k];
+                I[e] -= f * I[(size_t)c * m + k];
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < m * m; e += CB) inv[e] = (float)I[e];
+}
+
+// ---------------------------------------------------------------- cycle kernels
+
+__device__ __forceinline__ V3f ld3v(const float* p, int i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+__device__ __forceinline__ void st3v(float* p, int i, V3f a) { p[3 * i] = a.x; p[3 * i + 1] = a.y; p[3 * i + 2] = a.z; }
+__device__ __forceinline__ V3f mat3v(const float* m, V3f a) {
+    return mk(m[0] * a.x + m[1] * a.y + m[2] * a.z, m[3] * a.x + m[4] * a.y + m[5] * a.z,
+              m[6] * a.x + m[7] * a.y + m[8] * a.z);
+}
+
+// x = w D^-1 b
+__global__ void __launch_bounds__(AB) k_amg_pre(int n, const float* __restrict__ Dinv, const float* __restrict__ b,
+                                                float* __restrict__ x) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    st3v(x, i, AMG_OMEGA * mat3v(Dinv + 9 * (size_t)i, ld3v(b, i)));
+}
+
+// (A y)_i over level 0 (sliced ELL + contact CSR), y_j = x_j (+ e_agg(j) when PROLONG)
+template <bool PROLONG>
+__device__ __forceinline__ V3f row0(int i, const int* __restrict__ sliceOff, const int* __restrict__ sliceW,
+                                    const int* __restrict__ cols, const float* __restrict__ vals,
+                                    const int* __restrict__ cRowPtr, const int* __restrict__ cCol,
+                                    const float* __restrict__ cVal, const float* __restrict__ x,
+                                    const int* __restrict__ agg, const float* __restrict__ e) {
+    const int s = i / SLICE, lane = i % SLICE;
+    const int W = sliceW[s];
+    const size_t off = (size_t)sliceOff[s];
+    const int* cp = cols + off * SLICE + lane;
+    const float* vp = vals + off * 9 * SLICE + lane;
+    float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+    for (int k = 0; k < W; k++) {
+        const int j = cp[k * SLICE];
+        V3f xj = ld3v(x, j);
+        if (PROLONG) xj = xj + ld3v(e, agg[j]);
+        const float* a = vp + (size_t)k * 9 * SLICE;
+        y0 += a[0 * SLICE] * xj.x + a[1 * SLICE] * xj.y + a[2 * SLICE] * xj.z;
+        y1 += a[3 * SLICE] * xj.x + a[4 * SLICE] * xj.y + a[5 * SLICE] * xj.z;
+        y2 += a[6 * SLICE] * xj.x + a[7 * SLICE] * xj.y + a[8 * SLICE] * xj.z;
+    }
+    for (int k = cRowPtr[i], ce = cRowPtr[i + 1]; k < ce; k++) {
+        const int j = cCol[k];
+        V3f xj = ld3v(x, j);
+        if (PROLONG) xj = xj + ld3v(e, agg[j]);
+        const float* a = cVal + (size_t)k * 9;
+        y0 += a[0] * xj.x + a[1] * xj.y + a[2] * xj.z;
+        y1 += a[3] * xj.x + a[4] * xj.y + a[5] * xj.z;
+        y2 += a[6] * xj.x + a[7] * xj.y + a[8] * xj.z;
+    }
+    return mk(y0, y1, y2);
+}
+
+// (A y)_i over a block-CSR level
+template <bool PROLONG>
+__device__ __forceinline__ V3f rowc(int i, const int* __restrict__ rp, const int* __restrict__ ci,
+                                    const float* __restrict__ val, const float* __restrict__ x,
+                                    const int* __restrict__ agg, const float* __restrict__ e) {
+    float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+    for (int k = rp[i]; k < rp[i + 1]; k++) {
+        const int j = ci[k];
+        V3f xj = ld3v(x, j);
+        if (PROLONG) xj = xj + ld3v(e, agg[j]);
+        const float* a = val + (size_t)k * 9;
+        y0 += a[0] * xj.x + a[1] * xj.y + a[2] * xj.z;
+        y1 += a[3] * xj.x + a[4] * xj.y + a[5] * xj.z;
+        y2 += a[6] * xj.x + a[7] * xj.y + a[8] * xj.z;
+    }
+    return mk(y0, y1, y2);
+}
+
+struct Level0 {
+    const int *sliceOff, *sliceW, *cols, *cRowPtr, *cCol;
+    const float *vals, *cVal, *Minv;
+};
+
+// t = b - A x on level 0
+__global__ void __launch_bounds__(AB) k_amg_resid0(int nn, Level0 A, const float* __restrict__ b,
+                                                   const float* __restrict__ x, float* __restrict__ t) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= nn) return;
+    const V3f ax = row0<false>(i, A.sliceOff, A.sliceW, A.cols, A.vals, A.cRowPtr, A.cCol, A.cVal, x, nullptr, nullptr);
+    st3v(t, i, ld3v(b, i) - ax);
+}
+
+__global__ void __launch_bounds__(AB) k_amg_resid(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                  const float* __restrict__ val, const float* __restrict__ b,
+                                                  const float* __restrict__ x, float* __restrict__ t) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    st3v(t, i, ld3v(b, i) - rowc<false>(i, rp, ci, val, x, nullptr, nullptr));
+}
+
+// Pre-smoothing and residual of a block-CSR level in one pass: x = w D^-1 b
+// (stored for the own row, formed on the fly for the gathered ones) and
+// t = b - A x.  Coarse levels are small, so the extra gathers hit the cache and
+// one launch is saved.
+__global__ void __launch_bounds__(AB) k_amg_presid(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                   const float* __restrict__ val, const float* __restrict__ Dinv,
+                                                   const float* __restrict__ b, float* __restrict__ x,
+                                                   float* __restrict__ t) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+    for (int k = rp[i]; k < rp[i + 1]; k++) {
+        const int j = ci[k];
+        const V3f xj = AMG_OMEGA * mat3v(Dinv + 9 * (size_t)j, ld3v(b, j));
+        const float* a = val + (size_t)k * 9;
+        y0 += a[0] * xj.x + a[1] * xj.y + a[2] * xj.z;
+        y1 += a[3] * xj.x + a[4] * xj.y + a[5] * xj.z;
+        y2 += a[6] * xj.x + a[7] * xj.y + a[8] * xj.z;
+    }
+    const V3f bi = ld3v(b, i);
+    st3v(x, i, AMG_OMEGA * mat3v(Dinv + 9 * (size_t)i, bi));
+    st3v(t, i, bi - mk(y0, y1, y2));
+}
+
+// bc_I = sum of t over the members of aggregate I (ascending)
+__global__ void __launch_bounds__(AB) k_amg_restrict(int nc, const int* __restrict__ mptr, const int* __restrict__ midx,
+                                                     const float* __restrict__ t, float* __restrict__ bc) {
+    const int I = blockIdx.x * AB + threadIdx.x;
+    if (I >= nc) return;
+    V3f s = mk(0.0f, 0.0f, 0.0f);
+    for (int k = mptr[I]; k < mptr[I + 1]; k++) s = s + ld3v(t, midx[k]);
+    st3v(bc, I, s);
+}
+
+// x = inv b on the coarsest level: one workgroup of 16 waves, a wave per row,
+// the lanes of a wave over the row (coalesced) and a shuffle reduction
+constexpr int SB = 1024;
+__global__ void __launch_bounds__(SB) k_amg_coarse_solve(int m, const float* __restrict__ inv,
+                                                         const float* __restrict__ b, float* __restrict__ x) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int r = wave; r < m; r += SB / 64) {
+        float s = 0.0f;
+        for (int k = lane; k < m; k += 64) s += inv[(size_t)r * m + k] * b[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) x[r] = s;
+    }
+}
+
+// out = x' + w D^-1 (b - A x'), x' = x + e_agg, on a block-CSR level
+__global__ void __launch_bounds__(AB) k_amg_post(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                 const float* __restrict__ val, const float* __restrict__ Dinv,
+                                                 const int* __restrict__ agg, const float* __restrict__ e,
+                                                 const float* __restrict__ b, const float* __restrict__ x,
+                                                 float* __restrict__ out) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    const V3f ax = rowc<true>(i, rp, ci, val, x, agg, e);
+    const V3f xi = ld3v(x, i) + ld3v(e, agg[i]);
+    st3v(out, i, xi + AMG_OMEGA * mat3v(Dinv + 9 * (size_t)i, ld3v(b, i) - ax));
+}
+
+// AP = A_static P of level 0, per step: each AP slot sums the level-0 blocks of
+// its row whose column lies in its aggregate (slot order, fixed).
+__global__ void __launch_bounds__(AB) k_amg_ap_build(int nn, const int* __restrict__ sliceOff,
+                                                     const int* __restrict__ sliceW, const float* __restrict__ vals,
+                                                     const uint8_t* __restrict__ target,
+                                                     const int* __restrict__ apSliceOff,
+                                                     const int* __restrict__ apSliceW, float* __restrict__ apVals) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= nn) return;
+    const int s = i / SLICE, lane = i % SLICE;
+    const int W = sliceW[s], WA = apSliceW[s];
+    const size_t off = (size_t)sliceOff[s], aoff = (size_t)apSliceOff[s];
+    for (int m = 0; m < WA; m++) {
+        float a[9];
+#pragma unroll
+        for (int c = 0; c < 9; c++) a[c] = 0.0f;
+        for (int k = 0; k < W; k++) {
+            if (target[(off + k) * SLICE + lane] != m) continue;
+            const float* v = vals + (off + k) * 9 * SLICE + lane;
+#pragma unroll
+            for (int c = 0; c < 9; c++) a[c] += v[c * SLICE];
+        }
+        float* o = apVals + (aoff + m) * 9 * SLICE + lane;
+#pragma unroll
+        for (int c = 0; c < 9; c++) o[c * SLICE] = a[c];
+    }
+}
+
+// Level 0 post-smoothing from the stored residual t = r - A x of the same cycle:
+// r - A (x + P e) = t - (A P) e, with A P = AP (static part) plus the contact
+// blocks gathered through agg.  z = x + e_agg + w Minv (t - (A P) e); one partial
+// of r.z per block.
+__global__ void __launch_bounds__(AB) k_amg_post0ap(int nn, Level0 A, const int* __restrict__ apSliceOff,
+                                                    const int* __restrict__ apSliceW, const int* __restrict__ apCols,
+                                                    const float* __restrict__ apVals, const int* __restrict__ agg,
+                                                    const float* __restrict__ e, const float* __restrict__ r,
+                                                    const float* __restrict__ x, const float* __restrict__ t,
+                                                    float4* __restrict__ z, float* __restrict__ partRZ) {
+    __shared__ float lds[AB / 64];
+    float prz = 0.0f;
+    for (int i = blockIdx.x * AB + threadIdx.x; i < nn; i += gridDim.x * AB) {
+        const int s = i / SLICE, lane = i % SLICE;
+        const int WA = apSliceW[s];
+        const size_t aoff = (size_t)apSliceOff[s];
+        float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+        for (int m = 0; m < WA; m++) {
+            const V3f ej = ld3v(e, apCols[(aoff + m) * SLICE + lane]);
+            const float* a = apVals + (aoff + m) * 9 * SLICE + lane;
+            y0 += a[0 * SLICE] * ej.x + a[1 * SLICE] * ej.y + a[2 * SLICE] * ej.z;
+            y1 += a[3 * SLICE] * ej.x + a[4 * SLICE] * ej.y + a[5 * SLICE] * ej.z;
+            y2 += a[6 * SLICE] * ej.x + a[7 * SLICE] * ej.y + a[8 * SLICE] * ej.z;
+        }
+        for (int k = A.cRowPtr[i], ce = A.cRowPtr[i + 1]; k < ce; k++) {
+            const V3f ej = ld3v(e, agg[A.cCol[k]]);
+            const float* a = A.cVal + (size_t)k * 9;
+            y0 += a[0] * ej.x + a[1] * ej.y + a[2] * ej.z;
+            y1 += a[3] * ej.x + a[4] * ej.y + a[5] * ej.z;
+            y2 += a[6] * ej.x + a[7] * ej.y + a[8] * ej.z;
+        }
+        const V3f xi = ld3v(x, i) + ld3v(e, agg[i]);
+        const V3f zi = xi + AMG_OMEGA * mat3v(A.Minv + 9 * (size_t)i, ld3v(t, i) - mk(y0, y1, y2));
+        z[i] = make_float4(zi.x, zi.y, zi.z, 0.0f);
+        prz += dot(ld3v(r, i), zi);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) prz += __shfl_xor(prz, o, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = prz;
+    __syncthreads();
+    if (threadIdx.x == 0) partRZ[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+}
+
+// Level 0 post-smoothing: z = x' + w Minv (r - A x'); also one partial of r.z per block.
+__global__ void __launch_bounds__(AB) k_amg_post0(int nn, Level0 A, const int* __restrict__ agg,
+                                                  const float* __restrict__ e, const float* __restrict__ r,
+                                                  const float* __restrict__ x, float4* __restrict__ z,
+                                                  float* __restrict__ partRZ) {
+    __shared__ float lds[AB / 64];
+    float prz = 0.0f;
+    for (int i = blockIdx.x * AB + threadIdx.x; i < nn; i += gridDim.x * AB) {
+        const V3f ax = row0<true>(i, A.sliceOff, A.sliceW, A.cols, A.vals, A.cRowPtr, A.cCol, A.cVal, x, agg, e);
+        const V3f xi = ld3v(x, i) + ld3v(e, agg[i]);
+        const V3f ri = ld3v(r, i);
+        const V3f zi = xi + AMG_OMEGA * mat3v(A.Minv + 9 * (size_t)i, ri - ax);
+        z[i] = make_float4(zi.x, zi.y, zi.z, 0.0f);
+        prz += dot(ri, zi);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) prz += __shfl_xor(prz, o, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = prz;
+    __syncthreads();
+    if (threadIdx.x == 0) partRZ[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+}
+
+}  // namespace
+
+// Host topology of cloth c's hierarchy and its device index arrays (once; the mesh never changes).
+static void amg_build(cs_ctx* C, int c) {
+    ClothSys& S = C->sys[c];
+    AmgSys& G = S.amg;
+    const MeshInfo& M = C->meshes[S.mesh];
+    hipStream_t st = C->stream;
+    const int nn = M.nn;
+    // level-0 graph: nodes adjacent through a face
+    std::vector<std::vector<int>> adj(nn);
+    for (int f = M.f0; f < M.f0 + M.nf; f++)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                if (a != b) adj[C->hfn[3 * f + a] - M.n0].push_back(C->hfn[3 * f + b] - M.n0);
+    std::vector<int> gp(nn + 1, 0), gi;
+    for (int i = 0; i < nn; i++) {
+        std::sort(adj[i].begin(), adj[i].end());
+        adj[i].erase(std::unique(adj[i].begin(), adj[i].end()), adj[i].end());
+        gi.insert(gi.end(), adj[i].begin(), adj[i].end());
+        gp[i + 1] = (int)gi.size();
+    }
+    std::vector<int> sizes{nn};
+    std::vector<std::vector<int>> aggs;
+    int n = nn;
+    while (n > AMG_COARSEST) {
+        std::vector<int> agg, cp, cidx;
+        int na = 0;
+        aggregate(gp, gi, agg, na);
+        if (na >= n) break;
+        coarse_graph(gp, gi, agg, na, cp, cidx);
+        aggs.push_back(agg);
+        sizes.push_back(na);
+        gp.swap(cp);
+        gi.swap(cidx);
+        n = na;
+    }
+    G.built = true;
+    G.nl = (int)sizes.size();
+    if (sizes.back() > AMG_COARSEST || G.nl == 1) {  // too small, or cannot be coarsened: not usable
+        G.nl = 0;
+        return;
+    }
+    G.L.clear();
+    G.L.resize(G.nl);
+    for (int l = 0; l < G.nl; l++) G.L[l].n = sizes[l];
+    // level-0 static pattern with the value offset of each block in the sliced ELL
+    std::vector<int> frp = S.hRowPtr, fci = S.hCol, fid(S.hCol.size());
+    for (int r = 0; r < nn; r++) {
+        const int s = r / SLICE, lane = r % SLICE;
+        for (int k = S.hRowPtr[r]; k < S.hRowPtr[r + 1]; k++)
+            fid[k] = (int)(((size_t)S.hSliceOff[s] + (k - S.hRowPtr[r])) * 9 * SLICE + lane);
+    }
+    for (int l = 0; l + 1 < G.nl; l++) {
+        const std::vector<int>& agg = aggs[l];
+        const int nf = sizes[l], nc = sizes[l + 1];
+        // restriction lists and the prolongation map
+        std::vector<int> mptr(nc + 1, 0), midx(nf);
+        for (int i = 0; i < nf; i++) mptr[agg[i] + 1]++;
+        for (int a = 0; a < nc; a++) mptr[a + 1] += mptr[a];
+        std::vector<int> fill(mptr.begin(), mptr.end() - 1);
+        for (int i = 0; i < nf; i++) midx[fill[agg[i]]++] = i;
+        upload_vec(G.L[l].agg, agg, st);
+        upload_vec(G.L[l].mptr, mptr, st);
+        upload_vec(G.L[l].midx, midx, st);
+        // Galerkin pattern of level l+1 with contributor lists (fine order: row, then slot)
+        std::map<uint64_t, std::vector<int>> blocks;
+        for (int i = 0; i < nf; i++)
+            for (int k = frp[i]; k < frp[i + 1]; k++)
+                blocks[(uint64_t)agg[i] * (uint64_t)nc + (uint64_t)agg[fci[k]]].push_back(fid[k]);
+        std::vector<int> crp(nc + 1, 0), cci, cptr(1, 0), cidx, diag(nc, -1), cid;
+        for (const auto& kv : blocks) {
+            const int I = (int)(kv.first / nc), J = (int)(kv.first % nc);
+            crp[I + 1]++;
+            if (I == J) diag[I] = (int)cci.size();
+            cid.push_back((int)cci.size());
+            cci.push_back(J);
+            cidx.insert(cidx.end(), kv.second.begin(), kv.second.end());
+            cptr.push_back((int)cidx.size());
+        }
+        for (int a = 0; a < nc; a++) crp[a + 1] += crp[a];
+        AmgLevel& Lc = G.L[l + 1];
+        Lc.nnzb = (int)cci.size();
+        upload_vec(Lc.rp, crp, st);
+        upload_vec(Lc.ci, cci, st);
+        upload_vec(Lc.diag, diag, st);
+        upload_vec(Lc.cptr, cptr, st);
+        upload_vec(Lc.cidx, cidx, st);
+        Lc.val.as<float>(9 * (size_t)std::max(1, Lc.nnzb));
+        Lc.Dinv.as<float>(9 * (size_t)nc);
+        frp.swap(crp);
+        fci.swap(cci);
+        fid.swap(cid);
+    }
+    // level-0 A P: per row, one slot per distinct aggregate of its columns (ascending)
+    {
+        const std::vector<int>& agg = aggs[0];
+        const int ns = S.nslices;
+        std::vector<uint8_t> target((size_t)S.totalSlots * SLICE, 255);
+        std::vector<int> apOff(ns + 1, 0), apW(ns, 0);
+        std::vector<std::vector<int>> rowAggs(nn);
+        for (int r = 0; r < nn; r++) {
+            std::vector<int>& v = rowAggs[r];
+            for (int k = S.hRowPtr[r]; k < S.hRowPtr[r + 1]; k++) v.push_back(agg[S.hCol[k]]);
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            if (v.size() > 254) {  // a row this long has no uint8 slot index: not usable
+                G.nl = 0;
+                return;
+            }
+            apW[r / SLICE] = std::max(apW[r / SLICE], (int)v.size());
+        }
+        for (int s = 0; s < ns; s++) apOff[s + 1] = apOff[s] + apW[s];
+        std::vector<int> apCols((size_t)std::max(1, apOff[ns]) * SLICE, 0);
+        for (int r = 0; r < nn; r++) {
+            const int s = r / SLICE, lane = r % SLICE;
+            const std::vector<int>& v = rowAggs[r];
+            for (int m = 0; m < apW[s]; m++)
+                apCols[((size_t)apOff[s] + m) * SLICE + lane] = m < (int)v.size() ? v[m] : agg[r];
+            for (int k = S.hRowPtr[r]; k < S.hRowPtr[r + 1]; k++) {
+                const int m = (int)(std::lower_bound(v.begin(), v.end(), agg[S.hCol[k]]) - v.begin());
+                target[((size_t)S.hSliceOff[s] + (k - S.hRowPtr[r])) * SLICE + lane] = (uint8_t)m;
+            }
+        }
+        upload_vec(G.apTarget, target, st);
+        upload_vec(G.apSliceOff, apOff, st);
+        upload_vec(G.apSliceW, apW, st);
+        upload_vec(G.apCols, apCols, st);
+        G.apVals.as<float>((size_t)std::max(1, apOff[ns]) * SLICE * 9);
+        fill_f32(G.apVals.ptr<float>(), 0.0f, (long)std::max(1, apOff[ns]) * SLICE * 9, st);
+    }
+    for (int l = 0; l < G.nl; l++) {
+        AmgLevel& L = G.L[l];
+        if (l > 0) L.b.as<float>(3 * (size_t)L.n);
+        L.x.as<float>(3 * (size_t)L.n);
+        L.t.as<float>(3 * (size_t)L.n);
+    }
+    const int m = 3 * sizes.back();
+    G.work.as<double>(2 * (size_t)m * m);
+    G.inv.as<float>((size_t)m * m);
+}
+
+// Number of levels of cloth c's hierarchy (sizes[l] filled up to cap); 0 when the
+// cloth cannot use the preconditioner (<= 64 nodes, or no coarsening possible).
+int amg_levels(cs_ctx* C, int c, int* sizes, int cap) {
+    AmgSys& G = C->sys[c].amg;
+    if (!G.built) amg_build(C, c);
+    for (int l = 0; l < G.nl && l < cap; l++) sizes[l] = G.L[l].n;
+    return G.nl;
+}
+
+// Per-solve numeric setup: Galerkin operators of the static system, their
+// diagonal inverses, and the coarsest dense inverse.
+void amg_prepare(cs_ctx* C, int c) {
+    ClothSys& S = C->sys[c];
+    AmgSys& G = S.amg;
+    if (!G.built) amg_build(C, c);
+    hipStream_t s = C->stream;
+    for (int l = 1; l < G.nl; l++) {
+        AmgLevel& L = G.L[l];
+        if (l == 1)
+            hipLaunchKernelGGL(k_amg_galerkin<true>, dim3(cdiv(L.nnzb, AB)), dim3(AB), 0, s, L.nnzb, L.cptr.ptr<int>(),
+                               L.cidx.ptr<int>(), S.vals.ptr<float>(), L.val.ptr<float>());
+        else
+            hipLaunchKernelGGL(k_amg_galerkin<false>, dim3(cdiv(L.nnzb, AB)), dim3(AB), 0, s, L.nnzb,
+                               L.cptr.ptr<int>(), L.cidx.ptr<int>(), G.L[l - 1].val.ptr<float>(), L.val.ptr<float>());
+        CS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_amg_dinv, dim3(cdiv(L.n, AB)), dim3(AB), 0, s, L.n, L.diag.ptr<int>(), L.val.ptr<float>(),
+                           L.Dinv.ptr<float>());
+        CS_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_amg_ap_build, dim3(cdiv(S.nn, AB)), dim3(AB), 0, s, S.nn, S.sliceOff.ptr<int>(),
+                       S.sliceW.ptr<int>(), S.vals.ptr<float>(), G.apTarget.ptr<uint8_t>(), G.apSliceOff.ptr<int>(),
+                       G.apSliceW.ptr<int>(), G.apVals.ptr<float>());
+    CS_LAUNCH_CHECK();
+    const AmgLevel& Lc = G.L[G.nl - 1];
+    hipLaunchKernelGGL(k_amg_coarse_inverse, dim3(1), dim3(CB), 0, s, Lc.n, Lc.rp.ptr<int>(), Lc.ci.ptr<int>(),
+                       Lc.val.ptr<float>(), G.work.ptr<double>(), G.inv.ptr<float>());
+    CS_LAUNCH_CHECK();
+}
+
+// z = M^-1 r (one V(1,1) cycle); partRZ[b] = block b's share of r.z for the
+// nb blocks of the PCG grid (nb * 256 threads, rows in grid-stride order).
+void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int nb) {
+    ClothSys& S = C->sys[c];
+    AmgSys& G = S.amg;
+    hipStream_t s = C->stream;
+    Level0 A0;
+    A0.sliceOff = S.sliceOff.ptr<int>();
+    A0.sliceW = S.sliceW.ptr<int>();
+    A0.cols = S.cols.ptr<int>();
+    A0.vals = S.vals.ptr<float>();
+    A0.cRowPtr = S.cRowPtr.ptr<int>();
+    A0.cCol = S.nContact > 0 ? S.cCol.ptr<int>() : S.cRowPtr.ptr<int>();
+    A0.cVal = S.nContact > 0 ? S.cVal.ptr<float>() : S.vals.ptr<float>();
+    A0.Minv = S.Minv.ptr<float>();
+    // down: smooth, residual, restrict
+    for (int l = 0; l + 1 < G.nl; l++) {
+        AmgLevel& L = G.L[l];
+        const float* b = l == 0 ? r : L.b.ptr<float>();
+        const float* Dinv = l == 0 ? S.Minv.ptr<float>() : L.Dinv.ptr<float>();
+        const int g = cdiv(L.n, AB);
+        if (l == 0) {
+            hipLaunchKernelGGL(k_amg_pre, dim3(g), dim3(AB), 0, s, L.n, Dinv, b, L.x.ptr<float>());
+            hipLaunchKernelGGL(k_amg_resid0, dim3(g), dim3(AB), 0, s, L.n, A0, b, L.x.ptr<float>(), L.t.ptr<float>());
+        } else {
+            hipLaunchKernelGGL(k_amg_presid, dim3(g), dim3(AB), 0, s, L.n, L.rp.ptr<int>(), L.ci.ptr<int>(),
+                               L.val.ptr<float>(), Dinv, b, L.x.ptr<float>(), L.t.ptr<float>());
+        }
+        AmgLevel& N = G.L[l + 1];
+        hipLaunchKernelGGL(k_amg_restrict, dim3(cdiv(N.n, AB)), dim3(AB), 0, s, N.n, L.mptr.ptr<int>(),
+                           L.midx.ptr<int>(), L.t.ptr<float>(), N.b.ptr<float>());
+    }
+    AmgLevel& Lc = G.L[G.nl - 1];
+    hipLaunchKernelGGL(k_amg_coarse_solve, dim3(1), dim3(SB), 0, s, 3 * Lc.n, G.inv.ptr<float>(), Lc.b.ptr<float>(),
+                       Lc.x.ptr<float>());
+    // up: prolongate and post-smooth; the result of level l is in L[l].t (coarsest: x)
+    for (int l = G.nl - 2; l >= 0; l--) {
+        AmgLevel& L = G.L[l];
+        const AmgLevel& N = G.L[l + 1];
+        const float* e = (l + 1 == G.nl - 1) ? N.x.ptr<float>() : N.t.ptr<float>();
+        if (l == 0)
+            hipLaunchKernelGGL(k_amg_post0ap, dim3(nb), dim3(AB), 0, s, L.n, A0, G.apSliceOff.ptr<int>(),
+                               G.apSliceW.ptr<int>(), G.apCols.ptr<int>(), G.apVals.ptr<float>(), L.agg.ptr<int>(), e,
+                               r, L.x.ptr<float>(), L.t.ptr<float>(), z, partRZ);
+        else
+            hipLaunchKernelGGL(k_amg_post, dim3(cdiv(L.n, AB)), dim3(AB), 0, s, L.n, L.rp.ptr<int>(), L.ci.ptr<int>(),
+                               L.val.ptr<float>(), L.Dinv.ptr<float>(), L.agg.ptr<int>(), e, L.b.ptr<float>(),
+                               L.x.ptr<float>(), L.t.ptr<float>());
+    }
+    CS_LAUNCH_CHECK();
+}
+
+}  // namespace cs

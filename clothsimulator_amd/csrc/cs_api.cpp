@@ -70,7 +70,7 @@ template <class F> cs_status guard(cs_ctx* C, F&& f) {
 bool valid_params(const cs_params* p) {
     return p && p->dt > 0.0f && p->pcg_rel_tol > 0.0f && p->pcg_rel_tol < 1.0f && p->pcg_max_iters >= 0 &&
            p->max_collision_iters >= 0 && p->repulsion_thickness > 0.0f && p->collision_thickness >= 0.0f &&
-           (p->precond == CS_PRECOND_BLOCK_JACOBI || p->precond == CS_PRECOND_LEARNED);
+           (p->precond == CS_PRECOND_BLOCK_JACOBI || p->precond == CS_PRECOND_LEARNED || p->precond == CS_PRECOND_AMG);
 }
 
 const MeshInfo* mesh_of(cs_ctx* C, int kind, int id) {
@@ -566,6 +566,36 @@ cs_status cs_solve(cs_ctx* C, int cloth_id, float* dv, cs_stats* st) {
             st->pcg_converged = conv;
         }
         return conv ? (int)CS_OK : (int)CS_E_NOT_CONVERGED;
+    });
+}
+
+cs_status cs_precondition(cs_ctx* C, int cloth_id, const float* r, float* z, int* level_sizes, int cap,
+                          int* n_levels) {
+    if (!C || !n_levels || cap < 0 || (cap > 0 && !level_sizes) || (r && !z)) return CS_E_ARG;
+    if (cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
+    if (C->P.precond != CS_PRECOND_AMG) return CS_E_ARG;
+    if (r && !C->finalized) return CS_E_STATE;  // z needs an assembled system (cs_assemble)
+    return guard(C, [&]() {
+        host_finalize(C);
+        std::vector<int> sizes(64);
+        const int nl = amg_levels(C, cloth_id, sizes.data(), 64);
+        *n_levels = nl;
+        for (int l = 0; l < nl && l < cap; l++) level_sizes[l] = sizes[l];
+        if (!r) return (int)CS_OK;
+        if (nl <= 1) return (int)CS_E_STATE;
+        ClothSys& S = C->sys[cloth_id];
+        hipStream_t s = C->stream;
+        const size_t n3 = 3 * (size_t)S.nn;
+        CS_HIP(hipMemcpyAsync(S.r.ptr<float>(), r, sizeof(float) * n3, hipMemcpyHostToDevice, s));
+        amg_prepare(C, cloth_id);
+        const int nb = std::max(1, cdiv(S.nn, 256));
+        amg_apply(C, cloth_id, S.r.ptr<float>(), S.z.ptr<float4>(), C->ws.a.as<float>(nb), nb);
+        std::vector<float> z4(4 * (size_t)S.nn);
+        CS_HIP(hipMemcpyAsync(z4.data(), S.z.ptr<float4>(), sizeof(float) * z4.size(), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < S.nn; i++)
+            for (int k = 0; k < 3; k++) z[3 * i + k] = z4[4 * (size_t)i + k];
+        return (int)CS_OK;
     });
 }
 

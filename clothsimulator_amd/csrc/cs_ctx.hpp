@@ -62,6 +62,30 @@ struct Bvh {
     DBuf scene;            // float4 x2 (p, d)
 };
 
+// One level of the aggregation multigrid preconditioner.  Level 0 is the
+// cloth's own system (sliced ELL + contacts, block-Jacobi Minv); levels >= 1
+// hold block-CSR Galerkin operators of the static system.
+struct AmgLevel {
+    int n = 0, nnzb = 0;
+    DBuf rp, ci, diag;   // (l >= 1) block CSR, sorted columns; diagonal block index per row
+    DBuf val, Dinv;      // (l >= 1) float 9 per block / per node
+    DBuf cptr, cidx;     // (l >= 1) Galerkin contributors of each block: value ids in the finer level
+    DBuf agg;            // (not coarsest) node -> node of the next level
+    DBuf mptr, midx;     // (not coarsest) node of the next level -> its nodes here (ascending)
+    DBuf b, x, t;        // float 3n: right-hand side, smoothed iterate, residual / result
+};
+
+struct AmgSys {
+    bool built = false;
+    int nl = 0;
+    std::vector<AmgLevel> L;
+    DBuf work;  // double 2 m^2: Gauss-Jordan of the coarsest operator (m = 3 n of the last level)
+    DBuf inv;   // float m^2: its inverse
+    // A_static P of level 0 in sliced ELL (row i, one slot per distinct aggregate of
+    // its columns): apTarget[level-0 slot] = the AP slot it sums into (255: none)
+    DBuf apTarget, apSliceOff, apSliceW, apCols, apVals;
+};
+
 struct ClothSys {
     int mesh;
     int nn, nslices, totalSlots, maxWidth;
@@ -88,6 +112,7 @@ struct ClothSys {
     std::vector<int> hRowPtr, hCol;  // exact (unpadded) static pattern
     std::vector<int> hSliceOff, hSliceW, hRowLen;
     int nHandles = 0;
+    AmgSys amg;
 };
 
 struct PcgScalars {

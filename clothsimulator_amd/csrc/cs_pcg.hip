@@ -122,26 +122,30 @@ __device__ __forceinline__ void acc3(float& y0, float& y1, float& y2, const floa
     y2 += a[6] * v.x + a[7] * v.y + a[8] * v.z;
 }
 
+// x = 0, r = b; JACOBI: z = Minv r and the r.z partials (the multigrid path
+// computes z and r.z in its own last kernel).
+template <bool JACOBI>
 __global__ void __launch_bounds__(PB) k_pcg_init(int nn, const float* __restrict__ b, const float* __restrict__ bc,
                                                  const float* __restrict__ Minv, float* __restrict__ x,
                                                  float* __restrict__ r, float4* __restrict__ z,
-                                                 float4* __restrict__ d, float* __restrict__ part) {
+                                                 float* __restrict__ part) {
     __shared__ float lds[8];
     const int L = logical_block();
     float acc[2] = {0.0f, 0.0f};
     for (int i = L * PB + threadIdx.x; i < nn; i += gridDim.x * PB) {
         V3f bi = ld3v(b, i) + ld3v(bc, i);
-        V3f zi = mat3v(Minv + 9 * (size_t)i, bi);
         st3v(x, i, mk(0.0f, 0.0f, 0.0f));
         st3v(r, i, bi);
-        z[i] = f4(zi);
-        d[i] = f4(zi);
-        acc[0] += dot(bi, zi);
+        if (JACOBI) {
+            V3f zi = mat3v(Minv + 9 * (size_t)i, bi);
+            z[i] = f4(zi);
+            acc[0] += dot(bi, zi);
+        }
         acc[1] += dot(bi, bi);
     }
     block_sums<2>(acc, lds);
     if (threadIdx.x == 0) {
-        part[P_RZ0 * PMAX + L] = acc[0];
+        if (JACOBI) part[P_RZ0 * PMAX + L] = acc[0];
         part[P_BB * PMAX + L] = acc[1];
     }
 }
@@ -168,7 +172,7 @@ __global__ void __launch_bounds__(PB) k_pcg_setup(int nb, const float* __restric
 }
 
 // Iteration `it` over the sliced-ELL static part plus the contact CSR.  For
-// it == 0, d = z and q = A d.  For it > 0 one sweep accumulates s = A z and
+// it == 0, d = z (stored to dcur) and q = A d.  For it > 0 one sweep accumulates s = A z and
 // t = A d_prev; then the stop test on r.r of the current residual, beta, and
 // q = s + beta t, d = z + beta d_prev (stored to dcur for the block's rows).
 // The grid covers every row in one pass (no second round for a few blocks).
@@ -192,8 +196,8 @@ __global__ void __launch_bounds__(PB) k_pcg_spmv(int it, int nn, const int* __re
     const int slot[3] = {P_RR, P_RZ0 + p, P_RZ0 + (1 - p)};
     Partials<3> pr;
     if (!FIRST) pr.issue(part, slot, nb);
-    // FIRST: (s0,s1,s2) = A d.  Otherwise (s0,s1,s2) = A z and (t0,t1,t2) = A d_prev.
-    const float4* __restrict__ g0 = FIRST ? dprev : z;
+    // (s0,s1,s2) = A z; for it > 0 also (t0,t1,t2) = A d_prev.
+    const float4* __restrict__ g0 = z;
     const int i0 = L * PB + threadIdx.x;
     const int stride = gridDim.x * PB;
     float pdq = 0.0f;
@@ -249,8 +253,9 @@ __global__ void __launch_bounds__(PB) k_pcg_spmv(int it, int nn, const int* __re
         if (!live) continue;
         V3f di, qi;
         if (FIRST) {
-            di = v3(dprev[i]);
+            di = v3(z[i]);
             qi = mk(s0, s1, s2);
+            dcur[i] = f4(di);
         } else {
             const float4 zi = z[i], dp = dprev[i];
             di = mk(zi.x + beta * dp.x, zi.y + beta * dp.y, zi.z + beta * dp.z);
@@ -264,6 +269,9 @@ __global__ void __launch_bounds__(PB) k_pcg_spmv(int it, int nn, const int* __re
     if (threadIdx.x == 0) part[P_DQ * PMAX + L] = pdq;
 }
 
+// alpha, x += alpha d, r -= alpha q, partial r.r; JACOBI: also z = Minv r and
+// the r.z partials (the multigrid cycle that follows computes them otherwise).
+template <bool JACOBI>
 __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* __restrict__ Minv,
                                                    float* __restrict__ x, float* __restrict__ r,
                                                    float4* __restrict__ z, const float4* __restrict__ d,
@@ -288,8 +296,9 @@ __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* 
             qi = ld3v(q, i);
             xi = ld3v(x, i);
             ri = ld3v(r, i);
+            if (JACOBI)
 #pragma unroll
-            for (int c = 0; c < 9; c++) m[c] = Minv[9 * (size_t)i + c];
+                for (int c = 0; c < 9; c++) m[c] = Minv[9 * (size_t)i + c];
         }
         if (first_row) {
             if (sc->done) return;  // read after the row loads are in flight; the same in every thread
@@ -301,17 +310,19 @@ __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* 
         if (!live) continue;
         xi = xi + alpha * di;
         ri = ri - alpha * qi;
-        const V3f zi = mk(m[0] * ri.x + m[1] * ri.y + m[2] * ri.z, m[3] * ri.x + m[4] * ri.y + m[5] * ri.z,
-                          m[6] * ri.x + m[7] * ri.y + m[8] * ri.z);
         st3v(x, i, xi);
         st3v(r, i, ri);
-        z[i] = f4(zi);
-        acc[0] += dot(ri, zi);
+        if (JACOBI) {
+            const V3f zi = mk(m[0] * ri.x + m[1] * ri.y + m[2] * ri.z, m[3] * ri.x + m[4] * ri.y + m[5] * ri.z,
+                              m[6] * ri.x + m[7] * ri.y + m[8] * ri.z);
+            z[i] = f4(zi);
+            acc[0] += dot(ri, zi);
+        }
         acc[1] += dot(ri, ri);
     }
     block_sums<2>(acc, lds);
     if (threadIdx.x == 0) {
-        part[(P_RZ0 + (1 - p)) * PMAX + L] = acc[0];
+        if (JACOBI) part[(P_RZ0 + (1 - p)) * PMAX + L] = acc[0];
         part[P_RR * PMAX + L] = acc[1];
     }
 }
@@ -333,12 +344,23 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     const float* cVal = S.nContact > 0 ? S.cVal.ptr<float>() : S.vals.ptr<float>();
     float4* z = S.z.ptr<float4>();
     float4* dgen[2] = {S.d.ptr<float4>(), S.d.ptr<float4>() + nn};
-    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(), S.bc.ptr<float>(),
-                       S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, dgen[0], part);
-    CS_LAUNCH_CHECK();
+    int levels[16];
+    const bool amg = C->P.precond == CS_PRECOND_AMG && amg_levels(C, c, levels, 16) > 1;
+    if (amg) {
+        amg_prepare(C, c);
+        hipLaunchKernelGGL(k_pcg_init<false>, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(), S.bc.ptr<float>(),
+                           S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, part);
+        CS_LAUNCH_CHECK();
+        amg_apply(C, c, S.r.ptr<float>(), z, part + P_RZ0 * PMAX, nb);
+    } else {
+        hipLaunchKernelGGL(k_pcg_init<true>, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(), S.bc.ptr<float>(),
+                           S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, part);
+        CS_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_pcg_setup, dim3(1), dim3(PB), 0, s, nb, part, sc, tol * tol, maxit);
     CS_LAUNCH_CHECK();
-    const int chunk = 16;
+    // iterations per host poll: a multigrid iteration costs several times a Jacobi one
+    const int chunk = amg ? 4 : 16;
     int done = 0;
     PcgDev h{};
     int it = 0;
@@ -358,8 +380,14 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
                                    S.sliceW.ptr<int>(), S.cols.ptr<int>(), S.vals.ptr<float>(), cRowPtr, cCol, cVal,
                                    z, dgen[1 - (it & 1)], dcur, S.q.ptr<float>(), part, sc);
             ktimer_end(C, tok);
-            hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(), S.x.ptr<float>(),
-                               S.r.ptr<float>(), z, dcur, S.q.ptr<float>(), part, sc);
+            if (amg) {
+                hipLaunchKernelGGL(k_pcg_update<false>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
+                                   S.x.ptr<float>(), S.r.ptr<float>(), z, dcur, S.q.ptr<float>(), part, sc);
+                amg_apply(C, c, S.r.ptr<float>(), z, part + (P_RZ0 + (1 - (it & 1))) * PMAX, nb);
+            } else {
+                hipLaunchKernelGGL(k_pcg_update<true>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
+                                   S.x.ptr<float>(), S.r.ptr<float>(), z, dcur, S.q.ptr<float>(), part, sc);
+            }
         }
         CS_LAUNCH_CHECK();
         CS_HIP(hipMemcpyAsync(C->hScal, sc, sizeof(PcgDev), hipMemcpyDeviceToHost, s));

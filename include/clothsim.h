@@ -43,7 +43,11 @@ typedef enum cs_status {
 
 typedef struct cs_ctx cs_ctx; /* opaque context: device buffers + one HIP stream */
 
-enum { CS_PRECOND_BLOCK_JACOBI = 0, CS_PRECOND_LEARNED = 1 };
+/* PCG preconditioners.  BLOCK_JACOBI: inverse 3x3 diagonal blocks (the reference uses the
+ * scalar diagonal, src/Cloth.cu:766-809).  AMG: one V(1,1) cycle of aggregation multigrid
+ * (no counterpart in the reference; it changes the iteration count, not the system solved).
+ * LEARNED: reserved, rejected by this build. */
+enum { CS_PRECOND_BLOCK_JACOBI = 0, CS_PRECOND_LEARNED = 1, CS_PRECOND_AMG = 2 };
 enum { CS_MESH_CLOTH = 0, CS_MESH_OBSTACLE = 1 };
 
 /* cs_params.flags */
@@ -184,6 +188,12 @@ cs_status cs_assemble(cs_ctx* ctx, int cloth_id, int* row_ptr, int* col_idx, flo
                       int* nnzb_out, float* b);
 /* Solve the last assembled system of one cloth (src/Cloth.cu:766-809): dv (3n). */
 cs_status cs_solve(cs_ctx* ctx, int cloth_id, float* dv, cs_stats* stats);
+/* The CS_PRECOND_AMG preconditioner of the last assembled system (no counterpart in the
+ * reference, whose PCG uses the Jacobi diagonal): level sizes of the hierarchy
+ * (n_levels = 0 when the cloth cannot use it) and, when r != NULL, z = one V(1,1) cycle
+ * applied to r (3n each).  CS_E_ARG unless params.precond == CS_PRECOND_AMG. */
+cs_status cs_precondition(cs_ctx* ctx, int cloth_id, const float* r, float* z, int* level_sizes, int cap,
+                          int* n_levels);
 /* Simulator::findImpacts (src/Simulator.cu:383-392), deduplicated and sorted by (t, type, nodes). */
 cs_status cs_find_impacts(cs_ctx* ctx, float thickness, cs_impact_rec* out, int cap, int* n_out);
 /* Simulator::independentImpacts GPU (src/Simulator.cu:422-461) on caller-provided impacts

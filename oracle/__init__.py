@@ -61,6 +61,8 @@ def lib():
                                              C.POINTER(C.c_int)]
         L.or_solve_cubic.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.or_cubic_roots_unit.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.or_amg.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                             C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -269,6 +271,20 @@ class OracleScene:
         _check(L.or_assemble(self.h, cloth, iptr(rp), iptr(ci), dptr(vals), nnzb.value, C.byref(nnzb), dptr(b)),
                "assemble")
         return rp, ci, vals, b
+
+    def amg_levels(self, cloth=0):
+        sizes = (C.c_int * 64)()
+        nl = C.c_int()
+        _check(self.L.or_amg(self.h, cloth, sizes, 64, C.byref(nl), None, None), "amg")
+        return [sizes[i] for i in range(nl.value)]
+
+    def amg_apply(self, r, cloth=0):
+        """One V(1,1) cycle of the aggregation multigrid preconditioner applied to r."""
+        rr = np.ascontiguousarray(r, dtype=np.float64).ravel()
+        z = np.zeros_like(rr)
+        nl = C.c_int()
+        _check(self.L.or_amg(self.h, cloth, None, 0, C.byref(nl), dptr(rr), dptr(z)), "amg")
+        return z.reshape(-1, 3)
 
     def impacts(self, thickness, cap=1 << 20):
         out = np.zeros(cap, dtype=IMPACT_DTYPE)

@@ -8,6 +8,7 @@
 #include <functional>
 #include <set>
 
+#include "or_amg.hpp"
 #include "or_physics.hpp"
 
 namespace orc {
@@ -763,7 +764,28 @@ template <class T> int physics_step(Scene<T>& S, int directSolver, cs_stats* st)
             if (!direct_solve(rp, ci, val, sys.b, dv)) return CS_E_NOT_CONVERGED;
         } else {
             T rr = T(0);
-            int it = pcg_block_jacobi(rp, ci, val, sys.b, T(S.P.pcg_rel_tol), S.P.pcg_max_iters, dv, &rr);
+            int it;
+            if (S.P.precond == CS_PRECOND_AMG) {
+                System<T> ss;  // the static system (no proximity blocks) for the coarse operators
+                ss.n = sys.n;
+                ss.b.assign(sys.b.size(), T(0));
+                assemble(S, c, vector<Prox<T>>(), ss);
+                vector<int> srp, sci;
+                vector<T> sval;
+                ss.to_bsr(srp, sci, sval);
+                AmgTopo topo;
+                amg_cloth_topology(S, c, topo);
+                Amg M;
+                amg_setup(topo, rp, ci, val, srp, sci, sval, M);
+                it = pcg_with(rp, ci, val, sys.b, T(S.P.pcg_rel_tol), S.P.pcg_max_iters, dv, &rr,
+                              [&](const vector<T>& in, vector<T>& out) {
+                                  vector<double> bi(in.begin(), in.end()), xo;
+                                  amg_vcycle(M, 0, bi, xo);
+                                  out.assign(xo.begin(), xo.end());
+                              });
+            } else {
+                it = pcg_block_jacobi(rp, ci, val, sys.b, T(S.P.pcg_rel_tol), S.P.pcg_max_iters, dv, &rr);
+            }
             if (st) { st->pcg_iters += it; st->pcg_rel_residual = std::max(st->pcg_rel_residual, (float)rr); }
             if (it >= S.P.pcg_max_iters) { status = CS_E_NOT_CONVERGED; if (st) st->pcg_converged = 0; }
         }

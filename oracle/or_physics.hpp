@@ -326,26 +326,14 @@ template <class T> T chunked_dot(const vector<T>& p, const vector<T>& q) {
     return s;
 }
 
-template <class T> int pcg_block_jacobi(const vector<int>& rowPtr, const vector<int>& col, const vector<T>& val,
-                                        const vector<T>& b, T tol, int maxIters, vector<T>& x, T* relres) {
+// PCG with the preconditioner applyM(in, out) (out = M^-1 in).
+template <class T, class ApplyM>
+int pcg_with(const vector<int>& rowPtr, const vector<int>& col, const vector<T>& val, const vector<T>& b, T tol,
+             int maxIters, vector<T>& x, T* relres, ApplyM&& applyM) {
     int n = (int)rowPtr.size() - 1;
     int m = 3 * n;
-    vector<T> Minv(9 * n), r(b), z(m), d(m), q(m);
+    vector<T> r(b), z(m), d(m), q(m);
     x.assign(m, T(0));
-    for (int i = 0; i < n; i++) {
-        const T* dg = nullptr;
-        for (int k = rowPtr[i]; k < rowPtr[i + 1]; k++) if (col[k] == i) dg = &val[9 * k];
-        inverse3(dg, &Minv[9 * i]);
-    }
-    auto applyM = [&](const vector<T>& in, vector<T>& out) {
-#pragma omp parallel for schedule(static)
-        for (int i = 0; i < n; i++)
-            for (int a = 0; a < 3; a++) {
-                T s = T(0);
-                for (int c = 0; c < 3; c++) s += Minv[9 * i + 3 * a + c] * in[3 * i + c];
-                out[3 * i + a] = s;
-            }
-    };
     T bnorm = std::sqrt(chunked_dot(b, b));
     applyM(r, z);
     d = z;
@@ -374,6 +362,34 @@ template <class T> int pcg_block_jacobi(const vector<int>& rowPtr, const vector<
     }
     if (relres) *relres = bnorm > T(0) ? std::sqrt(chunked_dot(r, r)) / bnorm : T(0);
     return it;
+}
+
+// 3x3 block-Jacobi preconditioner of a BSR matrix
+template <class T> void block_jacobi(const vector<int>& rowPtr, const vector<int>& col, const vector<T>& val,
+                                     vector<T>& Minv) {
+    int n = (int)rowPtr.size() - 1;
+    Minv.assign(9 * (size_t)n, T(0));
+    for (int i = 0; i < n; i++) {
+        const T* dg = nullptr;
+        for (int k = rowPtr[i]; k < rowPtr[i + 1]; k++) if (col[k] == i) dg = &val[9 * k];
+        inverse3(dg, &Minv[9 * i]);
+    }
+}
+
+template <class T> int pcg_block_jacobi(const vector<int>& rowPtr, const vector<int>& col, const vector<T>& val,
+                                        const vector<T>& b, T tol, int maxIters, vector<T>& x, T* relres) {
+    const int n = (int)rowPtr.size() - 1;
+    vector<T> Minv;
+    block_jacobi(rowPtr, col, val, Minv);
+    return pcg_with(rowPtr, col, val, b, tol, maxIters, x, relres, [&](const vector<T>& in, vector<T>& out) {
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; i++)
+            for (int a = 0; a < 3; a++) {
+                T s = T(0);
+                for (int c = 0; c < 3; c++) s += Minv[9 * i + 3 * a + c] * in[3 * i + c];
+                out[3 * i + a] = s;
+            }
+    });
 }
 
 // Direct sparse LL^T (the reference CPU solve is Eigen SimplicialLLT,

@@ -336,6 +336,37 @@ int or_assemble(void* h, int cloth, int* rowPtr, int* col, double* vals, int cap
     });
 }
 
+// The aggregation multigrid preconditioner of cloth `cloth` at the current state:
+// level sizes (sizes != NULL: up to cap entries, *nlevels set) and, when r != NULL,
+// z = one V(1,1) cycle applied to r (3 * n_nodes doubles).
+int or_amg(void* h, int cloth, int* sizes, int cap, int* nlevels, const double* r, double* z) {
+    return dispatch(h, [&](auto& S) {
+        using T = std::decay_t<decltype(S.x[0][0])>;
+        if (cloth < 0 || cloth >= S.nCloths) return (int)CS_E_ARG;
+        AmgTopo topo;
+        amg_cloth_topology(S, cloth, topo);
+        *nlevels = (int)topo.sizes.size();
+        if (sizes)
+            for (int l = 0; l < *nlevels && l < cap; l++) sizes[l] = topo.sizes[l];
+        if (!r) return (int)CS_OK;
+        vector<Prox<T>> prox;
+        if (!(S.P.flags & CS_FLAG_NO_PROXIMITY)) prox = find_proximities(S);
+        System<T> sys, ss;
+        assemble(S, cloth, prox, sys);
+        assemble(S, cloth, vector<Prox<T>>(), ss);
+        vector<int> rp, ci, srp, sci;
+        vector<T> val, sval;
+        sys.to_bsr(rp, ci, val);
+        ss.to_bsr(srp, sci, sval);
+        Amg M;
+        amg_setup(topo, rp, ci, val, srp, sci, sval, M);
+        vector<double> bi(r, r + 3 * (size_t)topo.sizes[0]), xo;
+        amg_vcycle(M, 0, bi, xo);
+        for (size_t i = 0; i < xo.size(); i++) z[i] = xo[i];
+        return (int)CS_OK;
+    });
+}
+
 // Block-Jacobi PCG (direct == 0) or the banded LL^T (direct == 1) on a caller BSR system.
 int or_solve_bsr(int n, const int* rowPtr, const int* col, const double* vals, const double* b, int direct,
                  int fp64, double tol, int maxIters, double* x, int* iters, double* relres) {
