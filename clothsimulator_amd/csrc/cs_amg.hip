@@ -38,7 +38,6 @@ namespace cs {
 namespace {
 
 constexpr int AMG_COARSEST = 64;
-constexpr float AMG_OMEGA = 0.6f;
 constexpr int AB = 256;
 
 // ---------------------------------------------------------------- host topology
@@ -359,6 +358,76 @@ __global__ void __launch_bounds__(AB) k_amg_post(int n, const int* __restrict__ 
     st3v(out, i, xi + AMG_OMEGA * mat3v(Dinv + 9 * (size_t)i, ld3v(b, i) - ax));
 }
 
+// ---- the small levels in one workgroup ----
+// Levels of at most SMALL nodes cost one launch latency per kernel but little
+// work; one workgroup runs their whole sub-cycle (pre-smooth + residual,
+// restriction, coarsest solve, prolongation + post-smooth) with barriers
+// between the phases, in the same order and arithmetic as the per-level kernels.
+constexpr int SMALL = 512;  // measured: one workgroup is slower than per-level launches above this
+constexpr int MAXSMALL = 8;
+struct SmallLevel {
+    int n;
+    const int *rp, *ci, *agg, *mptr, *midx;
+    const float *val, *Dinv;
+    float *b, *x, *t;
+};
+struct SmallCycle {
+    int ns;  // levels in the sub-cycle; the last is the coarsest
+    int m;   // 3 n of the coarsest
+    const float* inv;
+    SmallLevel L[MAXSMALL];
+};
+
+__global__ void __launch_bounds__(SB) k_amg_small_cycle(SmallCycle Q) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int l = 0; l + 1 < Q.ns; l++) {
+        const SmallLevel& L = Q.L[l];
+        for (int i = tid; i < L.n; i += SB) {
+            float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+            for (int k = L.rp[i]; k < L.rp[i + 1]; k++) {
+                const int j = L.ci[k];
+                const V3f xj = AMG_OMEGA * mat3v(L.Dinv + 9 * (size_t)j, ld3v(L.b, j));
+                const float* a = L.val + (size_t)k * 9;
+                y0 += a[0] * xj.x + a[1] * xj.y + a[2] * xj.z;
+                y1 += a[3] * xj.x + a[4] * xj.y + a[5] * xj.z;
+                y2 += a[6] * xj.x + a[7] * xj.y + a[8] * xj.z;
+            }
+            const V3f bi = ld3v(L.b, i);
+            st3v(L.x, i, AMG_OMEGA * mat3v(L.Dinv + 9 * (size_t)i, bi));
+            st3v(L.t, i, bi - mk(y0, y1, y2));
+        }
+        __syncthreads();
+        const SmallLevel& N = Q.L[l + 1];
+        for (int I = tid; I < N.n; I += SB) {
+            V3f s = mk(0.0f, 0.0f, 0.0f);
+            for (int k = L.mptr[I]; k < L.mptr[I + 1]; k++) s = s + ld3v(L.t, L.midx[k]);
+            st3v(N.b, I, s);
+        }
+        __syncthreads();
+    }
+    {
+        const SmallLevel& C = Q.L[Q.ns - 1];
+        for (int r = wave; r < Q.m; r += SB / 64) {
+            float s = 0.0f;
+            for (int k = lane; k < Q.m; k += 64) s += Q.inv[(size_t)r * Q.m + k] * C.b[k];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+            if (lane == 0) C.x[r] = s;
+        }
+        __syncthreads();
+    }
+    for (int l = Q.ns - 2; l >= 0; l--) {
+        const SmallLevel& L = Q.L[l];
+        const float* e = (l + 1 == Q.ns - 1) ? Q.L[l + 1].x : Q.L[l + 1].t;
+        for (int i = tid; i < L.n; i += SB) {
+            const V3f ax = rowc<true>(i, L.rp, L.ci, L.val, L.x, L.agg, e);
+            const V3f xi = ld3v(L.x, i) + ld3v(e, L.agg[i]);
+            st3v(L.t, i, xi + AMG_OMEGA * mat3v(L.Dinv + 9 * (size_t)i, ld3v(L.b, i) - ax));
+        }
+        __syncthreads();
+    }
+}
+
 // AP = A_static P of level 0, per step: each AP slot sums the level-0 blocks of
 // its row whose column lies in its aggregate (slot order, fixed).
 __global__ void __launch_bounds__(AB) k_amg_ap_build(int nn, const int* __restrict__ sliceOff,
@@ -635,7 +704,9 @@ void amg_prepare(cs_ctx* C, int c) {
 
 // z = M^-1 r (one V(1,1) cycle); partRZ[b] = block b's share of r.z for the
 // nb blocks of the PCG grid (nb * 256 threads, rows in grid-stride order).
-void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int nb) {
+float* amg_level0_x(cs_ctx* C, int c) { return C->sys[c].amg.L[0].x.ptr<float>(); }
+
+void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int nb, bool presmoothed) {
     ClothSys& S = C->sys[c];
     AmgSys& G = S.amg;
     hipStream_t s = C->stream;
@@ -648,14 +719,19 @@ void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int n
     A0.cCol = S.nContact > 0 ? S.cCol.ptr<int>() : S.cRowPtr.ptr<int>();
     A0.cVal = S.nContact > 0 ? S.cVal.ptr<float>() : S.vals.ptr<float>();
     A0.Minv = S.Minv.ptr<float>();
+    // levels l0.. (the first with at most SMALL nodes, l0 >= 1) run in one workgroup
+    int l0 = G.nl - 1;
+    for (int l = 1; l < G.nl; l++)
+        if (G.L[l].n <= SMALL) { l0 = l; break; }
+    l0 = std::max(l0, G.nl - MAXSMALL);
     // down: smooth, residual, restrict
-    for (int l = 0; l + 1 < G.nl; l++) {
+    for (int l = 0; l < l0; l++) {
         AmgLevel& L = G.L[l];
         const float* b = l == 0 ? r : L.b.ptr<float>();
         const float* Dinv = l == 0 ? S.Minv.ptr<float>() : L.Dinv.ptr<float>();
         const int g = cdiv(L.n, AB);
         if (l == 0) {
-            hipLaunchKernelGGL(k_amg_pre, dim3(g), dim3(AB), 0, s, L.n, Dinv, b, L.x.ptr<float>());
+            if (!presmoothed) hipLaunchKernelGGL(k_amg_pre, dim3(g), dim3(AB), 0, s, L.n, Dinv, b, L.x.ptr<float>());
             hipLaunchKernelGGL(k_amg_resid0, dim3(g), dim3(AB), 0, s, L.n, A0, b, L.x.ptr<float>(), L.t.ptr<float>());
         } else {
             hipLaunchKernelGGL(k_amg_presid, dim3(g), dim3(AB), 0, s, L.n, L.rp.ptr<int>(), L.ci.ptr<int>(),
@@ -665,11 +741,29 @@ void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int n
         hipLaunchKernelGGL(k_amg_restrict, dim3(cdiv(N.n, AB)), dim3(AB), 0, s, N.n, L.mptr.ptr<int>(),
                            L.midx.ptr<int>(), L.t.ptr<float>(), N.b.ptr<float>());
     }
-    AmgLevel& Lc = G.L[G.nl - 1];
-    hipLaunchKernelGGL(k_amg_coarse_solve, dim3(1), dim3(SB), 0, s, 3 * Lc.n, G.inv.ptr<float>(), Lc.b.ptr<float>(),
-                       Lc.x.ptr<float>());
+    SmallCycle Q;
+    Q.ns = G.nl - l0;
+    Q.m = 3 * G.L[G.nl - 1].n;
+    Q.inv = G.inv.ptr<float>();
+    for (int k = 0; k < Q.ns; k++) {
+        AmgLevel& L = G.L[l0 + k];
+        const bool last = l0 + k == G.nl - 1;
+        SmallLevel& q = Q.L[k];
+        q.n = L.n;
+        q.rp = L.rp.ptr<int>();
+        q.ci = L.ci.ptr<int>();
+        q.val = L.val.ptr<float>();
+        q.Dinv = L.Dinv.ptr<float>();
+        q.agg = last ? nullptr : L.agg.ptr<int>();
+        q.mptr = last ? nullptr : L.mptr.ptr<int>();
+        q.midx = last ? nullptr : L.midx.ptr<int>();
+        q.b = L.b.ptr<float>();
+        q.x = L.x.ptr<float>();
+        q.t = L.t.ptr<float>();
+    }
+    hipLaunchKernelGGL(k_amg_small_cycle, dim3(1), dim3(SB), 0, s, Q);
     // up: prolongate and post-smooth; the result of level l is in L[l].t (coarsest: x)
-    for (int l = G.nl - 2; l >= 0; l--) {
+    for (int l = l0 - 1; l >= 0; l--) {
         AmgLevel& L = G.L[l];
         const AmgLevel& N = G.L[l + 1];
         const float* e = (l + 1 == G.nl - 1) ? N.x.ptr<float>() : N.t.ptr<float>();

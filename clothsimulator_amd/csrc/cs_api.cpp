@@ -589,7 +589,7 @@ cs_status cs_precondition(cs_ctx* C, int cloth_id, const float* r, float* z, int
         CS_HIP(hipMemcpyAsync(S.r.ptr<float>(), r, sizeof(float) * n3, hipMemcpyHostToDevice, s));
         amg_prepare(C, cloth_id);
         const int nb = std::max(1, cdiv(S.nn, 256));
-        amg_apply(C, cloth_id, S.r.ptr<float>(), S.z.ptr<float4>(), C->ws.a.as<float>(nb), nb);
+        amg_apply(C, cloth_id, S.r.ptr<float>(), S.z.ptr<float4>(), C->ws.a.as<float>(nb), nb, false);
         std::vector<float> z4(4 * (size_t)S.nn);
         CS_HIP(hipMemcpyAsync(z4.data(), S.z.ptr<float4>(), sizeof(float) * z4.size(), hipMemcpyDeviceToHost, s));
         CS_HIP(hipStreamSynchronize(s));

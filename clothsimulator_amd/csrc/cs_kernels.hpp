@@ -30,9 +30,12 @@ void launch_block_jacobi(cs_ctx* C, int c);
 // PCG (cs_pcg.hip)
 int pcg_solve(cs_ctx* C, int c, float* relres, int* converged);
 // aggregation multigrid preconditioner (cs_amg.hip)
+constexpr float AMG_OMEGA = 0.6f;  // damping of the block-Jacobi smoother
 int amg_levels(cs_ctx* C, int c, int* sizes, int cap);  // 0: not usable for this cloth
 void amg_prepare(cs_ctx* C, int c);
-void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int nb);
+// z = M^-1 r.  presmoothed: the caller already wrote x = w Minv r into amg_level0_x.
+void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int nb, bool presmoothed);
+float* amg_level0_x(cs_ctx* C, int c);
 
 // LBVH (cs_lbvh.hip)
 void bvh_build(cs_ctx* C, int mesh, bool ccd);

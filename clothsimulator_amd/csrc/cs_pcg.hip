@@ -122,13 +122,14 @@ __device__ __forceinline__ void acc3(float& y0, float& y1, float& y2, const floa
     y2 += a[6] * v.x + a[7] * v.y + a[8] * v.z;
 }
 
-// x = 0, r = b; JACOBI: z = Minv r and the r.z partials (the multigrid path
-// computes z and r.z in its own last kernel).
+// x = 0, r = b; JACOBI: z = Minv r and the r.z partials.  Otherwise the multigrid
+// cycle that follows computes z and r.z, and this kernel does its first step, the
+// damped Jacobi pre-smoothing xs = w Minv r of level 0.
 template <bool JACOBI>
 __global__ void __launch_bounds__(PB) k_pcg_init(int nn, const float* __restrict__ b, const float* __restrict__ bc,
                                                  const float* __restrict__ Minv, float* __restrict__ x,
                                                  float* __restrict__ r, float4* __restrict__ z,
-                                                 float* __restrict__ part) {
+                                                 float* __restrict__ xs, float* __restrict__ part) {
     __shared__ float lds[8];
     const int L = logical_block();
     float acc[2] = {0.0f, 0.0f};
@@ -136,10 +137,12 @@ __global__ void __launch_bounds__(PB) k_pcg_init(int nn, const float* __restrict
         V3f bi = ld3v(b, i) + ld3v(bc, i);
         st3v(x, i, mk(0.0f, 0.0f, 0.0f));
         st3v(r, i, bi);
+        const V3f zi = mat3v(Minv + 9 * (size_t)i, bi);
         if (JACOBI) {
-            V3f zi = mat3v(Minv + 9 * (size_t)i, bi);
             z[i] = f4(zi);
             acc[0] += dot(bi, zi);
+        } else {
+            st3v(xs, i, AMG_OMEGA * zi);
         }
         acc[1] += dot(bi, bi);
     }
@@ -270,13 +273,14 @@ __global__ void __launch_bounds__(PB) k_pcg_spmv(int it, int nn, const int* __re
 }
 
 // alpha, x += alpha d, r -= alpha q, partial r.r; JACOBI: also z = Minv r and
-// the r.z partials (the multigrid cycle that follows computes them otherwise).
+// the r.z partials.  Otherwise the multigrid cycle that follows computes them, and
+// this kernel writes its level-0 pre-smoothing xs = w Minv r.
 template <bool JACOBI>
 __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* __restrict__ Minv,
                                                    float* __restrict__ x, float* __restrict__ r,
-                                                   float4* __restrict__ z, const float4* __restrict__ d,
-                                                   const float* __restrict__ q, float* __restrict__ part,
-                                                   const PcgDev* sc) {
+                                                   float4* __restrict__ z, float* __restrict__ xs,
+                                                   const float4* __restrict__ d, const float* __restrict__ q,
+                                                   float* __restrict__ part, const PcgDev* sc) {
     __shared__ float lds[8];
     const int L = logical_block();
     const int nb = gridDim.x;
@@ -296,9 +300,8 @@ __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* 
             qi = ld3v(q, i);
             xi = ld3v(x, i);
             ri = ld3v(r, i);
-            if (JACOBI)
 #pragma unroll
-                for (int c = 0; c < 9; c++) m[c] = Minv[9 * (size_t)i + c];
+            for (int c = 0; c < 9; c++) m[c] = Minv[9 * (size_t)i + c];
         }
         if (first_row) {
             if (sc->done) return;  // read after the row loads are in flight; the same in every thread
@@ -312,11 +315,13 @@ __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* 
         ri = ri - alpha * qi;
         st3v(x, i, xi);
         st3v(r, i, ri);
+        const V3f zi = mk(m[0] * ri.x + m[1] * ri.y + m[2] * ri.z, m[3] * ri.x + m[4] * ri.y + m[5] * ri.z,
+                          m[6] * ri.x + m[7] * ri.y + m[8] * ri.z);
         if (JACOBI) {
-            const V3f zi = mk(m[0] * ri.x + m[1] * ri.y + m[2] * ri.z, m[3] * ri.x + m[4] * ri.y + m[5] * ri.z,
-                              m[6] * ri.x + m[7] * ri.y + m[8] * ri.z);
             z[i] = f4(zi);
             acc[0] += dot(ri, zi);
+        } else {
+            st3v(xs, i, AMG_OMEGA * zi);
         }
         acc[1] += dot(ri, ri);
     }
@@ -346,15 +351,17 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     float4* dgen[2] = {S.d.ptr<float4>(), S.d.ptr<float4>() + nn};
     int levels[16];
     const bool amg = C->P.precond == CS_PRECOND_AMG && amg_levels(C, c, levels, 16) > 1;
+    float* xs = nullptr;  // level-0 pre-smoothing iterate of the multigrid cycle
     if (amg) {
         amg_prepare(C, c);
+        xs = amg_level0_x(C, c);
         hipLaunchKernelGGL(k_pcg_init<false>, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(), S.bc.ptr<float>(),
-                           S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, part);
+                           S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, xs, part);
         CS_LAUNCH_CHECK();
-        amg_apply(C, c, S.r.ptr<float>(), z, part + P_RZ0 * PMAX, nb);
+        amg_apply(C, c, S.r.ptr<float>(), z, part + P_RZ0 * PMAX, nb, true);
     } else {
         hipLaunchKernelGGL(k_pcg_init<true>, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(), S.bc.ptr<float>(),
-                           S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, part);
+                           S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, xs, part);
         CS_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_pcg_setup, dim3(1), dim3(PB), 0, s, nb, part, sc, tol * tol, maxit);
@@ -382,11 +389,11 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
             ktimer_end(C, tok);
             if (amg) {
                 hipLaunchKernelGGL(k_pcg_update<false>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
-                                   S.x.ptr<float>(), S.r.ptr<float>(), z, dcur, S.q.ptr<float>(), part, sc);
-                amg_apply(C, c, S.r.ptr<float>(), z, part + (P_RZ0 + (1 - (it & 1))) * PMAX, nb);
+                                   S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
+                amg_apply(C, c, S.r.ptr<float>(), z, part + (P_RZ0 + (1 - (it & 1))) * PMAX, nb, true);
             } else {
                 hipLaunchKernelGGL(k_pcg_update<true>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
-                                   S.x.ptr<float>(), S.r.ptr<float>(), z, dcur, S.q.ptr<float>(), part, sc);
+                                   S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
             }
         }
         CS_LAUNCH_CHECK();
