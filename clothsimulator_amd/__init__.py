@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from ._cabi import (CS_E_NOT_CONVERGED, CS_MESH_CLOTH, CS_MESH_OBSTACLE, CS_OK, IMPACT_DTYPE, LIB_PATH, PROX_DTYPE,
+from ._cabi import (CS_E_NOT_CONVERGED, CS_MESH_CLOTH, CS_PRECOND_LEARNED, CS_MESH_OBSTACLE, CS_OK, IMPACT_DTYPE, LIB_PATH, PROX_DTYPE,
                     STATUS_NAMES, ImpactRec, MaterialIn, MeshArrays, MeshIn, Params, ProximityRec, SceneInfo,
                     Stats, default_params, fptr, iptr, material_struct)
 
@@ -22,7 +22,12 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_add_cloth", "cs_add_obstacle", "cs_add_handles", "cs_physics_step", "cs_collision_step", "cs_step",
             "cs_run", "cs_get_counts", "cs_get_state", "cs_set_state", "cs_get_geometry", "cs_material_table",
             "cs_lbvh_pairs", "cs_find_proximities", "cs_assemble", "cs_solve", "cs_find_impacts",
-            "cs_independent_impacts", "cs_kernel_time", "cs_load_scene", "cs_scene_error", "cs_precondition"]
+            "cs_independent_impacts", "cs_kernel_time", "cs_load_scene", "cs_scene_error", "cs_precondition",
+            "cs_lpc_weight_count", "cs_set_learned_weights", "cs_get_learned_weights", "cs_lpc_factor",
+            "cs_lpc_train_step"]
+
+# trained weights of the learned preconditioner shipped with the package (float32, .npy)
+LPC_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lpc_weights.npy")
 
 
 class ClothError(RuntimeError):
@@ -70,6 +75,11 @@ def load_library(path=LIB_PATH):
     L.cs_load_scene.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Params), C.POINTER(vp), C.POINTER(SceneInfo)]
     L.cs_scene_error.restype = C.c_char_p
     L.cs_precondition.argtypes = [vp, C.c_int, F, F, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
+    L.cs_lpc_weight_count.restype = C.c_int
+    L.cs_set_learned_weights.argtypes = [vp, F, C.c_int]
+    L.cs_get_learned_weights.argtypes = [vp, F, C.c_int]
+    L.cs_lpc_factor.argtypes = [vp, C.c_int, F, F, C.c_int, C.POINTER(C.c_int)]
+    L.cs_lpc_train_step.argtypes = [vp, C.c_int, C.c_float, F, F]
     _lib = L
     return L
 
@@ -82,6 +92,12 @@ def scene_info(config_path, root_dir=None):
     if st != CS_OK:
         raise ClothError(st, "cs_load_scene", L.cs_scene_error().decode())
     return info
+
+
+def default_learned_weights():
+    """The packaged trained weights of the learned preconditioner (float32 vector)."""
+    w = np.load(LPC_WEIGHTS, allow_pickle=False)
+    return np.ascontiguousarray(w, dtype=np.float32)
 
 
 def material_table(mat):
@@ -115,6 +131,8 @@ class Simulator:
                     self.add_handles(cid, c.handles)
             for o in spec.obstacles:
                 self.add_obstacle(o)
+        if self.params.precond == CS_PRECOND_LEARNED and os.path.exists(LPC_WEIGHTS):
+            self.set_learned_weights(default_learned_weights())
 
     @classmethod
     def from_config(cls, config_path, root_dir=None, params=None):
@@ -275,6 +293,36 @@ class Simulator:
         self._check(self.L.cs_precondition(self.h, cloth, fptr(rr), fptr(z), sizes, 64, C.byref(nl)),
                     "cs_precondition")
         return [sizes[i] for i in range(nl.value)], z.reshape(-1, 3)
+
+    def set_learned_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32).ravel()
+        self._check(self.L.cs_set_learned_weights(self.h, fptr(w), len(w)), "cs_set_learned_weights")
+
+    def learned_weights(self):
+        w = np.zeros(self.L.cs_lpc_weight_count(), dtype=np.float32)
+        self._check(self.L.cs_get_learned_weights(self.h, fptr(w), len(w)), "cs_get_learned_weights")
+        return w
+
+    def lpc_factor(self, cloth=0):
+        """G of the last assembled system: diagonal blocks (n, 3, 3) and lower blocks (n_lower, 3, 3)
+        in static-pattern order (rows ascending, columns ascending, j < i)."""
+        nl = C.c_int()
+        self._check(self.L.cs_lpc_factor(self.h, cloth, None, None, 0, C.byref(nl)), "cs_lpc_factor")
+        n = self.counts(CS_MESH_CLOTH, cloth)[0]
+        diag = np.zeros((n, 3, 3), dtype=np.float32)
+        lower = np.zeros((max(nl.value, 1), 3, 3), dtype=np.float32)
+        self._check(self.L.cs_lpc_factor(self.h, cloth, fptr(diag), fptr(lower), nl.value, C.byref(nl)),
+                    "cs_lpc_factor")
+        return diag, lower[:nl.value]
+
+    def lpc_train_step(self, cloth=0, lr=0.0, grad=False):
+        """Kaporin loss of the network's factor on the last assembled system, its gradient
+        (when grad) and one Adam step when lr > 0."""
+        loss = C.c_float()
+        g = np.zeros(self.L.cs_lpc_weight_count(), dtype=np.float32) if grad else None
+        self._check(self.L.cs_lpc_train_step(self.h, cloth, lr, C.byref(loss), fptr(g) if grad else None),
+                    "cs_lpc_train_step")
+        return (loss.value, g) if grad else loss.value
 
     def kernel_time(self, kernel_id, reset=False):
         """(summed ms, sampled launches) of a hot kernel; needs CS_FLAG_TIME_KERNELS in params.flags."""

@@ -6,6 +6,7 @@
 // collisionStep (src/Simulator.cu:571-684).
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 
 #include "cs_kernels.hpp"
@@ -227,7 +228,6 @@ cs_status cs_create(const cs_params* params, cs_ctx** ctx_out) {
     if (!ctx_out) return CS_E_ARG;
     *ctx_out = nullptr;
     if (!valid_params(params)) return CS_E_ARG;
-    if (params->precond == CS_PRECOND_LEARNED) return CS_E_ARG;  // no trained preconditioner bundled in this build
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CS_E_NO_DEVICE;
     cs_ctx* C = new cs_ctx();
@@ -268,7 +268,6 @@ const char* cs_last_error(const cs_ctx* C) { return C ? C->err.c_str() : "null c
 
 cs_status cs_set_params(cs_ctx* C, const cs_params* p) {
     if (!C || !valid_params(p)) return CS_E_ARG;
-    if (p->precond == CS_PRECOND_LEARNED) return CS_E_ARG;
     C->P = *p;
     return CS_OK;
 }
@@ -573,8 +572,27 @@ cs_status cs_precondition(cs_ctx* C, int cloth_id, const float* r, float* z, int
                           int* n_levels) {
     if (!C || !n_levels || cap < 0 || (cap > 0 && !level_sizes) || (r && !z)) return CS_E_ARG;
     if (cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
-    if (C->P.precond != CS_PRECOND_AMG) return CS_E_ARG;
+    if (C->P.precond != CS_PRECOND_AMG && C->P.precond != CS_PRECOND_LEARNED) return CS_E_ARG;
     if (r && !C->finalized) return CS_E_STATE;  // z needs an assembled system (cs_assemble)
+    if (C->P.precond == CS_PRECOND_LEARNED) {
+        *n_levels = 0;
+        if (!r) return CS_OK;
+        return guard(C, [&]() {
+            ClothSys& S = C->sys[cloth_id];
+            hipStream_t s = C->stream;
+            const size_t n3 = 3 * (size_t)S.nn;
+            CS_HIP(hipMemcpyAsync(S.r.ptr<float>(), r, sizeof(float) * n3, hipMemcpyHostToDevice, s));
+            lpc_prepare(C, cloth_id);
+            const int nb = std::max(1, cdiv(S.nn, 256));
+            lpc_apply(C, cloth_id, S.r.ptr<float>(), S.z.ptr<float4>(), C->ws.a.as<float>(nb), nb);
+            std::vector<float> z4(4 * (size_t)S.nn);
+            CS_HIP(hipMemcpyAsync(z4.data(), S.z.ptr<float4>(), sizeof(float) * z4.size(), hipMemcpyDeviceToHost, s));
+            CS_HIP(hipStreamSynchronize(s));
+            for (int i = 0; i < S.nn; i++)
+                for (int k = 0; k < 3; k++) z[3 * i + k] = z4[4 * (size_t)i + k];
+            return (int)CS_OK;
+        });
+    }
     return guard(C, [&]() {
         host_finalize(C);
         std::vector<int> sizes(64);
@@ -595,6 +613,54 @@ cs_status cs_precondition(cs_ctx* C, int cloth_id, const float* r, float* z, int
         CS_HIP(hipStreamSynchronize(s));
         for (int i = 0; i < S.nn; i++)
             for (int k = 0; k < 3; k++) z[3 * i + k] = z4[4 * (size_t)i + k];
+        return (int)CS_OK;
+    });
+}
+
+int cs_lpc_weight_count(void) { return lpc_weight_count(); }
+
+cs_status cs_set_learned_weights(cs_ctx* C, const float* w, int n) {
+    if (!C || !w || n != lpc_weight_count()) return CS_E_ARG;
+    for (int i = 0; i < n; i++)
+        if (!std::isfinite(w[i])) return CS_E_ARG;
+    return guard(C, [&]() {
+        lpc_set_weights(C, w);  // the factor is rebuilt from these at the next solve
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_get_learned_weights(cs_ctx* C, float* w, int n) {
+    if (!C || !w || n != lpc_weight_count()) return CS_E_ARG;
+    return guard(C, [&]() {
+        lpc_get_weights(C, w);
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_lpc_factor(cs_ctx* C, int cloth_id, float* diag, float* lower, int cap_lower, int* n_lower) {
+    if (!C || !n_lower || cap_lower < 0) return CS_E_ARG;
+    if (cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
+    if (!C->finalized) return CS_E_STATE;  // needs an assembled system
+    if (!C->lpcNet.loaded) return CS_E_STATE;
+    return guard(C, [&]() {
+        ClothSys& S = C->sys[cloth_id];
+        const int nl = ((int)S.hCol.size() - S.nn) / 2;
+        *n_lower = nl;
+        if (!diag || !lower) return (int)CS_OK;
+        if (nl > cap_lower) return (int)CS_E_CAPACITY;
+        lpc_prepare(C, cloth_id);
+        lpc_factor_blocks(C, cloth_id, diag, lower);
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_lpc_train_step(cs_ctx* C, int cloth_id, float lr, float* loss, float* grad) {
+    if (!C || !loss || !(lr >= 0.0f)) return CS_E_ARG;
+    if (cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
+    if (!C->finalized) return CS_E_STATE;
+    if (!C->lpcNet.loaded) return CS_E_STATE;
+    return guard(C, [&]() {
+        *loss = lpc_train_step(C, cloth_id, lr, grad);
         return (int)CS_OK;
     });
 }

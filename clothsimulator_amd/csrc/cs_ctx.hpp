@@ -86,6 +86,21 @@ struct AmgSys {
     DBuf apTarget, apSliceOff, apSliceW, apCols, apVals;
 };
 
+// Learned preconditioner of one cloth: M^-1 = G^T G with G block lower
+// triangular on the static pattern, stored in the value layout of the
+// sliced ELL (9 floats per slot): slot (i, j < i) holds G_ij, the diagonal
+// slot G_ii and slot (i, j > i) the transposed block G_ji^T, so both sweeps
+// only gather.  Network activations in 64-feature rows (see cs_lpc.hip).
+struct LpcSys {
+    bool built = false;
+    DBuf S;       // float 9 per node: S_i = chol(Minv_i)^T, row-major
+    DBuf mirror;  // int per slot: the slot of the transposed block (-1: padding or diagonal)
+    DBuf G;       // float 9 per slot, value layout
+    DBuf y;       // float4 per node: G r
+    DBuf h, P, Q, P2, Q2;  // float 64 per node row (rows padded to whole slices)
+    DBuf e;       // bf16 64 per slot: edge states
+};
+
 struct ClothSys {
     int mesh;
     int nn, nslices, totalSlots, maxWidth;
@@ -113,6 +128,18 @@ struct ClothSys {
     std::vector<int> hSliceOff, hSliceW, hRowLen;
     int nHandles = 0;
     AmgSys amg;
+    LpcSys lpc;
+};
+
+// Weights of the preconditioner network: float master copy and bfloat16
+// MFMA operand fragments (forward and transposed), each indexed like the flat
+// weight vector (cs_lpc.hip).
+struct LpcNet {
+    bool loaded = false;
+    DBuf w, frag, fragT;
+    // training: gradient, Adam moments and step count, activation arena, job table
+    DBuf grad, m, v, arena, jobs;
+    int t = 0;
 };
 
 struct PcgScalars {
@@ -161,6 +188,7 @@ struct cs_ctx {
     cs::DBuf faceRec, edgeRec;   // per-step element records (cloth faces / edges)
     std::vector<cs::ClothSys> sys;
     std::vector<cs::Bvh> bvh;     // one per mesh (rebuilt per stage)
+    cs::LpcNet lpcNet;            // learned preconditioner weights (CS_PRECOND_LEARNED)
 
     // proximity
     cs::DBuf pairs, pairCount;           // int2 pairs, uint32 counter(s)

@@ -36,6 +36,18 @@ void amg_prepare(cs_ctx* C, int c);
 // z = M^-1 r.  presmoothed: the caller already wrote x = w Minv r into amg_level0_x.
 void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int nb, bool presmoothed);
 float* amg_level0_x(cs_ctx* C, int c);
+// learned preconditioner (cs_lpc.hip)
+int lpc_weight_count();
+void lpc_set_weights(cs_ctx* C, const float* w);  // host array of lpc_weight_count() floats
+void lpc_get_weights(cs_ctx* C, float* w);
+void lpc_refresh_fragments(cs_ctx* C);            // after the float weights changed on the device
+void lpc_prepare(cs_ctx* C, int c);               // network on cloth c's system -> G
+// z = G^T G r and one r.z partial per block (nb blocks)
+void lpc_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int nb);
+// G's diagonal blocks (9 nn) and lower blocks (9 per lower entry, static-pattern row order)
+void lpc_factor_blocks(cs_ctx* C, int c, float* diag, float* lower);
+// one training step on cloth c's system: loss, gradient (optional host copy), Adam when lr > 0
+float lpc_train_step(cs_ctx* C, int c, float lr, float* grad);
 
 // LBVH (cs_lbvh.hip)
 void bvh_build(cs_ctx* C, int mesh, bool ccd);

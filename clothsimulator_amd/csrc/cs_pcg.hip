@@ -122,10 +122,15 @@ __device__ __forceinline__ void acc3(float& y0, float& y1, float& y2, const floa
     y2 += a[6] * v.x + a[7] * v.y + a[8] * v.z;
 }
 
-// x = 0, r = b; JACOBI: z = Minv r and the r.z partials.  Otherwise the multigrid
-// cycle that follows computes z and r.z, and this kernel does its first step, the
-// damped Jacobi pre-smoothing xs = w Minv r of level 0.
-template <bool JACOBI>
+// How z = M^-1 r is formed after the update of r:
+//   PC_JACOBI    z = Minv r and the r.z partials, in the same kernel;
+//   PC_SMOOTH    the multigrid cycle that follows computes z and r.z; this kernel
+//                does its first step, the damped Jacobi pre-smoothing xs = w Minv r;
+//   PC_EXTERNAL  a separate preconditioner pass computes z and r.z.
+enum { PC_JACOBI = 0, PC_SMOOTH = 1, PC_EXTERNAL = 2 };
+
+// x = 0, r = b, and the first preconditioner step as in k_pcg_update.
+template <int MODE>
 __global__ void __launch_bounds__(PB) k_pcg_init(int nn, const float* __restrict__ b, const float* __restrict__ bc,
                                                  const float* __restrict__ Minv, float* __restrict__ x,
                                                  float* __restrict__ r, float4* __restrict__ z,
@@ -137,18 +142,20 @@ __global__ void __launch_bounds__(PB) k_pcg_init(int nn, const float* __restrict
         V3f bi = ld3v(b, i) + ld3v(bc, i);
         st3v(x, i, mk(0.0f, 0.0f, 0.0f));
         st3v(r, i, bi);
-        const V3f zi = mat3v(Minv + 9 * (size_t)i, bi);
-        if (JACOBI) {
-            z[i] = f4(zi);
-            acc[0] += dot(bi, zi);
-        } else {
-            st3v(xs, i, AMG_OMEGA * zi);
+        if (MODE != PC_EXTERNAL) {
+            const V3f zi = mat3v(Minv + 9 * (size_t)i, bi);
+            if (MODE == PC_JACOBI) {
+                z[i] = f4(zi);
+                acc[0] += dot(bi, zi);
+            } else {
+                st3v(xs, i, AMG_OMEGA * zi);
+            }
         }
         acc[1] += dot(bi, bi);
     }
     block_sums<2>(acc, lds);
     if (threadIdx.x == 0) {
-        if (JACOBI) part[P_RZ0 * PMAX + L] = acc[0];
+        if (MODE == PC_JACOBI) part[P_RZ0 * PMAX + L] = acc[0];
         part[P_BB * PMAX + L] = acc[1];
     }
 }
@@ -272,10 +279,8 @@ __global__ void __launch_bounds__(PB) k_pcg_spmv(int it, int nn, const int* __re
     if (threadIdx.x == 0) part[P_DQ * PMAX + L] = pdq;
 }
 
-// alpha, x += alpha d, r -= alpha q, partial r.r; JACOBI: also z = Minv r and
-// the r.z partials.  Otherwise the multigrid cycle that follows computes them, and
-// this kernel writes its level-0 pre-smoothing xs = w Minv r.
-template <bool JACOBI>
+// alpha, x += alpha d, r -= alpha q, partial r.r, and the preconditioner step of MODE.
+template <int MODE>
 __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* __restrict__ Minv,
                                                    float* __restrict__ x, float* __restrict__ r,
                                                    float4* __restrict__ z, float* __restrict__ xs,
@@ -300,8 +305,9 @@ __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* 
             qi = ld3v(q, i);
             xi = ld3v(x, i);
             ri = ld3v(r, i);
+            if (MODE != PC_EXTERNAL)
 #pragma unroll
-            for (int c = 0; c < 9; c++) m[c] = Minv[9 * (size_t)i + c];
+                for (int c = 0; c < 9; c++) m[c] = Minv[9 * (size_t)i + c];
         }
         if (first_row) {
             if (sc->done) return;  // read after the row loads are in flight; the same in every thread
@@ -315,19 +321,21 @@ __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* 
         ri = ri - alpha * qi;
         st3v(x, i, xi);
         st3v(r, i, ri);
-        const V3f zi = mk(m[0] * ri.x + m[1] * ri.y + m[2] * ri.z, m[3] * ri.x + m[4] * ri.y + m[5] * ri.z,
-                          m[6] * ri.x + m[7] * ri.y + m[8] * ri.z);
-        if (JACOBI) {
-            z[i] = f4(zi);
-            acc[0] += dot(ri, zi);
-        } else {
-            st3v(xs, i, AMG_OMEGA * zi);
+        if (MODE != PC_EXTERNAL) {
+            const V3f zi = mk(m[0] * ri.x + m[1] * ri.y + m[2] * ri.z, m[3] * ri.x + m[4] * ri.y + m[5] * ri.z,
+                              m[6] * ri.x + m[7] * ri.y + m[8] * ri.z);
+            if (MODE == PC_JACOBI) {
+                z[i] = f4(zi);
+                acc[0] += dot(ri, zi);
+            } else {
+                st3v(xs, i, AMG_OMEGA * zi);
+            }
         }
         acc[1] += dot(ri, ri);
     }
     block_sums<2>(acc, lds);
     if (threadIdx.x == 0) {
-        if (JACOBI) part[(P_RZ0 + (1 - p)) * PMAX + L] = acc[0];
+        if (MODE == PC_JACOBI) part[(P_RZ0 + (1 - p)) * PMAX + L] = acc[0];
         part[P_RR * PMAX + L] = acc[1];
     }
 }
@@ -351,23 +359,30 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     float4* dgen[2] = {S.d.ptr<float4>(), S.d.ptr<float4>() + nn};
     int levels[16];
     const bool amg = C->P.precond == CS_PRECOND_AMG && amg_levels(C, c, levels, 16) > 1;
+    const bool learned = C->P.precond == CS_PRECOND_LEARNED;
     float* xs = nullptr;  // level-0 pre-smoothing iterate of the multigrid cycle
     if (amg) {
         amg_prepare(C, c);
         xs = amg_level0_x(C, c);
-        hipLaunchKernelGGL(k_pcg_init<false>, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(), S.bc.ptr<float>(),
+        hipLaunchKernelGGL(k_pcg_init<PC_SMOOTH>, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(), S.bc.ptr<float>(),
                            S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, xs, part);
         CS_LAUNCH_CHECK();
         amg_apply(C, c, S.r.ptr<float>(), z, part + P_RZ0 * PMAX, nb, true);
+    } else if (learned) {
+        lpc_prepare(C, c);
+        hipLaunchKernelGGL(k_pcg_init<PC_EXTERNAL>, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(),
+                           S.bc.ptr<float>(), S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, xs, part);
+        CS_LAUNCH_CHECK();
+        lpc_apply(C, c, S.r.ptr<float>(), z, part + P_RZ0 * PMAX, nb);
     } else {
-        hipLaunchKernelGGL(k_pcg_init<true>, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(), S.bc.ptr<float>(),
+        hipLaunchKernelGGL(k_pcg_init<PC_JACOBI>, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(), S.bc.ptr<float>(),
                            S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, xs, part);
         CS_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_pcg_setup, dim3(1), dim3(PB), 0, s, nb, part, sc, tol * tol, maxit);
     CS_LAUNCH_CHECK();
     // iterations per host poll: a multigrid iteration costs several times a Jacobi one
-    const int chunk = amg ? 4 : 16;
+    const int chunk = amg ? 4 : learned ? 8 : 16;
     int done = 0;
     PcgDev h{};
     int it = 0;
@@ -388,11 +403,15 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
                                    z, dgen[1 - (it & 1)], dcur, S.q.ptr<float>(), part, sc);
             ktimer_end(C, tok);
             if (amg) {
-                hipLaunchKernelGGL(k_pcg_update<false>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
+                hipLaunchKernelGGL(k_pcg_update<PC_SMOOTH>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
                                    S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
                 amg_apply(C, c, S.r.ptr<float>(), z, part + (P_RZ0 + (1 - (it & 1))) * PMAX, nb, true);
+            } else if (learned) {
+                hipLaunchKernelGGL(k_pcg_update<PC_EXTERNAL>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
+                                   S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
+                lpc_apply(C, c, S.r.ptr<float>(), z, part + (P_RZ0 + (1 - (it & 1))) * PMAX, nb);
             } else {
-                hipLaunchKernelGGL(k_pcg_update<true>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
+                hipLaunchKernelGGL(k_pcg_update<PC_JACOBI>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
                                    S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
             }
         }
