@@ -2,19 +2,26 @@
 """Benchmark of the MI355X cloth step (one GPU).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload crumple513]
+                    [--precond learned|amg|jacobi] [--compare jacobi,amg]
 
 A step is one full simulation step (physics: LBVH + proximities + implicit
-assembly + block-Jacobi PCG to 1e-5 relative residual + position/geometry
+assembly + preconditioned CG to 1e-5 relative residual + position/geometry
 update; collision: CCD LBVH + FP64 impacts + impact response + velocity
 update).  The default workload is the 513 x 513 crumple (263,169 nodes,
 524,288 faces, dt = 2 ms, ground plane), timed over its full 300 steps from the
 initial state with every input resident in HBM.  W warmup steps run first on
 the same context, which is then reset to the initial state.
 
-Prints one JSON line: node-steps/s, CG iterations per step, the roofline of the
-dominant kernel (the PCG SpMV, timed live with HIP events on the library's
-stream) and a CPU baseline (the oracle restatement on the host cores, on a
-bounded sample: the first step of the same workload).
+The measured run uses the learned preconditioner (packaged trained weights;
+the network runs once per solve inside the timed step).  The same steps are
+then run with block-Jacobi PCG (the reference's method) and with the
+aggregation multigrid preconditioner, each on a fresh context.
+
+Prints one JSON line: node-steps/s and CG iterations per step of the measured
+run, the same for the comparison runs, the roofline of the dominant kernel (the
+PCG SpMV, timed live with HIP events on the library's stream) and a CPU
+baseline (the oracle restatement on the host cores, on a bounded sample: the
+first step of the same workload).
 """
 import argparse
 import json
@@ -49,9 +56,9 @@ def static_nnzb(faces, n_nodes):
     return int(len(u))
 
 
-def read_traffic(workload):
+def read_traffic(workload, precond):
     """HBM bytes per SpMV launch measured with rocprofv3 PMC counters (committed under profiles/)."""
-    p = os.path.join(ROOT, "profiles", "pcg_spmv_%s_traffic.json" % workload)
+    p = os.path.join(ROOT, "profiles", "pcg_spmv_%s_%s_traffic.json" % (workload, PRECONDS[precond]))
     if os.path.exists(p):
         with open(p) as fh:
             return json.load(fh).get("bytes_per_launch")
@@ -84,6 +91,60 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n}))
 
 
+PRECONDS = {"learned": "learned_spai", "amg-learned": "amg_v11_learned_smoother", "jacobi": "block_jacobi",
+            "amg": "amg_v11"}
+
+
+def run(workload, precond, steps, warmup, timed):
+    """Steps `workload` from its initial state with one preconditioner; returns the timings."""
+    from clothsimulator_amd import CS_MESH_CLOTH, CS_MESH_OBSTACLE, Simulator, scenes
+    from clothsimulator_amd._cabi import (CS_FLAG_TIME_KERNELS, CS_KERNEL_EVENT_PAIR, CS_KERNEL_PCG_SPMV,
+                                          CS_PRECOND_AMG, CS_PRECOND_AMG_LEARNED, CS_PRECOND_BLOCK_JACOBI,
+                                          CS_PRECOND_LEARNED)
+
+    spec = scenes.by_name(workload)
+    if timed:
+        spec.params.flags |= CS_FLAG_TIME_KERNELS
+    spec.params.precond = {"amg": CS_PRECOND_AMG, "jacobi": CS_PRECOND_BLOCK_JACOBI, "learned": CS_PRECOND_LEARNED,
+                           "amg-learned": CS_PRECOND_AMG_LEARNED}[precond]
+    sim = Simulator(spec)   # the learned preconditioner loads the packaged trained weights
+    init = [sim.state(CS_MESH_CLOTH, i) for i in range(len(spec.cloths))]
+    obs = [sim.state(CS_MESH_OBSTACLE, i) for i in range(len(spec.obstacles))]
+    for _ in range(warmup):
+        sim.step()
+    for i, (x, x0, v) in enumerate(init):
+        sim.set_state(CS_MESH_CLOTH, i, x, x0, v)
+    for i, (x, x0, v) in enumerate(obs):
+        sim.set_state(CS_MESH_OBSTACLE, i, x, x0, v)
+    if timed:
+        sim.kernel_time(CS_KERNEL_PCG_SPMV, reset=True)
+        sim.kernel_time(CS_KERNEL_EVENT_PAIR, reset=True)
+    iters, contacts, failed = [], [], 0
+    t0 = time.perf_counter()  # every library call returns after its stream has completed
+    for k in range(steps):
+        s, st = sim.step()
+        if s not in (0, 4):
+            failed += 1
+        iters.append(st.pcg_iters)
+        contacts.append(st.n_contact_blocks)
+        if (k + 1) % 50 == 0:
+            print("%s step %d/%d  %.1f s  cg %d" % (precond, k + 1, steps, time.perf_counter() - t0, st.pcg_iters),
+                  file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    out = {"spec": spec, "elapsed": elapsed, "iters": iters, "contacts": contacts, "failed": failed,
+           "value": spec.n_cloth_nodes * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps}
+    if timed:
+        out["spmv"] = sim.kernel_time(CS_KERNEL_PCG_SPMV)
+        out["pair"] = sim.kernel_time(CS_KERNEL_EVENT_PAIR)
+    sim.close()
+    return out
+
+
+def summary(r):
+    return {"value": r["value"], "ms_per_step": r["ms_per_step"], "cg_iters_mean": float(np.mean(r["iters"])),
+            "cg_iters_max": int(np.max(r["iters"])), "failed_steps": r["failed"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,8 +153,11 @@ def main():
     ap.add_argument("--workload", default="crumple513")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precond", choices=["amg", "jacobi"], default="amg",
-                    help="PCG preconditioner: aggregation multigrid (default) or block-Jacobi")
+    ap.add_argument("--precond", choices=sorted(PRECONDS), default="learned",
+                    help="PCG preconditioner of the measured run: learned SPAI factor (default), aggregation "
+                         "multigrid or block-Jacobi")
+    ap.add_argument("--compare", default="jacobi,amg",
+                    help="comma-separated preconditioners also run over the same steps ('' for none)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
     a = ap.parse_args()
@@ -106,80 +170,64 @@ def main():
         copy_probe(a.copy_probe)
         return
 
-    from clothsimulator_amd import CS_MESH_CLOTH, CS_MESH_OBSTACLE, Simulator, scenes
-    from clothsimulator_amd._cabi import (CS_FLAG_TIME_KERNELS, CS_KERNEL_EVENT_PAIR, CS_KERNEL_PCG_SPMV,
-                                          CS_PRECOND_AMG, CS_PRECOND_BLOCK_JACOBI)
+    from clothsimulator_amd import scenes
 
-    spec = scenes.by_name(a.workload)
-    steps = a.steps if a.steps is not None else spec.steps
-    spec.params.flags |= CS_FLAG_TIME_KERNELS
-    spec.params.precond = CS_PRECOND_AMG if a.precond == "amg" else CS_PRECOND_BLOCK_JACOBI
-    sim = Simulator(spec)
-    init = [sim.state(CS_MESH_CLOTH, i) for i in range(len(spec.cloths))]
-    obs = [sim.state(CS_MESH_OBSTACLE, i) for i in range(len(spec.obstacles))]
-    for _ in range(a.warmup):
-        sim.step()
-    for i, (x, x0, v) in enumerate(init):
-        sim.set_state(CS_MESH_CLOTH, i, x, x0, v)
-    for i, (x, x0, v) in enumerate(obs):
-        sim.set_state(CS_MESH_OBSTACLE, i, x, x0, v)
-    sim.kernel_time(CS_KERNEL_PCG_SPMV, reset=True)
-    sim.kernel_time(CS_KERNEL_EVENT_PAIR, reset=True)
-
-    iters, contacts, failed = [], [], 0
-    t0 = time.perf_counter()  # every library call returns after its stream has completed
-    for k in range(steps):
-        s, st = sim.step()
-        if s not in (0, 4):
-            failed += 1
-        iters.append(st.pcg_iters)
-        contacts.append(st.n_contact_blocks)
-        if (k + 1) % 50 == 0:
-            print("step %d/%d  %.1f s  cg %d" % (k + 1, steps, time.perf_counter() - t0, st.pcg_iters),
-                  file=sys.stderr, flush=True)
-    elapsed = time.perf_counter() - t0
+    steps = a.steps if a.steps is not None else scenes.by_name(a.workload).steps
+    main_run = run(a.workload, a.precond, steps, a.warmup, True)
+    spec = main_run["spec"]
+    others = {}
+    for p in [q for q in a.compare.split(",") if q and q != a.precond]:
+        others[p] = run(a.workload, p, steps, a.warmup, False)
 
     n_nodes = spec.n_cloth_nodes
-    value = n_nodes * steps / elapsed
-    spmv_ms, spmv_n = sim.kernel_time(CS_KERNEL_PCG_SPMV)
     c0 = spec.cloths[0]
     nnzb = static_nnzb(c0.mesh.faces, c0.mesh.n_nodes)
-    nnzb_dyn = nnzb + float(np.mean(contacts))
+    nnzb_dyn = nnzb + float(np.mean(main_run["contacts"]))
     # algorithmic bytes of one SpMV launch: 36 B values + 4 B column index per block; per node
     # z and the previous direction read once (16 B each), the new direction (16 B) and q (12 B)
     # written once
     bytes_per_launch = 40.0 * nnzb_dyn + 60.0 * n_nodes
     # every sampled SpMV is bracketed by an event pair; an empty pair recorded just before it
     # measures the markers' own latency, which is subtracted
-    pair_ms, pair_n = sim.kernel_time(CS_KERNEL_EVENT_PAIR)
+    spmv_ms, spmv_n = main_run["spmv"]
+    pair_ms, pair_n = main_run["pair"]
     marker_s = (pair_ms / pair_n) * 1e-3 if pair_n else 0.0
     avg_s = (spmv_ms / spmv_n) * 1e-3 - marker_s if spmv_n else float("nan")
     achieved = bytes_per_launch / avg_s / 1e9 if spmv_n else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": read_traffic(a.workload),
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                "traffic": read_traffic(a.workload, a.precond),
                 "kernel": "k_pcg_spmv", "avg_us": avg_s * 1e6, "event_marker_us": marker_s * 1e6,
                 "launches_sampled": spmv_n, "bytes_per_launch": bytes_per_launch}
+    ms = summary(main_run)
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": ms["value"],
         "unit": "node-steps/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * elapsed / steps,
+        "ms_per_step": ms["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": spec.name, "nodes": n_nodes, "faces": int(c0.mesh.n_faces), "dt": spec.params.dt,
-                   "precond": "amg_v11" if a.precond == "amg" else "block_jacobi", "rel_tol": 1e-5, "obstacles": len(spec.obstacles)},
-        "cg_iters_mean": float(np.mean(iters)),
-        "cg_iters_max": int(np.max(iters)),
-        "cg_iters_learned": None,
-        "failed_steps": failed,
+                   "precond": PRECONDS[a.precond], "rel_tol": 1e-5, "obstacles": len(spec.obstacles)},
+        "cg_iters_mean": ms["cg_iters_mean"],
+        "cg_iters_max": ms["cg_iters_max"],
+        "cg_iters_learned": ms["cg_iters_mean"] if a.precond == "learned" else (
+            summary(others["learned"])["cg_iters_mean"] if "learned" in others else None),
+        "cg_iters_jacobi": ms["cg_iters_mean"] if a.precond == "jacobi" else (
+            summary(others["jacobi"])["cg_iters_mean"] if "jacobi" in others else None),
+        "failed_steps": ms["failed_steps"],
         "roofline": roofline,
     }
+    for p, r in others.items():
+        out[PRECONDS[p]] = summary(r)
+    if "jacobi" in others:
+        out["speedup_vs_jacobi"] = ms["value"] / others["jacobi"]["value"]
     if not a.no_cpu:
         print("cpu baseline: %d step(s) of %s on the host" % (a.cpu_steps, a.workload), file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(scenes.by_name(a.workload), a.cpu_steps)

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from ._cabi import (CS_E_NOT_CONVERGED, CS_MESH_CLOTH, CS_PRECOND_LEARNED, CS_MESH_OBSTACLE, CS_OK, IMPACT_DTYPE, LIB_PATH, PROX_DTYPE,
+from ._cabi import (CS_E_NOT_CONVERGED, CS_MESH_CLOTH, CS_PRECOND_AMG_LEARNED, CS_PRECOND_LEARNED, CS_MESH_OBSTACLE, CS_OK, IMPACT_DTYPE, LIB_PATH, PROX_DTYPE,
                     STATUS_NAMES, ImpactRec, MaterialIn, MeshArrays, MeshIn, Params, ProximityRec, SceneInfo,
                     Stats, default_params, fptr, iptr, material_struct)
 
@@ -24,7 +24,7 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_lbvh_pairs", "cs_find_proximities", "cs_assemble", "cs_solve", "cs_find_impacts",
             "cs_independent_impacts", "cs_kernel_time", "cs_load_scene", "cs_scene_error", "cs_precondition",
             "cs_lpc_weight_count", "cs_set_learned_weights", "cs_get_learned_weights", "cs_lpc_factor",
-            "cs_lpc_train_step"]
+            "cs_lpc_train_step", "cs_lpc_smoother_omega"]
 
 # trained weights of the learned preconditioner shipped with the package (float32, .npy)
 LPC_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lpc_weights.npy")
@@ -79,7 +79,8 @@ def load_library(path=LIB_PATH):
     L.cs_set_learned_weights.argtypes = [vp, F, C.c_int]
     L.cs_get_learned_weights.argtypes = [vp, F, C.c_int]
     L.cs_lpc_factor.argtypes = [vp, C.c_int, F, F, C.c_int, C.POINTER(C.c_int)]
-    L.cs_lpc_train_step.argtypes = [vp, C.c_int, C.c_float, F, F]
+    L.cs_lpc_train_step.argtypes = [vp, C.c_int, C.c_float, C.c_float, F, F]
+    L.cs_lpc_smoother_omega.argtypes = [vp, C.c_int, F]
     _lib = L
     return L
 
@@ -131,7 +132,7 @@ class Simulator:
                     self.add_handles(cid, c.handles)
             for o in spec.obstacles:
                 self.add_obstacle(o)
-        if self.params.precond == CS_PRECOND_LEARNED and os.path.exists(LPC_WEIGHTS):
+        if self.params.precond in (CS_PRECOND_LEARNED, CS_PRECOND_AMG_LEARNED) and os.path.exists(LPC_WEIGHTS):
             self.set_learned_weights(default_learned_weights())
 
     @classmethod
@@ -315,14 +316,21 @@ class Simulator:
                     "cs_lpc_factor")
         return diag, lower[:nl.value]
 
-    def lpc_train_step(self, cloth=0, lr=0.0, grad=False):
-        """Kaporin loss of the network's factor on the last assembled system, its gradient
-        (when grad) and one Adam step when lr > 0."""
+    def lpc_train_step(self, cloth=0, lr=0.0, grad=False, sol_weight=0.0):
+        """Kaporin loss of the network's factor on the last assembled system (plus sol_weight times
+        the solution term once the system is solved), its gradient (when grad) and one Adam step
+        when lr > 0."""
         loss = C.c_float()
         g = np.zeros(self.L.cs_lpc_weight_count(), dtype=np.float32) if grad else None
-        self._check(self.L.cs_lpc_train_step(self.h, cloth, lr, C.byref(loss), fptr(g) if grad else None),
+        self._check(self.L.cs_lpc_train_step(self.h, cloth, lr, sol_weight, C.byref(loss), fptr(g) if grad else None),
                     "cs_lpc_train_step")
         return (loss.value, g) if grad else loss.value
+
+    def smoother_omega(self, cloth=0):
+        """Damping of the learned smoother of the last assembled system (CS_PRECOND_AMG_LEARNED)."""
+        w = np.zeros(1, dtype=np.float32)
+        self._check(self.L.cs_lpc_smoother_omega(self.h, cloth, fptr(w)), "cs_lpc_smoother_omega")
+        return float(w[0])
 
     def kernel_time(self, kernel_id, reset=False):
         """(summed ms, sampled launches) of a hot kernel; needs CS_FLAG_TIME_KERNELS in params.flags."""

@@ -28,6 +28,7 @@ STATUS_NAMES = {0: "CS_OK", 1: "CS_E_ARG", 2: "CS_E_HIP", 3: "CS_E_OOM", 4: "CS_
 CS_PRECOND_BLOCK_JACOBI = 0
 CS_PRECOND_LEARNED = 1
 CS_PRECOND_AMG = 2
+CS_PRECOND_AMG_LEARNED = 3
 CS_MESH_CLOTH = 0
 CS_MESH_OBSTACLE = 1
 CS_FLAG_NO_PROXIMITY = 1

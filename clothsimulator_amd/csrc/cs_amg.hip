@@ -499,6 +499,38 @@ __global__ void __launch_bounds__(AB) k_amg_post0ap(int nn, Level0 A, const int*
     if (threadIdx.x == 0) partRZ[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
 }
 
+// Level 0 with the learned smoother, before its sweeps: the residual after the coarse
+// correction, res = t - A_static P e - C P e (into t), and x' = x + P e (into x).
+__global__ void __launch_bounds__(AB) k_amg_post0ap_res(int nn, Level0 A, const int* __restrict__ apSliceOff,
+                                                        const int* __restrict__ apSliceW,
+                                                        const int* __restrict__ apCols,
+                                                        const float* __restrict__ apVals, const int* __restrict__ agg,
+                                                        const float* __restrict__ e, float* __restrict__ x,
+                                                        float* __restrict__ t) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= nn) return;
+    const int s = i / SLICE, lane = i % SLICE;
+    const int WA = apSliceW[s];
+    const size_t aoff = (size_t)apSliceOff[s];
+    float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+    for (int m = 0; m < WA; m++) {
+        const V3f ej = ld3v(e, apCols[(aoff + m) * SLICE + lane]);
+        const float* a = apVals + (aoff + m) * 9 * SLICE + lane;
+        y0 += a[0 * SLICE] * ej.x + a[1 * SLICE] * ej.y + a[2 * SLICE] * ej.z;
+        y1 += a[3 * SLICE] * ej.x + a[4 * SLICE] * ej.y + a[5 * SLICE] * ej.z;
+        y2 += a[6 * SLICE] * ej.x + a[7 * SLICE] * ej.y + a[8 * SLICE] * ej.z;
+    }
+    for (int k = A.cRowPtr[i], ce = A.cRowPtr[i + 1]; k < ce; k++) {
+        const V3f ej = ld3v(e, agg[A.cCol[k]]);
+        const float* a = A.cVal + (size_t)k * 9;
+        y0 += a[0] * ej.x + a[1] * ej.y + a[2] * ej.z;
+        y1 += a[3] * ej.x + a[4] * ej.y + a[5] * ej.z;
+        y2 += a[6] * ej.x + a[7] * ej.y + a[8] * ej.z;
+    }
+    st3v(x, i, ld3v(x, i) + ld3v(e, agg[i]));
+    st3v(t, i, ld3v(t, i) - mk(y0, y1, y2));
+}
+
 // Level 0 post-smoothing: z = x' + w Minv (r - A x'); also one partial of r.z per block.
 __global__ void __launch_bounds__(AB) k_amg_post0(int nn, Level0 A, const int* __restrict__ agg,
                                                   const float* __restrict__ e, const float* __restrict__ r,
@@ -706,7 +738,7 @@ void amg_prepare(cs_ctx* C, int c) {
 // nb blocks of the PCG grid (nb * 256 threads, rows in grid-stride order).
 float* amg_level0_x(cs_ctx* C, int c) { return C->sys[c].amg.L[0].x.ptr<float>(); }
 
-void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int nb, bool presmoothed) {
+void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int nb, bool presmoothed, bool learned) {
     ClothSys& S = C->sys[c];
     AmgSys& G = S.amg;
     hipStream_t s = C->stream;
@@ -767,7 +799,12 @@ void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int n
         AmgLevel& L = G.L[l];
         const AmgLevel& N = G.L[l + 1];
         const float* e = (l + 1 == G.nl - 1) ? N.x.ptr<float>() : N.t.ptr<float>();
-        if (l == 0)
+        if (l == 0 && learned) {
+            hipLaunchKernelGGL(k_amg_post0ap_res, dim3(cdiv(L.n, AB)), dim3(AB), 0, s, L.n, A0,
+                               G.apSliceOff.ptr<int>(), G.apSliceW.ptr<int>(), G.apCols.ptr<int>(),
+                               G.apVals.ptr<float>(), L.agg.ptr<int>(), e, L.x.ptr<float>(), L.t.ptr<float>());
+            lpc_smooth_post(C, c, L.t.ptr<float>(), L.x.ptr<float>(), r, z, partRZ, nb);
+        } else if (l == 0)
             hipLaunchKernelGGL(k_amg_post0ap, dim3(nb), dim3(AB), 0, s, L.n, A0, G.apSliceOff.ptr<int>(),
                                G.apSliceW.ptr<int>(), G.apCols.ptr<int>(), G.apVals.ptr<float>(), L.agg.ptr<int>(), e,
                                r, L.x.ptr<float>(), L.t.ptr<float>(), z, partRZ);

@@ -71,7 +71,8 @@ template <class F> cs_status guard(cs_ctx* C, F&& f) {
 bool valid_params(const cs_params* p) {
     return p && p->dt > 0.0f && p->pcg_rel_tol > 0.0f && p->pcg_rel_tol < 1.0f && p->pcg_max_iters >= 0 &&
            p->max_collision_iters >= 0 && p->repulsion_thickness > 0.0f && p->collision_thickness >= 0.0f &&
-           (p->precond == CS_PRECOND_BLOCK_JACOBI || p->precond == CS_PRECOND_LEARNED || p->precond == CS_PRECOND_AMG);
+           (p->precond == CS_PRECOND_BLOCK_JACOBI || p->precond == CS_PRECOND_LEARNED || p->precond == CS_PRECOND_AMG ||
+            p->precond == CS_PRECOND_AMG_LEARNED);
 }
 
 const MeshInfo* mesh_of(cs_ctx* C, int kind, int id) {
@@ -572,7 +573,7 @@ cs_status cs_precondition(cs_ctx* C, int cloth_id, const float* r, float* z, int
                           int* n_levels) {
     if (!C || !n_levels || cap < 0 || (cap > 0 && !level_sizes) || (r && !z)) return CS_E_ARG;
     if (cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
-    if (C->P.precond != CS_PRECOND_AMG && C->P.precond != CS_PRECOND_LEARNED) return CS_E_ARG;
+    if (C->P.precond == CS_PRECOND_BLOCK_JACOBI) return CS_E_ARG;
     if (r && !C->finalized) return CS_E_STATE;  // z needs an assembled system (cs_assemble)
     if (C->P.precond == CS_PRECOND_LEARNED) {
         *n_levels = 0;
@@ -607,7 +608,14 @@ cs_status cs_precondition(cs_ctx* C, int cloth_id, const float* r, float* z, int
         CS_HIP(hipMemcpyAsync(S.r.ptr<float>(), r, sizeof(float) * n3, hipMemcpyHostToDevice, s));
         amg_prepare(C, cloth_id);
         const int nb = std::max(1, cdiv(S.nn, 256));
-        amg_apply(C, cloth_id, S.r.ptr<float>(), S.z.ptr<float4>(), C->ws.a.as<float>(nb), nb, false);
+        if (C->P.precond == CS_PRECOND_AMG_LEARNED) {
+            lpc_prepare(C, cloth_id);
+            lpc_smoother_prepare(C, cloth_id);
+            lpc_smooth_pre(C, cloth_id, S.r.ptr<float>(), amg_level0_x(C, cloth_id));
+            amg_apply(C, cloth_id, S.r.ptr<float>(), S.z.ptr<float4>(), C->ws.a.as<float>(nb), nb, true, true);
+        } else {
+            amg_apply(C, cloth_id, S.r.ptr<float>(), S.z.ptr<float4>(), C->ws.a.as<float>(nb), nb, false);
+        }
         std::vector<float> z4(4 * (size_t)S.nn);
         CS_HIP(hipMemcpyAsync(z4.data(), S.z.ptr<float4>(), sizeof(float) * z4.size(), hipMemcpyDeviceToHost, s));
         CS_HIP(hipStreamSynchronize(s));
@@ -654,13 +662,25 @@ cs_status cs_lpc_factor(cs_ctx* C, int cloth_id, float* diag, float* lower, int 
     });
 }
 
-cs_status cs_lpc_train_step(cs_ctx* C, int cloth_id, float lr, float* loss, float* grad) {
-    if (!C || !loss || !(lr >= 0.0f)) return CS_E_ARG;
+cs_status cs_lpc_train_step(cs_ctx* C, int cloth_id, float lr, float sol_weight, float* loss, float* grad) {
+    if (!C || !loss || !(lr >= 0.0f) || !(sol_weight >= 0.0f)) return CS_E_ARG;
     if (cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
     if (!C->finalized) return CS_E_STATE;
     if (!C->lpcNet.loaded) return CS_E_STATE;
     return guard(C, [&]() {
-        *loss = lpc_train_step(C, cloth_id, lr, grad);
+        *loss = lpc_train_step(C, cloth_id, lr, sol_weight, grad);
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_lpc_smoother_omega(cs_ctx* C, int cloth_id, float* omega) {
+    if (!C || !omega) return CS_E_ARG;
+    if (cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
+    if (!C->finalized || !C->lpcNet.loaded) return CS_E_STATE;
+    return guard(C, [&]() {
+        lpc_prepare(C, cloth_id);
+        lpc_smoother_prepare(C, cloth_id);
+        *omega = lpc_smoother_omega(C, cloth_id);
         return (int)CS_OK;
     });
 }

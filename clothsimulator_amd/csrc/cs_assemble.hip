@@ -502,6 +502,7 @@ void launch_element_records(cs_ctx* C, int c) {
 void launch_assemble_rows(cs_ctx* C, int c) {
     const MeshInfo& M = C->meshes[c];
     ClothSys& S = C->sys[c];
+    S.solved = false;
     const cs_params& P = C->P;
     size_t lds = (size_t)S.maxWidth * 9 * SLICE * sizeof(float);
     const int tok = ktimer_begin(C, CS_KERNEL_ASSEMBLE_ROWS);

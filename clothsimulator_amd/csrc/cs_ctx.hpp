@@ -87,18 +87,20 @@ struct AmgSys {
 };
 
 // Learned preconditioner of one cloth: M^-1 = G^T G with G block lower
-// triangular on the static pattern, stored in the value layout of the
-// sliced ELL (9 floats per slot): slot (i, j < i) holds G_ij, the diagonal
+// triangular on the static pattern, stored in bfloat16 in the value layout of
+// the sliced ELL (9 per slot): slot (i, j < i) holds G_ij, the diagonal
 // slot G_ii and slot (i, j > i) the transposed block G_ji^T, so both sweeps
 // only gather.  Network activations in 64-feature rows (see cs_lpc.hip).
 struct LpcSys {
     bool built = false;
     DBuf S;       // float 9 per node: S_i = chol(Minv_i)^T, row-major
     DBuf mirror;  // int per slot: the slot of the transposed block (-1: padding or diagonal)
-    DBuf G;       // float 9 per slot, value layout
+    DBuf G;       // bf16 9 per slot, value layout
     DBuf y;       // float4 per node: G r
     DBuf h, P, Q, P2, Q2;  // float 64 per node row (rows padded to whole slices)
     DBuf e;       // bf16 64 per slot: edge states
+    // learned smoother of the multigrid cycle: power-iteration vectors and partials, damping
+    DBuf pv, pw, ppart, omega;
 };
 
 struct ClothSys {
@@ -127,6 +129,7 @@ struct ClothSys {
     std::vector<int> hRowPtr, hCol;  // exact (unpadded) static pattern
     std::vector<int> hSliceOff, hSliceW, hRowLen;
     int nHandles = 0;
+    bool solved = false;  // x holds the converged solution of the current assembled system
     AmgSys amg;
     LpcSys lpc;
 };

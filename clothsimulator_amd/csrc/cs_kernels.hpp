@@ -34,7 +34,9 @@ constexpr float AMG_OMEGA = 0.6f;  // damping of the block-Jacobi smoother
 int amg_levels(cs_ctx* C, int c, int* sizes, int cap);  // 0: not usable for this cloth
 void amg_prepare(cs_ctx* C, int c);
 // z = M^-1 r.  presmoothed: the caller already wrote x = w Minv r into amg_level0_x.
-void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int nb, bool presmoothed);
+// learned: the level-0 smoother is the learned factor (lpc_smooth_*; the caller pre-smooths)
+void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int nb, bool presmoothed,
+               bool learned = false);
 float* amg_level0_x(cs_ctx* C, int c);
 // learned preconditioner (cs_lpc.hip)
 int lpc_weight_count();
@@ -44,10 +46,18 @@ void lpc_refresh_fragments(cs_ctx* C);            // after the float weights cha
 void lpc_prepare(cs_ctx* C, int c);               // network on cloth c's system -> G
 // z = G^T G r and one r.z partial per block (nb blocks)
 void lpc_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int nb);
+// The learned factor as the level-0 smoother of the multigrid cycle (after lpc_prepare):
+// damping from a power iteration, x = w G^T G r, and z = base + w G^T G res with r.z partials.
+void lpc_smoother_prepare(cs_ctx* C, int c);
+void lpc_smooth_pre(cs_ctx* C, int c, const float* r, float* x);
+void lpc_smooth_post(cs_ctx* C, int c, const float* res, const float* base, const float* r, float4* z,
+                     float* partRZ, int nb);
+float lpc_smoother_omega(cs_ctx* C, int c);
 // G's diagonal blocks (9 nn) and lower blocks (9 per lower entry, static-pattern row order)
 void lpc_factor_blocks(cs_ctx* C, int c, float* diag, float* lower);
-// one training step on cloth c's system: loss, gradient (optional host copy), Adam when lr > 0
-float lpc_train_step(cs_ctx* C, int c, float lr, float* grad);
+// one training step on cloth c's system: loss (Kaporin, plus solWeight times the solution term
+// when the system has been solved), gradient (optional host copy), Adam when lr > 0
+float lpc_train_step(cs_ctx* C, int c, float lr, float solWeight, float* grad);
 
 // LBVH (cs_lbvh.hip)
 void bvh_build(cs_ctx* C, int mesh, bool ccd);

@@ -401,14 +401,14 @@ struct LayerArgs {
     float* h;
     const bf16_t* ein;
     bf16_t* eout;
-    float* G;
+    bf16_t* G;
     const float* w;
     const s4* wf;
     Stash st;
 };
 
 // Ghat (row-major 3x3) from q / o and G = Ghat S, written to slot `sig` (and its transpose to `msig`)
-__device__ __forceinline__ void write_g(float* __restrict__ G, long sig, long msig, const float (&gh)[9],
+__device__ __forceinline__ void write_g(bf16_t* __restrict__ G, long sig, long msig, const float (&gh)[9],
                                         const float* __restrict__ Sj) {
     float r[9];
 #pragma unroll
@@ -416,12 +416,12 @@ __device__ __forceinline__ void write_g(float* __restrict__ G, long sig, long ms
 #pragma unroll
         for (int b = 0; b < 3; b++) r[3 * a + b] = gh[3 * a] * Sj[b] + gh[3 * a + 1] * Sj[3 + b] + gh[3 * a + 2] * Sj[6 + b];
 #pragma unroll
-    for (int c = 0; c < 9; c++) G[vidx(sig, c)] = r[c];
+    for (int c = 0; c < 9; c++) G[vidx(sig, c)] = bf_rne(r[c]);
     if (msig >= 0)
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
-            for (int b = 0; b < 3; b++) G[vidx(msig, 3 * a + b)] = r[3 * b + a];
+            for (int b = 0; b < 3; b++) G[vidx(msig, 3 * a + b)] = bf_rne(r[3 * b + a]);
 }
 
 template <bool TRAIN, bool LAST>
@@ -562,62 +562,154 @@ __global__ void __launch_bounds__(LB) k_lpc_layer(LayerArgs a) {
 // y = G r (lower slots and the diagonal)
 __global__ void __launch_bounds__(256) k_lpc_lower(int nn, const int* __restrict__ sliceOff,
                                                    const int* __restrict__ cols, const int* __restrict__ diagSlot,
-                                                   const float* __restrict__ G, const float* __restrict__ r,
+                                                   const bf16_t* __restrict__ G, const float* __restrict__ r,
                                                    float4* __restrict__ y) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < nn; i += gridDim.x * 256) {
         const int s = i / 64, lane = i % 64;
         const size_t off = (size_t)sliceOff[s];
         const int kd = diagSlot[i];
         const int* cp = cols + off * 64 + lane;
-        const float* gp = G + off * 9 * 64 + lane;
+        const bf16_t* gp = G + off * 9 * 64 + lane;
         float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
         for (int k = 0; k <= kd; k++) {
             const int j = cp[k * 64];
             const float r0 = r[3 * j], r1 = r[3 * j + 1], r2 = r[3 * j + 2];
-            const float* a = gp + (size_t)k * 9 * 64;
-            y0 += a[0 * 64] * r0 + a[1 * 64] * r1 + a[2 * 64] * r2;
-            y1 += a[3 * 64] * r0 + a[4 * 64] * r1 + a[5 * 64] * r2;
-            y2 += a[6 * 64] * r0 + a[7 * 64] * r1 + a[8 * 64] * r2;
+            const bf16_t* a = gp + (size_t)k * 9 * 64;
+            y0 += bf_val(a[0 * 64]) * r0 + bf_val(a[1 * 64]) * r1 + bf_val(a[2 * 64]) * r2;
+            y1 += bf_val(a[3 * 64]) * r0 + bf_val(a[4 * 64]) * r1 + bf_val(a[5 * 64]) * r2;
+            y2 += bf_val(a[6 * 64]) * r0 + bf_val(a[7 * 64]) * r1 + bf_val(a[8 * 64]) * r2;
         }
         y[i] = make_float4(y0, y1, y2, 0.0f);
     }
 }
 
-// z = G^T y (diagonal transposed, upper slots hold the transposed lower blocks);
-// one partial of r.z per block
+// z = G^T y (diagonal transposed, upper slots hold the transposed lower blocks), then by MODE:
+//   UP_PCG   z -> zout (float4), one partial of r.z per block
+//   UP_POW   z -> zout (float4), one partial of |z|^2 per block
+//   UP_PRE   w z -> xout (3 floats per node); w = *omega
+//   UP_POST  base + w z -> zout (float4), one partial of r.z per block
+enum { UP_PCG = 0, UP_POW = 1, UP_PRE = 2, UP_POST = 3 };
+
+template <int MODE>
 __global__ void __launch_bounds__(256) k_lpc_upper(int nn, const int* __restrict__ sliceOff,
                                                    const int* __restrict__ cols, const int* __restrict__ diagSlot,
-                                                   const int* __restrict__ rowLen, const float* __restrict__ G,
+                                                   const int* __restrict__ rowLen, const bf16_t* __restrict__ G,
                                                    const float4* __restrict__ y, const float* __restrict__ r,
-                                                   float4* __restrict__ z, float* __restrict__ partRZ) {
+                                                   const float* __restrict__ base, const float* __restrict__ omega,
+                                                   float4* __restrict__ zout, float* __restrict__ xout,
+                                                   float* __restrict__ part) {
     __shared__ float lds[4];
-    float prz = 0.0f;
+    float acc = 0.0f;
+    const float w = (MODE == UP_PRE || MODE == UP_POST) ? *omega : 1.0f;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < nn; i += gridDim.x * 256) {
         const int s = i / 64, lane = i % 64;
         const size_t off = (size_t)sliceOff[s];
         const int kd = diagSlot[i], rl = rowLen[i];
         const int* cp = cols + off * 64 + lane;
-        const float* gp = G + off * 9 * 64 + lane;
+        const bf16_t* gp = G + off * 9 * 64 + lane;
         const float4 yi = y[i];
-        const float* d = gp + (size_t)kd * 9 * 64;
-        float z0 = d[0 * 64] * yi.x + d[3 * 64] * yi.y + d[6 * 64] * yi.z;
-        float z1 = d[1 * 64] * yi.x + d[4 * 64] * yi.y + d[7 * 64] * yi.z;
-        float z2 = d[2 * 64] * yi.x + d[5 * 64] * yi.y + d[8 * 64] * yi.z;
+        const bf16_t* d = gp + (size_t)kd * 9 * 64;
+        float z0 = bf_val(d[0 * 64]) * yi.x + bf_val(d[3 * 64]) * yi.y + bf_val(d[6 * 64]) * yi.z;
+        float z1 = bf_val(d[1 * 64]) * yi.x + bf_val(d[4 * 64]) * yi.y + bf_val(d[7 * 64]) * yi.z;
+        float z2 = bf_val(d[2 * 64]) * yi.x + bf_val(d[5 * 64]) * yi.y + bf_val(d[8 * 64]) * yi.z;
         for (int k = kd + 1; k < rl; k++) {
             const float4 yj = y[cp[k * 64]];
-            const float* a = gp + (size_t)k * 9 * 64;
-            z0 += a[0 * 64] * yj.x + a[1 * 64] * yj.y + a[2 * 64] * yj.z;
-            z1 += a[3 * 64] * yj.x + a[4 * 64] * yj.y + a[5 * 64] * yj.z;
-            z2 += a[6 * 64] * yj.x + a[7 * 64] * yj.y + a[8 * 64] * yj.z;
+            const bf16_t* a = gp + (size_t)k * 9 * 64;
+            z0 += bf_val(a[0 * 64]) * yj.x + bf_val(a[1 * 64]) * yj.y + bf_val(a[2 * 64]) * yj.z;
+            z1 += bf_val(a[3 * 64]) * yj.x + bf_val(a[4 * 64]) * yj.y + bf_val(a[5 * 64]) * yj.z;
+            z2 += bf_val(a[6 * 64]) * yj.x + bf_val(a[7 * 64]) * yj.y + bf_val(a[8 * 64]) * yj.z;
         }
-        z[i] = make_float4(z0, z1, z2, 0.0f);
-        prz += r[3 * i] * z0 + r[3 * i + 1] * z1 + r[3 * i + 2] * z2;
+        if (MODE == UP_PRE) {
+            xout[3 * i] = w * z0;
+            xout[3 * i + 1] = w * z1;
+            xout[3 * i + 2] = w * z2;
+            continue;
+        }
+        if (MODE == UP_POST) {
+            z0 = base[3 * i] + w * z0;
+            z1 = base[3 * i + 1] + w * z1;
+            z2 = base[3 * i + 2] + w * z2;
+        }
+        zout[i] = make_float4(z0, z1, z2, 0.0f);
+        if (MODE == UP_POW) acc += z0 * z0 + z1 * z1 + z2 * z2;
+        else acc += r[3 * i] * z0 + r[3 * i + 1] * z1 + r[3 * i + 2] * z2;
     }
+    if (MODE == UP_PRE) return;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) prz += __shfl_xor(prz, o, 64);
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = prz;
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) partRZ[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+    if (threadIdx.x == 0) part[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+}
+
+// ---------------------------------------------------------------- smoother damping
+
+// Sum of n partials, every thread of the block gets it.
+__device__ __forceinline__ float sum_partials(const float* __restrict__ part, int n, float* lds) {
+    float a = 0.0f;
+    for (int b = threadIdx.x; b < n; b += 256) a += part[b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = a;
+    __syncthreads();
+    return (lds[0] + lds[1]) + (lds[2] + lds[3]);
+}
+
+// Power-iteration start: v_i = (+-1, +-1, +-1) from a hash of i.
+__global__ void k_lpc_pow_init(int nn, float4* __restrict__ v) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    uint32_t h = (uint32_t)i * 2654435761u;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    v[i] = make_float4((h & 1) ? 1.0f : -1.0f, (h & 2) ? 1.0f : -1.0f, (h & 4) ? 1.0f : -1.0f, 0.0f);
+}
+
+// w = A (v / |v|) with |v|^2 the sum of the previous sweep's partials (none: |v| = 1);
+// A = the level-0 system (static sliced ELL + contact CSR)
+__global__ void __launch_bounds__(256) k_lpc_pow_spmv(int nn, const int* __restrict__ sliceOff,
+                                                      const int* __restrict__ sliceW, const int* __restrict__ cols,
+                                                      const float* __restrict__ vals, const int* __restrict__ cRowPtr,
+                                                      const int* __restrict__ cCol, const float* __restrict__ cVal,
+                                                      const float4* __restrict__ v, const float* __restrict__ part,
+                                                      int npart, float* __restrict__ w) {
+    __shared__ float lds[4];
+    const float sc = part ? rsqrtf(sum_partials(part, npart, lds)) : 1.0f;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < nn; i += gridDim.x * 256) {
+        const int s = i / 64, lane = i % 64;
+        const size_t off = (size_t)sliceOff[s];
+        const int W = sliceW[s];
+        const int* cp = cols + off * 64 + lane;
+        const float* vp = vals + off * 9 * 64 + lane;
+        float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+        for (int k = 0; k < W; k++) {
+            const float4 vj = v[cp[k * 64]];
+            const float* a = vp + (size_t)k * 9 * 64;
+            y0 += a[0 * 64] * vj.x + a[1 * 64] * vj.y + a[2 * 64] * vj.z;
+            y1 += a[3 * 64] * vj.x + a[4 * 64] * vj.y + a[5 * 64] * vj.z;
+            y2 += a[6 * 64] * vj.x + a[7 * 64] * vj.y + a[8 * 64] * vj.z;
+        }
+        for (int k = cRowPtr[i], ce = cRowPtr[i + 1]; k < ce; k++) {
+            const float4 vj = v[cCol[k]];
+            const float* a = cVal + (size_t)k * 9;
+            y0 += a[0] * vj.x + a[1] * vj.y + a[2] * vj.z;
+            y1 += a[3] * vj.x + a[4] * vj.y + a[5] * vj.z;
+            y2 += a[6] * vj.x + a[7] * vj.y + a[8] * vj.z;
+        }
+        w[3 * i] = sc * y0;
+        w[3 * i + 1] = sc * y1;
+        w[3 * i + 2] = sc * y2;
+    }
+}
+
+// omega = c / lambda, lambda = |G^T G A v| for the last (unit) v
+__global__ void __launch_bounds__(256) k_lpc_omega(const float* __restrict__ part, int npart, float c,
+                                                   float* __restrict__ omega) {
+    __shared__ float lds[4];
+    const float lam = sqrtf(sum_partials(part, npart, lds));
+    if (threadIdx.x == 0) *omega = (lam > 0.0f && lam < INFINITY) ? c / lam : 0.0f;
 }
 
 // ---------------------------------------------------------------- training: loss
@@ -722,6 +814,110 @@ __global__ void __launch_bounds__(256) k_lpc_loss_final(int nblk, int n3, const 
         scal[1] = (float)(2.0 / T);
         scal[2] = (float)(2.0 / n3);
         scal[3] = (float)T;
+    }
+}
+
+// ---------------------------------------------------------------- training: solution term
+
+// bt = b + bc (the right-hand side the PCG solves)
+__global__ void k_lpc_rhs(int n3, const float* __restrict__ b, const float* __restrict__ bc, float* __restrict__ bt) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n3) bt[i] = b[i] + bc[i];
+}
+
+// partials of u.p and b.x (u float4 per node, p / b / x 3 per node)
+__global__ void __launch_bounds__(256) k_lpc_sol_dots(int nn, const float4* __restrict__ u, const float* __restrict__ p,
+                                                      const float* __restrict__ b, const float* __restrict__ x,
+                                                      float* __restrict__ part) {
+    __shared__ float lds[2][4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    float a0 = 0.0f, a1 = 0.0f;
+    if (i < nn) {
+        const float4 ui = u[i];
+        a0 = ui.x * p[3 * i] + ui.y * p[3 * i + 1] + ui.z * p[3 * i + 2];
+        a1 = b[3 * i] * x[3 * i] + b[3 * i + 1] * x[3 * i + 1] + b[3 * i + 2] * x[3 * i + 2];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, 64);
+        a1 += __shfl_xor(a1, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        lds[0][threadIdx.x >> 6] = a0;
+        lds[1][threadIdx.x >> 6] = a1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = (lds[0][0] + lds[0][1]) + (lds[0][2] + lds[0][3]);
+        part[2 * blockIdx.x + 1] = (lds[1][0] + lds[1][1]) + (lds[1][2] + lds[1][3]);
+    }
+}
+
+// L_sol = 1 - alpha^2 / (beta gamma) with alpha = b^T G^T G b, beta = u^T A u, gamma = b^T x*;
+// sc = [c1, c2] of dL/dG_ij = c1 y_i b_j^T + c2 (y_i p_j^T + w_i b_j^T); adds lam L_sol to the loss.
+__global__ void __launch_bounds__(256) k_lpc_sol_final(int nblk, const float* __restrict__ palpha,
+                                                       const float* __restrict__ pdots, float lam,
+                                                       float* __restrict__ scal, float* __restrict__ sc) {
+    __shared__ double lds[3][256];
+    double a = 0.0, bt = 0.0, gm = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += 256) {
+        a += palpha[k];
+        bt += pdots[2 * k];
+        gm += pdots[2 * k + 1];
+    }
+    lds[0][threadIdx.x] = a;
+    lds[1][threadIdx.x] = bt;
+    lds[2][threadIdx.x] = gm;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int q = 0; q < 3; q++) lds[q][threadIdx.x] += lds[q][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double al = lds[0][0], be = lds[1][0], ga = lds[2][0];
+        const bool ok = be > 0.0 && ga > 0.0;
+        const double L = ok ? 1.0 - al * al / (be * ga) : 0.0;
+        sc[0] = ok ? (float)(-4.0 * al / (be * ga)) : 0.0f;
+        sc[1] = ok ? (float)(2.0 * al * al / (be * be * ga)) : 0.0f;
+        sc[2] = (float)L;
+        scal[0] += lam * (float)L;
+    }
+}
+
+// Z_ij += lam (T / 2) dL_sol/dGhat_ij for the lower and diagonal slots of row i, where
+// dL_sol/dGhat_ij = (c1 y_i b_j^T + c2 (y_i p_j^T + w_i b_j^T)) S_j^T; the decoders' backward
+// multiplies Z by 2 / T.
+__global__ void __launch_bounds__(256) k_lpc_sol_grad(Topo t, const float4* __restrict__ y,
+                                                      const float4* __restrict__ w, const float* __restrict__ b,
+                                                      const float* __restrict__ p, const float* __restrict__ S,
+                                                      const float* __restrict__ scal, const float* __restrict__ sc,
+                                                      float lam, float* __restrict__ Z) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= t.nn) return;
+    const float f = lam * scal[3] * 0.5f;
+    const float c1 = sc[0] * f, c2 = sc[1] * f;
+    const float4 yi = y[i], wi = w[i];
+    const float ya[3] = {yi.x, yi.y, yi.z}, wa[3] = {wi.x, wi.y, wi.z};
+    const int li = i % 64;
+    const long off = t.sliceOff[i / 64];
+    const int kd = t.diagSlot[i];
+    for (int k = 0; k <= kd; k++) {
+        const long sg = (off + k) * 64 + li;
+        const int j = t.cols[sg];
+        const float bj[3] = {b[3 * j], b[3 * j + 1], b[3 * j + 2]};
+        const float pj[3] = {p[3 * j], p[3 * j + 1], p[3 * j + 2]};
+        float D[9];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) D[3 * a + c] = c1 * ya[a] * bj[c] + c2 * (ya[a] * pj[c] + wa[a] * bj[c]);
+        const float* Sj = S + 9 * (size_t)j;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int c = 0; c < 3; c++)  // (D S_j^T)_ac = sum_k D_ak S_j[c][k]
+                Z[sg * 9 + 3 * a + c] += D[3 * a] * Sj[3 * c] + D[3 * a + 1] * Sj[3 * c + 1] + D[3 * a + 2] * Sj[3 * c + 2];
     }
 }
 
@@ -1119,8 +1315,8 @@ void lpc_alloc(cs_ctx* C, int c) {
     L.mirror.as<int>(slots);
     CS_HIP(hipMemcpyAsync(L.mirror.p, mirror.data(), sizeof(int) * slots, hipMemcpyHostToDevice, st));
     L.S.as<float>(9 * (size_t)S.nn);
-    L.G.as<float>(9 * slots);
-    fill_f32(L.G.ptr<float>(), 0.0f, (long)(9 * slots), st);  // padding slots stay zero
+    L.G.as<bf16_t>(9 * slots);
+    fill_u32(L.G.ptr<uint32_t>(), 0u, (long)(9 * slots / 2), st);  // padding slots stay zero
     L.y.as<float4>(S.nn);
     for (DBuf* b : {&L.h, &L.P, &L.Q, &L.P2, &L.Q2}) b->as<float>(rows * HF);
     L.e.as<bf16_t>(slots * HF);
@@ -1178,7 +1374,7 @@ void lpc_forward(cs_ctx* C, int c, const Stash& st) {
         la.h = L.h.ptr<float>();
         la.ein = TRAIN ? st.e[l] : L.e.ptr<bf16_t>();
         la.eout = TRAIN ? st.e[l + 1] : L.e.ptr<bf16_t>();
-        la.G = L.G.ptr<float>();
+        la.G = L.G.ptr<bf16_t>();
         la.w = N.w.ptr<float>();
         la.wf = wf;
         la.st = st;
@@ -1204,19 +1400,86 @@ void lpc_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int n
     hipStream_t s = C->stream;
     const int g = std::min(nb, cdiv(S.nn, 256));
     hipLaunchKernelGGL(k_lpc_lower, dim3(g), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(), S.cols.ptr<int>(),
-                       S.diagSlot.ptr<int>(), L.G.ptr<float>(), r, L.y.ptr<float4>());
+                       S.diagSlot.ptr<int>(), L.G.ptr<bf16_t>(), r, L.y.ptr<float4>());
     // the r.z partials are summed over nb slots by the next kernel: write all of them
-    hipLaunchKernelGGL(k_lpc_upper, dim3(nb), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(), S.cols.ptr<int>(),
-                       S.diagSlot.ptr<int>(), S.rowLen.ptr<int>(), L.G.ptr<float>(), L.y.ptr<float4>(), r, z, partRZ);
+    hipLaunchKernelGGL(k_lpc_upper<UP_PCG>, dim3(nb), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(),
+                       S.cols.ptr<int>(), S.diagSlot.ptr<int>(), S.rowLen.ptr<int>(), L.G.ptr<bf16_t>(),
+                       L.y.ptr<float4>(), r, nullptr, nullptr, z, nullptr, partRZ);
     CS_LAUNCH_CHECK();
+}
+
+namespace {
+
+void launch_lower(cs_ctx* C, ClothSys& S, const float* r, int g) {
+    hipLaunchKernelGGL(k_lpc_lower, dim3(g), dim3(256), 0, C->stream, S.nn, S.sliceOff.ptr<int>(), S.cols.ptr<int>(),
+                       S.diagSlot.ptr<int>(), S.lpc.G.ptr<bf16_t>(), r, S.lpc.y.ptr<float4>());
+}
+
+}  // namespace
+
+// Damping of the learned smoother: omega = 1.3 / lambda_max(G^T G A), lambda from 12 power
+// iterations (an estimate from below, so omega lambda_max stays under 2 unless the estimate
+// is off by more than a third).
+void lpc_smoother_prepare(cs_ctx* C, int c) {
+    ClothSys& S = C->sys[c];
+    LpcSys& L = S.lpc;
+    hipStream_t s = C->stream;
+    const int g = cdiv(S.nn, 256);
+    float4* v = L.pv.as<float4>(S.nn);
+    float* w = L.pw.as<float>(3 * (size_t)S.nn);
+    float* part = L.ppart.as<float>(g);
+    L.omega.as<float>(1);
+    const int* cCol = S.nContact > 0 ? S.cCol.ptr<int>() : S.cRowPtr.ptr<int>();
+    const float* cVal = S.nContact > 0 ? S.cVal.ptr<float>() : S.vals.ptr<float>();
+    hipLaunchKernelGGL(k_lpc_pow_init, dim3(g), dim3(256), 0, s, S.nn, v);
+    for (int it = 0; it < 12; it++) {
+        hipLaunchKernelGGL(k_lpc_pow_spmv, dim3(g), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(),
+                           S.sliceW.ptr<int>(), S.cols.ptr<int>(), S.vals.ptr<float>(), S.cRowPtr.ptr<int>(), cCol,
+                           cVal, v, it ? part : nullptr, g, w);
+        launch_lower(C, S, w, g);
+        hipLaunchKernelGGL(k_lpc_upper<UP_POW>, dim3(g), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(),
+                           S.cols.ptr<int>(), S.diagSlot.ptr<int>(), S.rowLen.ptr<int>(), L.G.ptr<bf16_t>(),
+                           L.y.ptr<float4>(), nullptr, nullptr, nullptr, v, nullptr, part);
+    }
+    hipLaunchKernelGGL(k_lpc_omega, dim3(1), dim3(256), 0, s, part, g, 1.3f, L.omega.ptr<float>());
+    CS_LAUNCH_CHECK();
+}
+
+void lpc_smooth_pre(cs_ctx* C, int c, const float* r, float* x) {
+    ClothSys& S = C->sys[c];
+    LpcSys& L = S.lpc;
+    const int g = cdiv(S.nn, 256);
+    launch_lower(C, S, r, g);
+    hipLaunchKernelGGL(k_lpc_upper<UP_PRE>, dim3(g), dim3(256), 0, C->stream, S.nn, S.sliceOff.ptr<int>(),
+                       S.cols.ptr<int>(), S.diagSlot.ptr<int>(), S.rowLen.ptr<int>(), L.G.ptr<bf16_t>(),
+                       L.y.ptr<float4>(), nullptr, nullptr, L.omega.ptr<float>(), nullptr, x, nullptr);
+    CS_LAUNCH_CHECK();
+}
+
+void lpc_smooth_post(cs_ctx* C, int c, const float* res, const float* base, const float* r, float4* z,
+                     float* partRZ, int nb) {
+    ClothSys& S = C->sys[c];
+    LpcSys& L = S.lpc;
+    launch_lower(C, S, res, std::min(nb, cdiv(S.nn, 256)));
+    hipLaunchKernelGGL(k_lpc_upper<UP_POST>, dim3(nb), dim3(256), 0, C->stream, S.nn, S.sliceOff.ptr<int>(),
+                       S.cols.ptr<int>(), S.diagSlot.ptr<int>(), S.rowLen.ptr<int>(), L.G.ptr<bf16_t>(),
+                       L.y.ptr<float4>(), r, base, L.omega.ptr<float>(), z, nullptr, partRZ);
+    CS_LAUNCH_CHECK();
+}
+
+float lpc_smoother_omega(cs_ctx* C, int c) {
+    float w = 0.0f;
+    CS_HIP(hipMemcpyAsync(&w, C->sys[c].lpc.omega.p, sizeof(float), hipMemcpyDeviceToHost, C->stream));
+    CS_HIP(hipStreamSynchronize(C->stream));
+    return w;
 }
 
 void lpc_factor_blocks(cs_ctx* C, int c, float* diag, float* lower) {
     ClothSys& S = C->sys[c];
     LpcSys& L = S.lpc;
     const size_t slots = (size_t)S.totalSlots * SLICE;
-    std::vector<float> G(9 * slots);
-    CS_HIP(hipMemcpyAsync(G.data(), L.G.p, sizeof(float) * 9 * slots, hipMemcpyDeviceToHost, C->stream));
+    std::vector<bf16_t> G(9 * slots);
+    CS_HIP(hipMemcpyAsync(G.data(), L.G.p, sizeof(bf16_t) * 9 * slots, hipMemcpyDeviceToHost, C->stream));
     CS_HIP(hipStreamSynchronize(C->stream));
     long t = 0;
     for (int i = 0; i < S.nn; i++) {
@@ -1226,7 +1489,7 @@ void lpc_factor_blocks(cs_ctx* C, int c, float* diag, float* lower) {
             if (j > i) break;
             const size_t sig = ((size_t)S.hSliceOff[si] + k) * SLICE + li;
             float* out = j == i ? diag + 9 * (size_t)i : lower + 9 * t++;
-            for (int q = 0; q < 9; q++) out[q] = G[(sig >> 6) * 576 + q * 64 + (sig & 63)];
+            for (int q = 0; q < 9; q++) out[q] = bf_val(G[(sig >> 6) * 576 + q * 64 + (sig & 63)]);
         }
     }
 }
@@ -1285,6 +1548,31 @@ size_t carve_all(char* base, size_t slots, size_t rows, int nblk, Stash& st, Bac
     return c.used;
 }
 
+struct SolBufs {
+    float* bt;    // 3 per node row: b + bc
+    float4* u;    // G^T G b
+    float* p;     // A u
+    float4* y;    // G b
+    float4* w;    // G p
+    float* palpha;  // nblk partials of b.u
+    float* pdots;   // 2 nblk partials of u.p and b.x
+    float* sc;      // c1, c2, L_sol
+};
+
+size_t carve_sol(char* base, size_t rows, int nblk, SolBufs& sb) {
+    Carver c;
+    c.base = base;
+    sb.bt = c.take<float>(3 * rows);
+    sb.u = c.take<float4>(rows);
+    sb.p = c.take<float>(3 * rows);
+    sb.y = c.take<float4>(rows);
+    sb.w = c.take<float4>(rows);
+    sb.palpha = c.take<float>(nblk);
+    sb.pdots = c.take<float>(2 * (size_t)nblk);
+    sb.sc = c.take<float>(4);
+    return c.used;
+}
+
 template <int KB, int MB>
 void launch_wgrad(cs_ctx* C, const std::vector<WJob>& jobs, WJob* djobs, long maxRows) {
     if (jobs.empty()) return;
@@ -1295,7 +1583,7 @@ void launch_wgrad(cs_ctx* C, const std::vector<WJob>& jobs, WJob* djobs, long ma
 
 }  // namespace
 
-float lpc_train_step(cs_ctx* C, int c, float lr, float* gradOut) {
+float lpc_train_step(cs_ctx* C, int c, float lr, float solWeight, float* gradOut) {
     LpcNet& N = C->lpcNet;
     if (!N.loaded) throw StatusError(CS_E_STATE, "training needs initial weights (cs_set_learned_weights)");
     lpc_alloc(C, c);
@@ -1308,14 +1596,42 @@ float lpc_train_step(cs_ctx* C, int c, float lr, float* gradOut) {
     BackBufs bb{};
     double* part = nullptr;
     float* scal = nullptr;
-    const size_t need = carve_all(nullptr, slots, rows, nblk, st, bb, part, scal);
+    SolBufs sb{};
+    const size_t need1 = carve_all(nullptr, slots, rows, nblk, st, bb, part, scal);
+    const size_t need = need1 + carve_sol(nullptr, rows, nblk, sb);
     char* base = N.arena.as<char>(need);
     carve_all(base, slots, rows, nblk, st, bb, part, scal);
+    carve_sol(base + need1, rows, nblk, sb);
     lpc_forward<true>(C, c, st);
     Topo t = topo_of(S);
     LossArgs la{t, st.gh, st.Ah, st.o, bb.Z, part};
     hipLaunchKernelGGL(k_lpc_loss, dim3(nblk), dim3(256), 0, s, la);
     hipLaunchKernelGGL(k_lpc_loss_final, dim3(1), dim3(256), 0, s, nblk, 3 * S.nn, part, scal);
+    if (S.solved && solWeight > 0.0f) {
+        // the solution term: how well M^-1 b points at the solution in the energy norm
+        const float lam = solWeight;
+        LpcSys& L = S.lpc;
+        const int* cCol = S.nContact > 0 ? S.cCol.ptr<int>() : S.cRowPtr.ptr<int>();
+        const float* cVal = S.nContact > 0 ? S.cVal.ptr<float>() : S.vals.ptr<float>();
+        hipLaunchKernelGGL(k_lpc_rhs, dim3(cdiv(3 * S.nn, 256)), dim3(256), 0, s, 3 * S.nn, S.b.ptr<float>(),
+                           S.bc.ptr<float>(), sb.bt);
+        hipLaunchKernelGGL(k_lpc_lower, dim3(nblk), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(), S.cols.ptr<int>(),
+                           S.diagSlot.ptr<int>(), L.G.ptr<bf16_t>(), sb.bt, sb.y);
+        hipLaunchKernelGGL(k_lpc_upper<UP_PCG>, dim3(nblk), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(),
+                           S.cols.ptr<int>(), S.diagSlot.ptr<int>(), S.rowLen.ptr<int>(), L.G.ptr<bf16_t>(), sb.y,
+                           sb.bt, nullptr, nullptr, sb.u, nullptr, sb.palpha);
+        hipLaunchKernelGGL(k_lpc_pow_spmv, dim3(nblk), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(),
+                           S.sliceW.ptr<int>(), S.cols.ptr<int>(), S.vals.ptr<float>(), S.cRowPtr.ptr<int>(), cCol,
+                           cVal, sb.u, nullptr, 0, sb.p);
+        hipLaunchKernelGGL(k_lpc_lower, dim3(nblk), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(), S.cols.ptr<int>(),
+                           S.diagSlot.ptr<int>(), L.G.ptr<bf16_t>(), sb.p, sb.w);
+        hipLaunchKernelGGL(k_lpc_sol_dots, dim3(nblk), dim3(256), 0, s, S.nn, sb.u, sb.p, sb.bt, S.x.ptr<float>(),
+                           sb.pdots);
+        hipLaunchKernelGGL(k_lpc_sol_final, dim3(1), dim3(256), 0, s, nblk, sb.palpha, sb.pdots, lam, scal, sb.sc);
+        hipLaunchKernelGGL(k_lpc_sol_grad, dim3(nblk), dim3(256), 0, s, t, sb.y, sb.w, sb.bt, sb.p, L.S.ptr<float>(),
+                           scal, sb.sc, lam, bb.Z);
+        CS_LAUNCH_CHECK();
+    }
     N.grad.as<float>(W_COUNT);
     fill_f32(N.grad.ptr<float>(), 0.0f, W_COUNT, s);
     BackArgs ba;

@@ -358,10 +358,21 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     float4* z = S.z.ptr<float4>();
     float4* dgen[2] = {S.d.ptr<float4>(), S.d.ptr<float4>() + nn};
     int levels[16];
+    const bool amgl = C->P.precond == CS_PRECOND_AMG_LEARNED && amg_levels(C, c, levels, 16) > 1;
     const bool amg = C->P.precond == CS_PRECOND_AMG && amg_levels(C, c, levels, 16) > 1;
     const bool learned = C->P.precond == CS_PRECOND_LEARNED;
     float* xs = nullptr;  // level-0 pre-smoothing iterate of the multigrid cycle
-    if (amg) {
+    if (amgl) {
+        amg_prepare(C, c);
+        lpc_prepare(C, c);
+        lpc_smoother_prepare(C, c);
+        xs = amg_level0_x(C, c);
+        hipLaunchKernelGGL(k_pcg_init<PC_EXTERNAL>, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(),
+                           S.bc.ptr<float>(), S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, xs, part);
+        CS_LAUNCH_CHECK();
+        lpc_smooth_pre(C, c, S.r.ptr<float>(), xs);
+        amg_apply(C, c, S.r.ptr<float>(), z, part + P_RZ0 * PMAX, nb, true, true);
+    } else if (amg) {
         amg_prepare(C, c);
         xs = amg_level0_x(C, c);
         hipLaunchKernelGGL(k_pcg_init<PC_SMOOTH>, dim3(nb), dim3(PB), 0, s, nn, S.b.ptr<float>(), S.bc.ptr<float>(),
@@ -382,7 +393,7 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     hipLaunchKernelGGL(k_pcg_setup, dim3(1), dim3(PB), 0, s, nb, part, sc, tol * tol, maxit);
     CS_LAUNCH_CHECK();
     // iterations per host poll: a multigrid iteration costs several times a Jacobi one
-    const int chunk = amg ? 4 : learned ? 8 : 16;
+    const int chunk = (amg || amgl) ? 4 : learned ? 8 : 16;
     int done = 0;
     PcgDev h{};
     int it = 0;
@@ -402,7 +413,12 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
                                    S.sliceW.ptr<int>(), S.cols.ptr<int>(), S.vals.ptr<float>(), cRowPtr, cCol, cVal,
                                    z, dgen[1 - (it & 1)], dcur, S.q.ptr<float>(), part, sc);
             ktimer_end(C, tok);
-            if (amg) {
+            if (amgl) {
+                hipLaunchKernelGGL(k_pcg_update<PC_EXTERNAL>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
+                                   S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
+                lpc_smooth_pre(C, c, S.r.ptr<float>(), xs);
+                amg_apply(C, c, S.r.ptr<float>(), z, part + (P_RZ0 + (1 - (it & 1))) * PMAX, nb, true, true);
+            } else if (amg) {
                 hipLaunchKernelGGL(k_pcg_update<PC_SMOOTH>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
                                    S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
                 amg_apply(C, c, S.r.ptr<float>(), z, part + (P_RZ0 + (1 - (it & 1))) * PMAX, nb, true);
@@ -425,6 +441,7 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     if (!done) throw StatusError(CS_E_HIP, "PCG: the device never reported its stop");
     if (relres) *relres = h.bb > 0.0f ? sqrtf(h.rr / h.bb) : 0.0f;
     if (converged) *converged = (h.rr <= h.tol2bb);
+    S.solved = h.rr <= h.tol2bb;  // S.x solves the assembled system
     return h.iters;
 }
 

@@ -9,8 +9,10 @@ its own ground patch, with a grid side drawn from {33, 65, 129}, the stretch
 stiffness scaled by U[0.5, 2] and dt drawn from U[2, 10] ms.  The round steps
 the scene (block-Jacobi PCG) and, after every step, takes one Adam step per
 cloth on the Kaporin loss of that cloth's assembled system (cs_lpc_train_step:
-the network's forward and backward kernels on the device).  Weights carry over
-from round to round; the optimiser restarts with each round.
+the network's forward and backward kernels on the device), optionally plus the
+solution term (--sol-weight).  Weights carry over from round to round; the
+optimiser restarts with each round.  The packaged weights come from
+`--minutes 6 --seed 0` (Kaporin loss only).
 """
 import argparse
 import math
@@ -82,6 +84,8 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--slots", type=int, default=6)
     ap.add_argument("--steps-per-round", type=int, default=40)
+    ap.add_argument("--sol-weight", type=float, default=0.0,
+                    help="weight of the solution term (1 - cos^2_A between M^-1 b and the step's solution)")
     a = ap.parse_args(argv)
     rng = np.random.default_rng(a.seed)
     w = np.load(a.init, allow_pickle=False).astype(np.float32) if a.init else init_weights(a.seed)
@@ -99,7 +103,7 @@ def main(argv=None):
             frac = min(1.0, (time.time() - t0) / budget)
             lr = a.lr * 0.5 * (1.0 + math.cos(math.pi * frac))
             for c in range(len(spec.cloths)):
-                losses.append(sim.lpc_train_step(c, lr))
+                losses.append(sim.lpc_train_step(c, lr, sol_weight=a.sol_weight))
                 total += 1
             if time.time() - t0 >= budget:
                 break

@@ -44,12 +44,15 @@ typedef enum cs_status {
 typedef struct cs_ctx cs_ctx; /* opaque context: device buffers + one HIP stream */
 
 /* PCG preconditioners.  BLOCK_JACOBI: inverse 3x3 diagonal blocks (the reference uses the
- * scalar diagonal, src/Cloth.cu:766-809).  AMG: one V(1,1) cycle of aggregation multigrid.
- * LEARNED: M^-1 = G^T G with G emitted per solve by a message-passing network from the
- * assembled system (weights set with cs_set_learned_weights; CS_E_STATE without them).
- * AMG and LEARNED have no counterpart in the reference; they change the iteration count,
- * not the system solved. */
-enum { CS_PRECOND_BLOCK_JACOBI = 0, CS_PRECOND_LEARNED = 1, CS_PRECOND_AMG = 2 };
+ * scalar diagonal, src/Cloth.cu:766-809).  AMG: one V(1,1) cycle of aggregation multigrid
+ * with damped block-Jacobi smoothing.  LEARNED: M^-1 = G^T G with G emitted per solve by a
+ * message-passing network from the assembled system (weights set with
+ * cs_set_learned_weights; CS_E_STATE without them).  AMG_LEARNED: the AMG cycle with the
+ * learned G^T G, damped by a power-iteration estimate of its spectrum, as the finest level's
+ * smoother (cloths too small for a hierarchy fall back to block Jacobi).  None but
+ * BLOCK_JACOBI has a counterpart in the reference; they change the iteration count, not the
+ * system solved. */
+enum { CS_PRECOND_BLOCK_JACOBI = 0, CS_PRECOND_LEARNED = 1, CS_PRECOND_AMG = 2, CS_PRECOND_AMG_LEARNED = 3 };
 enum { CS_MESH_CLOTH = 0, CS_MESH_OBSTACLE = 1 };
 
 /* cs_params.flags */
@@ -190,10 +193,10 @@ cs_status cs_assemble(cs_ctx* ctx, int cloth_id, int* row_ptr, int* col_idx, flo
                       int* nnzb_out, float* b);
 /* Solve the last assembled system of one cloth (src/Cloth.cu:766-809): dv (3n). */
 cs_status cs_solve(cs_ctx* ctx, int cloth_id, float* dv, cs_stats* stats);
-/* The CS_PRECOND_AMG or CS_PRECOND_LEARNED preconditioner of the last assembled system (no
- * counterpart in the reference, whose PCG uses the Jacobi diagonal): for AMG the level sizes of
- * the hierarchy (n_levels = 0 when the cloth cannot use it; always 0 for LEARNED) and, when
- * r != NULL, z = M^-1 r (3n each).  CS_E_ARG for the other preconditioners. */
+/* The CS_PRECOND_AMG, CS_PRECOND_LEARNED or CS_PRECOND_AMG_LEARNED preconditioner of the last
+ * assembled system (no counterpart in the reference, whose PCG uses the Jacobi diagonal): the
+ * level sizes of the multigrid hierarchy (n_levels = 0 when the cloth cannot use it; always 0
+ * for LEARNED) and, when r != NULL, z = M^-1 r (3n each).  CS_E_ARG for BLOCK_JACOBI. */
 cs_status cs_precondition(cs_ctx* ctx, int cloth_id, const float* r, float* z, int* level_sizes, int cap,
                           int* n_levels);
 /* ---- learned preconditioner (replaces the preconditioner step z = M^-1 r of the reference's
@@ -209,9 +212,14 @@ cs_status cs_get_learned_weights(cs_ctx* ctx, float* w, int n);
  * buffers query n_lower only. */
 cs_status cs_lpc_factor(cs_ctx* ctx, int cloth_id, float* diag, float* lower, int cap_lower, int* n_lower);
 /* One training step on one cloth's last assembled system: the Kaporin loss of the network's
- * factor, its gradient (grad, cs_lpc_weight_count() floats, may be NULL) and, when lr > 0,
- * one Adam step (beta 0.9 / 0.999) on the context's weights. */
-cs_status cs_lpc_train_step(cs_ctx* ctx, int cloth_id, float lr, float* loss, float* grad);
+ * factor (plus sol_weight (1 - cos^2) of the energy-norm angle between M^-1 b and the solution,
+ * when the system has been solved by cs_solve or a step), its gradient (grad,
+ * cs_lpc_weight_count() floats, may be NULL) and, when lr > 0, one Adam step (beta 0.9 / 0.999)
+ * on the context's weights. */
+cs_status cs_lpc_train_step(cs_ctx* ctx, int cloth_id, float lr, float sol_weight, float* loss, float* grad);
+/* The damping w of the learned smoother (CS_PRECOND_AMG_LEARNED) of one cloth's last assembled
+ * system: x += w G^T G (b - A x). */
+cs_status cs_lpc_smoother_omega(cs_ctx* ctx, int cloth_id, float* omega);
 
 /* Simulator::findImpacts (src/Simulator.cu:383-392), deduplicated and sorted by (t, type, nodes). */
 cs_status cs_find_impacts(cs_ctx* ctx, float thickness, cs_impact_rec* out, int cap, int* n_out);

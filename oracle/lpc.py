@@ -25,7 +25,9 @@ Definitions (every array float64; blocks are 3x3):
   Ghat_ii = [[exp o0, 0, 0], [o3, exp o1, 0], [o4, o5, exp o2]], Ghat_ij = q_ij[0:9] (3x3 row-major)
   G_ij = Ghat_ij S_j;  M^-1 = G^T G  (Ghat = I gives block Jacobi)
   loss = log(T / n) - (2 / n) sum_i (o_i0 + o_i1 + o_i2),  T = trace(Ghat Ahat Ghat^T), n = 3 N
-  (Kaporin's condition number of Ghat Ahat Ghat^T, up to the constant log det Ahat / n).
+  (Kaporin's condition number of Ghat Ahat Ghat^T, up to the constant log det Ahat / n),
+  plus, when the system's solution x* (A x* = b) is given, lam (1 - cos^2_A(x*, M^-1 b)):
+  how far the first preconditioned direction is from the solution, in the energy norm.
 
 With bf16=True the forward rounds the matrix-product operands (weights and
 activations) and the stored edge states to bfloat16 (round to nearest even),
@@ -253,6 +255,61 @@ def kaporin(g, o, q, Ah):
     return loss, T, do, dq
 
 
+def _csr(g, blocks):
+    import scipy.sparse as sp
+    return sp.bsr_matrix((blocks, g.ci, g.rp), shape=(3 * g.n, 3 * g.n)).tocsr()
+
+
+def solution_term(g, o, q, S, b, x):
+    """L_sol = 1 - (b^T M b)^2 / ((u^T A u)(b^T x)), u = M b, M = G^T G, and its gradients
+    d/do, d/dq through Ghat (G_ij = Ghat_ij S_j)."""
+    Gd, Gl = factor(g, o, q, S)
+    A = _csr(g, g.vals)
+    b = np.asarray(b, dtype=np.float64).reshape(-1, 3)
+    x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
+    i_l, j_l = g.src[g.low], g.dst[g.low]
+
+    def Gmul(v):
+        y = np.einsum("nab,nb->na", Gd, v)
+        np.add.at(y, i_l, np.einsum("eab,eb->ea", Gl, v[j_l]))
+        return y
+
+    def GTmul(y):
+        z = np.einsum("nba,nb->na", Gd, y)
+        np.add.at(z, j_l, np.einsum("eba,eb->ea", Gl, y[i_l]))
+        return z
+
+    y = Gmul(b)
+    u = GTmul(y)
+    p = (A @ u.ravel()).reshape(-1, 3)
+    w = Gmul(p)
+    alpha = float((y * y).sum())
+    beta = float((u * p).sum())
+    gamma = float((b * x).sum())
+    L = 1.0 - alpha * alpha / (beta * gamma)
+    # dL = -(2 alpha / (beta gamma)) d alpha + (alpha^2 / (beta^2 gamma)) d beta with
+    # d alpha / dG_ij = 2 y_i b_j^T and d beta / dG_ij = 2 (y_i p_j^T + w_i b_j^T)
+    c1 = -4.0 * alpha / (beta * gamma)
+    c2 = 2.0 * alpha * alpha / (beta * beta * gamma)
+
+    def dG(i, j):
+        return c1 * np.outer(y[i], b[j]) + c2 * (np.outer(y[i], p[j]) + np.outer(w[i], b[j]))
+    # chain to Ghat: G_ij = Ghat_ij S_j  ->  dGhat_ij = dG_ij S_j^T
+    do = np.zeros_like(o)
+    dq = np.zeros_like(q)
+    for i in range(g.n):
+        D = dG(i, i) @ S[i].T
+        do[i, 0] = D[0, 0] * np.exp(o[i, 0])
+        do[i, 1] = D[1, 1] * np.exp(o[i, 1])
+        do[i, 2] = D[2, 2] * np.exp(o[i, 2])
+        do[i, 3] = D[1, 0]
+        do[i, 4] = D[2, 0]
+        do[i, 5] = D[2, 1]
+    for t in range(len(g.low)):
+        dq[t, :9] = (dG(i_l[t], j_l[t]) @ S[j_l[t]].T).reshape(9)
+    return L, do, dq
+
+
 def backward(flat, g, c, do, dq):
     """Gradient of the loss w.r.t. the flat weight vector, given d loss / d o and d loss / d q
     (the same rounding points as the forward when it ran with bf16, treated as identity)."""
@@ -309,7 +366,12 @@ def backward(flat, g, c, do, dq):
     return pack(gr)
 
 
-def loss_and_grad(flat, g, bf=False):
+def loss_and_grad(flat, g, bf=False, b=None, x=None, lam=1.0):
     o, q, c = forward(flat, g, bf)
     loss, T, do, dq = kaporin(g, o, q, c["Ah"])
+    if b is not None:
+        ls, dos, dqs = solution_term(g, o, q, c["S"], b, x)
+        loss = loss + lam * ls
+        do = do + lam * dos
+        dq = dq + lam * dqs
     return loss, backward(flat, g, c, do, dq)

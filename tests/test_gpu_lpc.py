@@ -11,8 +11,8 @@ import pytest
 import oracle
 from oracle import lpc
 from clothsimulator_amd import CS_MESH_CLOTH, LPC_WEIGHTS, Simulator, default_learned_weights, scenes
-from clothsimulator_amd._cabi import (CS_FLAG_NO_PROXIMITY, CS_PRECOND_BLOCK_JACOBI, CS_PRECOND_LEARNED,
-                                      material_struct)
+from clothsimulator_amd._cabi import (CS_FLAG_NO_PROXIMITY, CS_PRECOND_AMG, CS_PRECOND_AMG_LEARNED,
+                                      CS_PRECOND_BLOCK_JACOBI, CS_PRECOND_LEARNED, material_struct)
 
 pytestmark = pytest.mark.gpu
 
@@ -67,14 +67,16 @@ def test_factor_matches_oracle():
 
 
 def test_untrained_network_is_block_jacobi():
-    """Zero output layers: Ghat = I, so G^T G = Minv exactly as block Jacobi."""
-    W = lpc.unpack(lpc.init_weights(3))
-    sim, g, _ = system(weights=lpc.pack(W).astype(np.float32))
-    rng = np.random.default_rng(4)
-    r = rng.normal(0, 1, (g.n, 3)).astype(np.float32)
-    _, z = sim.precondition(0, r)
-    zj = np.einsum("nab,nb->na", g.Minv, r.astype(np.float64))
-    assert np.abs(z - zj).max() <= 1e-4 * np.abs(zj).max()
+    """Zero output layers: Ghat = I, so G = blockdiag(S) (stored in bfloat16) and G^T G = Minv
+    up to that rounding: block Jacobi."""
+    W = lpc.init_weights(3)
+    sim, g, _ = system(weights=W.astype(np.float32))
+    diag, lower = sim.lpc_factor(0)
+    assert np.all(lower == 0)
+    S = np.transpose(np.linalg.cholesky(g.Minv), (0, 2, 1))
+    assert np.abs(diag - S).max() <= 8e-3 * np.abs(S).max()   # within one bfloat16 step
+    GtG = np.einsum("nba,nbc->nac", diag.astype(np.float64), diag.astype(np.float64))
+    assert np.abs(GtG - g.Minv).max() <= 2e-2 * np.abs(g.Minv).max()
 
 
 def test_preconditioner_apply_and_symmetry():
@@ -92,10 +94,16 @@ def test_preconditioner_apply_and_symmetry():
     assert float((r1 * z1).sum()) > 0
 
 
-def test_loss_and_gradient_match_oracle():
+@pytest.mark.parametrize("solved", [False, True])
+def test_loss_and_gradient_match_oracle(solved):
+    """The Kaporin loss alone (assembled system) and with the solution term (after a solve)."""
     sim, g, w = system(seed=1)
-    loss, grad = sim.lpc_train_step(0, lr=0.0, grad=True)
-    lo, go = lpc.loss_and_grad(w, g, bf=True)
+    if solved:
+        b = sim.assemble(0)[3]   # the same state: the same system
+        dv, st = sim.solve(0)
+        assert st.pcg_converged
+    loss, grad = sim.lpc_train_step(0, lr=0.0, grad=True, sol_weight=1.0)
+    lo, go = lpc.loss_and_grad(w, g, bf=True, b=b if solved else None, x=dv if solved else None)
     assert abs(loss - lo) <= 1e-2 * abs(lo) + 1e-4, (loss, lo)
     rel = np.linalg.norm(grad - go) / np.linalg.norm(go)
     assert rel < 5e-2, rel
@@ -176,14 +184,142 @@ def test_device_training_lowers_loss_and_iterations():
     assert lrn < 0.85 * jac, (lrn, jac)
 
 
+def stepped_state(name, steps):
+    """A simulated state of a scene (block-Jacobi steps from its initial state)."""
+    spec = scenes.by_name(name)
+    spec.params.precond = CS_PRECOND_BLOCK_JACOBI
+    sim = Simulator(spec)
+    for _ in range(steps):
+        sim.step()
+    x, x0, v = sim.state(CS_MESH_CLOTH, 0)
+    sim.close()
+    return x, v
+
+
 @pytest.mark.skipif(not os.path.exists(LPC_WEIGHTS), reason="no packaged weights")
-def test_packaged_weights_gain_over_block_jacobi():
-    """The shipped network on held-out systems: fewer PCG iterations than block Jacobi (the
-    method without it) at the same 1e-5 relative residual."""
+@pytest.mark.parametrize("name,steps", [("crumple97", 15), ("drape97", 60)])
+def test_packaged_weights_gain_over_block_jacobi(name, steps):
+    """The shipped network on held-out systems (grid sides not in the training set, at
+    simulated states: a crumple falling, a drape in contact with its sphere): fewer PCG
+    iterations than block Jacobi (the method without it) at the same 1e-5 relative residual.
+    (A flat sheet in free fall, whose normal direction decouples, is the known case where it
+    loses to block Jacobi.)"""
     w = default_learned_weights()
-    for name, seed in (("crumple65", 11), ("drape65", 12)):
-        spec = scenes.by_name(name)
-        x, v = perturbed(spec, seed=seed, amp=0.003, vel=0.05)
-        jac = iterations(scenes.by_name(name), CS_PRECOND_BLOCK_JACOBI, x, v)
-        lrn = iterations(scenes.by_name(name), CS_PRECOND_LEARNED, x, v, w)
-        assert lrn < 0.7 * jac, (name, lrn, jac)
+    x, v = stepped_state(name, steps)
+    jac = iterations(scenes.by_name(name), CS_PRECOND_BLOCK_JACOBI, x, v)
+    lrn = iterations(scenes.by_name(name), CS_PRECOND_LEARNED, x, v, w)
+    assert lrn < 0.75 * jac, (name, lrn, jac)
+
+
+# ---------------------------------------------------------------- multigrid with the learned smoother
+
+def dense_blocks(rp, ci, vals, n):
+    A = np.zeros((3 * n, 3 * n))
+    row = np.repeat(np.arange(n), np.diff(rp))
+    for k, (i, j) in enumerate(zip(row, ci)):
+        A[3 * i:3 * i + 3, 3 * j:3 * j + 3] = vals[k]
+    return A
+
+
+def amgl_system(name="hang33", seed=0, weights=None):
+    spec = scenes.by_name(name)
+    spec.params.precond = CS_PRECOND_AMG_LEARNED
+    spec.params.flags |= CS_FLAG_NO_PROXIMITY
+    sim = Simulator(spec)
+    sim.set_learned_weights(rand_weights(seed=5, out_scale=0.05) if weights is None else weights)
+    x, v = perturbed(spec, seed)
+    sim.set_state(CS_MESH_CLOTH, 0, x, x, v)
+    rp, ci, vals, b = sim.assemble(0)
+    return spec, sim, rp, ci, vals.reshape(-1, 3, 3).astype(np.float64)
+
+
+def test_amg_learned_cycle_matches_dense_restatement():
+    """One V(1,1) cycle whose finest smoother is w G^T G (G and w read back from the device),
+    coarse levels damped block Jacobi on Galerkin operators, against dense algebra."""
+    from test_oracle_amg import OMEGA, block_diag_inv, hierarchy
+    spec, sim, rp, ci, vals = amgl_system()
+    n = len(rp) - 1
+    A = dense_blocks(rp, ci, vals, n)
+    diag, lower = sim.lpc_factor(0)
+    w = sim.smoother_omega(0)
+    row = np.repeat(np.arange(n), np.diff(rp))
+    Gm = np.zeros((3 * n, 3 * n))
+    for i in range(n):
+        Gm[3 * i:3 * i + 3, 3 * i:3 * i + 3] = diag[i]
+    low = [(i, j) for i, j in zip(row, ci) if j < i]
+    for t, (i, j) in enumerate(low):
+        Gm[3 * i:3 * i + 3, 3 * j:3 * j + 3] = lower[t]
+    S0 = w * Gm.T @ Gm
+    # the damping keeps the smoother convergent: w lambda_max(G^T G A) < 2
+    lam = np.max(np.linalg.eigvals(Gm.T @ Gm @ A).real)
+    assert 0.5 < w * lam < 2.0, (w, lam)
+    sizes, aggs = hierarchy(n, spec.cloths[0].mesh.faces)
+    assert sim.precondition(0)[0] == sizes
+    Ps = []
+    for agg in aggs:
+        P1 = np.zeros((len(agg), agg.max() + 1))
+        P1[np.arange(len(agg)), agg] = 1.0
+        Ps.append(np.kron(P1, np.eye(3)))
+    As = [A]
+    for P in Ps:
+        As.append(P.T @ As[-1] @ P)
+
+    def op(l):
+        Al = As[l]
+        if l == len(As) - 1:
+            return np.linalg.inv(Al)
+        Sm = S0 if l == 0 else OMEGA * block_diag_inv(Al)
+        I = np.eye(Al.shape[0])
+        Cc = Ps[l] @ op(l + 1) @ Ps[l].T
+        pre = Sm + Cc @ (I - Al @ Sm)
+        return pre + Sm @ (I - Al @ pre)
+    M = op(0)
+    rng = np.random.default_rng(6)
+    for trial in range(2):
+        r = rng.normal(0, 1, (n, 3)).astype(np.float32)
+        _, zg = sim.precondition(0, r)
+        zd = (M @ r.astype(np.float64).ravel()).reshape(-1, 3)
+        err = np.abs(zg - zd).max() / np.abs(zd).max()
+        assert err < 2e-3, (trial, err)
+
+
+def test_amg_learned_is_spd():
+    spec, sim, rp, ci, vals = amgl_system(seed=2)
+    n = len(rp) - 1
+    rng = np.random.default_rng(7)
+    r1 = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    r2 = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    _, z1 = sim.precondition(0, r1)
+    _, z2 = sim.precondition(0, r2)
+    a, b = float((r2 * z1).sum()), float((r1 * z2).sum())
+    assert abs(a - b) <= 1e-4 * max(abs(a), abs(b))
+    assert float((r1 * z1).sum()) > 0 and float((r2 * z2).sum()) > 0
+
+
+@pytest.mark.parametrize("name", ["hang33", "drape33"])
+def test_amg_learned_steps_match_direct_solve(name):
+    spec = scenes.by_name(name)
+    spec.params.precond = CS_PRECOND_AMG_LEARNED
+    sim = Simulator(spec)
+    sim.set_learned_weights(rand_weights(seed=3, out_scale=0.02))
+    o = oracle.OracleScene(scenes.by_name(name), fp64=True)
+    for k in range(3):
+        x, x0, v = o.state(fp64=False)
+        sim.set_state(CS_MESH_CLOTH, 0, x, x0, v)
+        s, st = sim.physics_step()
+        assert s == 0, s
+        o.physics_step(direct=True)
+        xg, _, _ = sim.state(CS_MESH_CLOTH, 0)
+        xo, _, _ = o.state(fp64=False)
+        assert np.abs(xg - xo).max() <= 1e-5, (k, np.abs(xg - xo).max())
+
+
+@pytest.mark.skipif(not os.path.exists(LPC_WEIGHTS), reason="no packaged weights")
+def test_learned_smoother_gains_over_jacobi_smoother():
+    """The shipped network as the finest smoother of the multigrid cycle: fewer PCG iterations
+    than the same cycle with damped block-Jacobi smoothing, on a held-out falling crumple."""
+    w = default_learned_weights()
+    x, v = stepped_state("crumple97", 15)
+    amg = iterations(scenes.by_name("crumple97"), CS_PRECOND_AMG, x, v)
+    amgl = iterations(scenes.by_name("crumple97"), CS_PRECOND_AMG_LEARNED, x, v, w)
+    assert amgl < amg, (amgl, amg)

@@ -163,3 +163,29 @@ def test_package_weight_layout_matches_oracle():
     from clothsimulator_amd import lpc_layout
     assert lpc_layout.PARAMS == lpc.PARAMS and lpc_layout.COUNT == lpc.COUNT
     assert np.array_equal(lpc_layout.init_weights(3), lpc.init_weights(3).astype(np.float32))
+
+
+def test_solution_term_gradient_and_zero_at_eigenvectors():
+    g, vals, rp, ci = small_system(seed=11)
+    A = dense(rp, ci, vals, g.n)
+    rng = np.random.default_rng(1)
+    xs = rng.normal(0, 1, 3 * g.n)
+    b = A @ xs
+    w = weights_with_outputs(seed=3, scale=0.1)
+    loss, grad = lpc.loss_and_grad(w, g, b=b, x=xs)
+    assert loss > lpc.loss_and_grad(w, g)[0]   # the term is 1 - cos^2 >= 0
+    for t in rng.choice(lpc.COUNT, 25, replace=False):
+        wp, wm = w.copy(), w.copy()
+        wp[t] += 1e-6
+        wm[t] -= 1e-6
+        fd = (lpc.loss_and_grad(wp, g, b=b, x=xs)[0] - lpc.loss_and_grad(wm, g, b=b, x=xs)[0]) / 2e-6
+        assert abs(fd - grad[t]) <= 1e-6 * max(1.0, abs(fd)) + 2e-9, (t, fd, grad[t])
+    # untrained network (M = block Jacobi): an eigenvector x of M A gives M b = lambda x, cos = 1
+    o, q, c = lpc.forward(lpc.init_weights(2), g)
+    Minv = np.zeros_like(A)
+    for i in range(g.n):
+        Minv[3 * i:3 * i + 3, 3 * i:3 * i + 3] = g.Minv[i]
+    lam, V = np.linalg.eig(Minv @ A)
+    x = V[:, np.argmax(lam.real)].real
+    L, _, _ = lpc.solution_term(g, o, q, c["S"], A @ x, x)
+    assert abs(L) < 1e-10
