@@ -124,6 +124,32 @@ __global__ void __launch_bounds__(AB) k_amg_galerkin(int nnzb, const int* __rest
     for (int c = 0; c < 9; c++) out[(size_t)k * 9 + c] = s[c];
 }
 
+// Level-1 diagonal blocks += the contact diagonal blocks of their level-0 nodes (in
+// ascending node order): obstacle contacts and each node's share of a self-contact stiffen
+// its aggregate, so the coarse correction does not move what the contacts hold.
+__global__ void __launch_bounds__(AB) k_amg_contact_diag(int n1, const int* __restrict__ mptr,
+                                                         const int* __restrict__ midx, const int* __restrict__ diag1,
+                                                         const int* __restrict__ cDiag, const float* __restrict__ cVal,
+                                                         float* __restrict__ val1) {
+    const int a = blockIdx.x * AB + threadIdx.x;
+    if (a >= n1) return;
+    float s[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) s[c] = 0.0f;
+    bool any = false;
+    for (int t = mptr[a]; t < mptr[a + 1]; t++) {
+        const int k = cDiag[midx[t]];
+        if (k < 0) continue;
+        any = true;
+#pragma unroll
+        for (int c = 0; c < 9; c++) s[c] += cVal[(size_t)k * 9 + c];
+    }
+    if (!any) return;
+    float* d = val1 + (size_t)diag1[a] * 9;
+#pragma unroll
+    for (int c = 0; c < 9; c++) d[c] += s[c];
+}
+
 __global__ void __launch_bounds__(AB) k_amg_dinv(int n, const int* __restrict__ diag, const float* __restrict__ val,
                                                  float* __restrict__ Dinv) {
     const int i = blockIdx.x * AB + threadIdx.x;
@@ -719,6 +745,10 @@ void amg_prepare(cs_ctx* C, int c) {
         else
             hipLaunchKernelGGL(k_amg_galerkin<false>, dim3(cdiv(L.nnzb, AB)), dim3(AB), 0, s, L.nnzb,
                                L.cptr.ptr<int>(), L.cidx.ptr<int>(), G.L[l - 1].val.ptr<float>(), L.val.ptr<float>());
+        if (l == 1 && S.nContact > 0)
+            hipLaunchKernelGGL(k_amg_contact_diag, dim3(cdiv(L.n, AB)), dim3(AB), 0, s, L.n, G.L[0].mptr.ptr<int>(),
+                               G.L[0].midx.ptr<int>(), L.diag.ptr<int>(), S.cDiag.ptr<int>(), S.cVal.ptr<float>(),
+                               L.val.ptr<float>());
         CS_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_amg_dinv, dim3(cdiv(L.n, AB)), dim3(AB), 0, s, L.n, L.diag.ptr<int>(), L.val.ptr<float>(),
                            L.Dinv.ptr<float>());

@@ -190,11 +190,23 @@ void amg_setup(const AmgTopo& topo, const vector<int>& rp, const vector<int>& ci
     L0.ci = ci;
     L0.val.assign(val.begin(), val.end());
     amg_dinv(L0);
+    // coarse operators: Galerkin products of the static system plus the contact blocks'
+    // diagonal part (a contact with an obstacle, or a node's own share of a self-contact, stiffens
+    // its aggregate; contact couplings between distant nodes are left out of the coarse levels)
     AmgLevel stat;
     stat.n = L0.n;
     stat.rp = srp;
     stat.ci = sci;
     stat.val.assign(sval.begin(), sval.end());
+    for (int i = 0; i < L0.n; i++) {
+        int ds = -1, df = -1;
+        for (int k = srp[i]; k < srp[i + 1]; k++)
+            if (sci[k] == i) ds = k;
+        for (int k = rp[i]; k < rp[i + 1]; k++)
+            if (ci[k] == i) df = k;
+        if (ds >= 0 && df >= 0)
+            for (int c = 0; c < 9; c++) stat.val[9 * (size_t)ds + c] = val[9 * (size_t)df + c];
+    }
     const AmgLevel* src = &stat;
     for (int l = 1; l < nl; l++) {
         amg_galerkin(*src, topo.agg[l - 1], topo.sizes[l], M.L[l]);

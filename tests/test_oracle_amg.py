@@ -133,3 +133,38 @@ def test_oracle_amg_hierarchy_of_a_larger_sheet():
     o = oracle.OracleScene(spec, fp64=True)
     sizes, _ = hierarchy(65 * 65, spec.cloths[0].mesh.faces)
     assert o.amg_levels(0) == sizes and sizes[-1] <= COARSEST and len(sizes) == 3
+
+
+def test_oracle_amg_with_contacts_matches_dense_restatement():
+    """A perturbed sheet with self-proximities: level 0 is the full system, the coarse levels are
+    Galerkin products of the static system plus the contact blocks' diagonal part."""
+    from clothsimulator_amd._cabi import CS_FLAG_NO_PROXIMITY
+    spec = hang(33)
+    spec.params.precond = CS_PRECOND_AMG
+    o = oracle.OracleScene(spec, fp64=True)
+    rng = np.random.default_rng(0)
+    x, x0, v = o.state(fp64=True)
+    x = x + rng.normal(0, 0.01, x.shape)
+    v = rng.normal(0, 0.1, x.shape)
+    o.set_state(0, 0, x, x, v)
+    assert len(o.proximities()) > 0
+    n = x.shape[0]
+    rp, ci, vals, b = o.assemble(0)
+    p2 = hang(33).params
+    p2.flags |= CS_FLAG_NO_PROXIMITY
+    o2 = oracle.OracleScene(hang(33), fp64=True, params=p2)
+    o2.set_state(0, 0, x, x, v)
+    rs, cs_, vs, _ = o2.assemble(0)
+    Af, As = dense(rp, ci, vals, n), dense(rs, cs_, vs, n)
+    Cd = np.zeros_like(Af)
+    for i in range(n):
+        s = slice(3 * i, 3 * i + 3)
+        Cd[s, s] = Af[s, s] - As[s, s]
+    assert np.abs(Cd).max() > 0
+    _, aggs = hierarchy(n, spec.cloths[0].mesh.faces)
+    M = vcycle_matrix(Af, As + Cd, aggs)
+    for _ in range(2):
+        r = rng.normal(0, 1, (n, 3))
+        z = o.amg_apply(r, 0)
+        zd = (M @ r.ravel()).reshape(-1, 3)
+        assert np.allclose(z, zd, rtol=1e-8, atol=1e-9 * np.abs(zd).max())
