@@ -12,11 +12,13 @@ update).  The default workload is the 513 x 513 crumple (263,169 nodes,
 initial state with every input resident in HBM.  W warmup steps run first on
 the same context, which is then reset to the initial state.
 
-The measured run uses the learned preconditioner M^-1 = G^T G (packaged
-trained weights; the network runs once per solve inside the timed step).  The
-same steps are then run with block-Jacobi PCG (the reference's method), with
-the aggregation-multigrid V-cycle whose finest smoother is the learned factor,
-and with the same cycle smoothed by block Jacobi, each on a fresh context.
+The measured run uses the learned preconditioner in its fastest form: the
+aggregation-multigrid V-cycle whose finest-level smoother is the network's
+factor G^T G (packaged trained weights; the network and the smoother's damping
+estimate run once per solve inside the timed step).  The same steps are then
+run with block-Jacobi PCG (the reference's method), with the learned factor as
+the whole preconditioner (M^-1 = G^T G), and with the multigrid cycle smoothed
+by block Jacobi, each on a fresh context.
 
 Prints one JSON line: node-steps/s and CG iterations per step of the measured
 run, the same for the comparison runs, the roofline of the dominant kernel (the
@@ -154,10 +156,10 @@ def main():
     ap.add_argument("--workload", default="crumple513")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precond", choices=sorted(PRECONDS), default="learned",
-                    help="PCG preconditioner of the measured run: learned SPAI factor (default), multigrid with "
-                         "the learned smoother, multigrid with block-Jacobi smoothing or block-Jacobi")
-    ap.add_argument("--compare", default="jacobi,amg-learned,amg",
+    ap.add_argument("--precond", choices=sorted(PRECONDS), default="amg-learned",
+                    help="PCG preconditioner of the measured run: multigrid with the learned smoother (default), "
+                         "learned SPAI factor, multigrid with block-Jacobi smoothing or block-Jacobi")
+    ap.add_argument("--compare", default="jacobi,learned,amg",
                     help="comma-separated preconditioners also run over the same steps ('' for none)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
