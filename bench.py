@@ -143,6 +143,33 @@ def run(workload, precond, steps, warmup, timed):
     return out
 
 
+def sequential_sample(spec, precond, steps, k, batched_value):
+    """K instances of a batch (evenly spaced in the list), each run alone on its own context for
+    the same steps: their summed node-steps over their summed wall time is the throughput of
+    running the instances one after another."""
+    from clothsimulator_amd import Simulator, scenes
+    from clothsimulator_amd._cabi import CS_PRECOND_BLOCK_JACOBI
+    n = len(spec.cloths)
+    picks = sorted(set(int(i * n / k) for i in range(k)))
+    nodes, secs = 0, 0.0
+    for i in picks:
+        p = scenes.default_params(dt=spec.params.dt)
+        p.precond = CS_PRECOND_BLOCK_JACOBI
+        one = scenes.SceneSpec("instance%d" % i, p, [spec.cloths[i]], [spec.obstacles[i]], steps)
+        sim = Simulator(one)
+        sim.step()   # first-step allocations outside the timing
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.step()
+        secs += time.perf_counter() - t0
+        nodes += one.n_cloth_nodes * steps
+        sim.close()
+        print("sequential sample: instance %d (%d nodes) done" % (i, one.n_cloth_nodes), file=sys.stderr, flush=True)
+    seq = nodes / secs
+    return {"instances_sampled": len(picks), "node_steps_per_s": seq, "batched_speedup": batched_value / seq,
+            "note": "instances run one after another, each on its own context, block-Jacobi PCG"}
+
+
 def summary(r):
     return {"value": r["value"], "ms_per_step": r["ms_per_step"], "cg_iters_mean": float(np.mean(r["iters"])),
             "cg_iters_max": int(np.max(r["iters"])), "failed_steps": r["failed"]}
@@ -161,6 +188,9 @@ def main():
                          "learned SPAI factor, multigrid with block-Jacobi smoothing or block-Jacobi")
     ap.add_argument("--compare", default="jacobi,learned,amg",
                     help="comma-separated preconditioners also run over the same steps ('' for none)")
+    ap.add_argument("--sequential-sample", type=int, default=16, metavar="K",
+                    help="batch workloads: also run K of the instances one at a time, each on its own context, "
+                         "and report the sequential throughput they imply")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
     a = ap.parse_args()
@@ -184,7 +214,7 @@ def main():
 
     n_nodes = spec.n_cloth_nodes
     c0 = spec.cloths[0]
-    nnzb = static_nnzb(c0.mesh.faces, c0.mesh.n_nodes)
+    nnzb = sum(static_nnzb(c.mesh.faces, c.mesh.n_nodes) for c in spec.cloths)
     nnzb_dyn = nnzb + float(np.mean(main_run["contacts"]))
     # algorithmic bytes of one SpMV launch: 36 B values + 4 B column index per block; per node
     # z and the previous direction read once (16 B each), the new direction (16 B) and q (12 B)
@@ -229,6 +259,9 @@ def main():
     }
     for p, r in others.items():
         out[PRECONDS[p]] = summary(r)
+    if a.workload.startswith("batch") and a.sequential_sample > 0:
+        out["sequential"] = sequential_sample(spec, a.precond, steps, a.sequential_sample, ms["value"])
+        out["config"]["instances"] = len(spec.cloths)
     if "jacobi" in others:
         out["speedup_vs_jacobi"] = ms["value"] / others["jacobi"]["value"]
     if not a.no_cpu:

@@ -103,19 +103,40 @@ int physics_step(cs_ctx* C, cs_stats* st) {
     local.ms_proximity = T.lap(s);
     int status = CS_OK;
     local.pcg_converged = 1;
-    for (int c = 0; c < C->nCloths; c++) {
-        launch_element_records(C, c);
-        launch_assemble_rows(C, c);
-        local.n_contact_blocks += assemble_contacts(C, c);
-        launch_block_jacobi(C, c);
+    if ((C->P.flags & CS_FLAG_INSTANCES) && C->P.precond == CS_PRECOND_BLOCK_JACOBI && C->nCloths > 1) {
+        // independent instances: assemble every cloth, then one batched solve
+        for (int c = 0; c < C->nCloths; c++) {
+            launch_element_records(C, c);
+            launch_assemble_rows(C, c);
+            local.n_contact_blocks += assemble_contacts(C, c);
+            launch_block_jacobi(C, c);
+        }
         local.ms_assemble += T.lap(s);
-        float rel = 0.0f;
-        int conv = 0;
-        local.pcg_iters += pcg_solve(C, c, &rel, &conv);
-        local.pcg_rel_residual = std::max(local.pcg_rel_residual, rel);
-        if (!conv) { local.pcg_converged = 0; status = CS_E_NOT_CONVERGED; }
-        launch_apply_dv(C, c);
+        std::vector<int> its(C->nCloths), conv(C->nCloths);
+        std::vector<float> rel(C->nCloths);
+        pcg_solve_batched(C, its.data(), rel.data(), conv.data());
+        for (int c = 0; c < C->nCloths; c++) {
+            local.pcg_iters += its[c];
+            local.pcg_rel_residual = std::max(local.pcg_rel_residual, rel[c]);
+            if (!conv[c]) { local.pcg_converged = 0; status = CS_E_NOT_CONVERGED; }
+            launch_apply_dv(C, c);
+        }
         local.ms_solve += T.lap(s);
+    } else {
+        for (int c = 0; c < C->nCloths; c++) {
+            launch_element_records(C, c);
+            launch_assemble_rows(C, c);
+            local.n_contact_blocks += assemble_contacts(C, c);
+            launch_block_jacobi(C, c);
+            local.ms_assemble += T.lap(s);
+            float rel = 0.0f;
+            int conv = 0;
+            local.pcg_iters += pcg_solve(C, c, &rel, &conv);
+            local.pcg_rel_residual = std::max(local.pcg_rel_residual, rel);
+            if (!conv) { local.pcg_converged = 0; status = CS_E_NOT_CONVERGED; }
+            launch_apply_dv(C, c);
+            local.ms_solve += T.lap(s);
+        }
     }
     launch_update_positions(C, 0, C->N);
     launch_face_geometry(C, 0, C->F);
@@ -140,7 +161,7 @@ int collision_step(cs_ctx* C, cs_stats* st) {
     host_finalize(C);
     hipStream_t s = C->stream;
     StageTimer T;
-    for (size_t m = 0; m < C->meshes.size(); m++) bvh_build(C, (int)m, true);
+    bvh_build_all(C, true);
     const float th = C->P.collision_thickness;
     float obstacleMass = 1e3f;
     bool success = false;
@@ -157,9 +178,9 @@ int collision_step(cs_ctx* C, cs_stats* st) {
             C->nImpAcc += m;
             collision_response(C, C->nImpAcc, deform, obstacleMass);
             iters++;
-            for (int c = 0; c < C->nCloths; c++) bvh_refit(C, c);
+            bvh_refit_range(C, 0, C->nCloths);
             if (deform == 1) {
-                for (int o = C->nCloths; o < (int)C->meshes.size(); o++) bvh_refit(C, o);
+                bvh_refit_range(C, C->nCloths, (int)C->meshes.size());
                 obstacleMass *= 0.5f;
             }
         }
@@ -175,12 +196,10 @@ int collision_step(cs_ctx* C, cs_stats* st) {
         if (st) st->ms_collision = T.lap(s);
         return CS_E_COLLISION_FAILED;
     }
-    for (int c = 0; c < C->nCloths; c++) {
-        const MeshInfo& M = C->meshes[c];
-        launch_face_geometry(C, M.f0, M.nf);
-        launch_node_geometry(C, M.n0, M.nn);
-        launch_update_velocities(C, M.n0, M.nn);
-    }
+    // cloths come first in the node and face arrays: one launch each over all of them
+    launch_face_geometry(C, 0, C->FC);
+    launch_node_geometry(C, 0, C->NC);
+    launch_update_velocities(C, 0, C->NC);
     if (deform == 1)
         for (int o = C->nCloths; o < (int)C->meshes.size(); o++) {
             const MeshInfo& M = C->meshes[o];
@@ -256,6 +275,7 @@ void cs_destroy(cs_ctx* C) {
     if (!C) return;
     if (C->stream) (void)hipStreamSynchronize(C->stream);
     if (C->hScal) (void)hipHostFree(C->hScal);
+    if (C->hBatch) (void)hipHostFree(C->hBatch);
     if (C->hCount) (void)hipHostFree(C->hCount);
     for (hipEvent_t e : C->kt.events) (void)hipEventDestroy(e);
     cs_ctx* c = C;
@@ -460,7 +480,7 @@ cs_status cs_lbvh_pairs(cs_ctx* C, int ccd, float thickness, int* pairs, int cap
     if (C->nCloths == 0) return CS_E_STATE;
     return guard(C, [&]() {
         host_finalize(C);
-        for (size_t m = 0; m < C->meshes.size(); m++) bvh_build(C, (int)m, ccd != 0);
+        bvh_build_all(C, ccd != 0);
         long n = bvh_pairs_ordered(C, thickness, pairs, pairs ? cap : 0);
         *n_out = (int)n;
         return (pairs && n > cap) ? (int)CS_E_CAPACITY : (int)CS_OK;
@@ -690,7 +710,7 @@ cs_status cs_find_impacts(cs_ctx* C, float thickness, cs_impact_rec* out, int ca
     if (C->nCloths == 0) return CS_E_STATE;
     return guard(C, [&]() {
         host_finalize(C);
-        for (size_t m = 0; m < C->meshes.size(); m++) bvh_build(C, (int)m, true);
+        bvh_build_all(C, true);
         int n = find_impacts(C, thickness);
         *n_out = n;
         if (!out) return (int)CS_OK;

@@ -191,6 +191,7 @@ struct cs_ctx {
     cs::DBuf faceRec, edgeRec;   // per-step element records (cloth faces / edges)
     std::vector<cs::ClothSys> sys;
     std::vector<cs::Bvh> bvh;     // one per mesh (rebuilt per stage)
+    cs::DBuf bvhJobs, bvhSortTmp; // batched build of the small meshes' trees
     cs::LpcNet lpcNet;            // learned preconditioner weights (CS_PRECOND_LEARNED)
 
     // proximity
@@ -204,6 +205,10 @@ struct cs_ctx {
     // PCG
     cs::DBuf pcgPartial, pcgScal, pcgCounter;
     cs::PcgScalars* hScal = nullptr;  // pinned
+    // batched solve of independent cloths: per-cloth table, block -> cloth map, partials, scalars
+    cs::DBuf bTab, bMap, bPart, bScal;
+    void* hBatch = nullptr;  // pinned copy of the per-cloth scalars
+    int hBatchCap = 0;
     // CCD / impacts
     cs::DBuf impacts, impCount, impKeys, impKeysTmp, impIdx, impIdxTmp, impSorted;
     // per face, from its mesh's last-built LBVH: float4 (leaf box lo, mesh id bits),

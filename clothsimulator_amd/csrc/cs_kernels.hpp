@@ -29,6 +29,8 @@ void launch_block_jacobi(cs_ctx* C, int c);
 
 // PCG (cs_pcg.hip)
 int pcg_solve(cs_ctx* C, int c, float* relres, int* converged);
+// block-Jacobi PCG of every cloth at once (independent instances); per-cloth results
+void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged);
 // aggregation multigrid preconditioner (cs_amg.hip)
 constexpr float AMG_OMEGA = 0.6f;  // damping of the block-Jacobi smoother
 int amg_levels(cs_ctx* C, int c, int* sizes, int cap);  // 0: not usable for this cloth
@@ -61,7 +63,9 @@ float lpc_train_step(cs_ctx* C, int c, float lr, float solWeight, float* grad);
 
 // LBVH (cs_lbvh.hip)
 void bvh_build(cs_ctx* C, int mesh, bool ccd);
+void bvh_build_all(cs_ctx* C, bool ccd);  // every mesh; small meshes in one batched launch
 void bvh_refit(cs_ctx* C, int mesh);
+void bvh_refit_range(cs_ctx* C, int m0, int m1);  // meshes [m0, m1); small meshes in one batched launch
 // Candidate face pairs of all traversals (cloth self, cloth x earlier cloth,
 // cloth x obstacle) appended to C->pairs; returns the count.
 long bvh_all_pairs(cs_ctx* C, float thickness);

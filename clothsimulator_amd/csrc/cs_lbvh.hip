@@ -245,11 +245,9 @@ __device__ __forceinline__ LBox short_run(const BNode* nodes, const LBox* leafTa
 }
 
 // internal node boxes from their leaf ranges
-__global__ void __launch_bounds__(256) k_internal_boxes(int nf, int nblk, const int2* __restrict__ range,
-                                                        const LBox* __restrict__ leafTab,
-                                                        const LBox* __restrict__ blockTab, BNode* __restrict__ nodes) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nf - 1) return;
+__device__ __forceinline__ void internal_box(int i, int nf, int nblk, const int2* __restrict__ range,
+                                             const LBox* __restrict__ leafTab, const LBox* __restrict__ blockTab,
+                                             BNode* __restrict__ nodes) {
     const int2 rg = range[i];
     const int a = rg.x, b = rg.y;
     LBox box;
@@ -273,6 +271,203 @@ __global__ void __launch_bounds__(256) k_internal_boxes(int nf, int nblk, const 
     n.maxIndex = b;
     n.face = -1;
     nodes[i] = n;
+}
+
+__global__ void __launch_bounds__(256) k_internal_boxes(int nf, int nblk, const int2* __restrict__ range,
+                                                        const LBox* __restrict__ leafTab,
+                                                        const LBox* __restrict__ blockTab, BNode* __restrict__ nodes) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nf - 1) return;
+    internal_box(i, nf, nblk, range, leafTab, blockTab, nodes);
+}
+
+// ---------------------------------------------------------------- batched build of small meshes
+// One 1024-thread workgroup builds one mesh's whole tree, every small mesh in
+// one launch: leaf boxes, scene box, Morton codes, a bitonic sort of the unique
+// keys code<<32 | face (the order of the stable radix sort above), leaves,
+// Karras nodes, then the same range-union tables and internal boxes as
+// k_leaf_table / k_block_table / k_internal_boxes, level by level behind
+// workgroup barriers.  Every table entry is the union of a fixed run of sorted
+// leaves and min/max are exact, so the trees equal those of the per-mesh
+// kernels bit for bit.  Refit of small meshes runs the same way.
+constexpr int BB = 1024;
+
+struct BuildJob {
+    int f0, nf, mesh, pow2;  // faces, first face, mesh id, sort length (power of two >= nf)
+    int nblk, levels;        // 64-leaf blocks and block-table levels
+    int ccd;                 // leaf boxes span x and x0
+    uint64_t* keys;          // nf: code<<32 | face of the sorted leaves
+    uint64_t* tmp;           // pow2: sort buffer
+    BNode* nodes;
+    int2* child;
+    int2* range;
+    float4* fbox;
+    LBox* leafTab;
+    LBox* blockTab;
+};
+
+// range-union tables and internal boxes of job J's tree from its current leaf
+// boxes and stored ranges (the whole workgroup; ends behind a barrier)
+__device__ void tree_boxes(const BuildJob& J, int t) {
+    const int nf = J.nf;
+    if (nf <= 1) return;
+    for (int k = 1; k <= LEAF_LEVELS; k++) {
+        const int h = 1 << (k - 1);
+        LBox* cur = J.leafTab + (size_t)(k - 1) * nf;
+        const LBox* prev = k == 1 ? nullptr : J.leafTab + (size_t)(k - 2) * nf;
+        for (int j = t; j < nf; j += BB) {
+            const LBox a = k == 1 ? leaf_lbox(J.nodes, nf, j) : prev[j];
+            const LBox b = j + h >= nf ? lbox_empty() : (k == 1 ? leaf_lbox(J.nodes, nf, j + h) : prev[j + h]);
+            cur[j] = lbox_union(a, b);
+        }
+        __syncthreads();
+    }
+    const LBox* run32 = J.leafTab + (size_t)(LEAF_LEVELS - 1) * nf;
+    for (int b = t; b < J.nblk; b += BB) {
+        const int j = 64 * b;
+        J.blockTab[b] = lbox_union(run32[j], j + 32 < nf ? run32[j + 32] : lbox_empty());
+    }
+    __syncthreads();
+    for (int k = 1; k < J.levels; k++) {
+        const LBox* prev = J.blockTab + (size_t)(k - 1) * J.nblk;
+        LBox* cur = J.blockTab + (size_t)k * J.nblk;
+        for (int i = t; i + (1 << k) <= J.nblk; i += BB) cur[i] = lbox_union(prev[i], prev[i + (1 << (k - 1))]);
+        __syncthreads();
+    }
+    for (int i = t; i < nf - 1; i += BB) internal_box(i, nf, J.nblk, J.range, J.leafTab, J.blockTab, J.nodes);
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(BB) k_bvh_build_batch(const BuildJob* __restrict__ jobs, bool ccd,
+                                                        const float4* __restrict__ x, const float4* __restrict__ x0,
+                                                        const int4* __restrict__ fnodes,
+                                                        float4* __restrict__ faceBox) {
+    __shared__ float red[6][BB / 64];
+    __shared__ float scene[6];
+    const BuildJob J = jobs[blockIdx.x];
+    const int t = threadIdx.x;
+    // leaf boxes and the scene box
+    Box acc = box_empty();
+    for (int f = t; f < J.nf; f += BB) {
+        const Box b = face_box(x, x0, fnodes[J.f0 + f], ccd);
+        J.fbox[2 * f] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.0f);
+        J.fbox[2 * f + 1] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.0f);
+        box_add(acc, b);
+    }
+    for (int k = 0; k < 3; k++)
+        for (int o = 32; o > 0; o >>= 1) {
+            acc.lo[k] = tmin(acc.lo[k], __shfl_xor(acc.lo[k], o, 64));
+            acc.hi[k] = tmax(acc.hi[k], __shfl_xor(acc.hi[k], o, 64));
+        }
+    if ((t & 63) == 0)
+        for (int k = 0; k < 3; k++) { red[k][t >> 6] = acc.lo[k]; red[3 + k][t >> 6] = acc.hi[k]; }
+    __syncthreads();
+    if (t == 0) {
+        Box b = box_empty();
+        for (int w = 0; w < BB / 64; w++)
+            for (int k = 0; k < 3; k++) { b.lo[k] = tmin(b.lo[k], red[k][w]); b.hi[k] = tmax(b.hi[k], red[3 + k][w]); }
+        for (int k = 0; k < 3; k++) { scene[k] = b.lo[k]; scene[3 + k] = b.hi[k] - b.lo[k]; }
+    }
+    __syncthreads();
+    // Morton codes (as k_morton) into the sort buffer, padded with the largest key
+    for (int f = t; f < J.pow2; f += BB) {
+        uint64_t key = ~0ull;
+        if (f < J.nf) {
+            const float4 lo = J.fbox[2 * f], hi = J.fbox[2 * f + 1];
+            const float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+            uint32_t e[3];
+            for (int i = 0; i < 3; i++) {
+                float q = scene[3 + i] == 0.0f ? 0.0f : (c[i] - scene[i]) / scene[3 + i];
+                q = fminf(fmaxf(q * 1024.0f, 0.0f), 1023.0f);
+                e[i] = expand_bits((uint32_t)q);
+            }
+            key = ((uint64_t)((e[0] << 2) + (e[1] << 1) + e[2]) << 32) | (uint64_t)f;
+        }
+        J.tmp[f] = key;
+    }
+    __syncthreads();
+    // bitonic sort, ascending
+    for (int k = 2; k <= J.pow2; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = t; i < J.pow2; i += BB) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint64_t a = J.tmp[i], b = J.tmp[l];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        J.tmp[i] = b;
+                        J.tmp[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    // leaves (as k_fill_leaves)
+    for (int i = t; i < J.nf; i += BB) {
+        const uint64_t key = J.tmp[i];
+        const uint32_t f = (uint32_t)key;
+        J.keys[i] = key;
+        const float4 lo = J.fbox[2 * f], hi = J.fbox[2 * f + 1];
+        faceBox[2 * (J.f0 + (int)f)] = make_float4(lo.x, lo.y, lo.z, __int_as_float(J.mesh));
+        faceBox[2 * (J.f0 + (int)f) + 1] = make_float4(hi.x, hi.y, hi.z, __int_as_float(i));
+        BNode n;
+        n.lo[0] = lo.x; n.lo[1] = lo.y; n.lo[2] = lo.z;
+        n.hi[0] = hi.x; n.hi[1] = hi.y; n.hi[2] = hi.z;
+        n.maxIndex = i;
+        n.face = J.f0 + (int)f;
+        J.nodes[J.nf - 1 + i] = n;
+    }
+    __syncthreads();
+    // internal nodes (as k_karras)
+    const int ni = J.nf - 1;
+    const uint64_t* k = J.keys;
+    for (int i = t; i < ni; i += BB) {
+        int left, right;
+        if (i == 0) {
+            left = 0;
+            right = ni;
+        } else {
+            const uint64_t code = k[i];
+            const int dl = clz64(code ^ k[i - 1]);
+            const int dr = clz64(code ^ k[i + 1]);
+            if (dl > dr) {
+                int l = -1, r = i - 1;
+                while (r - l > 1) { int mid = (l + r) >> 1; if (clz64(code ^ k[mid]) > dr) r = mid; else l = mid; }
+                left = r; right = i;
+            } else {
+                int l = i + 1, r = ni + 1;
+                while (r - l > 1) { int mid = (l + r) >> 1; if (clz64(code ^ k[mid]) > dl) l = mid; else r = mid; }
+                left = i; right = l;
+            }
+        }
+        const uint64_t code = k[left];
+        const int dd = clz64(code ^ k[right]);
+        int l = left, r = right;
+        while (r - l > 1) { int mid = (l + r) >> 1; if (clz64(code ^ k[mid]) > dd) l = mid; else r = mid; }
+        const int mid = l;
+        J.child[i] = make_int2(mid == left ? ni + mid : mid, mid + 1 == right ? ni + mid + 1 : mid + 1);
+        J.range[i] = make_int2(left, right);
+    }
+    __syncthreads();
+    tree_boxes(J, t);
+}
+
+// Refit of small meshes, one workgroup each (as k_refit_leaves + internal_boxes).
+__global__ void __launch_bounds__(BB) k_bvh_refit_batch(const BuildJob* __restrict__ jobs, const float4* __restrict__ x,
+                                                        const float4* __restrict__ x0, const int4* __restrict__ fnodes,
+                                                        float4* __restrict__ faceBox) {
+    const BuildJob J = jobs[blockIdx.x];
+    const int t = threadIdx.x;
+    for (int i = t; i < J.nf; i += BB) {
+        BNode& n = J.nodes[J.nf - 1 + i];
+        const Box b = face_box(x, x0, fnodes[n.face], J.ccd != 0);
+        for (int c = 0; c < 3; c++) { n.lo[c] = b.lo[c]; n.hi[c] = b.hi[c]; }
+        float4* r = faceBox + 2 * (size_t)n.face;
+        r[0] = make_float4(b.lo[0], b.lo[1], b.lo[2], r[0].w);
+        r[1] = make_float4(b.hi[0], b.hi[1], b.hi[2], r[1].w);
+    }
+    __syncthreads();
+    tree_boxes(J, t);
 }
 
 // refit: recompute leaf boxes in place (sorted order, maxIndex unchanged)
@@ -431,6 +626,85 @@ void bvh_build(cs_ctx* C, int mesh, bool ccd) {
     }
 }
 
+constexpr int SMALL_BVH = 1 << 15;  // meshes up to this many faces build and refit in one batched launch
+
+static BuildJob small_job(const MeshInfo& M, Bvh& B) {
+    BuildJob j;
+    j.f0 = M.f0;
+    j.nf = M.nf;
+    j.mesh = B.mesh;
+    j.pow2 = 1;
+    while (j.pow2 < M.nf) j.pow2 <<= 1;
+    j.nblk = cdiv(M.nf, 64);
+    j.levels = floor_log2_host(j.nblk) + 1;
+    j.ccd = B.ccd ? 1 : 0;
+    j.keys = B.keys.ptr<uint64_t>();
+    j.tmp = nullptr;
+    j.nodes = B.nodes.ptr<BNode>();
+    j.child = B.child.ptr<int2>();
+    j.range = B.range.ptr<int2>();
+    j.fbox = B.fbox.ptr<float4>();
+    j.leafTab = B.leafTab.ptr<LBox>();
+    j.blockTab = B.blockTab.ptr<LBox>();
+    return j;
+}
+
+// Launches kern over the job table (one workgroup per job) and waits for it:
+// the host copy of the table must outlive the copy to the device.
+template <class K, class... A>
+static void launch_jobs(cs_ctx* C, const std::vector<BuildJob>& jobs, K kern, A... args) {
+    hipStream_t s = C->stream;
+    BuildJob* dj = C->bvhJobs.as<BuildJob>(jobs.size());
+    CS_HIP(hipMemcpyAsync(dj, jobs.data(), sizeof(BuildJob) * jobs.size(), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(kern, dim3((unsigned)jobs.size()), dim3(BB), 0, s, dj, args...);
+    CS_LAUNCH_CHECK();
+    CS_HIP(hipStreamSynchronize(s));
+}
+
+// Every mesh's tree: small meshes in one batched launch, larger ones with the
+// per-mesh kernels.
+void bvh_build_all(cs_ctx* C, bool ccd) {
+    std::vector<BuildJob> jobs;
+    size_t tmpLen = 0;
+    for (size_t m = 0; m < C->meshes.size(); m++) {
+        const MeshInfo& M = C->meshes[m];
+        if (M.nf > SMALL_BVH || M.nf < 1) {
+            bvh_build(C, (int)m, ccd);
+            continue;
+        }
+        Bvh& B = C->bvh[m];
+        B.mesh = (int)m;
+        B.ccd = ccd;
+        ensure_bvh(C, B, M.nf);
+        jobs.push_back(small_job(M, B));
+        tmpLen += jobs.back().pow2;
+    }
+    if (jobs.empty()) return;
+    uint64_t* tmp = C->bvhSortTmp.as<uint64_t>(tmpLen);
+    size_t o = 0;
+    for (BuildJob& j : jobs) { j.tmp = tmp + o; o += j.pow2; }
+    float4* faceBox = C->faceBox.as<float4>(2 * (size_t)C->F);
+    launch_jobs(C, jobs, k_bvh_build_batch, ccd, (const float4*)C->x.ptr<float4>(), (const float4*)C->x0.ptr<float4>(),
+                (const int4*)C->fnodes.ptr<int4>(), faceBox);
+}
+
+// Refit of meshes [m0, m1) (each built before): small meshes batched.
+void bvh_refit_range(cs_ctx* C, int m0, int m1) {
+    std::vector<BuildJob> jobs;
+    for (int m = m0; m < m1; m++) {
+        Bvh& B = C->bvh[m];
+        if (B.nf < 1) continue;
+        if (B.nf > SMALL_BVH) {
+            bvh_refit(C, m);
+            continue;
+        }
+        jobs.push_back(small_job(C->meshes[m], B));
+    }
+    if (jobs.empty()) return;
+    launch_jobs(C, jobs, k_bvh_refit_batch, (const float4*)C->x.ptr<float4>(), (const float4*)C->x0.ptr<float4>(),
+                (const int4*)C->fnodes.ptr<int4>(), C->faceBox.ptr<float4>());
+}
+
 void bvh_refit(cs_ctx* C, int mesh) {
     Bvh& B = C->bvh[mesh];
     hipLaunchKernelGGL(k_refit_leaves, dim3(cdiv(B.nf, 256)), dim3(256), 0, C->stream, B.nf, B.ccd,
@@ -441,8 +715,16 @@ void bvh_refit(cs_ctx* C, int mesh) {
 }
 
 // Simulator::traverse order (src/Simulator.cu:179-196): for each cloth, itself,
-// then every earlier cloth, then every obstacle.
+// then every earlier cloth, then every obstacle.  Independent instances
+// (CS_FLAG_INSTANCES): cloth c against itself and obstacle c only.
 template <class F> static void for_each_traversal(cs_ctx* C, F&& f) {
+    if (C->P.flags & CS_FLAG_INSTANCES) {
+        for (int c = 0; c < C->nCloths; c++) {
+            f(c, c, true);
+            if (C->nCloths + c < (int)C->meshes.size()) f(c, C->nCloths + c, false);
+        }
+        return;
+    }
     for (int c = 0; c < C->nCloths; c++) {
         f(c, c, true);
         for (int j = 0; j < c; j++) f(c, j, false);
@@ -451,51 +733,63 @@ template <class F> static void for_each_traversal(cs_ctx* C, F&& f) {
 }
 
 // All candidate pairs of the traversal order, written to C->pairs without any
-// atomic: per traversal a count pass, an exclusive scan of the per-leaf
-// counts, and an emit pass at the scanned offsets (so the order is
-// deterministic: traversal, then query leaf, then tree order).
+// atomic: a count pass per traversal into one concatenated array of per-leaf
+// counts, one exclusive scan over all of it (so the offsets are global pair
+// positions), one host read of the total, and an emit pass per traversal at the
+// scanned offsets.  The order is deterministic: traversal, then query leaf,
+// then tree order.  One host synchronisation per call, however many meshes.
 long bvh_all_pairs(cs_ctx* C, float th) {
     hipStream_t s = C->stream;
     unsigned* flag = C->pairCount.as<unsigned>(4);
     fill_u32(flag, 0u, 4, s);
-    long total = 0;
+    struct Trav { int q, t; bool self; long off; };
+    std::vector<Trav> trav;
+    long len = 0;
     for_each_traversal(C, [&](int q, int t, bool self) {
-        Bvh& Q = C->bvh[q];
-        Bvh& T = C->bvh[t];
+        trav.push_back({q, t, self, len});
+        len += C->bvh[q].nf;
+    });
+    if (len == 0) return 0;
+    uint32_t* num = C->ws.a.as<uint32_t>(len + 1);
+    uint32_t* offs = C->ws.b.as<uint32_t>(len + 1);
+    for (const Trav& v : trav) {
+        Bvh& Q = C->bvh[v.q];
+        Bvh& T = C->bvh[v.t];
         dim3 g(cdiv(Q.nf, TB));
-        uint32_t* num = C->ws.a.as<uint32_t>(Q.nf + 1);
-        if (self)
+        if (v.self)
             hipLaunchKernelGGL((k_traverse<true, 0>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
-                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, num, nullptr, nullptr, nullptr, 0L,
-                               flag + 1);
+                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, num + v.off, nullptr, nullptr,
+                               nullptr, 0L, flag + 1);
         else
             hipLaunchKernelGGL((k_traverse<false, 0>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
-                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, num, nullptr, nullptr, nullptr, 0L,
-                               flag + 1);
-        CS_LAUNCH_CHECK();
-        uint32_t* offs = C->ws.b.as<uint32_t>(Q.nf + 1);
-        scan_exclusive(num, offs, Q.nf, C->ws, s);
-        CS_HIP(hipMemcpyAsync(C->hCount, offs + Q.nf, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        CS_HIP(hipStreamSynchronize(s));
-        const long n = C->hCount[0];
-        if (n == 0) return;
-        C->pairs.grow_keep(sizeof(int2) * (size_t)(total + n), sizeof(int2) * (size_t)total, s);
-        C->pairCap = C->pairs.bytes / sizeof(int2);
-        int2* out = C->pairs.ptr<int2>() + total;
-        if (self)
+                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, num + v.off, nullptr, nullptr,
+                               nullptr, 0L, flag + 1);
+    }
+    CS_LAUNCH_CHECK();
+    scan_exclusive(num, offs, (int)len, C->ws, s);
+    CS_HIP(hipMemcpyAsync(C->hCount, offs + len, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    CS_HIP(hipMemcpyAsync(C->hCount + 1, flag + 1, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    CS_HIP(hipStreamSynchronize(s));
+    if (C->hCount[1]) throw StatusError(CS_E_CAPACITY, "LBVH traversal stack overflow (tree deeper than 64)");
+    const long total = C->hCount[0];
+    if (total == 0) return 0;
+    C->pairs.grow_keep(sizeof(int2) * (size_t)total, 0, s);
+    C->pairCap = C->pairs.bytes / sizeof(int2);
+    int2* out = C->pairs.ptr<int2>();
+    for (const Trav& v : trav) {
+        Bvh& Q = C->bvh[v.q];
+        Bvh& T = C->bvh[v.t];
+        dim3 g(cdiv(Q.nf, TB));
+        if (v.self)
             hipLaunchKernelGGL((k_traverse<true, 1>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
-                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, offs, out, nullptr, 0L,
-                               flag + 1);
+                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, offs + v.off, out,
+                               nullptr, 0L, flag + 1);
         else
             hipLaunchKernelGGL((k_traverse<false, 1>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
-                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, offs, out, nullptr, 0L,
-                               flag + 1);
-        CS_LAUNCH_CHECK();
-        total += n;
-    });
-    CS_HIP(hipMemcpyAsync(C->hCount, flag + 1, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    CS_HIP(hipStreamSynchronize(s));
-    if (C->hCount[0]) throw StatusError(CS_E_CAPACITY, "LBVH traversal stack overflow (tree deeper than 64)");
+                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, offs + v.off, out,
+                               nullptr, 0L, flag + 1);
+    }
+    CS_LAUNCH_CHECK();
     return total;
 }
 

@@ -340,6 +340,172 @@ __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* 
     }
 }
 
+
+// ---------------------------------------------------------------- batched solve
+// Independent cloths solved together: every launch covers the rows of all
+// cloths (256 rows per block, a block never spans two cloths), and each cloth
+// keeps its own partial sums, alpha, beta and stop flag, so the iterates are
+// those of separate solves while a whole batch costs two launches per iteration.
+
+struct BCloth {
+    int nn, nb, blk0, pad;  // rows, blocks, first block of the batch grid
+    const int *sliceOff, *sliceW, *cols, *cRowPtr, *cCol;
+    const float *vals, *cVal, *Minv, *b, *bc;
+    float *x, *r, *q;
+    float4 *z, *d0, *d1;
+    float* part;  // P_COUNT slots of nb partials: slot s of block k at part[s * nb + k]
+    PcgDev* sc;
+};
+
+// block-sum of the nb partials of one slot (every thread gets it)
+__device__ __forceinline__ float slot_sum(const float* __restrict__ p, int nb, float* lds) {
+    float a = 0.0f;
+    for (int k = threadIdx.x; k < nb; k += PB) a += p[k];
+    return block_sum(a, lds);
+}
+
+__global__ void __launch_bounds__(PB) k_bpcg_init(const BCloth* __restrict__ cl, const int* __restrict__ bmap) {
+    __shared__ float lds[8];
+    const BCloth c = cl[bmap[blockIdx.x]];
+    const int lb = blockIdx.x - c.blk0;
+    const int i = lb * PB + threadIdx.x;
+    float acc[2] = {0.0f, 0.0f};
+    if (i < c.nn) {
+        const V3f bi = ld3v(c.b, i) + ld3v(c.bc, i);
+        st3v(c.x, i, mk(0.0f, 0.0f, 0.0f));
+        st3v(c.r, i, bi);
+        const V3f zi = mat3v(c.Minv + 9 * (size_t)i, bi);
+        c.z[i] = f4(zi);
+        acc[0] = dot(bi, zi);
+        acc[1] = dot(bi, bi);
+    }
+    block_sums<2>(acc, lds);
+    if (threadIdx.x == 0) {
+        c.part[P_RZ0 * c.nb + lb] = acc[0];
+        c.part[P_BB * c.nb + lb] = acc[1];
+    }
+}
+
+// one block per cloth
+__global__ void __launch_bounds__(PB) k_bpcg_setup(const BCloth* __restrict__ cl, float tol2, int maxit) {
+    __shared__ float lds[8];
+    const BCloth c = cl[blockIdx.x];
+    const float bb = slot_sum(c.part + P_BB * c.nb, c.nb, lds);
+    const float rz = slot_sum(c.part + P_RZ0 * c.nb, c.nb, lds);
+    if (threadIdx.x == 0) {
+        c.sc->iters = 0;
+        c.sc->maxit = maxit;
+        c.sc->bb = bb;
+        c.sc->rr = bb;
+        c.sc->rz = rz;
+        c.sc->tol2bb = tol2 * bb;
+        c.sc->done = (bb <= tol2 * bb) || maxit <= 0;
+    }
+}
+
+// Iteration `it` of every unfinished cloth: as k_pcg_spmv (stop test from the r.r partials,
+// beta, q = A z + beta A d_prev, d = z + beta d_prev, partial d.q).
+template <bool FIRST>
+__global__ void __launch_bounds__(PB) k_bpcg_spmv(const BCloth* __restrict__ cl, const int* __restrict__ bmap,
+                                                  int it) {
+    __shared__ float lds[16];
+    const BCloth c = cl[bmap[blockIdx.x]];
+    if (c.sc->done) return;
+    const int lb = blockIdx.x - c.blk0;
+    const int p = it & 1;
+    float beta = 0.0f;
+    if (!FIRST) {
+        const float rr = slot_sum(c.part + P_RR * c.nb, c.nb, lds);
+        if (rr <= c.sc->tol2bb || it >= c.sc->maxit) {
+            if (lb == 0 && threadIdx.x == 0) {
+                c.sc->iters = it;
+                c.sc->rr = rr;
+                c.sc->done = 1;
+            }
+            return;
+        }
+        const float rzn = slot_sum(c.part + (P_RZ0 + p) * c.nb, c.nb, lds);
+        const float rzo = slot_sum(c.part + (P_RZ0 + (1 - p)) * c.nb, c.nb, lds);
+        beta = rzn / rzo;
+    }
+    float4* dcur = FIRST ? c.d0 : (p ? c.d1 : c.d0);
+    const float4* dprev = p ? c.d0 : c.d1;
+    const int i = lb * PB + threadIdx.x;
+    float pdq = 0.0f;
+    if (i < c.nn) {
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+        const int sl = i / SLICE, lane = i % SLICE;
+        const int W = c.sliceW[sl];
+        const size_t off = (size_t)c.sliceOff[sl];
+        const int* cp = c.cols + off * SLICE + lane;
+        const float* vp = c.vals + off * 9 * SLICE + lane;
+        for (int k = 0; k < W; k++) {
+            const int j = cp[k * SLICE];
+            const float* ap = vp + (size_t)k * 9 * SLICE;
+            float a[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) a[q] = ap[q * SLICE];
+            acc3(s0, s1, s2, a, v3(c.z[j]));
+            if (!FIRST) acc3(t0, t1, t2, a, v3(dprev[j]));
+        }
+        for (int kc = c.cRowPtr[i], ce = c.cRowPtr[i + 1]; kc < ce; kc++) {
+            const int jj = c.cCol[kc];
+            float a[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) a[q] = c.cVal[(size_t)kc * 9 + q];
+            acc3(s0, s1, s2, a, v3(c.z[jj]));
+            if (!FIRST) acc3(t0, t1, t2, a, v3(dprev[jj]));
+        }
+        const float4 zi = c.z[i];
+        V3f di, qi;
+        if (FIRST) {
+            di = v3(zi);
+            qi = mk(s0, s1, s2);
+        } else {
+            const float4 dp = dprev[i];
+            di = mk(zi.x + beta * dp.x, zi.y + beta * dp.y, zi.z + beta * dp.z);
+            qi = mk(s0 + beta * t0, s1 + beta * t1, s2 + beta * t2);
+        }
+        dcur[i] = f4(di);
+        st3v(c.q, i, qi);
+        pdq = dot(di, qi);
+    }
+    pdq = block_sum(pdq, lds);
+    if (threadIdx.x == 0) c.part[P_DQ * c.nb + lb] = pdq;
+}
+
+// alpha, x += alpha d, r -= alpha q, z = Minv r, partials r.z and r.r
+__global__ void __launch_bounds__(PB) k_bpcg_update(const BCloth* __restrict__ cl, const int* __restrict__ bmap,
+                                                    int it) {
+    __shared__ float lds[8];
+    const BCloth c = cl[bmap[blockIdx.x]];
+    if (c.sc->done) return;
+    const int lb = blockIdx.x - c.blk0;
+    const int p = it & 1;
+    const float dq = slot_sum(c.part + P_DQ * c.nb, c.nb, lds);
+    const float rz = slot_sum(c.part + (P_RZ0 + p) * c.nb, c.nb, lds);
+    const float alpha = rz / dq;
+    const float4* d = it == 0 ? c.d0 : (p ? c.d1 : c.d0);
+    const int i = lb * PB + threadIdx.x;
+    float acc[2] = {0.0f, 0.0f};
+    if (i < c.nn) {
+        const V3f di = v3(d[i]);
+        const V3f xi = ld3v(c.x, i) + alpha * di;
+        const V3f ri = ld3v(c.r, i) - alpha * ld3v(c.q, i);
+        st3v(c.x, i, xi);
+        st3v(c.r, i, ri);
+        const V3f zi = mat3v(c.Minv + 9 * (size_t)i, ri);
+        c.z[i] = f4(zi);
+        acc[0] = dot(ri, zi);
+        acc[1] = dot(ri, ri);
+    }
+    block_sums<2>(acc, lds);
+    if (threadIdx.x == 0) {
+        c.part[(P_RZ0 + (1 - p)) * c.nb + lb] = acc[0];
+        c.part[P_RR * c.nb + lb] = acc[1];
+    }
+}
+
 }  // namespace
 
 int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
@@ -443,6 +609,95 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     if (converged) *converged = (h.rr <= h.tol2bb);
     S.solved = h.rr <= h.tol2bb;  // S.x solves the assembled system
     return h.iters;
+}
+
+// All cloths' systems (assembled, Minv current) solved together with block-Jacobi PCG.
+// Per cloth: iterations, relative residual and convergence, as pcg_solve.
+void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged) {
+    hipStream_t s = C->stream;
+    const int ncl = C->nCloths;
+    std::vector<BCloth> tab(ncl);
+    std::vector<int> bmap;
+    size_t partTotal = 0;
+    for (int c = 0; c < ncl; c++) {
+        const int nb = std::max(1, cdiv(C->sys[c].nn, PB));
+        tab[c].nb = nb;
+        tab[c].blk0 = (int)bmap.size();
+        for (int k = 0; k < nb; k++) bmap.push_back(c);
+        partTotal += (size_t)P_COUNT * nb;
+    }
+    float* part = C->bPart.as<float>(partTotal);
+    PcgDev* sc = C->bScal.as<PcgDev>(ncl);
+    size_t po = 0;
+    for (int c = 0; c < ncl; c++) {
+        ClothSys& S = C->sys[c];
+        BCloth& b = tab[c];
+        b.nn = S.nn;
+        b.sliceOff = S.sliceOff.ptr<int>();
+        b.sliceW = S.sliceW.ptr<int>();
+        b.cols = S.cols.ptr<int>();
+        b.vals = S.vals.ptr<float>();
+        b.cRowPtr = S.cRowPtr.ptr<int>();
+        b.cCol = S.nContact > 0 ? S.cCol.ptr<int>() : S.cRowPtr.ptr<int>();
+        b.cVal = S.nContact > 0 ? S.cVal.ptr<float>() : S.vals.ptr<float>();
+        b.Minv = S.Minv.ptr<float>();
+        b.b = S.b.ptr<float>();
+        b.bc = S.bc.ptr<float>();
+        b.x = S.x.ptr<float>();
+        b.r = S.r.ptr<float>();
+        b.q = S.q.ptr<float>();
+        b.z = S.z.ptr<float4>();
+        b.d0 = S.d.ptr<float4>();
+        b.d1 = S.d.ptr<float4>() + S.nn;
+        b.part = part + po;
+        po += (size_t)P_COUNT * b.nb;
+        b.sc = sc + c;
+    }
+    const int NB = (int)bmap.size();
+    BCloth* dtab = C->bTab.as<BCloth>(ncl);
+    int* dmap = C->bMap.as<int>(NB);
+    CS_HIP(hipMemcpyAsync(dtab, tab.data(), sizeof(BCloth) * ncl, hipMemcpyHostToDevice, s));
+    CS_HIP(hipMemcpyAsync(dmap, bmap.data(), sizeof(int) * NB, hipMemcpyHostToDevice, s));
+    const float tol = C->P.pcg_rel_tol;
+    const int maxit = C->P.pcg_max_iters;
+    hipLaunchKernelGGL(k_bpcg_init, dim3(NB), dim3(PB), 0, s, dtab, dmap);
+    hipLaunchKernelGGL(k_bpcg_setup, dim3(ncl), dim3(PB), 0, s, dtab, tol * tol, maxit);
+    CS_LAUNCH_CHECK();
+    if (C->hBatchCap < ncl) {
+        if (C->hBatch) CS_HIP(hipHostFree(C->hBatch));
+        C->hBatch = nullptr;
+        CS_HIP(hipHostMalloc(reinterpret_cast<void**>(&C->hBatch), sizeof(PcgDev) * ncl, hipHostMallocDefault));
+        C->hBatchCap = ncl;
+    }
+    const PcgDev* h = reinterpret_cast<const PcgDev*>(C->hBatch);
+    const int chunk = 16;
+    int it = 0;
+    bool all = false;
+    while (!all && it <= maxit + 2 * chunk) {
+        for (int k = 0; k < chunk; k++, it++) {
+            if (k == 1) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
+            const int tok = k == 1 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
+            if (it == 0)
+                hipLaunchKernelGGL(k_bpcg_spmv<true>, dim3(NB), dim3(PB), 0, s, dtab, dmap, it);
+            else
+                hipLaunchKernelGGL(k_bpcg_spmv<false>, dim3(NB), dim3(PB), 0, s, dtab, dmap, it);
+            ktimer_end(C, tok);
+            hipLaunchKernelGGL(k_bpcg_update, dim3(NB), dim3(PB), 0, s, dtab, dmap, it);
+        }
+        CS_LAUNCH_CHECK();
+        CS_HIP(hipMemcpyAsync(C->hBatch, sc, sizeof(PcgDev) * ncl, hipMemcpyDeviceToHost, s));
+        CS_HIP(hipStreamSynchronize(s));
+        ktimer_collect(C);
+        all = true;
+        for (int c = 0; c < ncl; c++) all = all && h[c].done;
+    }
+    if (!all) throw StatusError(CS_E_HIP, "batched PCG: the device never reported every stop");
+    for (int c = 0; c < ncl; c++) {
+        iters[c] = h[c].iters;
+        relres[c] = h[c].bb > 0.0f ? sqrtf(h[c].rr / h[c].bb) : 0.0f;
+        converged[c] = h[c].rr <= h[c].tol2bb;
+        C->sys[c].solved = converged[c] != 0;
+    }
 }
 
 }  // namespace cs

@@ -187,7 +187,7 @@ __global__ void k_prox_records(long nTotal, long nNodeSlots, long nEdgeSlots, co
 int find_proximities(cs_ctx* C) {
     hipStream_t s = C->stream;
     const float rep = C->P.repulsion_thickness;
-    for (size_t m = 0; m < C->meshes.size(); m++) bvh_build(C, (int)m, false);
+    bvh_build_all(C, false);
     long nPairs = bvh_all_pairs(C, 2.0f * rep);
     fill_u64(C->bestNode.ptr<uint64_t>(), ~0ull, 2L * C->N, s);
     fill_u64(C->bestEdge.ptr<uint64_t>(), ~0ull, 2L * C->E, s);

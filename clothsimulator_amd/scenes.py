@@ -168,12 +168,38 @@ def crumple(n=513, steps=300, seed=0, height=0.5, sigma=1e-3):
                      [ground_plane()], steps)
 
 
+def instances(count=256, seed=0, side_lo=17, side_hi=129, steps=200, spacing=3.0):
+    """Independent drapes in one context (CS_FLAG_INSTANCES): instance k is a 1 m square
+    grid with side drawn uniformly from {side_lo..side_hi} (uniform over the side, not the node
+    count), stretch stiffness scaled by U[0.5, 2], 0.5 m above its own icosphere of radius
+    U[0.2, 0.4] m whose centre is offset by U[-0.2, 0.2] m in x and y; instances sit on a
+    square lattice `spacing` m apart.  Cloth k and obstacle k form instance k."""
+    from ._cabi import CS_FLAG_INSTANCES
+    rng = np.random.default_rng(seed)
+    per_row = int(math.ceil(math.sqrt(count)))
+    sphere_x, sphere_f = icosphere(3, 1.0)
+    cloths, obstacles = [], []
+    for k in range(count):
+        n = int(rng.integers(side_lo, side_hi + 1))
+        stiff = float(rng.uniform(0.5, 2.0))
+        radius = float(rng.uniform(0.2, 0.4))
+        off = rng.uniform(-0.2, 0.2, 2)
+        cx, cy = spacing * (k % per_row), spacing * (k // per_row)
+        x, f = grid_mesh(n, 1.0, origin=(cx - 0.5, cy - 0.5, 0.5))
+        cloths.append(ClothSpec(MeshArrays(x, f), material_struct(stretch=5e3 * stiff)))
+        sx = sphere_x * radius + np.array([cx + off[0], cy + off[1], 0.0])
+        obstacles.append(MeshArrays(sx, sphere_f))
+    p = default_params(dt=0.005)
+    p.flags |= CS_FLAG_INSTANCES
+    return SceneSpec("batch%d" % count, p, cloths, obstacles, steps)
+
+
 def by_name(name):
     table = {"hang33": lambda: hang(33), "drape129": lambda: drape(129), "crumple513": lambda: crumple(513)}
     if name in table:
         return table[name]()
     # parametrised small variants: hangN, drapeN, crumpleN
-    for prefix, fn in (("hang", hang), ("drape", drape), ("crumple", crumple)):
+    for prefix, fn in (("hang", hang), ("drape", drape), ("crumple", crumple), ("batch", instances)):
         if name.startswith(prefix) and name[len(prefix):].isdigit():
             return fn(int(name[len(prefix):]))
     raise KeyError(name)

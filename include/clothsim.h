@@ -59,7 +59,11 @@ enum { CS_MESH_CLOTH = 0, CS_MESH_OBSTACLE = 1 };
 enum {
     CS_FLAG_NO_PROXIMITY = 1, /* skip repulsion/friction proximities (testing only; the reference always runs them) */
     CS_FLAG_NO_COLLISION = 2, /* cs_step skips the collision step (testing only) */
-    CS_FLAG_TIME_KERNELS = 4  /* sample HIP-event durations of the hot kernels (see cs_kernel_time) */
+    CS_FLAG_TIME_KERNELS = 4, /* sample HIP-event durations of the hot kernels (see cs_kernel_time) */
+    CS_FLAG_INSTANCES = 8     /* independent instances: cloth i collides only with itself and with obstacle i
+                                 (the reference traverses every cloth pair, src/Simulator.cu:179-196, which finds
+                                 nothing between instances placed apart), and with block-Jacobi PCG all cloths'
+                                 solves advance together in batched launches (no counterpart) */
 };
 
 /* kernel ids for cs_kernel_time.  CS_KERNEL_EVENT_PAIR times an empty event pair recorded on

@@ -190,3 +190,22 @@ def test_cpp_mirror_runs_frames(tmp_path):
     v2 = [l for l in open(out / "frame00002_00.obj") if l.startswith("v ")]
     x2 = np.array([[float(t) for t in l.split()[1:]] for l in v2])
     assert np.all(np.isfinite(x2)) and not np.allclose(x2, x0)  # 16 steps of gravity moved it
+
+
+def test_instances_scene_generator():
+    from clothsimulator_amd import scenes
+    from clothsimulator_amd._cabi import CS_FLAG_INSTANCES
+    spec = scenes.instances(count=256, seed=0)
+    sides = np.array([int(round(np.sqrt(c.mesh.n_nodes))) for c in spec.cloths])
+    assert len(spec.cloths) == len(spec.obstacles) == 256
+    assert sides.min() >= 17 and sides.max() <= 129 and len(np.unique(sides)) > 80
+    assert spec.params.flags & CS_FLAG_INSTANCES and spec.params.dt == np.float32(0.005)
+    # every cloth sits above its own sphere and away from the others
+    for k in (0, 17, 255):
+        c = spec.cloths[k].mesh.x.mean(axis=0)
+        o = spec.obstacles[k].x
+        r = (o.max(axis=0) - o.min(axis=0))[0] / 2
+        assert 0.2 - 1e-9 <= r <= 0.4 + 1e-9
+        assert np.linalg.norm(o.mean(axis=0)[:2] - c[:2]) <= 0.2 * np.sqrt(2) + 1e-9
+        assert spec.cloths[k].mesh.x[:, 2].min() > o[:, 2].max()
+    assert scenes.instances(count=256, seed=0).n_cloth_nodes == spec.n_cloth_nodes   # deterministic
