@@ -51,7 +51,8 @@ def build(jobs=8, verbose=False):
             print("=== %s\n%s" % (src, log), file=sys.stderr)
         raise RuntimeError("hipcc failed on %d file(s)" % len(failed))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    # --no-undefined: a symbol missing from the objects fails here, not at load time
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         print(r.stdout + r.stderr, file=sys.stderr)

@@ -46,6 +46,17 @@ void ktimer_collect(cs_ctx* C) {
     K.pending.clear();
 }
 
+void* pinned_batch(cs_ctx* C, size_t bytes) {
+    if (C->hBatchCap < bytes) {
+        if (C->hBatch) CS_HIP(hipHostFree(C->hBatch));
+        C->hBatch = nullptr;
+        C->hBatchCap = 0;
+        CS_HIP(hipHostMalloc(&C->hBatch, bytes, hipHostMallocDefault));
+        C->hBatchCap = bytes;
+    }
+    return C->hBatch;
+}
+
 }  // namespace cs
 
 namespace {
@@ -105,12 +116,7 @@ int physics_step(cs_ctx* C, cs_stats* st) {
     local.pcg_converged = 1;
     if ((C->P.flags & CS_FLAG_INSTANCES) && C->P.precond == CS_PRECOND_BLOCK_JACOBI && C->nCloths > 1) {
         // independent instances: assemble every cloth, then one batched solve
-        for (int c = 0; c < C->nCloths; c++) {
-            launch_element_records(C, c);
-            launch_assemble_rows(C, c);
-            local.n_contact_blocks += assemble_contacts(C, c);
-            launch_block_jacobi(C, c);
-        }
+        local.n_contact_blocks += assemble_all(C);
         local.ms_assemble += T.lap(s);
         std::vector<int> its(C->nCloths), conv(C->nCloths);
         std::vector<float> rel(C->nCloths);

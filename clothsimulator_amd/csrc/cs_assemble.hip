@@ -230,18 +230,36 @@ __device__ __forceinline__ V3f edge_rhs(const float4* r, int a) {
     return mk(q6.y, q6.z, q6.w);
 }
 
+// The rows of one cloth's system.
+struct RowArgs {
+    int n0, nn;
+    const int *sliceOff, *sliceW, *incPtr, *diagSlot;
+    const int2* inc;
+    const float* hcount;
+    const float4* hpos;
+    float *vals, *b;
+};
+
+// Element records and per-node data shared by every cloth.
+struct RowShared {
+    float dt, gx, gy, gz, hk;
+    const float4 *faceRec, *finv, *edgeRec;  // records indexed by global face / edge id
+    const float* mass;
+    const float4 *x, *v;
+};
+
 // One thread per row (node) of one cloth; one wavefront per 64-row slice.
 // acc[(slot*9 + c)*64 + lane] accumulates the row's blocks in LDS.
-__global__ void __launch_bounds__(64) k_assemble_rows(
-    int n0, int nn, int f0, int e0, float dt, float gx, float gy, float gz, float hk, const int* __restrict__ sliceOff,
-    const int* __restrict__ sliceW, const int* __restrict__ incPtr, const int2* __restrict__ inc,
-    const float4* __restrict__ faceRec, const float4* __restrict__ finv, const float4* __restrict__ edgeRec,
-    const int* __restrict__ diagSlot, const float* __restrict__ mass, const float* __restrict__ hcount,
-    const float4* __restrict__ hpos, const float4* __restrict__ x, const float4* __restrict__ v,
-    float* __restrict__ vals, float* __restrict__ bout) {
-    extern __shared__ float acc[];
-    const int lane = threadIdx.x, s = blockIdx.x, row = s * SLICE + lane;
-    const int W = sliceW[s];
+__device__ __forceinline__ void assemble_slice(const RowArgs& R, const RowShared& G, int s, float* acc) {
+    const int nn = R.nn, n0 = R.n0;
+    const float dt = G.dt, hk = G.hk;
+    const int* __restrict__ incPtr = R.incPtr;
+    const int2* __restrict__ inc = R.inc;
+    const float4* __restrict__ faceRec = G.faceRec;
+    const float4* __restrict__ finv = G.finv;
+    const float4* __restrict__ edgeRec = G.edgeRec;
+    const int lane = threadIdx.x, row = s * SLICE + lane;
+    const int W = R.sliceW[s];
     for (int k = 0; k < W * 9; k++) acc[k * SLICE + lane] = 0.0f;
     V3f bacc = mk(0.0f, 0.0f, 0.0f);
     if (row < nn) {
@@ -251,7 +269,7 @@ __global__ void __launch_bounds__(64) k_assemble_rows(
             const int a = ic.y & 3;
             if (ic.x >= 0) {
                 const int f = ic.x;
-                const float4* r = faceRec + (size_t)(f - f0) * FACE_REC;
+                const float4* r = faceRec + (size_t)f * FACE_REC;
                 float4 r0 = r[0], r1 = r[1], r2 = r[2];
                 float4 Y = finv[f];
                 FaceCoef c;
@@ -271,7 +289,7 @@ __global__ void __launch_bounds__(64) k_assemble_rows(
                 bacc += ld3(r[3 + a]);
             } else {
                 const int e = ic.x & 0x7fffffff;
-                const float4* r = edgeRec + (size_t)(e - e0) * EDGE_REC;
+                const float4* r = edgeRec + (size_t)e * EDGE_REC;
                 float4 ga4 = r[a];
                 V3f ga = ld3(ga4);
                 float cc = r[0].w;
@@ -290,47 +308,74 @@ __global__ void __launch_bounds__(64) k_assemble_rows(
         }
         // mass, gravity and handle springs (src/PhysicsHelper.cu:152-176,379-396)
         const int g = n0 + row;
-        const float m = mass[g];
-        const float hc = hcount[row];
-        const int ds = diagSlot[row];
+        const float m = G.mass[g];
+        const float hc = R.hcount[row];
+        const int ds = R.diagSlot[row];
         const float dg = m + hc * (dt * dt * hk);
         acc[(ds * 9 + 0) * SLICE + lane] += dg;
         acc[(ds * 9 + 4) * SLICE + lane] += dg;
         acc[(ds * 9 + 8) * SLICE + lane] += dg;
-        bacc += (dt * m) * mk(gx, gy, gz);
+        bacc += (dt * m) * mk(G.gx, G.gy, G.gz);
         if (hc > 0.0f) {
-            V3f hp = ld3(hpos[row]);
-            bacc += (dt * ((hp - hc * ld3(x[g])) - (hc * dt) * ld3(v[g]))) * hk;
+            V3f hp = ld3(R.hpos[row]);
+            bacc += (dt * ((hp - hc * ld3(G.x[g])) - (hc * dt) * ld3(G.v[g]))) * hk;
         }
-        bout[3 * row + 0] = bacc.x;
-        bout[3 * row + 1] = bacc.y;
-        bout[3 * row + 2] = bacc.z;
+        R.b[3 * row + 0] = bacc.x;
+        R.b[3 * row + 1] = bacc.y;
+        R.b[3 * row + 2] = bacc.z;
     }
-    float* out = vals + (size_t)sliceOff[s] * 9 * SLICE;
+    float* out = R.vals + (size_t)R.sliceOff[s] * 9 * SLICE;
     for (int k = 0; k < W * 9; k++) out[k * SLICE + lane] = acc[k * SLICE + lane];
 }
 
+__global__ void __launch_bounds__(64) k_assemble_rows(RowArgs R, RowShared G) {
+    extern __shared__ float acc[];
+    assemble_slice(R, G, blockIdx.x, acc);
+}
+
+// Per-cloth view of the batched assembly of independent instances.
+struct AsmJob {
+    RowArgs R;
+    int slice0, blk0;  // first slice / first 256-row block of the cloth in the batch grids
+    int u0, nu;        // its contact blocks in the batch's sorted unique blocks
+    const int* cDiagSrc;
+    float *bc, *Minv, *cVal;
+    int *cRowPtr, *cDiag, *cCol;
+};
+
+// the job whose range [first(j), first(j + 1)) holds v (first() ascending)
+template <class F> __device__ __forceinline__ int find_job(int njobs, int v, F&& first) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first(mid) <= v) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(64) k_assemble_rows_batch(const AsmJob* __restrict__ jobs, int njobs, RowShared G) {
+    extern __shared__ float acc[];
+    const int j = find_job(njobs, (int)blockIdx.x, [&](int m) { return jobs[m].slice0; });
+    assemble_slice(jobs[j].R, G, (int)blockIdx.x - jobs[j].slice0, acc);
+}
+
 // ---------------------------------------------------------------- contacts
-// One thread per proximity: impulse and friction blocks (src/PhysicsHelper.cu:398-468)
-// emitted as 16 records keyed by (row, col) of this cloth; the RHS of row a
-// rides on the (a, a) record.
-__global__ void __launch_bounds__(256) k_contact_emit(int nP, const cs_proximity_rec* __restrict__ prox, int n0, int nn,
-                                                      int NC, float dt, float thickness, const float4* __restrict__ x,
-                                                      const float4* __restrict__ v, const float* __restrict__ mass,
-                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ idx,
-                                                      float* __restrict__ recs) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nP) return;
+// The impulse and friction blocks of proximity p (src/PhysicsHelper.cu:398-468) as 16
+// records keyed by keyOf(node a, node b), the block's (row, col) key or `invalid` for a
+// block of no system; the RHS of row a rides on the (a, a) record.
+template <class KeyFn>
+__device__ __forceinline__ void emit_contact(int p, const cs_proximity_rec* __restrict__ prox, int NC, float dt,
+                                             float thickness, const float4* __restrict__ x,
+                                             const float4* __restrict__ v, const float* __restrict__ mass,
+                                             uint64_t* __restrict__ keys, uint32_t* __restrict__ idx,
+                                             float* __restrict__ recs, uint64_t invalid, KeyFn&& keyOf) {
     const cs_proximity_rec P = prox[p];
-    const uint64_t invalid = (uint64_t)nn * (uint64_t)nn;
-    int loc[4];
     V3f xs[4], vs[4];
     V3f n = mk(P.n[0], P.n[1], P.n[2]);
     float d = -thickness;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         int nd = P.nodes[k];
-        loc[k] = (nd >= n0 && nd < n0 + nn) ? nd - n0 : -1;
         xs[k] = ld3(x[nd]);
         vs[k] = ld3(v[nd]);
         d += P.w[k] * dot(n, xs[k]);
@@ -361,13 +406,10 @@ __global__ void __launch_bounds__(256) k_contact_emit(int nP, const cs_proximity
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             const int slot = 16 * p + 4 * a + b;
-            if (loc[a] < 0 || loc[b] < 0) {
-                keys[slot] = invalid;
-                idx[slot] = slot;
-                continue;
-            }
-            keys[slot] = (uint64_t)loc[a] * (uint64_t)nn + (uint64_t)loc[b];
+            const uint64_t key = keyOf(P.nodes[a], P.nodes[b]);
+            keys[slot] = key;
             idx[slot] = slot;
+            if (key == invalid) continue;
             float* rr = recs + (size_t)slot * 12;
             const float wab = P.w[a] * P.w[b];
             const float ci = -dt * dt * jimp * wab;  // impulse: -dt^2 J
@@ -387,6 +429,41 @@ __global__ void __launch_bounds__(256) k_contact_emit(int nP, const cs_proximity
                 rr[9] = rr[10] = rr[11] = 0.0f;
             }
         }
+}
+
+// One thread per proximity: impulse and friction blocks (src/PhysicsHelper.cu:398-468) of
+// the cloth whose rows are [n0, n0 + nn), keyed by local (row, col).
+__global__ void __launch_bounds__(256) k_contact_emit(int nP, const cs_proximity_rec* __restrict__ prox, int n0, int nn,
+                                                      int NC, float dt, float thickness, const float4* __restrict__ x,
+                                                      const float4* __restrict__ v, const float* __restrict__ mass,
+                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ idx,
+                                                      float* __restrict__ recs) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nP) return;
+    const uint64_t invalid = (uint64_t)nn * (uint64_t)nn;
+    emit_contact(p, prox, NC, dt, thickness, x, v, mass, keys, idx, recs, invalid, [&](int na, int nb) {
+        const bool in = na >= n0 && na < n0 + nn && nb >= n0 && nb < n0 + nn;
+        return in ? (uint64_t)(na - n0) * (uint64_t)nn + (uint64_t)(nb - n0) : invalid;
+    });
+}
+
+// The same for every cloth of a batch of independent instances at once: keyed by the
+// global (row, col) node pair when both nodes are rows of one cloth.
+__global__ void __launch_bounds__(256) k_contact_emit_all(int nP, const cs_proximity_rec* __restrict__ prox,
+                                                          const AsmJob* __restrict__ jobs, int njobs, int NC,
+                                                          float dt, float thickness, const float4* __restrict__ x,
+                                                          const float4* __restrict__ v, const float* __restrict__ mass,
+                                                          uint64_t* __restrict__ keys, uint32_t* __restrict__ idx,
+                                                          float* __restrict__ recs) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nP) return;
+    const uint64_t invalid = (uint64_t)NC * (uint64_t)NC;
+    emit_contact(p, prox, NC, dt, thickness, x, v, mass, keys, idx, recs, invalid, [&](int na, int nb) {
+        if (na >= NC || nb >= NC) return invalid;
+        const int ca = find_job(njobs, na, [&](int m) { return jobs[m].R.n0; });
+        const int cb = find_job(njobs, nb, [&](int m) { return jobs[m].R.n0; });
+        return ca == cb ? (uint64_t)na * (uint64_t)NC + (uint64_t)nb : invalid;
+    });
 }
 
 __global__ void k_contact_heads(int n, const uint64_t* __restrict__ keys, uint64_t invalid, uint32_t* __restrict__ flag) {
@@ -456,25 +533,102 @@ __device__ __forceinline__ void inv3(const float a[9], float o[9]) {
     o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
 }
 
-__global__ void k_block_jacobi(int nn, const int* __restrict__ sliceOff, const int* __restrict__ diagSlot,
-                               const float* __restrict__ vals, const int* __restrict__ cdiag,
-                               const float* __restrict__ cval, float* __restrict__ Minv) {
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= nn) return;
+__device__ __forceinline__ void block_jacobi_row(int row, const int* __restrict__ sliceOff,
+                                                 const int* __restrict__ diagSlot, const float* __restrict__ vals,
+                                                 const float* __restrict__ cblk, float* __restrict__ Minv) {
     const int s = row / SLICE, lane = row % SLICE;
     const float* base = vals + ((size_t)sliceOff[s] + diagSlot[row]) * 9 * SLICE + lane;
     float a[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) a[q] = base[q * SLICE];
-    const int cd = cdiag[row];
-    if (cd >= 0) {
+    if (cblk) {
 #pragma unroll
-        for (int q = 0; q < 9; q++) a[q] += cval[(size_t)cd * 9 + q];
+        for (int q = 0; q < 9; q++) a[q] += cblk[q];
     }
     float o[9];
     inv3(a, o);
 #pragma unroll
     for (int q = 0; q < 9; q++) Minv[(size_t)row * 9 + q] = o[q];
+}
+
+__global__ void k_block_jacobi(int nn, const int* __restrict__ sliceOff, const int* __restrict__ diagSlot,
+                               const float* __restrict__ vals, const int* __restrict__ cdiag,
+                               const float* __restrict__ cval, float* __restrict__ Minv) {
+    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nn) return;
+    const int cd = cdiag[row];
+    block_jacobi_row(row, sliceOff, diagSlot, vals, cd >= 0 ? cval + (size_t)cd * 9 : nullptr, Minv);
+}
+
+// Batched instances, one thread per cloth row: the row's slice of its cloth's contact CSR
+// (row pointer, diagonal block, contact RHS) from the batch-wide reduction, and its
+// block-Jacobi inverse.  nu == 0: no contact anywhere in the batch.
+__global__ void __launch_bounds__(256) k_contact_rows_all(const AsmJob* __restrict__ jobs, int njobs, int NC, int nu,
+                                                          const uint64_t* __restrict__ ukeys,
+                                                          const int* __restrict__ gdiag, const float* __restrict__ gval,
+                                                          const float* __restrict__ gbc) {
+    const int j = find_job(njobs, (int)blockIdx.x, [&](int m) { return jobs[m].blk0; });
+    const AsmJob& J = jobs[j];
+    const int r = ((int)blockIdx.x - J.blk0) * 256 + threadIdx.x;
+    if (r >= J.R.nn) return;
+    const int g = J.R.n0 + r;
+    const float* cblk = nullptr;
+    if (nu > 0) {
+        // first unique block of the cloth with row >= r
+        const uint64_t target = (uint64_t)g * (uint64_t)NC;
+        int lo = J.u0, hi = J.u0 + J.nu;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ukeys[mid] < target) lo = mid + 1; else hi = mid;
+        }
+        J.cRowPtr[r] = lo - J.u0;
+        if (r == J.R.nn - 1) J.cRowPtr[J.R.nn] = J.nu;
+        const int cd = gdiag[g];
+        J.cDiag[r] = cd >= 0 ? cd - J.u0 : -1;
+        J.bc[3 * r + 0] = gbc[3 * (size_t)g + 0];
+        J.bc[3 * r + 1] = gbc[3 * (size_t)g + 1];
+        J.bc[3 * r + 2] = gbc[3 * (size_t)g + 2];
+        if (cd >= 0) cblk = gval + (size_t)cd * 9;
+    } else {
+        J.cRowPtr[r] = 0;
+        if (r == J.R.nn - 1) J.cRowPtr[J.R.nn] = 0;
+        J.cDiag[r] = -1;
+        J.bc[3 * r + 0] = 0.0f;
+        J.bc[3 * r + 1] = 0.0f;
+        J.bc[3 * r + 2] = 0.0f;
+    }
+    block_jacobi_row(r, J.R.sliceOff, J.R.diagSlot, J.R.vals, cblk, J.Minv);
+}
+
+// Batched instances, one thread per unique contact block: its cloth-local column and value.
+__global__ void __launch_bounds__(256) k_contact_blocks_all(const AsmJob* __restrict__ jobs, int njobs, int NC, int nu,
+                                                            const uint64_t* __restrict__ ukeys,
+                                                            const int* __restrict__ gcol,
+                                                            const float* __restrict__ gval) {
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    if (u >= nu) return;
+    const int j = find_job(njobs, u, [&](int m) { return jobs[m].u0; });
+    const AsmJob& J = jobs[j];
+    const int k = u - J.u0;
+    J.cCol[k] = gcol[u] - J.R.n0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) J.cVal[(size_t)k * 9 + q] = gval[(size_t)u * 9 + q];
+}
+
+// u0[c]: first unique block of cloth c (rows from n0[c]); u0[njobs] = nu = pos[n].
+__global__ void k_contact_bounds(const AsmJob* __restrict__ jobs, int njobs, int NC, const uint32_t* __restrict__ total,
+                                 const uint64_t* __restrict__ ukeys, int* __restrict__ u0) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > njobs) return;
+    const int nu = (int)*total;
+    if (c == njobs) { u0[c] = nu; return; }
+    const uint64_t target = (uint64_t)jobs[c].R.n0 * (uint64_t)NC;
+    int lo = 0, hi = nu;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ukeys[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    u0[c] = lo;
 }
 
 }  // namespace
@@ -499,22 +653,171 @@ void launch_element_records(cs_ctx* C, int c) {
     CS_LAUNCH_CHECK();
 }
 
-void launch_assemble_rows(cs_ctx* C, int c) {
+static RowArgs row_args(cs_ctx* C, int c) {
     const MeshInfo& M = C->meshes[c];
     ClothSys& S = C->sys[c];
-    S.solved = false;
+    RowArgs R;
+    R.n0 = M.n0;
+    R.nn = M.nn;
+    R.sliceOff = S.sliceOff.ptr<int>();
+    R.sliceW = S.sliceW.ptr<int>();
+    R.incPtr = S.incPtr.ptr<int>();
+    R.diagSlot = S.diagSlot.ptr<int>();
+    R.inc = S.inc.ptr<int2>();
+    R.hcount = S.hcount.ptr<float>();
+    R.hpos = S.hpos.ptr<float4>();
+    R.vals = S.vals.ptr<float>();
+    R.b = S.b.ptr<float>();
+    return R;
+}
+
+static RowShared row_shared(cs_ctx* C) {
     const cs_params& P = C->P;
+    RowShared G;
+    G.dt = P.dt;
+    G.gx = P.gravity[0];
+    G.gy = P.gravity[1];
+    G.gz = P.gravity[2];
+    G.hk = P.handle_stiffness;
+    G.faceRec = C->faceRec.ptr<float4>();
+    G.finv = C->finv.ptr<float4>();
+    G.edgeRec = C->edgeRec.ptr<float4>();
+    G.mass = C->mass.ptr<float>();
+    G.x = C->x.ptr<float4>();
+    G.v = C->v.ptr<float4>();
+    return G;
+}
+
+void launch_assemble_rows(cs_ctx* C, int c) {
+    ClothSys& S = C->sys[c];
+    S.solved = false;
     size_t lds = (size_t)S.maxWidth * 9 * SLICE * sizeof(float);
     const int tok = ktimer_begin(C, CS_KERNEL_ASSEMBLE_ROWS);
-    hipLaunchKernelGGL(k_assemble_rows, dim3(S.nslices), dim3(SLICE), lds, C->stream, M.n0, M.nn, M.f0, M.e0, P.dt,
-                       P.gravity[0], P.gravity[1], P.gravity[2], P.handle_stiffness, S.sliceOff.ptr<int>(),
-                       S.sliceW.ptr<int>(), S.incPtr.ptr<int>(), S.inc.ptr<int2>(),
-                       C->faceRec.ptr<float4>() + (size_t)M.f0 * FACE_REC, C->finv.ptr<float4>(),
-                       C->edgeRec.ptr<float4>() + (size_t)M.e0 * EDGE_REC, S.diagSlot.ptr<int>(), C->mass.ptr<float>(),
-                       S.hcount.ptr<float>(), S.hpos.ptr<float4>(), C->x.ptr<float4>(), C->v.ptr<float4>(),
-                       S.vals.ptr<float>(), S.b.ptr<float>());
+    hipLaunchKernelGGL(k_assemble_rows, dim3(S.nslices), dim3(SLICE), lds, C->stream, row_args(C, c), row_shared(C));
     CS_LAUNCH_CHECK();
     ktimer_end(C, tok);
+}
+
+// Every cloth's system of a batch of independent instances (each cloth's rows, contacts and
+// block-Jacobi inverse equal those of launch_element_records + launch_assemble_rows +
+// assemble_contacts + launch_block_jacobi on that cloth): one launch per pass over all
+// cloths and one host synchronisation.  Returns the number of contact blocks.
+int assemble_all(cs_ctx* C) {
+    hipStream_t s = C->stream;
+    const cs_params& P = C->P;
+    const int ncl = C->nCloths;
+    // element records of every cloth face and edge (cloths come first in both arrays)
+    hipLaunchKernelGGL(k_face_records, dim3(cdiv(C->FC, 256)), dim3(256), 0, s, 0, C->FC, P.dt, P.wind_density,
+                       P.wind_drag, P.wind_velocity[0], P.wind_velocity[1], P.wind_velocity[2], C->x.ptr<float4>(),
+                       C->v.ptr<float4>(), C->fnodes.ptr<int4>(), C->finv.ptr<float4>(), C->farea.ptr<float>(),
+                       C->fnrm.ptr<float4>(), C->fmat.ptr<int>(), C->matTable.ptr<float>(), C->faceRec.ptr<float4>());
+    CS_LAUNCH_CHECK();
+    if (C->EC > 0) {
+        hipLaunchKernelGGL(k_edge_records, dim3(cdiv(C->EC, 256)), dim3(256), 0, s, 0, C->EC, P.dt, C->x.ptr<float4>(),
+                           C->v.ptr<float4>(), C->enodes.ptr<int2>(), C->eopp.ptr<int2>(), C->eadj.ptr<int2>(),
+                           C->farea.ptr<float>(), C->fnrm.ptr<float4>(), C->ebias.ptr<float4>(), C->fmat.ptr<int>(),
+                           C->matBend.ptr<float>(), C->edgeRec.ptr<float4>());
+        CS_LAUNCH_CHECK();
+    }
+    std::vector<AsmJob> jobs(ncl);
+    int slices = 0, blocks = 0, maxW = 1;
+    for (int c = 0; c < ncl; c++) {
+        ClothSys& S = C->sys[c];
+        S.solved = false;
+        AsmJob& J = jobs[c];
+        J.R = row_args(C, c);
+        J.slice0 = slices;
+        J.blk0 = blocks;
+        J.u0 = J.nu = 0;
+        J.cDiagSrc = nullptr;
+        J.bc = S.bc.ptr<float>();
+        J.Minv = S.Minv.ptr<float>();
+        J.cRowPtr = S.cRowPtr.ptr<int>();
+        J.cDiag = S.cDiag.ptr<int>();
+        J.cCol = nullptr;
+        J.cVal = nullptr;
+        slices += S.nslices;
+        blocks += cdiv(S.nn, 256);
+        maxW = std::max(maxW, S.maxWidth);
+    }
+    AsmJob* dj = C->asmJobs.as<AsmJob>(ncl);
+    CS_HIP(hipMemcpyAsync(dj, jobs.data(), sizeof(AsmJob) * ncl, hipMemcpyHostToDevice, s));
+    const int tok = ktimer_begin(C, CS_KERNEL_ASSEMBLE_ROWS);
+    hipLaunchKernelGGL(k_assemble_rows_batch, dim3(slices), dim3(SLICE), (size_t)maxW * 9 * SLICE * sizeof(float), s,
+                       (const AsmJob*)dj, ncl, row_shared(C));
+    CS_LAUNCH_CHECK();
+    ktimer_end(C, tok);
+    const int NC = C->NC;
+    const int nP = C->nProx;
+    int nu = 0;
+    const uint64_t* ukeys = nullptr;
+    const int* gdiag = nullptr;
+    const float *gval = nullptr, *gbc = nullptr;
+    if (nP > 0) {
+        const long nr = 16L * nP;
+        uint64_t* keys = C->cKeys.as<uint64_t>(nr);
+        uint64_t* keysTmp = C->cKeysTmp.as<uint64_t>(nr);
+        uint32_t* idx = C->cIdx.as<uint32_t>(nr);
+        uint32_t* idxTmp = C->cIdxTmp.as<uint32_t>(nr);
+        float* recs = C->cRec.as<float>(nr * 12);
+        uint32_t* flag = C->cFlag.as<uint32_t>(nr + 1);
+        uint32_t* pos = C->cPos.as<uint32_t>(nr + 1);
+        int* col = C->gcCol.as<int>(nr);
+        float* val = C->gcVal.as<float>(9 * nr);
+        float* bc = C->gcBc.as<float>(3L * NC);
+        int* diag = C->gcDiag.as<int>(NC);
+        int* u0 = C->gcU0.as<int>(ncl + 1);
+        fill_f32(bc, 0.0f, 3L * NC, s);
+        fill_u32(reinterpret_cast<uint32_t*>(diag), 0xffffffffu, NC, s);
+        hipLaunchKernelGGL(k_contact_emit_all, dim3(cdiv(nP, 256)), dim3(256), 0, s, nP,
+                           C->prox.ptr<cs_proximity_rec>(), (const AsmJob*)dj, ncl, NC, P.dt, P.repulsion_thickness,
+                           C->x.ptr<float4>(), C->v.ptr<float4>(), C->mass.ptr<float>(), keys, idx, recs);
+        CS_LAUNCH_CHECK();
+        const uint64_t invalid = (uint64_t)NC * (uint64_t)NC;
+        int bits = 1;
+        while (bits < 64 && (invalid >> bits) != 0) bits++;
+        radix_sort_pairs(keys, idx, keysTmp, idxTmp, (int)nr, 0, bits, C->ws, s);
+        hipLaunchKernelGGL(k_contact_heads, dim3(cdiv(nr, 256)), dim3(256), 0, s, (int)nr, keys, invalid, flag);
+        CS_LAUNCH_CHECK();
+        scan_exclusive(flag, pos, (int)nr, C->ws, s);
+        hipLaunchKernelGGL(k_contact_reduce, dim3(cdiv(nr, 256)), dim3(256), 0, s, (int)nr, NC, keys, idx, flag, pos,
+                           recs, col, val, keysTmp, bc, diag);
+        CS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_contact_bounds, dim3(cdiv(ncl + 1, 256)), dim3(256), 0, s, (const AsmJob*)dj, ncl, NC,
+                           pos + nr, keysTmp, u0);
+        CS_LAUNCH_CHECK();
+        int* hu = reinterpret_cast<int*>(pinned_batch(C, sizeof(int) * (ncl + 1)));
+        CS_HIP(hipMemcpyAsync(hu, u0, sizeof(int) * (ncl + 1), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipStreamSynchronize(s));
+        nu = hu[ncl];
+        for (int c = 0; c < ncl; c++) {
+            ClothSys& S = C->sys[c];
+            AsmJob& J = jobs[c];
+            J.u0 = hu[c];
+            J.nu = hu[c + 1] - hu[c];
+            S.nContact = J.nu;
+            J.cCol = S.cCol.as<int>(std::max(J.nu, 1));
+            J.cVal = S.cVal.as<float>(9L * std::max(J.nu, 1));
+        }
+        CS_HIP(hipMemcpyAsync(dj, jobs.data(), sizeof(AsmJob) * ncl, hipMemcpyHostToDevice, s));
+        ukeys = keysTmp;
+        gdiag = diag;
+        gval = val;
+        gbc = bc;
+        if (nu > 0) {
+            hipLaunchKernelGGL(k_contact_blocks_all, dim3(cdiv(nu, 256)), dim3(256), 0, s, (const AsmJob*)dj, ncl, NC,
+                               nu, ukeys, (const int*)col, gval);
+            CS_LAUNCH_CHECK();
+        }
+    } else {
+        for (int c = 0; c < ncl; c++) C->sys[c].nContact = 0;
+    }
+    hipLaunchKernelGGL(k_contact_rows_all, dim3(blocks), dim3(256), 0, s, (const AsmJob*)dj, ncl, NC, nu, ukeys, gdiag,
+                       gval, gbc);
+    CS_LAUNCH_CHECK();
+    // the job table's host copy must outlive its copies to the device
+    CS_HIP(hipStreamSynchronize(s));
+    return nu;
 }
 
 int assemble_contacts(cs_ctx* C, int c) {

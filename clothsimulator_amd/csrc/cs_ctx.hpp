@@ -191,7 +191,7 @@ struct cs_ctx {
     cs::DBuf faceRec, edgeRec;   // per-step element records (cloth faces / edges)
     std::vector<cs::ClothSys> sys;
     std::vector<cs::Bvh> bvh;     // one per mesh (rebuilt per stage)
-    cs::DBuf bvhJobs, bvhSortTmp; // batched build of the small meshes' trees
+    cs::DBuf bvhJobs, bvhSortTmp, travJobs; // job tables of the batched tree builds and traversals
     cs::LpcNet lpcNet;            // learned preconditioner weights (CS_PRECOND_LEARNED)
 
     // proximity
@@ -207,8 +207,10 @@ struct cs_ctx {
     cs::PcgScalars* hScal = nullptr;  // pinned
     // batched solve of independent cloths: per-cloth table, block -> cloth map, partials, scalars
     cs::DBuf bTab, bMap, bPart, bScal;
-    void* hBatch = nullptr;  // pinned copy of the per-cloth scalars
-    int hBatchCap = 0;
+    void* hBatch = nullptr;  // pinned readback of per-cloth data (scalars, contact bounds)
+    size_t hBatchCap = 0;    // bytes
+    cs::DBuf asmJobs;        // per-cloth table of the batched assembly
+    cs::DBuf gcCol, gcVal, gcBc, gcDiag, gcU0;  // batch-wide contact reduction
     // CCD / impacts
     cs::DBuf impacts, impCount, impKeys, impKeysTmp, impIdx, impIdxTmp, impSorted;
     // per face, from its mesh's last-built LBVH: float4 (leaf box lo, mesh id bits),

@@ -26,6 +26,8 @@ void launch_element_records(cs_ctx* C, int c);
 void launch_assemble_rows(cs_ctx* C, int c);
 int assemble_contacts(cs_ctx* C, int c);  // returns number of contact blocks
 void launch_block_jacobi(cs_ctx* C, int c);
+int assemble_all(cs_ctx* C);  // batched instances: every cloth's system, contacts and block Jacobi
+void* pinned_batch(cs_ctx* C, size_t bytes);  // the context's pinned readback buffer, grown to `bytes`
 
 // PCG (cs_pcg.hip)
 int pcg_solve(cs_ctx* C, int c, float* relres, int* converged);

@@ -525,30 +525,43 @@ __device__ __forceinline__ bool traverse(const BNode* __restrict__ tn, const int
     return true;
 }
 
-// mode 0: count into num[qi]; mode 1: emit at offs[qi]; mode 2: append with an atomic counter
-template <bool SELF, int MODE>
-__global__ void __launch_bounds__(TB) k_traverse(int qnf, const BNode* __restrict__ qn, const BNode* __restrict__ tn,
-                                                 const int2* __restrict__ tchild, int tnf, float th,
-                                                 uint32_t* __restrict__ num, const uint32_t* __restrict__ offs,
-                                                 int2* __restrict__ out, unsigned* __restrict__ counter, long cap,
-                                                 unsigned* __restrict__ overflow) {
+// One traversal of a batch: the leaves of query tree q against tree t.
+struct TravJob {
+    int qnf, tnf, self, blk0;  // query leaves, target leaves, q == t, first block of the batch grid
+    long off;                  // position of query leaf 0 in the concatenated per-leaf arrays
+    const BNode* qn;
+    const BNode* tn;
+    const int2* tchild;
+};
+
+// Every traversal of the batch in one grid (each block belongs to one job, found
+// by binary search over the jobs' first blocks).  MODE 0: count each query
+// leaf's pairs into num[off + qi]; MODE 1: emit them at offs[off + qi].
+template <int MODE>
+__global__ void __launch_bounds__(TB) k_traverse_batch(const TravJob* __restrict__ jobs, int njobs, float th,
+                                                       uint32_t* __restrict__ num, const uint32_t* __restrict__ offs,
+                                                       int2* __restrict__ out, unsigned* __restrict__ overflow) {
     __shared__ int stk[STACK * TB];
-    const int qi = blockIdx.x * TB + threadIdx.x;
-    if (qi >= qnf) return;
-    const BNode q = qn[qnf - 1 + qi];
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].blk0 <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+    }
+    const TravJob J = jobs[lo];
+    const int qi = ((int)blockIdx.x - J.blk0) * TB + threadIdx.x;
+    if (qi >= J.qnf) return;
+    const BNode q = J.qn[J.qnf - 1 + qi];
     const Box qb = node_box(q);
     const int qface = q.face;
     uint32_t cnt = 0;
-    uint32_t base = MODE == 1 ? offs[qi] : 0;
-    bool ok = traverse<SELF>(tn, tchild, tnf, qb, qi, th, stk + threadIdx.x, [&](int f) {
+    const uint32_t base = MODE == 1 ? offs[J.off + qi] : 0;
+    auto emit = [&](int f) {
         if (MODE == 1) out[base + cnt] = make_int2(qface, f);
-        if (MODE == 2) {
-            unsigned slot = atomicAdd(counter, 1u);
-            if ((long)slot < cap) out[slot] = make_int2(qface, f);
-        }
         cnt++;
-    });
-    if (MODE == 0) num[qi] = cnt;
+    };
+    const bool ok = J.self ? traverse<true>(J.tn, J.tchild, J.tnf, qb, qi, th, stk + threadIdx.x, emit)
+                           : traverse<false>(J.tn, J.tchild, J.tnf, qb, qi, th, stk + threadIdx.x, emit);
+    if (MODE == 0) num[J.off + qi] = cnt;
     if (!ok) atomicOr(overflow, 1u);
 }
 
@@ -742,53 +755,46 @@ long bvh_all_pairs(cs_ctx* C, float th) {
     hipStream_t s = C->stream;
     unsigned* flag = C->pairCount.as<unsigned>(4);
     fill_u32(flag, 0u, 4, s);
-    struct Trav { int q, t; bool self; long off; };
-    std::vector<Trav> trav;
+    std::vector<TravJob> jobs;
     long len = 0;
+    int blocks = 0;
     for_each_traversal(C, [&](int q, int t, bool self) {
-        trav.push_back({q, t, self, len});
-        len += C->bvh[q].nf;
+        const Bvh& Q = C->bvh[q];
+        const Bvh& T = C->bvh[t];
+        if (Q.nf < 1 || T.nf < 1) return;
+        TravJob j;
+        j.qnf = Q.nf;
+        j.tnf = T.nf;
+        j.self = self ? 1 : 0;
+        j.blk0 = blocks;
+        j.off = len;
+        j.qn = Q.nodes.ptr<BNode>();
+        j.tn = T.nodes.ptr<BNode>();
+        j.tchild = T.child.ptr<int2>();
+        jobs.push_back(j);
+        len += Q.nf;
+        blocks += cdiv(Q.nf, TB);
     });
     if (len == 0) return 0;
     uint32_t* num = C->ws.a.as<uint32_t>(len + 1);
     uint32_t* offs = C->ws.b.as<uint32_t>(len + 1);
-    for (const Trav& v : trav) {
-        Bvh& Q = C->bvh[v.q];
-        Bvh& T = C->bvh[v.t];
-        dim3 g(cdiv(Q.nf, TB));
-        if (v.self)
-            hipLaunchKernelGGL((k_traverse<true, 0>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
-                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, num + v.off, nullptr, nullptr,
-                               nullptr, 0L, flag + 1);
-        else
-            hipLaunchKernelGGL((k_traverse<false, 0>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
-                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, num + v.off, nullptr, nullptr,
-                               nullptr, 0L, flag + 1);
-    }
+    TravJob* dj = C->travJobs.as<TravJob>(jobs.size());
+    CS_HIP(hipMemcpyAsync(dj, jobs.data(), sizeof(TravJob) * jobs.size(), hipMemcpyHostToDevice, s));
+    const int nj = (int)jobs.size();
+    hipLaunchKernelGGL(k_traverse_batch<0>, dim3(blocks), dim3(TB), 0, s, (const TravJob*)dj, nj, th, num,
+                       (const uint32_t*)nullptr, (int2*)nullptr, flag + 1);
     CS_LAUNCH_CHECK();
     scan_exclusive(num, offs, (int)len, C->ws, s);
     CS_HIP(hipMemcpyAsync(C->hCount, offs + len, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     CS_HIP(hipMemcpyAsync(C->hCount + 1, flag + 1, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    CS_HIP(hipStreamSynchronize(s));
+    CS_HIP(hipStreamSynchronize(s));  // also ends the job table's host-to-device copy
     if (C->hCount[1]) throw StatusError(CS_E_CAPACITY, "LBVH traversal stack overflow (tree deeper than 64)");
     const long total = C->hCount[0];
     if (total == 0) return 0;
     C->pairs.grow_keep(sizeof(int2) * (size_t)total, 0, s);
     C->pairCap = C->pairs.bytes / sizeof(int2);
-    int2* out = C->pairs.ptr<int2>();
-    for (const Trav& v : trav) {
-        Bvh& Q = C->bvh[v.q];
-        Bvh& T = C->bvh[v.t];
-        dim3 g(cdiv(Q.nf, TB));
-        if (v.self)
-            hipLaunchKernelGGL((k_traverse<true, 1>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
-                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, offs + v.off, out,
-                               nullptr, 0L, flag + 1);
-        else
-            hipLaunchKernelGGL((k_traverse<false, 1>), g, dim3(TB), 0, s, Q.nf, Q.nodes.ptr<BNode>(),
-                               T.nodes.ptr<BNode>(), T.child.ptr<int2>(), T.nf, th, nullptr, offs + v.off, out,
-                               nullptr, 0L, flag + 1);
-    }
+    hipLaunchKernelGGL(k_traverse_batch<1>, dim3(blocks), dim3(TB), 0, s, (const TravJob*)dj, nj, th,
+                       (uint32_t*)nullptr, (const uint32_t*)offs, C->pairs.ptr<int2>(), flag + 1);
     CS_LAUNCH_CHECK();
     return total;
 }

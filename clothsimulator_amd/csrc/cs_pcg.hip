@@ -663,13 +663,7 @@ void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged) {
     hipLaunchKernelGGL(k_bpcg_init, dim3(NB), dim3(PB), 0, s, dtab, dmap);
     hipLaunchKernelGGL(k_bpcg_setup, dim3(ncl), dim3(PB), 0, s, dtab, tol * tol, maxit);
     CS_LAUNCH_CHECK();
-    if (C->hBatchCap < ncl) {
-        if (C->hBatch) CS_HIP(hipHostFree(C->hBatch));
-        C->hBatch = nullptr;
-        CS_HIP(hipHostMalloc(reinterpret_cast<void**>(&C->hBatch), sizeof(PcgDev) * ncl, hipHostMallocDefault));
-        C->hBatchCap = ncl;
-    }
-    const PcgDev* h = reinterpret_cast<const PcgDev*>(C->hBatch);
+    const PcgDev* h = reinterpret_cast<const PcgDev*>(pinned_batch(C, sizeof(PcgDev) * ncl));
     const int chunk = 16;
     int it = 0;
     bool all = false;

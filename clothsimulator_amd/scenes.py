@@ -168,11 +168,12 @@ def crumple(n=513, steps=300, seed=0, height=0.5, sigma=1e-3):
                      [ground_plane()], steps)
 
 
-def instances(count=256, seed=0, side_lo=17, side_hi=129, steps=200, spacing=3.0):
+def instances(count=256, seed=0, side_lo=17, side_hi=129, steps=200, spacing=3.0, gap=None):
     """Independent drapes in one context (CS_FLAG_INSTANCES): instance k is a 1 m square
     grid with side drawn uniformly from {side_lo..side_hi} (uniform over the side, not the node
     count), stretch stiffness scaled by U[0.5, 2], 0.5 m above its own icosphere of radius
-    U[0.2, 0.4] m whose centre is offset by U[-0.2, 0.2] m in x and y; instances sit on a
+    U[0.2, 0.4] m whose centre (at height 0) is offset by U[-0.2, 0.2] m in x and y (with
+    `gap`, the cloth starts that far above its sphere's top instead); instances sit on a
     square lattice `spacing` m apart.  Cloth k and obstacle k form instance k."""
     from ._cabi import CS_FLAG_INSTANCES
     rng = np.random.default_rng(seed)
@@ -185,7 +186,8 @@ def instances(count=256, seed=0, side_lo=17, side_hi=129, steps=200, spacing=3.0
         radius = float(rng.uniform(0.2, 0.4))
         off = rng.uniform(-0.2, 0.2, 2)
         cx, cy = spacing * (k % per_row), spacing * (k // per_row)
-        x, f = grid_mesh(n, 1.0, origin=(cx - 0.5, cy - 0.5, 0.5))
+        z = 0.5 if gap is None else radius + gap
+        x, f = grid_mesh(n, 1.0, origin=(cx - 0.5, cy - 0.5, z))
         cloths.append(ClothSpec(MeshArrays(x, f), material_struct(stretch=5e3 * stiff)))
         sx = sphere_x * radius + np.array([cx + off[0], cy + off[1], 0.0])
         obstacles.append(MeshArrays(sx, sphere_f))

@@ -39,6 +39,32 @@ def test_batched_instances_match_separate_runs():
     assert abs(its_batch - its_single) <= max(4, 0.02 * its_single), (its_batch, its_single)
 
 
+def test_batched_contacts_match_separate_runs():
+    """Instances resting on their spheres: the batched contact assembly (one sort over every
+    cloth's blocks) gives each cloth the contact blocks of its own run."""
+    spec = scenes.instances(count=3, seed=5, side_lo=17, side_hi=25, gap=0.004)
+    batch = Simulator(spec)
+    contacts_batch = 0
+    for _ in range(12):
+        s, st = batch.step()
+        assert s == 0, s
+        contacts_batch += st.n_contact_blocks
+    assert contacts_batch > 0
+    contacts_single = 0
+    for k in range(3):
+        sim = Simulator(single(spec, k))
+        for _ in range(12):
+            s, st = sim.step()
+            assert s == 0, s
+            contacts_single += st.n_contact_blocks
+        xs, _, vs = sim.state(CS_MESH_CLOTH, 0)
+        xb, _, vb = batch.state(CS_MESH_CLOTH, k)
+        print("instance %d: max|dx| %.3g max|dv| %.3g" % (k, np.abs(xb - xs).max(), np.abs(vb - vs).max()))
+        assert np.abs(xb - xs).max() <= 1e-5, (k, np.abs(xb - xs).max())
+        assert np.abs(vb - vs).max() <= 2e-3, (k, np.abs(vb - vs).max())
+    assert contacts_batch == contacts_single, (contacts_batch, contacts_single)
+
+
 def test_instance_pairs_stay_inside_their_instance():
     spec = scenes.instances(count=3, seed=4, side_lo=17, side_hi=21, spacing=1.0)   # neighbouring cloths touch
     sim = Simulator(spec)
