@@ -146,9 +146,8 @@ struct FilterArgs {
     const int2 *enodes, *eadj;
     const int *nfPtr, *nfIdx;
     const float4* fbox;  // per face: (leaf box lo, mesh bits), (leaf box hi, sorted index bits)
-    uint64_t *candVF, *candEE;
-    unsigned* count;  // [0] vertex-face, [1] edge-edge
-    long cap;
+    uint16_t* mask;      // per pair: survivors (bit t for test t)
+    uint32_t *blkVF, *blkEE;  // per 256-pair block: survivors of each kind
 };
 
 __device__ __forceinline__ void swept_box(Box& b, const float4* x, const float4* x0, int n) {
@@ -220,93 +219,133 @@ __device__ bool ee_owner(const FilterArgs& A, int e0, int e1, int p0, int p1) {
 
 __device__ __forceinline__ int pick3(int4 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
-// The 15 tests of a pair: t < 3 node t of fa against fb, t < 6 node t-3 of fb
-// against fa, else edge (t-6)/3 of fa against edge (t-6)%3 of fb.
-__device__ bool filter_test(const FilterArgs& A, int t, int fa, int4 na, const FaceRec& ra, int fb, int4 nb,
-                            const FaceRec& rb, uint64_t& key) {
-    if (t < 6) {
-        const bool fromA = t < 3;
-        const int n = fromA ? pick3(na, t) : pick3(nb, t - 3);
-        const int p = fromA ? fa : fb, f = fromA ? fb : fa;
-        const int4 ff = fromA ? nb : na;
-        const FaceRec& rf = fromA ? rb : ra;
-        if (n == ff.x || n == ff.y || n == ff.z) return false;
-        if (!box_overlap(node_sbox(A, n), rf.b, A.th)) return false;
-        if (!vf_owner(A, n, p, rf)) return false;
-        key = ((uint64_t)n << A.keyBits) | (uint64_t)f;
-        return true;
-    }
-    int e0 = pick3(A.fedges[fa], (t - 6) / 3), e1 = pick3(A.fedges[fb], (t - 6) % 3);
-    int p0 = fa, p1 = fb;
-    if (e1 < e0) { int s = e0; e0 = e1; e1 = s; s = p0; p0 = p1; p1 = s; }
-    const int2 a = A.enodes[e0], b = A.enodes[e1];
-    if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) return false;
-    Box b0 = box_empty(), b1 = box_empty();
-    swept_box(b0, A.x, A.x0, a.x);
-    swept_box(b0, A.x, A.x0, a.y);
-    swept_box(b1, A.x, A.x0, b.x);
-    swept_box(b1, A.x, A.x0, b.y);
-    if (!box_overlap(b0, b1, A.th)) return false;
-    if (!ee_owner(A, e0, e1, p0, p1)) return false;
-    key = (1ull << (2 * A.keyBits)) | ((uint64_t)e0 << A.keyBits) | (uint64_t)e1;
-    return true;
+__device__ __forceinline__ Box box_union(const Box& a, const Box& b) {
+    Box c;
+    for (int k = 0; k < 3; k++) { c.lo[k] = tmin(a.lo[k], b.lo[k]); c.hi[k] = tmax(a.hi[k], b.hi[k]); }
+    return c;
+}
+// the swept box of node id, one of the face's nodes fn whose swept boxes are b[0..2]
+__device__ __forceinline__ Box pick_box(int id, int4 fn, const Box b[3]) {
+    return id == fn.x ? b[0] : (id == fn.y ? b[1] : b[2]);
 }
 
-// wave-aggregated append of c items per lane; returns this lane's first slot
-__device__ __forceinline__ unsigned wave_append(unsigned* counter, int c) {
-    const int lane = threadIdx.x & 63;
-    int incl = c;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int v = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += v;
-    }
-    const int total = __shfl(incl, 63, 64);
-    unsigned base = 0;
-    if (lane == 63 && total > 0) base = atomicAdd(counter, (unsigned)total);
-    base = __shfl(base, 63, 64);
-    return base + (unsigned)(incl - c);
-}
-
-// One thread per face pair.  Survivors are kept as a 15-bit mask and their
-// keys recomputed at write time (no dynamically indexed private array).
+// Pass 1, one thread per face pair: the 15 tests (t < 3: node t of fa against
+// fb; t < 6: node t - 3 of fb against fa; else edge (t - 6) / 3 of fa against
+// edge (t - 6) % 3 of fb) through the filters and ownership, with the six
+// nodes' swept boxes loaded once.  Writes the pair's survivor mask and each
+// block's survivor counts; no atomics.
 __global__ void __launch_bounds__(256) k_ccd_filter(FilterArgs A) {
+    __shared__ uint32_t red[2][4];
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
-    unsigned mvf = 0, mee = 0;
-    int fa = 0, fb = 0;
-    int4 na = make_int4(0, 0, 0, 0), nb = na;
+    unsigned m = 0;
     if (p < A.nPairs) {
         const int2 pr = A.pairs[p];
-        fa = pr.x;
-        fb = pr.y;
-        na = A.fnodes[fa];
-        nb = A.fnodes[fb];
+        const int fa = pr.x, fb = pr.y;
+        const int4 na = A.fnodes[fa], nb = A.fnodes[fb];
         const FaceRec ra = face_rec(A, fa), rb = face_rec(A, fb);
-        uint64_t key;
-        for (int t = 0; t < 6; t++)
-            if (filter_test(A, t, fa, na, ra, fb, nb, rb, key)) mvf |= 1u << t;
-        for (int t = 6; t < 15; t++)
-            if (filter_test(A, t, fa, na, ra, fb, nb, rb, key)) mee |= 1u << t;
+        Box ba[3], bb[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            ba[k] = node_sbox(A, pick3(na, k));
+            bb[k] = node_sbox(A, pick3(nb, k));
+        }
+#pragma unroll
+        for (int t = 0; t < 6; t++) {
+            const bool fromA = t < 3;
+            const int n = fromA ? pick3(na, t) : pick3(nb, t - 3);
+            const int4 ff = fromA ? nb : na;
+            if (n == ff.x || n == ff.y || n == ff.z) continue;
+            const FaceRec& rf = fromA ? rb : ra;
+            if (!box_overlap(fromA ? ba[t] : bb[t - 3], rf.b, A.th)) continue;
+            if (vf_owner(A, n, fromA ? fa : fb, rf)) m |= 1u << t;
+        }
+        const int4 ea = A.fedges[fa], eb = A.fedges[fb];
+#pragma unroll
+        for (int t = 6; t < 15; t++) {
+            int e0 = pick3(ea, (t - 6) / 3), e1 = pick3(eb, (t - 6) % 3);
+            const int2 a = A.enodes[e0], b = A.enodes[e1];
+            if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) continue;
+            Box b0 = box_union(pick_box(a.x, na, ba), pick_box(a.y, na, ba));
+            Box b1 = box_union(pick_box(b.x, nb, bb), pick_box(b.y, nb, bb));
+            int p0 = fa, p1 = fb;
+            if (e1 < e0) {  // the lower edge id first
+                int sw = e0; e0 = e1; e1 = sw;
+                sw = p0; p0 = p1; p1 = sw;
+                const Box tb = b0; b0 = b1; b1 = tb;
+            }
+            if (!box_overlap(b0, b1, A.th)) continue;
+            if (ee_owner(A, e0, e1, p0, p1)) m |= 1u << t;
+        }
+        A.mask[p] = (uint16_t)m;
     }
-    const unsigned ovf = wave_append(&A.count[0], __popc(mvf));
-    const unsigned oee = wave_append(&A.count[1], __popc(mee));
-    unsigned k = 0;
+    unsigned cv = __popc(m & 63u), ce = __popc(m >> 6);
+    for (int o = 32; o > 0; o >>= 1) {
+        cv += __shfl_xor(cv, o, 64);
+        ce += __shfl_xor(ce, o, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][wave] = cv; red[1][wave] = ce; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        A.blkVF[blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        A.blkEE[blockIdx.x] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    }
+}
+
+struct EmitArgs {
+    const int2* pairs;
+    long nPairs;
+    int keyBits;
+    const int4 *fnodes, *fedges;
+    const uint16_t* mask;
+    const uint32_t *offVF, *offEE;  // exclusive scans of the block counts
+    uint64_t *candVF, *candEE;
+};
+
+// inclusive scan of v over the 64 lanes of the wave
+__device__ __forceinline__ unsigned wave_incl_scan(unsigned v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// Pass 2: the survivors' keys at their positions in pair order.
+__global__ void __launch_bounds__(256) k_ccd_emit(EmitArgs A) {
+    __shared__ uint32_t wsum[2][4];
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    const unsigned m = p < A.nPairs ? A.mask[p] : 0u;
+    const unsigned cv = __popc(m & 63u), ce = __popc(m >> 6);
+    const unsigned iv = wave_incl_scan(cv), ie = wave_incl_scan(ce);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) { wsum[0][wave] = iv; wsum[1][wave] = ie; }
+    __syncthreads();
+    if (m == 0) return;
+    unsigned ov = A.offVF[blockIdx.x] + iv - cv, oe = A.offEE[blockIdx.x] + ie - ce;
+    for (int w = 0; w < wave; w++) { ov += wsum[0][w]; oe += wsum[1][w]; }
+    const int2 pr = A.pairs[p];
+    const int fa = pr.x, fb = pr.y;
+    const int4 na = A.fnodes[fa], nb = A.fnodes[fb];
+#pragma unroll
     for (int t = 0; t < 6; t++)
-        if (mvf >> t & 1u) {
+        if (m >> t & 1u) {
             const int n = t < 3 ? pick3(na, t) : pick3(nb, t - 3);
             const int f = t < 3 ? fb : fa;
-            if ((long)(ovf + k) < A.cap) A.candVF[ovf + k] = ((uint64_t)n << A.keyBits) | (uint64_t)f;
-            k++;
+            A.candVF[ov++] = ((uint64_t)n << A.keyBits) | (uint64_t)f;
         }
-    k = 0;
-    for (int t = 6; t < 15; t++)
-        if (mee >> t & 1u) {
-            int e0 = pick3(A.fedges[fa], (t - 6) / 3), e1 = pick3(A.fedges[fb], (t - 6) % 3);
-            if (e1 < e0) { int s = e0; e0 = e1; e1 = s; }
-            if ((long)(oee + k) < A.cap)
-                A.candEE[oee + k] = (1ull << (2 * A.keyBits)) | ((uint64_t)e0 << A.keyBits) | (uint64_t)e1;
-            k++;
-        }
+    if (m >> 6) {
+        const int4 ea = A.fedges[fa], eb = A.fedges[fb];
+#pragma unroll
+        for (int t = 6; t < 15; t++)
+            if (m >> t & 1u) {
+                int e0 = pick3(ea, (t - 6) / 3), e1 = pick3(eb, (t - 6) % 3);
+                if (e1 < e0) { const int sw = e0; e0 = e1; e1 = sw; }
+                A.candEE[oe++] = (1ull << (2 * A.keyBits)) | ((uint64_t)e0 << A.keyBits) | (uint64_t)e1;
+            }
+    }
 }
 
 // ---- stage 2: the FP64 test of every distinct candidate ----
@@ -480,12 +519,12 @@ int find_impacts(cs_ctx* C, float th) {
     long mx = std::max(std::max((long)C->N, (long)C->F), (long)C->E);
     while ((1L << keyBits) <= mx) keyBits++;
     unsigned* count = C->impCount.as<unsigned>(4);
-    // stage 1: distinct candidate tests (vertex-face and edge-edge lists)
-    long ccap = std::max<long>(4096, 2 * nPairs);
-    long nVF = 0, nEE = 0;
-    for (int attempt = 0; attempt < 2; attempt++) {
-        uint64_t* cand = C->ccdCand.as<uint64_t>(2 * ccap);
-        fill_u32(count, 0u, 2, s);
+    // stage 1: distinct candidate tests (vertex-face and edge-edge lists), in pair order
+    const int nblk = (int)cdiv(nPairs, 256);
+    uint16_t* mask = C->ccdMask.as<uint16_t>(nPairs);
+    uint32_t* blk = C->ccdBlk.as<uint32_t>(4L * (nblk + 1));
+    uint32_t *blkVF = blk, *blkEE = blk + (nblk + 1), *offVF = blk + 2L * (nblk + 1), *offEE = blk + 3L * (nblk + 1);
+    {
         FilterArgs A;
         A.pairs = C->pairs.ptr<int2>();
         A.nPairs = nPairs;
@@ -496,16 +535,33 @@ int find_impacts(cs_ctx* C, float th) {
         A.fnodes = C->fnodes.ptr<int4>(); A.fedges = C->fedges.ptr<int4>();
         A.enodes = C->enodes.ptr<int2>(); A.eadj = C->eadj.ptr<int2>();
         A.nfPtr = C->nfPtr.ptr<int>(); A.nfIdx = C->nfIdx.ptr<int>(); A.fbox = C->faceBox.ptr<float4>();
-        A.candVF = cand; A.candEE = cand + ccap;
-        A.count = count; A.cap = ccap;
-        hipLaunchKernelGGL(k_ccd_filter, dim3(cdiv(nPairs, 256)), dim3(256), 0, s, A);
+        A.mask = mask;
+        A.blkVF = blkVF;
+        A.blkEE = blkEE;
+        hipLaunchKernelGGL(k_ccd_filter, dim3(nblk), dim3(256), 0, s, A);
         CS_LAUNCH_CHECK();
-        CS_HIP(hipMemcpyAsync(C->hCount, count, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        CS_HIP(hipStreamSynchronize(s));
-        nVF = C->hCount[0];
-        nEE = C->hCount[1];
-        if (nVF <= ccap && nEE <= ccap) break;
-        ccap = std::max(nVF, nEE) + 4096;
+    }
+    scan_exclusive(blkVF, offVF, nblk, C->ws, s);
+    scan_exclusive(blkEE, offEE, nblk, C->ws, s);
+    CS_HIP(hipMemcpyAsync(C->hCount, offVF + nblk, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    CS_HIP(hipMemcpyAsync(C->hCount + 1, offEE + nblk, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    CS_HIP(hipStreamSynchronize(s));
+    const long nVF = C->hCount[0], nEE = C->hCount[1];
+    const long ccap = std::max(1L, std::max(nVF, nEE));
+    if (nVF + nEE > 0) {
+        uint64_t* cand = C->ccdCand.as<uint64_t>(2 * ccap);
+        EmitArgs E;
+        E.pairs = C->pairs.ptr<int2>();
+        E.nPairs = nPairs;
+        E.keyBits = keyBits;
+        E.fnodes = C->fnodes.ptr<int4>(); E.fedges = C->fedges.ptr<int4>();
+        E.mask = mask;
+        E.offVF = offVF;
+        E.offEE = offEE;
+        E.candVF = cand;
+        E.candEE = cand + ccap;
+        hipLaunchKernelGGL(k_ccd_emit, dim3(nblk), dim3(256), 0, s, E);
+        CS_LAUNCH_CHECK();
     }
     // stage 2: FP64 tests; impacts are appended with their keys
     long cap = std::max<long>(4096, (nVF + nEE) / 8);

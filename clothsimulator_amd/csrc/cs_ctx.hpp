@@ -217,6 +217,7 @@ struct cs_ctx {
     // float4 (leaf box hi, sorted leaf index bits)
     cs::DBuf faceBox;
     cs::DBuf ccdCand;   // uint64 candidate keys of the CCD prefilter (one per distinct test)
+    cs::DBuf ccdMask, ccdBlk;  // per pair survivor masks; per block counts and their scans
     cs::DBuf impAcc;  // accumulated impacts of the deform pass
     int nImpAcc = 0;
     cs::DBuf nodeMin, nodeRemoved, impState;
