@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* 
 // those of separate solves while a whole batch costs two launches per iteration.
 
 struct BCloth {
-    int nn, nb, blk0, pad;  // rows, blocks, first block of the batch grid
+    int nn, nb, pad0, pad1;  // rows, blocks
     const int *sliceOff, *sliceW, *cols, *cRowPtr, *cCol;
     const float *vals, *cVal, *Minv, *b, *bc;
     float *x, *r, *q;
@@ -357,17 +357,53 @@ struct BCloth {
     PcgDev* sc;
 };
 
-// block-sum of the nb partials of one slot (every thread gets it)
-__device__ __forceinline__ float slot_sum(const float* __restrict__ p, int nb, float* lds) {
-    float a = 0.0f;
-    for (int k = threadIdx.x; k < nb; k += PB) a += p[k];
-    return block_sum(a, lds);
+// Partials of NS slots of one cloth (nb <= 4 PB blocks in the fast path):
+// issue() loads this thread's share unconditionally, sums() reduces them.
+template <int NS>
+struct BPartials {
+    float raw[NS][4];
+    __device__ __forceinline__ void issue(const float* __restrict__ part, const int (&slot)[NS], int nb) {
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) raw[s][u] = part[slot[s] * nb + min((int)threadIdx.x + u * PB, nb - 1)];
+    }
+    __device__ __forceinline__ void sums(const float* __restrict__ part, const int (&slot)[NS], int nb, float* lds,
+                                         float (&out)[NS]) {
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) asm volatile("" : "+v"(raw[s][u]));
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            float a = 0.0f;
+#pragma unroll
+            for (int u = 0; u < 4; u++) a += (int)threadIdx.x + u * PB < nb ? raw[s][u] : 0.0f;
+            for (int b = threadIdx.x + 4 * PB; b < nb; b += PB) a += part[slot[s] * nb + b];
+            out[s] = a;
+        }
+        block_sums<NS>(out, lds);
+    }
+};
+
+// The batch grid covers the unfinished cloths only (rebuilt at every host
+// poll), padded to a multiple of 8 blocks and walked in logical order
+// (logical_block): each XCD gets one contiguous run of the cloths' blocks, so
+// neighbouring rows' gathers share that XCD's L2.  bmap[L] = (cloth, block of
+// the cloth) of logical block L (cloth -1: padding).
+
+// (cloth, block) of this workgroup, as wave-uniform values (scalar loads of the cloth's table)
+__device__ __forceinline__ int2 batch_block(const int2* __restrict__ bmap) {
+    const int2 m = bmap[logical_block()];
+    return make_int2(__builtin_amdgcn_readfirstlane(m.x), __builtin_amdgcn_readfirstlane(m.y));
 }
 
-__global__ void __launch_bounds__(PB) k_bpcg_init(const BCloth* __restrict__ cl, const int* __restrict__ bmap) {
+__global__ void __launch_bounds__(PB) k_bpcg_init(const BCloth* __restrict__ cl, const int2* __restrict__ bmap) {
     __shared__ float lds[8];
-    const BCloth c = cl[bmap[blockIdx.x]];
-    const int lb = blockIdx.x - c.blk0;
+    const int2 m = batch_block(bmap);
+    if (m.x < 0) return;
+    const BCloth& c = cl[m.x];
+    const int lb = m.y;
     const int i = lb * PB + threadIdx.x;
     float acc[2] = {0.0f, 0.0f};
     if (i < c.nn) {
@@ -390,50 +426,44 @@ __global__ void __launch_bounds__(PB) k_bpcg_init(const BCloth* __restrict__ cl,
 __global__ void __launch_bounds__(PB) k_bpcg_setup(const BCloth* __restrict__ cl, float tol2, int maxit) {
     __shared__ float lds[8];
     const BCloth c = cl[blockIdx.x];
-    const float bb = slot_sum(c.part + P_BB * c.nb, c.nb, lds);
-    const float rz = slot_sum(c.part + P_RZ0 * c.nb, c.nb, lds);
+    const int slot[2] = {P_BB, P_RZ0};
+    BPartials<2> pr;
+    pr.issue(c.part, slot, c.nb);
+    float v[2];
+    pr.sums(c.part, slot, c.nb, lds, v);
     if (threadIdx.x == 0) {
         c.sc->iters = 0;
         c.sc->maxit = maxit;
-        c.sc->bb = bb;
-        c.sc->rr = bb;
-        c.sc->rz = rz;
-        c.sc->tol2bb = tol2 * bb;
-        c.sc->done = (bb <= tol2 * bb) || maxit <= 0;
+        c.sc->bb = v[0];
+        c.sc->rr = v[0];
+        c.sc->rz = v[1];
+        c.sc->tol2bb = tol2 * v[0];
+        c.sc->done = (v[0] <= tol2 * v[0]) || maxit <= 0;
     }
 }
 
-// Iteration `it` of every unfinished cloth: as k_pcg_spmv (stop test from the r.r partials,
-// beta, q = A z + beta A d_prev, d = z + beta d_prev, partial d.q).
+// Iteration `it` of every unfinished cloth, as k_pcg_spmv: the sweep s = A z,
+// t = A d_prev first, then the stop test from the r.r partials (loaded at entry),
+// beta, q = s + beta t, d = z + beta d_prev and the partial d.q.  A stopped
+// cloth's partials no longer change, so its blocks keep deciding to stop.
 template <bool FIRST>
-__global__ void __launch_bounds__(PB) k_bpcg_spmv(const BCloth* __restrict__ cl, const int* __restrict__ bmap,
+__global__ void __launch_bounds__(PB) k_bpcg_spmv(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
                                                   int it) {
     __shared__ float lds[16];
-    const BCloth c = cl[bmap[blockIdx.x]];
-    if (c.sc->done) return;
-    const int lb = blockIdx.x - c.blk0;
+    const int2 m = batch_block(bmap);
+    if (m.x < 0) return;
+    const BCloth& c = cl[m.x];
+    const int lb = m.y;
     const int p = it & 1;
-    float beta = 0.0f;
-    if (!FIRST) {
-        const float rr = slot_sum(c.part + P_RR * c.nb, c.nb, lds);
-        if (rr <= c.sc->tol2bb || it >= c.sc->maxit) {
-            if (lb == 0 && threadIdx.x == 0) {
-                c.sc->iters = it;
-                c.sc->rr = rr;
-                c.sc->done = 1;
-            }
-            return;
-        }
-        const float rzn = slot_sum(c.part + (P_RZ0 + p) * c.nb, c.nb, lds);
-        const float rzo = slot_sum(c.part + (P_RZ0 + (1 - p)) * c.nb, c.nb, lds);
-        beta = rzn / rzo;
-    }
-    float4* dcur = FIRST ? c.d0 : (p ? c.d1 : c.d0);
+    const int slot[3] = {P_RR, P_RZ0 + p, P_RZ0 + (1 - p)};
+    BPartials<3> pr;
+    if (!FIRST) pr.issue(c.part, slot, c.nb);
+    if (FIRST && c.sc->done) return;
     const float4* dprev = p ? c.d0 : c.d1;
     const int i = lb * PB + threadIdx.x;
-    float pdq = 0.0f;
-    if (i < c.nn) {
-        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+    const bool live = i < c.nn;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+    if (live) {
         const int sl = i / SLICE, lane = i % SLICE;
         const int W = c.sliceW[sl];
         const size_t off = (size_t)c.sliceOff[sl];
@@ -456,6 +486,24 @@ __global__ void __launch_bounds__(PB) k_bpcg_spmv(const BCloth* __restrict__ cl,
             acc3(s0, s1, s2, a, v3(c.z[jj]));
             if (!FIRST) acc3(t0, t1, t2, a, v3(dprev[jj]));
         }
+    }
+    float beta = 0.0f;
+    if (!FIRST) {
+        float v[3];
+        pr.sums(c.part, slot, c.nb, lds, v);
+        if (v[0] <= c.sc->tol2bb || it >= c.sc->maxit) {  // the same decision in every block of the cloth
+            if (lb == 0 && threadIdx.x == 0 && !c.sc->done) {
+                c.sc->iters = it;
+                c.sc->rr = v[0];
+                c.sc->done = 1;
+            }
+            return;
+        }
+        beta = v[1] / v[2];
+    }
+    float pdq = 0.0f;
+    if (live) {
+        float4* dcur = FIRST ? c.d0 : (p ? c.d1 : c.d0);
         const float4 zi = c.z[i];
         V3f di, qi;
         if (FIRST) {
@@ -474,24 +522,37 @@ __global__ void __launch_bounds__(PB) k_bpcg_spmv(const BCloth* __restrict__ cl,
     if (threadIdx.x == 0) c.part[P_DQ * c.nb + lb] = pdq;
 }
 
-// alpha, x += alpha d, r -= alpha q, z = Minv r, partials r.z and r.r
-__global__ void __launch_bounds__(PB) k_bpcg_update(const BCloth* __restrict__ cl, const int* __restrict__ bmap,
+// alpha, x += alpha d, r -= alpha q, z = Minv r, partials r.z and r.r (the row
+// loads are issued before the partials are reduced)
+__global__ void __launch_bounds__(PB) k_bpcg_update(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
                                                     int it) {
     __shared__ float lds[8];
-    const BCloth c = cl[bmap[blockIdx.x]];
-    if (c.sc->done) return;
-    const int lb = blockIdx.x - c.blk0;
+    const int2 m = batch_block(bmap);
+    if (m.x < 0) return;
+    const BCloth& c = cl[m.x];
+    const int lb = m.y;
     const int p = it & 1;
-    const float dq = slot_sum(c.part + P_DQ * c.nb, c.nb, lds);
-    const float rz = slot_sum(c.part + (P_RZ0 + p) * c.nb, c.nb, lds);
-    const float alpha = rz / dq;
+    const int slot[2] = {P_DQ, P_RZ0 + p};
+    BPartials<2> pr;
+    pr.issue(c.part, slot, c.nb);
     const float4* d = it == 0 ? c.d0 : (p ? c.d1 : c.d0);
     const int i = lb * PB + threadIdx.x;
+    const bool live = i < c.nn;
+    V3f di = mk(0.0f, 0.0f, 0.0f), xi = di, ri = di, qi = di;
+    if (live) {
+        di = v3(d[i]);
+        xi = ld3v(c.x, i);
+        ri = ld3v(c.r, i);
+        qi = ld3v(c.q, i);
+    }
+    if (c.sc->done) return;
+    float v[2];
+    pr.sums(c.part, slot, c.nb, lds, v);
+    const float alpha = v[1] / v[0];
     float acc[2] = {0.0f, 0.0f};
-    if (i < c.nn) {
-        const V3f di = v3(d[i]);
-        const V3f xi = ld3v(c.x, i) + alpha * di;
-        const V3f ri = ld3v(c.r, i) - alpha * ld3v(c.q, i);
+    if (live) {
+        xi = xi + alpha * di;
+        ri = ri - alpha * qi;
         st3v(c.x, i, xi);
         st3v(c.r, i, ri);
         const V3f zi = mat3v(c.Minv + 9 * (size_t)i, ri);
@@ -567,8 +628,10 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
         for (int k = 0; k < chunk; k++, it++) {
             // the second SpMV of a chunk is sampled (the first follows a host wait), after an
             // empty pair that measures the markers' own latency
-            if (k == 1) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
-            const int tok = k == 1 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
+            // sampled: the first full-batch SpMV after the initial one (every cloth still active),
+            // after an empty pair that measures the markers' own latency
+            if (it == 1) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
+            const int tok = it == 1 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
             float4* dcur = it == 0 ? dgen[0] : dgen[it & 1];
             if (it == 0)
                 hipLaunchKernelGGL(k_pcg_spmv<true>, dim3(nb), dim3(PB), 0, s, it, nn, S.sliceOff.ptr<int>(),
@@ -617,14 +680,10 @@ void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged) {
     hipStream_t s = C->stream;
     const int ncl = C->nCloths;
     std::vector<BCloth> tab(ncl);
-    std::vector<int> bmap;
     size_t partTotal = 0;
     for (int c = 0; c < ncl; c++) {
-        const int nb = std::max(1, cdiv(C->sys[c].nn, PB));
-        tab[c].nb = nb;
-        tab[c].blk0 = (int)bmap.size();
-        for (int k = 0; k < nb; k++) bmap.push_back(c);
-        partTotal += (size_t)P_COUNT * nb;
+        tab[c].nb = std::max(1, cdiv(C->sys[c].nn, PB));
+        partTotal += (size_t)P_COUNT * tab[c].nb;
     }
     float* part = C->bPart.as<float>(partTotal);
     PcgDev* sc = C->bScal.as<PcgDev>(ncl);
@@ -653,11 +712,21 @@ void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged) {
         po += (size_t)P_COUNT * b.nb;
         b.sc = sc + c;
     }
-    const int NB = (int)bmap.size();
+    // (cloth, block) of every logical block of the unfinished cloths, padded to a multiple of 8
+    std::vector<int2> bmap;
+    auto build_map = [&](const PcgDev* done) {
+        bmap.clear();
+        for (int c = 0; c < ncl; c++)
+            if (!done || !done[c].done)
+                for (int k = 0; k < tab[c].nb; k++) bmap.push_back(make_int2(c, k));
+        while (bmap.size() % 8) bmap.push_back(make_int2(-1, 0));
+    };
+    build_map(nullptr);
+    int NB = (int)bmap.size();
     BCloth* dtab = C->bTab.as<BCloth>(ncl);
-    int* dmap = C->bMap.as<int>(NB);
+    int2* dmap = C->bMap.as<int2>(NB);
     CS_HIP(hipMemcpyAsync(dtab, tab.data(), sizeof(BCloth) * ncl, hipMemcpyHostToDevice, s));
-    CS_HIP(hipMemcpyAsync(dmap, bmap.data(), sizeof(int) * NB, hipMemcpyHostToDevice, s));
+    CS_HIP(hipMemcpyAsync(dmap, bmap.data(), sizeof(int2) * NB, hipMemcpyHostToDevice, s));
     const float tol = C->P.pcg_rel_tol;
     const int maxit = C->P.pcg_max_iters;
     hipLaunchKernelGGL(k_bpcg_init, dim3(NB), dim3(PB), 0, s, dtab, dmap);
@@ -669,8 +738,10 @@ void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged) {
     bool all = false;
     while (!all && it <= maxit + 2 * chunk) {
         for (int k = 0; k < chunk; k++, it++) {
-            if (k == 1) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
-            const int tok = k == 1 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
+            // sampled: the first full-batch SpMV after the initial one (every cloth still active),
+            // after an empty pair that measures the markers' own latency
+            if (it == 1) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
+            const int tok = it == 1 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
             if (it == 0)
                 hipLaunchKernelGGL(k_bpcg_spmv<true>, dim3(NB), dim3(PB), 0, s, dtab, dmap, it);
             else
@@ -683,7 +754,16 @@ void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged) {
         CS_HIP(hipStreamSynchronize(s));
         ktimer_collect(C);
         all = true;
-        for (int c = 0; c < ncl; c++) all = all && h[c].done;
+        int active = 0;
+        for (int c = 0; c < ncl; c++) {
+            all = all && h[c].done;
+            active += h[c].done ? 0 : tab[c].nb;
+        }
+        if (!all && active + 8 <= NB) {  // drop the finished cloths from the grid
+            build_map(h);
+            NB = (int)bmap.size();
+            CS_HIP(hipMemcpyAsync(dmap, bmap.data(), sizeof(int2) * NB, hipMemcpyHostToDevice, s));
+        }
     }
     if (!all) throw StatusError(CS_E_HIP, "batched PCG: the device never reported every stop");
     for (int c = 0; c < ncl; c++) {
