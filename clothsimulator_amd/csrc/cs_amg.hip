@@ -162,73 +162,78 @@ __global__ void __launch_bounds__(AB) k_amg_dinv(int n, const int* __restrict__ 
     for (int c = 0; c < 9; c++) Dinv[(size_t)i * 9 + c] = o[c];
 }
 
-// Dense inverse of the coarsest operator (m = 3n <= 192): Gauss-Jordan with
-// partial pivoting (ties: lowest row) in FP64 in one workgroup; A and the
-// identity live in `work` (2 m^2 doubles, L2-resident).
+// Dense inverse of the coarsest operator (m = 3n <= 192, symmetric positive
+// definite) by the symmetric sweep operator in FP64 (Gauss-Jordan without
+// pivoting that keeps the matrix symmetric), the lower triangle held in
+// registers: entry e = r (r + 1) / 2 + k of the triangle is slot e / 1024 of
+// thread e % 1024.  Per pivot c, column c goes through LDS and
+//   a_cc = -1/p,  a_rc = a_rc/p (r != c),  a_rk -= a_rc (a_kc/p) (r, k != c);
+// after every pivot the triangle holds -A^-1.  The operator is SPD, so the
+// pivots are positive Schur-complement diagonals.
 constexpr int CB = 1024;
+constexpr int CT = (3 * AMG_COARSEST * (3 * AMG_COARSEST + 1) / 2 + CB - 1) / CB;  // triangle slots per thread
 __global__ void __launch_bounds__(CB) k_amg_coarse_inverse(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                                            const float* __restrict__ val, double* __restrict__ work,
                                                            float* __restrict__ inv) {
-    __shared__ double sv[CB];
-    __shared__ int si[CB];
-    __shared__ double fac[3 * AMG_COARSEST];
+    __shared__ double colc[3 * AMG_COARSEST];
     const int m = 3 * n, tid = threadIdx.x;
-    double* A = work;
-    double* I = work + (size_t)m * m;
-    for (int e = tid; e < m * m; e += CB) {
-        A[e] = 0.0;
-        I[e] = (e / m == e % m) ? 1.0 : 0.0;
-    }
+    const int T = m * (m + 1) / 2;
+    for (int e = tid; e < m * m; e += CB) work[e] = 0.0;
     __syncthreads();
     for (int i = tid; i < n; i += CB)
-        for (int k = rp[i]; k < rp[i + 1]; k++)
+        for (int q = rp[i]; q < rp[i + 1]; q++)
             for (int a = 0; a < 3; a++)
-                for (int b = 0; b < 3; b++) A[(size_t)(3 * i + a) * m + 3 * ci[k] + b] = val[(size_t)k * 9 + 3 * a + b];
+                for (int b = 0; b < 3; b++) work[(size_t)(3 * i + a) * m + 3 * ci[q] + b] = val[(size_t)q * 9 + 3 * a + b];
     __syncthreads();
+    double v[CT];
+    int rk[CT];  // r << 8 | k (-1: no entry)
+#pragma unroll
+    for (int u = 0; u < CT; u++) {
+        const int e = tid + u * CB;
+        int r = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+        while (r * (r + 1) / 2 > e) r--;
+        while ((r + 1) * (r + 2) / 2 <= e) r++;
+        const int k = e - r * (r + 1) / 2;
+        const bool ok = e < T;
+        rk[u] = ok ? (r << 8 | k) : -1;
+        v[u] = ok ? work[(size_t)r * m + k] : 0.0;
+    }
     for (int c = 0; c < m; c++) {
-        double best = -1.0;
-        int br = m;
-        for (int r = c + tid; r < m; r += CB) {
-            const double v = fabs(A[(size_t)r * m + c]);
-            if (v > best) { best = v; br = r; }
-        }
-        sv[tid] = best;
-        si[tid] = br;
-        __syncthreads();
-        for (int s = CB / 2; s > 0; s >>= 1) {
-            if (tid < s && (sv[tid + s] > sv[tid] || (sv[tid + s] == sv[tid] && si[tid + s] < si[tid]))) {
-                sv[tid] = sv[tid + s];
-                si[tid] = si[tid + s];
-            }
-            __syncthreads();
-        }
-        const int p = si[0];
-        if (p != c)
-            for (int k = tid; k < m; k += CB) {
-                double t = A[(size_t)p * m + k]; A[(size_t)p * m + k] = A[(size_t)c * m + k]; A[(size_t)c * m + k] = t;
-                t = I[(size_t)p * m + k]; I[(size_t)p * m + k] = I[(size_t)c * m + k]; I[(size_t)c * m + k] = t;
-            }
-        __syncthreads();
-        const double d = 1.0 / A[(size_t)c * m + c];
-        __syncthreads();
-        for (int k = tid; k < m; k += CB) {
-            A[(size_t)c * m + k] *= d;
-            I[(size_t)c * m + k] *= d;
+#pragma unroll
+        for (int u = 0; u < CT; u++) {
+            int q = rk[u];
+            asm volatile("" : "+v"(q));  // keeps the derived LDS addresses out of registers across pivots
+            if (q < 0) continue;
+            const int r = q >> 8, k = q & 255;
+            if (k == c) colc[r] = v[u];       // (r, c), r >= c
+            else if (r == c) colc[k] = v[u];  // (c, k), k < c
         }
         __syncthreads();
-        for (int r = tid; r < m; r += CB) fac[r] = r == c ? 0.0 : A[(size_t)r * m + c];
-        __syncthreads();
-        for (int e = tid; e < m * m; e += CB) {
-            const double f = fac[e / m];
-            if (f != 0.0) {
-                const int k = e % m;
-                A[e] -= f * A[(size_t)c * m + k];
-                I[e] -= f * I[(size_t)c * m + k];
-            }
+        const double d = 1.0 / colc[c];
+#pragma unroll
+        for (int u = 0; u < CT; u++) {
+            int q = rk[u];
+            asm volatile("" : "+v"(q));
+            if (q < 0) continue;
+            const int r = q >> 8, k = q & 255;
+            double nv;
+            if (r == c && k == c) nv = -d;
+            else if (r == c) nv = colc[k] * d;
+            else if (k == c) nv = colc[r] * d;
+            else nv = v[u] - colc[r] * (colc[k] * d);
+            v[u] = nv;
+            asm volatile("" ::: "memory");  // one slot's LDS reads at a time (register pressure)
         }
         __syncthreads();
     }
-    for (int e = tid; e < m * m; e += CB) inv[e] = (float)I[e];
+#pragma unroll
+    for (int u = 0; u < CT; u++) {
+        if (rk[u] < 0) continue;
+        const int r = rk[u] >> 8, k = rk[u] & 255;
+        const float x = (float)(-v[u]);
+        inv[(size_t)r * m + k] = x;
+        inv[(size_t)k * m + r] = x;
+    }
 }
 
 // ---------------------------------------------------------------- cycle kernels
@@ -389,69 +394,102 @@ __global__ void __launch_bounds__(AB) k_amg_post(int n, const int* __restrict__ 
 // work; one workgroup runs their whole sub-cycle (pre-smooth + residual,
 // restriction, coarsest solve, prolongation + post-smooth) with barriers
 // between the phases, in the same order and arithmetic as the per-level kernels.
+// Every vector of the sub-cycle lives in LDS (SMAXN nodes over all its levels),
+// so a row's gathers are LDS reads, and each row issues the column and value
+// loads of four blocks at a time.
 constexpr int SMALL = 512;  // measured: one workgroup is slower than per-level launches above this
 constexpr int MAXSMALL = 8;
+constexpr int SMAXN = 1024;
 struct SmallLevel {
-    int n;
+    int n, off;  // nodes, first node in the LDS vectors
     const int *rp, *ci, *agg, *mptr, *midx;
     const float *val, *Dinv;
-    float *b, *x, *t;
 };
 struct SmallCycle {
     int ns;  // levels in the sub-cycle; the last is the coarsest
     int m;   // 3 n of the coarsest
     const float* inv;
+    const float* b0;  // right-hand side of the first level
+    float* out0;      // its result (the coarsest's solution when it is the only level)
     SmallLevel L[MAXSMALL];
 };
 
+// sum over the blocks k0..k1-1 (ascending) of val_k v[ci_k], v an LDS vector
+__device__ __forceinline__ V3f row_lds(int k0, int k1, const int* __restrict__ ci, const float* __restrict__ val,
+                                       const float* v) {
+    float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+    for (int k = k0; k < k1; k += 4) {
+        int j[4];
+        float a[4][9];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int kk = min(k + u, k1 - 1);
+            j[u] = ci[kk];
+#pragma unroll
+            for (int c = 0; c < 9; c++) a[u][c] = val[(size_t)kk * 9 + c];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (k + u < k1) {
+                const float* x = v + 3 * j[u];
+                const float x0 = x[0], x1 = x[1], x2 = x[2];
+                y0 += a[u][0] * x0 + a[u][1] * x1 + a[u][2] * x2;
+                y1 += a[u][3] * x0 + a[u][4] * x1 + a[u][5] * x2;
+                y2 += a[u][6] * x0 + a[u][7] * x1 + a[u][8] * x2;
+            }
+    }
+    return mk(y0, y1, y2);
+}
+__device__ __forceinline__ V3f ld3s(const float* v, int i) { return mk(v[3 * i], v[3 * i + 1], v[3 * i + 2]); }
+__device__ __forceinline__ void st3s(float* v, int i, V3f a) { v[3 * i] = a.x; v[3 * i + 1] = a.y; v[3 * i + 2] = a.z; }
+
 __global__ void __launch_bounds__(SB) k_amg_small_cycle(SmallCycle Q) {
+    __shared__ float vb[3 * SMAXN], vx[3 * SMAXN], vt[3 * SMAXN];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int e = tid; e < 3 * Q.L[0].n; e += SB) vb[3 * Q.L[0].off + e] = Q.b0[e];
+    __syncthreads();
     for (int l = 0; l + 1 < Q.ns; l++) {
         const SmallLevel& L = Q.L[l];
-        for (int i = tid; i < L.n; i += SB) {
-            float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
-            for (int k = L.rp[i]; k < L.rp[i + 1]; k++) {
-                const int j = L.ci[k];
-                const V3f xj = AMG_OMEGA * mat3v(L.Dinv + 9 * (size_t)j, ld3v(L.b, j));
-                const float* a = L.val + (size_t)k * 9;
-                y0 += a[0] * xj.x + a[1] * xj.y + a[2] * xj.z;
-                y1 += a[3] * xj.x + a[4] * xj.y + a[5] * xj.z;
-                y2 += a[6] * xj.x + a[7] * xj.y + a[8] * xj.z;
-            }
-            const V3f bi = ld3v(L.b, i);
-            st3v(L.x, i, AMG_OMEGA * mat3v(L.Dinv + 9 * (size_t)i, bi));
-            st3v(L.t, i, bi - mk(y0, y1, y2));
-        }
+        float *b = vb + 3 * L.off, *x = vx + 3 * L.off, *t = vt + 3 * L.off;
+        for (int i = tid; i < L.n; i += SB) st3s(x, i, AMG_OMEGA * mat3v(L.Dinv + 9 * (size_t)i, ld3s(b, i)));
+        __syncthreads();
+        for (int i = tid; i < L.n; i += SB) st3s(t, i, ld3s(b, i) - row_lds(L.rp[i], L.rp[i + 1], L.ci, L.val, x));
         __syncthreads();
         const SmallLevel& N = Q.L[l + 1];
+        float* nb = vb + 3 * N.off;
         for (int I = tid; I < N.n; I += SB) {
-            V3f s = mk(0.0f, 0.0f, 0.0f);
-            for (int k = L.mptr[I]; k < L.mptr[I + 1]; k++) s = s + ld3v(L.t, L.midx[k]);
-            st3v(N.b, I, s);
+            V3f sum = mk(0.0f, 0.0f, 0.0f);
+            for (int k = L.mptr[I]; k < L.mptr[I + 1]; k++) sum = sum + ld3s(t, L.midx[k]);
+            st3s(nb, I, sum);
         }
         __syncthreads();
     }
     {
         const SmallLevel& C = Q.L[Q.ns - 1];
+        const float* cb = vb + 3 * C.off;
         for (int r = wave; r < Q.m; r += SB / 64) {
-            float s = 0.0f;
-            for (int k = lane; k < Q.m; k += 64) s += Q.inv[(size_t)r * Q.m + k] * C.b[k];
+            float sum = 0.0f;
+            for (int k = lane; k < Q.m; k += 64) sum += Q.inv[(size_t)r * Q.m + k] * cb[k];
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-            if (lane == 0) C.x[r] = s;
+            for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+            if (lane == 0) vx[3 * C.off + r] = sum;
         }
         __syncthreads();
     }
+    // the result of every level is left in vx
     for (int l = Q.ns - 2; l >= 0; l--) {
         const SmallLevel& L = Q.L[l];
-        const float* e = (l + 1 == Q.ns - 1) ? Q.L[l + 1].x : Q.L[l + 1].t;
+        const float* e = vx + 3 * Q.L[l + 1].off;
+        float *b = vb + 3 * L.off, *x = vx + 3 * L.off, *xp = vt + 3 * L.off;
+        for (int i = tid; i < L.n; i += SB) st3s(xp, i, ld3s(x, i) + ld3s(e, L.agg[i]));
+        __syncthreads();
         for (int i = tid; i < L.n; i += SB) {
-            const V3f ax = rowc<true>(i, L.rp, L.ci, L.val, L.x, L.agg, e);
-            const V3f xi = ld3v(L.x, i) + ld3v(e, L.agg[i]);
-            st3v(L.t, i, xi + AMG_OMEGA * mat3v(L.Dinv + 9 * (size_t)i, ld3v(L.b, i) - ax));
+            const V3f ax = row_lds(L.rp[i], L.rp[i + 1], L.ci, L.val, xp);
+            st3s(x, i, ld3s(xp, i) + AMG_OMEGA * mat3v(L.Dinv + 9 * (size_t)i, ld3s(b, i) - ax));
         }
         __syncthreads();
     }
+    for (int e = tid; e < 3 * Q.L[0].n; e += SB) Q.out0[e] = vx[3 * Q.L[0].off + e];
 }
 
 // AP = A_static P of level 0, per step: each AP slot sums the level-0 blocks of
@@ -786,6 +824,12 @@ void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int n
     for (int l = 1; l < G.nl; l++)
         if (G.L[l].n <= SMALL) { l0 = l; break; }
     l0 = std::max(l0, G.nl - MAXSMALL);
+    for (;;) {  // the small levels' vectors must fit the workgroup's LDS
+        int tot = 0;
+        for (int l = l0; l < G.nl; l++) tot += G.L[l].n;
+        if (tot <= SMAXN || l0 == G.nl - 1) break;
+        l0++;
+    }
     // down: smooth, residual, restrict
     for (int l = 0; l < l0; l++) {
         AmgLevel& L = G.L[l];
@@ -807,11 +851,16 @@ void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int n
     Q.ns = G.nl - l0;
     Q.m = 3 * G.L[G.nl - 1].n;
     Q.inv = G.inv.ptr<float>();
+    Q.b0 = G.L[l0].b.ptr<float>();
+    Q.out0 = l0 == G.nl - 1 ? G.L[l0].x.ptr<float>() : G.L[l0].t.ptr<float>();
+    int off = 0;
     for (int k = 0; k < Q.ns; k++) {
         AmgLevel& L = G.L[l0 + k];
         const bool last = l0 + k == G.nl - 1;
         SmallLevel& q = Q.L[k];
         q.n = L.n;
+        q.off = off;
+        off += L.n;
         q.rp = L.rp.ptr<int>();
         q.ci = L.ci.ptr<int>();
         q.val = L.val.ptr<float>();
@@ -819,9 +868,6 @@ void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int n
         q.agg = last ? nullptr : L.agg.ptr<int>();
         q.mptr = last ? nullptr : L.mptr.ptr<int>();
         q.midx = last ? nullptr : L.midx.ptr<int>();
-        q.b = L.b.ptr<float>();
-        q.x = L.x.ptr<float>();
-        q.t = L.t.ptr<float>();
     }
     hipLaunchKernelGGL(k_amg_small_cycle, dim3(1), dim3(SB), 0, s, Q);
     // up: prolongate and post-smooth; the result of level l is in L[l].t (coarsest: x)
