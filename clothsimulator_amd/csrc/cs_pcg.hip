@@ -19,6 +19,7 @@
 // at kernel entry and reduced after the kernel's memory sweep (SpMV) or after
 // its row loads (update), so their latency hides behind the sweep.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "cs_kernels.hpp"
@@ -39,8 +40,10 @@ struct PcgDev {
     int iters;  // iterations completed when the loop stopped
     int done;
     int maxit;
-    int pad;
+    int last;   // iteration whose stop test produced rrcur
     float tol2bb, bb, rr, rz;
+    float rrcur;  // r.r seen by the latest stop test
+    float pad[3];
 };
 
 // Workgroups are dealt to the 8 XCDs round-robin.  Logical block L gives each
@@ -249,6 +252,10 @@ __global__ void __launch_bounds__(PB) k_pcg_spmv(int it, int nn, const int* __re
         if (!FIRST && first_row) {
             float tot[3];
             pr.sums(part, slot, nb, lds, tot);
+            if (L == 0 && threadIdx.x == 0) {  // progress for the host's choice of chunk length
+                sc->rrcur = tot[0];
+                sc->last = it;
+            }
             if (tot[0] <= sc->tol2bb || it >= sc->maxit) {  // the same decision in every block
                 if (L == 0 && threadIdx.x == 0 && !sc->done) {
                     sc->iters = it;
@@ -620,16 +627,23 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     hipLaunchKernelGGL(k_pcg_setup, dim3(1), dim3(PB), 0, s, nb, part, sc, tol * tol, maxit);
     CS_LAUNCH_CHECK();
     // iterations per host poll: a multigrid iteration costs several times a Jacobi one
-    const int chunk = (amg || amgl) ? 4 : learned ? 8 : 16;
+    // Iterations per host poll.  The first chunk has a fixed length; later ones
+    // aim at the stop: the host extrapolates the r.r decrease between its last two
+    // polls to the iteration that reaches tol^2 b.b (between 1 and 4x the first
+    // length).  Each
+    // iteration launched past the stop costs a full multigrid or learned
+    // iteration (only the SpMV returns early), each poll one host round trip.
+    const int chunk0 = (amg || amgl) ? 4 : learned ? 8 : 16;
+    int chunk = chunk0;
     int done = 0;
     PcgDev h{};
     int it = 0;
-    while (!done && it <= maxit + 2 * chunk) {
+    float rr0 = -1.0f;  // r.r at the previous poll, and its iteration
+    int it0 = 0;
+    while (!done && it <= maxit + 8 * chunk0) {
         for (int k = 0; k < chunk; k++, it++) {
-            // the second SpMV of a chunk is sampled (the first follows a host wait), after an
-            // empty pair that measures the markers' own latency
-            // sampled: the first full-batch SpMV after the initial one (every cloth still active),
-            // after an empty pair that measures the markers' own latency
+            // sampled: the SpMV of iteration 1 of every solve, after an empty pair that
+            // measures the markers' own latency
             if (it == 1) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
             const int tok = it == 1 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
             float4* dcur = it == 0 ? dgen[0] : dgen[it & 1];
@@ -666,6 +680,17 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
         std::memcpy(&h, C->hScal, sizeof(PcgDev));
         done = h.done;
         ktimer_collect(C);
+        if (done) break;
+        chunk = chunk0;
+        if (rr0 > 0.0f && h.rrcur > 0.0f && rr0 > h.rrcur && h.last > it0) {
+            // geometric extrapolation of the latest r.r decrease
+            const double rate = std::log((double)h.rrcur / rr0) / (h.last - it0);  // < 0
+            const double need = std::log((double)h.tol2bb / h.rrcur) / rate;       // iterations to go
+            const int left = (int)std::ceil(need) - (it - 1 - h.last);             // beyond those launched
+            chunk = std::max(1, std::min(4 * chunk0, left));
+        }
+        rr0 = h.rrcur;
+        it0 = h.last;
     }
     if (!done) throw StatusError(CS_E_HIP, "PCG: the device never reported its stop");
     if (relres) *relres = h.bb > 0.0f ? sqrtf(h.rr / h.bb) : 0.0f;
