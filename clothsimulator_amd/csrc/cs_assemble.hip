@@ -691,6 +691,7 @@ static RowShared row_shared(cs_ctx* C) {
 void launch_assemble_rows(cs_ctx* C, int c) {
     ClothSys& S = C->sys[c];
     S.solved = false;
+    S.asmEpoch++;
     size_t lds = (size_t)S.maxWidth * 9 * SLICE * sizeof(float);
     const int tok = ktimer_begin(C, CS_KERNEL_ASSEMBLE_ROWS);
     hipLaunchKernelGGL(k_assemble_rows, dim3(S.nslices), dim3(SLICE), lds, C->stream, row_args(C, c), row_shared(C));
@@ -724,6 +725,7 @@ int assemble_all(cs_ctx* C) {
     for (int c = 0; c < ncl; c++) {
         ClothSys& S = C->sys[c];
         S.solved = false;
+        S.asmEpoch++;
         AsmJob& J = jobs[c];
         J.R = row_args(C, c);
         J.slice0 = slices;

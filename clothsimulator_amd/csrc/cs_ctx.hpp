@@ -101,10 +101,12 @@ struct LpcSys {
     DBuf e;       // bf16 64 per slot: edge states
     // learned smoother of the multigrid cycle: power-iteration vectors and partials, damping
     DBuf pv, pw, ppart, omega;
+    unsigned omegaAsm = ~0u, omegaNet = ~0u;  // assembly and weights the current omega was estimated for
 };
 
 struct ClothSys {
     int mesh;
+    unsigned asmEpoch = 0;  // incremented by every assembly of the system
     int nn, nslices, totalSlots, maxWidth;
     // static sliced-ELL pattern
     DBuf sliceOff;  // int[nslices+1] (in slots)
@@ -143,6 +145,7 @@ struct LpcNet {
     // training: gradient, Adam moments and step count, activation arena, job table
     DBuf grad, m, v, arena, jobs;
     int t = 0;
+    unsigned version = 0;  // incremented whenever the weights change
 };
 
 struct PcgScalars {

@@ -1276,6 +1276,7 @@ void lpc_set_weights(cs_ctx* C, const float* w) {
 
 void lpc_refresh_fragments(cs_ctx* C) {
     LpcNet& N = C->lpcNet;
+    N.version++;  // every weight change ends here
     for (const MatDesc& m : kMats) {
         const int nt = m.K * m.N / 4;
         hipLaunchKernelGGL(k_lpc_frag, dim3(cdiv(nt, 256)), dim3(256), 0, C->stream, N.w.ptr<float>(),
@@ -1431,6 +1432,9 @@ void lpc_smoother_prepare(cs_ctx* C, int c) {
     L.omega.as<float>(1);
     const int* cCol = S.nContact > 0 ? S.cCol.ptr<int>() : S.cRowPtr.ptr<int>();
     const float* cVal = S.nContact > 0 ? S.cVal.ptr<float>() : S.vals.ptr<float>();
+    // 12 sweeps from the hashed start, once per assembled system and set of weights.  (A
+    // warm start from the previous step's vector with fewer sweeps misses the new
+    // dominant modes that fresh contact blocks bring, and the smoother then diverges.)
     hipLaunchKernelGGL(k_lpc_pow_init, dim3(g), dim3(256), 0, s, S.nn, v);
     for (int it = 0; it < 12; it++) {
         hipLaunchKernelGGL(k_lpc_pow_spmv, dim3(g), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(),
@@ -1443,6 +1447,8 @@ void lpc_smoother_prepare(cs_ctx* C, int c) {
     }
     hipLaunchKernelGGL(k_lpc_omega, dim3(1), dim3(256), 0, s, part, g, 1.3f, L.omega.ptr<float>());
     CS_LAUNCH_CHECK();
+    L.omegaAsm = S.asmEpoch;
+    L.omegaNet = C->lpcNet.version;
 }
 
 void lpc_smooth_pre(cs_ctx* C, int c, const float* r, float* x) {
