@@ -288,19 +288,42 @@ __device__ __forceinline__ V3f row0(int i, const int* __restrict__ sliceOff, con
 }
 
 // (A y)_i over a block-CSR level
+// Row i of a block-CSR level times x (PROLONG: times x + e gathered through agg),
+// blocks in ascending order; the column, value and gather loads of four blocks
+// are issued together (the sum is formed in the same order as one at a time).
 template <bool PROLONG>
 __device__ __forceinline__ V3f rowc(int i, const int* __restrict__ rp, const int* __restrict__ ci,
                                     const float* __restrict__ val, const float* __restrict__ x,
                                     const int* __restrict__ agg, const float* __restrict__ e) {
     float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
-    for (int k = rp[i]; k < rp[i + 1]; k++) {
-        const int j = ci[k];
-        V3f xj = ld3v(x, j);
-        if (PROLONG) xj = xj + ld3v(e, agg[j]);
-        const float* a = val + (size_t)k * 9;
-        y0 += a[0] * xj.x + a[1] * xj.y + a[2] * xj.z;
-        y1 += a[3] * xj.x + a[4] * xj.y + a[5] * xj.z;
-        y2 += a[6] * xj.x + a[7] * xj.y + a[8] * xj.z;
+    const int k0 = rp[i], k1 = rp[i + 1];
+    for (int k = k0; k < k1; k += 4) {
+        int j[4];
+        float a[4][9];
+        V3f xj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int kk = min(k + u, k1 - 1);
+            j[u] = ci[kk];
+#pragma unroll
+            for (int c = 0; c < 9; c++) a[u][c] = val[(size_t)kk * 9 + c];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) xj[u] = ld3v(x, j[u]);
+        if (PROLONG) {
+            int g[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) g[u] = agg[j[u]];
+#pragma unroll
+            for (int u = 0; u < 4; u++) xj[u] = xj[u] + ld3v(e, g[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (k + u < k1) {
+                y0 += a[u][0] * xj[u].x + a[u][1] * xj[u].y + a[u][2] * xj[u].z;
+                y1 += a[u][3] * xj[u].x + a[u][4] * xj[u].y + a[u][5] * xj[u].z;
+                y2 += a[u][6] * xj[u].x + a[u][7] * xj[u].y + a[u][8] * xj[u].z;
+            }
     }
     return mk(y0, y1, y2);
 }
@@ -338,13 +361,27 @@ __global__ void __launch_bounds__(AB) k_amg_presid(int n, const int* __restrict_
     const int i = blockIdx.x * AB + threadIdx.x;
     if (i >= n) return;
     float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
-    for (int k = rp[i]; k < rp[i + 1]; k++) {
-        const int j = ci[k];
-        const V3f xj = AMG_OMEGA * mat3v(Dinv + 9 * (size_t)j, ld3v(b, j));
-        const float* a = val + (size_t)k * 9;
-        y0 += a[0] * xj.x + a[1] * xj.y + a[2] * xj.z;
-        y1 += a[3] * xj.x + a[4] * xj.y + a[5] * xj.z;
-        y2 += a[6] * xj.x + a[7] * xj.y + a[8] * xj.z;
+    const int k0 = rp[i], k1 = rp[i + 1];
+    for (int k = k0; k < k1; k += 4) {  // four blocks' loads at a time, summed in order
+        int j[4];
+        float a[4][9];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int kk = min(k + u, k1 - 1);
+            j[u] = ci[kk];
+#pragma unroll
+            for (int c = 0; c < 9; c++) a[u][c] = val[(size_t)kk * 9 + c];
+        }
+        V3f xj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) xj[u] = AMG_OMEGA * mat3v(Dinv + 9 * (size_t)j[u], ld3v(b, j[u]));
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (k + u < k1) {
+                y0 += a[u][0] * xj[u].x + a[u][1] * xj[u].y + a[u][2] * xj[u].z;
+                y1 += a[u][3] * xj[u].x + a[u][4] * xj[u].y + a[u][5] * xj[u].z;
+                y2 += a[u][6] * xj[u].x + a[u][7] * xj[u].y + a[u][8] * xj[u].z;
+            }
     }
     const V3f bi = ld3v(b, i);
     st3v(x, i, AMG_OMEGA * mat3v(Dinv + 9 * (size_t)i, bi));
@@ -357,7 +394,15 @@ __global__ void __launch_bounds__(AB) k_amg_restrict(int nc, const int* __restri
     const int I = blockIdx.x * AB + threadIdx.x;
     if (I >= nc) return;
     V3f s = mk(0.0f, 0.0f, 0.0f);
-    for (int k = mptr[I]; k < mptr[I + 1]; k++) s = s + ld3v(t, midx[k]);
+    const int k0 = mptr[I], k1 = mptr[I + 1];
+    for (int k = k0; k < k1; k += 4) {  // four members' loads at a time, summed in order
+        V3f m[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) m[u] = ld3v(t, midx[min(k + u, k1 - 1)]);
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (k + u < k1) s = s + m[u];
+    }
     st3v(bc, I, s);
 }
 
