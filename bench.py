@@ -139,6 +139,8 @@ def run(workload, precond, steps, warmup, timed):
     if timed:
         out["spmv"] = sim.kernel_time(CS_KERNEL_PCG_SPMV)
         out["pair"] = sim.kernel_time(CS_KERNEL_EVENT_PAIR)
+    if len(spec.cloths) > 1:
+        out["batch_tuning"] = sim.batch_tuning()
     sim.close()
     return out
 
@@ -262,6 +264,8 @@ def main():
     if a.workload.startswith("batch") and a.sequential_sample > 0:
         out["sequential"] = sequential_sample(spec, a.precond, steps, a.sequential_sample, ms["value"])
         out["config"]["instances"] = len(spec.cloths)
+        rows, chunk, trials = main_run["batch_tuning"]
+        out["batch_launch"] = {"rows_per_block": rows, "iterations_per_poll": chunk, "solves_timed_by_autotuner": trials}
     if "jacobi" in others:
         out["speedup_vs_jacobi"] = ms["value"] / others["jacobi"]["value"]
     if not a.no_cpu:

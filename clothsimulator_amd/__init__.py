@@ -24,7 +24,7 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_lbvh_pairs", "cs_find_proximities", "cs_assemble", "cs_solve", "cs_find_impacts",
             "cs_independent_impacts", "cs_kernel_time", "cs_load_scene", "cs_scene_error", "cs_precondition",
             "cs_lpc_weight_count", "cs_set_learned_weights", "cs_get_learned_weights", "cs_lpc_factor",
-            "cs_lpc_train_step", "cs_lpc_smoother_omega"]
+            "cs_lpc_train_step", "cs_lpc_smoother_omega", "cs_batch_tuning"]
 
 # trained weights of the learned preconditioner shipped with the package (float32, .npy)
 LPC_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lpc_weights.npy")
@@ -81,6 +81,7 @@ def load_library(path=LIB_PATH):
     L.cs_lpc_factor.argtypes = [vp, C.c_int, F, F, C.c_int, C.POINTER(C.c_int)]
     L.cs_lpc_train_step.argtypes = [vp, C.c_int, C.c_float, C.c_float, F, F]
     L.cs_lpc_smoother_omega.argtypes = [vp, C.c_int, F]
+    L.cs_batch_tuning.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     _lib = L
     return L
 
@@ -331,6 +332,12 @@ class Simulator:
         w = np.zeros(1, dtype=np.float32)
         self._check(self.L.cs_lpc_smoother_omega(self.h, cloth, fptr(w)), "cs_lpc_smoother_omega")
         return float(w[0])
+
+    def batch_tuning(self):
+        """(rows per workgroup, iterations per poll, solves timed) of the batched instance solve."""
+        r, c, t = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.L.cs_batch_tuning(self.h, C.byref(r), C.byref(c), C.byref(t)), "cs_batch_tuning")
+        return r.value, c.value, t.value
 
     def kernel_time(self, kernel_id, reset=False):
         """(summed ms, sampled launches) of a hot kernel; needs CS_FLAG_TIME_KERNELS in params.flags."""

@@ -699,6 +699,14 @@ cs_status cs_lpc_train_step(cs_ctx* C, int cloth_id, float lr, float sol_weight,
     });
 }
 
+cs_status cs_batch_tuning(const cs_ctx* C, int* rows_per_block, int* chunk, int* trials) {
+    if (!C || !rows_per_block || !chunk || !trials) return CS_E_ARG;
+    *rows_per_block = C->batchTune.rows;
+    *chunk = C->batchTune.chunk;
+    *trials = C->batchTune.trials;
+    return CS_OK;
+}
+
 cs_status cs_lpc_smoother_omega(cs_ctx* C, int cloth_id, float* omega) {
     if (!C || !omega) return CS_E_ARG;
     if (cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;

@@ -210,6 +210,11 @@ struct cs_ctx {
     cs::PcgScalars* hScal = nullptr;  // pinned
     // batched solve of independent cloths: per-cloth table, block -> cloth map, partials, scalars
     cs::DBuf bTab, bMap, bPart, bScal;
+    struct BatchTune {
+        int trials = 0, choice = 0;    // solves tried so far, best candidate
+        double score[4] = {0, 0, 0, 0};  // seconds per node-iteration, summed over trials
+        int rows = 256, chunk = 16;      // the setting in use
+    } batchTune;
     void* hBatch = nullptr;  // pinned readback of per-cloth data (scalars, contact bounds)
     size_t hBatchCap = 0;    // bytes
     cs::DBuf asmJobs;        // per-cloth table of the batched assembly

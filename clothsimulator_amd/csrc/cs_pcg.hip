@@ -19,6 +19,7 @@
 // at kernel entry and reduced after the kernel's memory sweep (SpMV) or after
 // its row loads (update), so their latency hides behind the sweep.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -364,16 +365,43 @@ struct BCloth {
     PcgDev* sc;
 };
 
-// Partials of NS slots of one cloth (nb <= 4 PB blocks in the fast path):
+// Sum of NS values over a BP-thread block (every thread gets the sums).
+template <int NS, int BP>
+__device__ __forceinline__ void bsums(float (&v)[NS], float* lds) {
+    constexpr int NW = BP / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[s] += __shfl_xor(v[s], o, 64);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int s = 0; s < NS; s++) lds[NW * s + wave] = v[s];
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NS; s++) {  // pairwise tree over the waves (block_sums' order at 4 waves)
+        float a[NW];
+#pragma unroll
+        for (int w = 0; w < NW; w++) a[w] = lds[NW * s + w];
+#pragma unroll
+        for (int h = NW / 2; h > 0; h >>= 1)
+#pragma unroll
+            for (int w = 0; w < h; w++) a[w] = a[2 * w] + a[2 * w + 1];
+        v[s] = a[0];
+    }
+}
+
+// Partials of NS slots of one cloth (nb <= 4 BP blocks in the fast path):
 // issue() loads this thread's share unconditionally, sums() reduces them.
-template <int NS>
+template <int NS, int BP>
 struct BPartials {
     float raw[NS][4];
     __device__ __forceinline__ void issue(const float* __restrict__ part, const int (&slot)[NS], int nb) {
 #pragma unroll
         for (int s = 0; s < NS; s++)
 #pragma unroll
-            for (int u = 0; u < 4; u++) raw[s][u] = part[slot[s] * nb + min((int)threadIdx.x + u * PB, nb - 1)];
+            for (int u = 0; u < 4; u++) raw[s][u] = part[slot[s] * nb + min((int)threadIdx.x + u * BP, nb - 1)];
     }
     __device__ __forceinline__ void sums(const float* __restrict__ part, const int (&slot)[NS], int nb, float* lds,
                                          float (&out)[NS]) {
@@ -385,11 +413,11 @@ struct BPartials {
         for (int s = 0; s < NS; s++) {
             float a = 0.0f;
 #pragma unroll
-            for (int u = 0; u < 4; u++) a += (int)threadIdx.x + u * PB < nb ? raw[s][u] : 0.0f;
-            for (int b = threadIdx.x + 4 * PB; b < nb; b += PB) a += part[slot[s] * nb + b];
+            for (int u = 0; u < 4; u++) a += (int)threadIdx.x + u * BP < nb ? raw[s][u] : 0.0f;
+            for (int b = threadIdx.x + 4 * BP; b < nb; b += BP) a += part[slot[s] * nb + b];
             out[s] = a;
         }
-        block_sums<NS>(out, lds);
+        bsums<NS, BP>(out, lds);
     }
 };
 
@@ -405,13 +433,14 @@ __device__ __forceinline__ int2 batch_block(const int2* __restrict__ bmap) {
     return make_int2(__builtin_amdgcn_readfirstlane(m.x), __builtin_amdgcn_readfirstlane(m.y));
 }
 
-__global__ void __launch_bounds__(PB) k_bpcg_init(const BCloth* __restrict__ cl, const int2* __restrict__ bmap) {
-    __shared__ float lds[8];
+template <int BP>
+__global__ void __launch_bounds__(BP) k_bpcg_init(const BCloth* __restrict__ cl, const int2* __restrict__ bmap) {
+    __shared__ float lds[2 * BP / 64];
     const int2 m = batch_block(bmap);
     if (m.x < 0) return;
     const BCloth& c = cl[m.x];
     const int lb = m.y;
-    const int i = lb * PB + threadIdx.x;
+    const int i = lb * BP + threadIdx.x;
     float acc[2] = {0.0f, 0.0f};
     if (i < c.nn) {
         const V3f bi = ld3v(c.b, i) + ld3v(c.bc, i);
@@ -422,7 +451,7 @@ __global__ void __launch_bounds__(PB) k_bpcg_init(const BCloth* __restrict__ cl,
         acc[0] = dot(bi, zi);
         acc[1] = dot(bi, bi);
     }
-    block_sums<2>(acc, lds);
+    bsums<2, BP>(acc, lds);
     if (threadIdx.x == 0) {
         c.part[P_RZ0 * c.nb + lb] = acc[0];
         c.part[P_BB * c.nb + lb] = acc[1];
@@ -430,11 +459,12 @@ __global__ void __launch_bounds__(PB) k_bpcg_init(const BCloth* __restrict__ cl,
 }
 
 // one block per cloth
-__global__ void __launch_bounds__(PB) k_bpcg_setup(const BCloth* __restrict__ cl, float tol2, int maxit) {
-    __shared__ float lds[8];
+template <int BP>
+__global__ void __launch_bounds__(BP) k_bpcg_setup(const BCloth* __restrict__ cl, float tol2, int maxit) {
+    __shared__ float lds[2 * BP / 64];
     const BCloth c = cl[blockIdx.x];
     const int slot[2] = {P_BB, P_RZ0};
-    BPartials<2> pr;
+    BPartials<2, BP> pr;
     pr.issue(c.part, slot, c.nb);
     float v[2];
     pr.sums(c.part, slot, c.nb, lds, v);
@@ -453,21 +483,21 @@ __global__ void __launch_bounds__(PB) k_bpcg_setup(const BCloth* __restrict__ cl
 // t = A d_prev first, then the stop test from the r.r partials (loaded at entry),
 // beta, q = s + beta t, d = z + beta d_prev and the partial d.q.  A stopped
 // cloth's partials no longer change, so its blocks keep deciding to stop.
-template <bool FIRST>
-__global__ void __launch_bounds__(PB) k_bpcg_spmv(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
+template <int BP, bool FIRST>
+__global__ void __launch_bounds__(BP) k_bpcg_spmv(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
                                                   int it) {
-    __shared__ float lds[16];
+    __shared__ float lds[3 * BP / 64];
     const int2 m = batch_block(bmap);
     if (m.x < 0) return;
     const BCloth& c = cl[m.x];
     const int lb = m.y;
     const int p = it & 1;
     const int slot[3] = {P_RR, P_RZ0 + p, P_RZ0 + (1 - p)};
-    BPartials<3> pr;
+    BPartials<3, BP> pr;
     if (!FIRST) pr.issue(c.part, slot, c.nb);
     if (FIRST && c.sc->done) return;
     const float4* dprev = p ? c.d0 : c.d1;
-    const int i = lb * PB + threadIdx.x;
+    const int i = lb * BP + threadIdx.x;
     const bool live = i < c.nn;
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
     if (live) {
@@ -525,25 +555,30 @@ __global__ void __launch_bounds__(PB) k_bpcg_spmv(const BCloth* __restrict__ cl,
         st3v(c.q, i, qi);
         pdq = dot(di, qi);
     }
-    pdq = block_sum(pdq, lds);
+    {
+        float t1[1] = {pdq};
+        bsums<1, BP>(t1, lds);
+        pdq = t1[0];
+    }
     if (threadIdx.x == 0) c.part[P_DQ * c.nb + lb] = pdq;
 }
 
 // alpha, x += alpha d, r -= alpha q, z = Minv r, partials r.z and r.r (the row
 // loads are issued before the partials are reduced)
-__global__ void __launch_bounds__(PB) k_bpcg_update(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
+template <int BP>
+__global__ void __launch_bounds__(BP) k_bpcg_update(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
                                                     int it) {
-    __shared__ float lds[8];
+    __shared__ float lds[2 * BP / 64];
     const int2 m = batch_block(bmap);
     if (m.x < 0) return;
     const BCloth& c = cl[m.x];
     const int lb = m.y;
     const int p = it & 1;
     const int slot[2] = {P_DQ, P_RZ0 + p};
-    BPartials<2> pr;
+    BPartials<2, BP> pr;
     pr.issue(c.part, slot, c.nb);
     const float4* d = it == 0 ? c.d0 : (p ? c.d1 : c.d0);
-    const int i = lb * PB + threadIdx.x;
+    const int i = lb * BP + threadIdx.x;
     const bool live = i < c.nn;
     V3f di = mk(0.0f, 0.0f, 0.0f), xi = di, ri = di, qi = di;
     if (live) {
@@ -567,7 +602,7 @@ __global__ void __launch_bounds__(PB) k_bpcg_update(const BCloth* __restrict__ c
         acc[0] = dot(ri, zi);
         acc[1] = dot(ri, ri);
     }
-    block_sums<2>(acc, lds);
+    bsums<2, BP>(acc, lds);
     if (threadIdx.x == 0) {
         c.part[(P_RZ0 + (1 - p)) * c.nb + lb] = acc[0];
         c.part[P_RR * c.nb + lb] = acc[1];
@@ -699,15 +734,15 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     return h.iters;
 }
 
-// All cloths' systems (assembled, Minv current) solved together with block-Jacobi PCG.
-// Per cloth: iterations, relative residual and convergence, as pcg_solve.
-void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged) {
+// The batched solve with BP rows per block and `chunk` iterations per host poll.
+template <int BP>
+static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* converged) {
     hipStream_t s = C->stream;
     const int ncl = C->nCloths;
     std::vector<BCloth> tab(ncl);
     size_t partTotal = 0;
     for (int c = 0; c < ncl; c++) {
-        tab[c].nb = std::max(1, cdiv(C->sys[c].nn, PB));
+        tab[c].nb = std::max(1, cdiv(C->sys[c].nn, BP));
         partTotal += (size_t)P_COUNT * tab[c].nb;
     }
     float* part = C->bPart.as<float>(partTotal);
@@ -754,11 +789,10 @@ void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged) {
     CS_HIP(hipMemcpyAsync(dmap, bmap.data(), sizeof(int2) * NB, hipMemcpyHostToDevice, s));
     const float tol = C->P.pcg_rel_tol;
     const int maxit = C->P.pcg_max_iters;
-    hipLaunchKernelGGL(k_bpcg_init, dim3(NB), dim3(PB), 0, s, dtab, dmap);
-    hipLaunchKernelGGL(k_bpcg_setup, dim3(ncl), dim3(PB), 0, s, dtab, tol * tol, maxit);
+    hipLaunchKernelGGL(k_bpcg_init<BP>, dim3(NB), dim3(BP), 0, s, dtab, dmap);
+    hipLaunchKernelGGL(k_bpcg_setup<BP>, dim3(ncl), dim3(BP), 0, s, dtab, tol * tol, maxit);
     CS_LAUNCH_CHECK();
     const PcgDev* h = reinterpret_cast<const PcgDev*>(pinned_batch(C, sizeof(PcgDev) * ncl));
-    const int chunk = 16;
     int it = 0;
     bool all = false;
     while (!all && it <= maxit + 2 * chunk) {
@@ -768,11 +802,11 @@ void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged) {
             if (it == 1) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
             const int tok = it == 1 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
             if (it == 0)
-                hipLaunchKernelGGL(k_bpcg_spmv<true>, dim3(NB), dim3(PB), 0, s, dtab, dmap, it);
+                hipLaunchKernelGGL((k_bpcg_spmv<BP, true>), dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
             else
-                hipLaunchKernelGGL(k_bpcg_spmv<false>, dim3(NB), dim3(PB), 0, s, dtab, dmap, it);
+                hipLaunchKernelGGL((k_bpcg_spmv<BP, false>), dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
             ktimer_end(C, tok);
-            hipLaunchKernelGGL(k_bpcg_update, dim3(NB), dim3(PB), 0, s, dtab, dmap, it);
+            hipLaunchKernelGGL(k_bpcg_update<BP>, dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
         }
         CS_LAUNCH_CHECK();
         CS_HIP(hipMemcpyAsync(C->hBatch, sc, sizeof(PcgDev) * ncl, hipMemcpyDeviceToHost, s));
@@ -797,6 +831,43 @@ void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged) {
         converged[c] = h[c].rr <= h[c].tol2bb;
         C->sys[c].solved = converged[c] != 0;
     }
+}
+
+// Batched-solve autotuner: the first solves of a context try every (rows per
+// block, iterations per poll) candidate twice, round robin, and score each by
+// host wall time per node-iteration (sum over cloths of iterations x rows); the
+// best is kept for the context's later solves.  The rows per block set the
+// grouping of the dot-product partials, so results of different candidates agree
+// to rounding; CS_FLAG_FIXED_TUNING keeps (256, 16) for bit-reproducible runs.
+static const int kTuneRows[] = {256, 128, 256, 128};
+static const int kTuneChunk[] = {16, 16, 32, 32};
+constexpr int kTuneN = 4, kTuneRounds = 2;
+
+// All cloths' systems (assembled, Minv current) solved together with block-Jacobi PCG.
+// Per cloth: iterations, relative residual and convergence, as pcg_solve.
+void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged) {
+    auto& T = C->batchTune;
+    int cand = T.choice;
+    const bool tuning = !(C->P.flags & CS_FLAG_FIXED_TUNING) && T.trials < kTuneN * kTuneRounds;
+    if (C->P.flags & CS_FLAG_FIXED_TUNING) cand = 0;
+    else if (tuning) cand = T.trials % kTuneN;
+    const auto t0 = std::chrono::steady_clock::now();
+    if (kTuneRows[cand] == 128) solve_batched<128>(C, kTuneChunk[cand], iters, relres, converged);
+    else solve_batched<256>(C, kTuneChunk[cand], iters, relres, converged);
+    if (tuning) {
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        double work = 0.0;
+        for (int c = 0; c < C->nCloths; c++) work += (double)iters[c] * C->sys[c].nn;
+        T.score[cand] += work > 0.0 ? sec / work : 0.0;
+        if (++T.trials == kTuneN * kTuneRounds) {
+            int best = 0;
+            for (int k = 1; k < kTuneN; k++)
+                if (T.score[k] < T.score[best]) best = k;
+            T.choice = best;
+        }
+    }
+    T.rows = kTuneRows[T.choice];
+    T.chunk = kTuneChunk[T.choice];
 }
 
 }  // namespace cs

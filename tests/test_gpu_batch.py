@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from clothsimulator_amd import CS_MESH_CLOTH, Simulator, scenes
-from clothsimulator_amd._cabi import CS_FLAG_INSTANCES
+from clothsimulator_amd._cabi import CS_FLAG_FIXED_TUNING, CS_FLAG_INSTANCES
 
 pytestmark = pytest.mark.gpu
 
@@ -18,6 +18,7 @@ def single(spec, k):
 
 def test_batched_instances_match_separate_runs():
     spec = scenes.instances(count=4, seed=3, side_lo=17, side_hi=29)
+    spec.params.flags |= CS_FLAG_FIXED_TUNING  # the single-cloth grouping of partial sums: comparable to 1e-5
     batch = Simulator(spec)
     its_batch = 0
     for _ in range(6):
@@ -43,6 +44,7 @@ def test_batched_contacts_match_separate_runs():
     """Instances resting on their spheres: the batched contact assembly (one sort over every
     cloth's blocks) gives each cloth the contact blocks of its own run."""
     spec = scenes.instances(count=3, seed=5, side_lo=17, side_hi=25, gap=0.004)
+    spec.params.flags |= CS_FLAG_FIXED_TUNING
     batch = Simulator(spec)
     contacts_batch = 0
     for _ in range(12):
@@ -82,3 +84,22 @@ def test_instance_pairs_stay_inside_their_instance():
         sim2.step()
     p2 = sim2.lbvh_pairs(True, 0.02)
     assert np.any(owner[p2[:, 0]] != owner[p2[:, 1]])
+
+
+def test_batched_solve_autotuner_settles():
+    """The batched solve times its candidate launch shapes over the first 8 solves and keeps
+    one; with CS_FLAG_FIXED_TUNING it keeps (256 rows, 16 iterations per poll)."""
+    spec = scenes.instances(count=6, seed=7, side_lo=17, side_hi=33)
+    sim = Simulator(spec)
+    for k in range(10):
+        s, st = sim.step()
+        assert s == 0, s
+    rows, chunk, trials = sim.batch_tuning()
+    print("tuned: %d rows per block, %d iterations per poll after %d timed solves" % (rows, chunk, trials))
+    assert trials == 8 and rows in (128, 256) and chunk in (16, 32)
+    spec.params.flags |= CS_FLAG_FIXED_TUNING
+    fixed = Simulator(spec)
+    for k in range(2):
+        s, st = fixed.step()
+        assert s == 0, s
+    assert fixed.batch_tuning() == (256, 16, 0)
