@@ -51,6 +51,7 @@ def lib():
         L.or_lbvh_keys.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_uint64)]
         L.or_lbvh_pairs.argtypes = [vp, C.c_int, C.c_float, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
         L.or_find_proximities.argtypes = [vp, C.POINTER(ProximityRec), C.c_int, C.POINTER(C.c_int)]
+        L.or_find_proximities_brute.argtypes = [vp, C.c_int, C.POINTER(ProximityRec), C.c_int, C.POINTER(C.c_int)]
         L.or_assemble.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int,
                                   C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.or_solve_bsr.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
@@ -252,11 +253,13 @@ class OracleScene:
                "lbvh_pairs")
         return pairs[:n.value]
 
-    def proximities(self, cap=1 << 20):
+    def proximities(self, cap=1 << 20, brute=False):
+        """Proximities of the current state (brute: every face pair of the traversed mesh pairs
+        instead of the tree's candidate pairs)."""
         out = np.zeros(cap, dtype=PROX_DTYPE)
         n = C.c_int()
-        _check(self.L.or_find_proximities(self.h, out.ctypes.data_as(C.POINTER(ProximityRec)), cap, C.byref(n)),
-               "find_proximities")
+        _check(self.L.or_find_proximities_brute(self.h, int(brute), out.ctypes.data_as(C.POINTER(ProximityRec)), cap,
+                                                C.byref(n)), "find_proximities")
         return out[:n.value].copy()
 
     def assemble(self, cloth=0):

@@ -391,16 +391,42 @@ template <class T> Prox<T> make_ee_prox(const Scene<T>& S, int e0, int e1) {
     return p;
 }
 
-// Simulator::findProximitiesGpu (src/Simulator.cu:300-349)
-template <class T> vector<Prox<T>> find_proximities(const Scene<T>& S) {
-    vector<BVH<T>> bvhs;
-    build_all(S, false, bvhs);
+// Every face pair of the mesh pairs Simulator::traverse visits (a mesh against
+// itself, a cloth against earlier cloths and against obstacles), with no box
+// test: the tree-free candidate set that pins find_proximities.  A proximity
+// closer than 2 thickness lies between faces whose dilated boxes overlap, so
+// the nearest candidate of every slot below the limit is the same in both.
+template <class T> vector<std::pair<int, int>> all_face_pairs(const Scene<T>& S) {
+    vector<std::pair<int, int>> all;
+    for (int c = 0; c < S.nCloths; c++) {
+        const int f0 = S.meshes[c].f0, nf = S.meshes[c].nf;
+        for (int i = 0; i < nf; i++)
+            for (int j = i + 1; j < nf; j++) all.push_back({f0 + i, f0 + j});
+        for (int o = 0; o < (int)S.meshes.size(); o++) {
+            if (o == c || (o < S.nCloths && o > c)) continue;
+            for (int i = 0; i < nf; i++)
+                for (int j = 0; j < S.meshes[o].nf; j++) all.push_back({f0 + i, S.meshes[o].f0 + j});
+        }
+    }
+    return all;
+}
+
+// Simulator::findProximitiesGpu (src/Simulator.cu:300-349); brute: over all_face_pairs
+// instead of the tree's candidate pairs
+template <class T> vector<Prox<T>> find_proximities(const Scene<T>& S, bool brute = false) {
     T thick = T(S.P.repulsion_thickness);
     ProxCand<T> C;
     C.node.assign(2 * S.x.size(), Best<T>());
     C.edge.assign(2 * S.en.size(), Best<T>());
     C.face.assign(2 * S.fn.size(), Best<T>());
-    vector<std::pair<int, int>> pairs = collect_pairs(S, bvhs, T(2) * thick);
+    vector<std::pair<int, int>> pairs;
+    if (brute) {
+        pairs = all_face_pairs(S);
+    } else {
+        vector<BVH<T>> bvhs;
+        build_all(S, false, bvhs);
+        pairs = collect_pairs(S, bvhs, T(2) * thick);
+    }
     const long np = (long)pairs.size(), CH = 4096, nch = (np + CH - 1) / CH;
     vector<CandList<T>> parts(nch);
 #pragma omp parallel for schedule(dynamic, 4)

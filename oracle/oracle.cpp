@@ -303,14 +303,19 @@ int or_lbvh_pairs(void* h, int ccd, float thickness, int brute, int* pairs, int 
     });
 }
 
-int or_find_proximities(void* h, cs_proximity_rec* out, int cap, int* n) {
+// brute != 0: over every face pair of the traversed mesh pairs instead of the tree's pairs
+int or_find_proximities_brute(void* h, int brute, cs_proximity_rec* out, int cap, int* n) {
     return dispatch(h, [&](auto& S) {
-        auto prox = find_proximities(S);
+        auto prox = find_proximities(S, brute != 0);
         *n = (int)prox.size();
         if ((int)prox.size() > cap) return (int)CS_E_CAPACITY;
         for (size_t i = 0; i < prox.size(); i++) prox_out(prox[i], out[i]);
         return (int)CS_OK;
     });
+}
+
+int or_find_proximities(void* h, cs_proximity_rec* out, int cap, int* n) {
+    return or_find_proximities_brute(h, 0, out, cap, n);
 }
 
 // Assembled system of one cloth on the current state (proximities detected

@@ -13,6 +13,7 @@ oracle's own formula:
                 pattern size N + 2E + 2E_int
   solvers       dense numpy solve; FP32 and FP64 PCG iteration counts
   LBVH          tree traversal == brute force over all leaf pairs; Morton codes re-interleaved
+  proximities   tree candidates == every face pair of the traversed mesh pairs (no box test)
   CCD           vertex through a static triangle (t = h0/(h0-h1)); crossing edges; cubic roots
                 against numpy on random cubics; degenerate (linear / zero) cubics
   independent   greedy in sorted order restated in Python
@@ -371,3 +372,64 @@ def test_independent_set_is_greedy():
             want.append(nd)
             taken.update(nd)
         assert [list(g) for g in got["nodes"]] == want
+
+
+def _prox_scenes():
+    """Self-proximities (two parallel sheets in one mesh), cloth-cloth (two sheets as two cloths)
+    and cloth-obstacle (a drape lowered onto its sphere)."""
+    n, h = 9, 0.1 / 8
+    xa, fa = grid_mesh(n, 0.1, origin=(0.0, 0.0, 0.0))
+    xb, fb = grid_mesh(n, 0.1, origin=(0.3 * h, 0.3 * h, 1.5e-3))
+    p = default_params(dt=0.005)
+    one = SceneSpec("two_sheets", p, [ClothSpec(MeshArrays(np.vstack([xa, xb]), np.vstack([fa, fb + n * n])),
+                                                material_struct())], [], 1)
+    two = SceneSpec("two_cloths", default_params(dt=0.005),
+                    [ClothSpec(MeshArrays(xa, fa), material_struct()), ClothSpec(MeshArrays(xb, fb), material_struct())],
+                    [], 1)
+    yield one, None
+    yield two, None
+    spec = drape(17)
+    x = spec.cloths[0].mesh.x.copy()
+    x[:, 2] -= 0.1005  # the cloth through the sphere's pole vertex by 0.5 mm, within 2 mm of the facets around it
+    yield spec, x
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_proximities_tree_equal_brute_force(case):
+    spec, x = list(_prox_scenes())[case]
+    o = oracle.OracleScene(spec)
+    if x is not None:
+        _, _, v = o.state()
+        o.set_state(0, 0, x, x, v)
+    t = o.proximities()
+    b = o.proximities(brute=True)
+    print("%s: %d proximities" % (spec.name, len(t)))
+    assert len(t) > 0
+    assert len(t) == len(b)
+    for f in ("nodes", "w", "n", "stiffness", "mu"):
+        assert np.array_equal(t[f], b[f]), f
+
+
+def test_vertex_face_proximity_closed_form():
+    """A small cloth triangle 1 mm above a large obstacle triangle: each cloth node's
+    proximity is its vertical drop onto the obstacle (distance 1 mm, normal +z, weights
+    (1, -barycentric of the foot point)), with stiffness k min(node area, face area) and the
+    obstacle friction."""
+    cx = np.array([[0.10, 0.10, 1e-3], [0.14, 0.11, 1e-3], [0.12, 0.15, 1e-3]])
+    ox = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    p = default_params(dt=0.005)
+    spec = SceneSpec("tri_over_tri", p, [ClothSpec(MeshArrays(cx, np.array([[0, 1, 2]])), material_struct())],
+                     [MeshArrays(ox, np.array([[0, 1, 2]]))], 1)
+    o = oracle.OracleScene(spec, fp64=True)
+    pr = o.proximities()
+    assert len(pr) == 3
+    area = 0.5 * np.linalg.norm(np.cross(cx[1] - cx[0], cx[2] - cx[0]))
+    for rec in pr:
+        i = rec["nodes"][0]
+        assert list(rec["nodes"][1:]) == [3, 4, 5]  # the obstacle's nodes follow the cloth's
+        px, py = cx[i, 0], cx[i, 1]
+        bary = np.array([1.0 - px - py, px, py])  # foot point on the unit right triangle
+        assert np.allclose(rec["w"], np.r_[1.0, -bary], atol=1e-6)
+        assert np.allclose(rec["n"], [0.0, 0.0, 1.0], atol=1e-6)
+        assert np.isclose(rec["stiffness"], p.collision_stiffness * area / 3.0, rtol=1e-5)
+        assert np.isclose(rec["mu"], p.obstacle_friction)
