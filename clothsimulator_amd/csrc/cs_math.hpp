@@ -15,6 +15,15 @@
 
 namespace cs {
 
+// Pointers read from memory (job tables, per-cloth descriptors) are generic, and
+// generic accesses compile to flat instructions; device code casts them to the
+// global address space so they load and store as global memory.
+template <class T> using G1 = __attribute__((address_space(1))) T;
+template <class T> __device__ __forceinline__ G1<T>* g1(T* p) { return (G1<T>*)p; }
+// float4 as a builtin vector (copyable from any address space, unlike the float4 class)
+typedef float f4n __attribute__((ext_vector_type(4)));
+
+
 template <class T> CS_HD T tmin(T a, T b) { return a < b ? a : b; }
 template <class T> CS_HD T tmax(T a, T b) { return a > b ? a : b; }
 template <class T> CS_HD T tabs(T x) { return x < T(0) ? -x : x; }

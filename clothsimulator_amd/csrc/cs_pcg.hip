@@ -111,13 +111,15 @@ struct Partials {
     }
 };
 
-__device__ __forceinline__ V3f mat3v(const float* __restrict__ m, V3f a) {
+template <class P> __device__ __forceinline__ V3f mat3v(P m, V3f a) {
     return mk(m[0] * a.x + m[1] * a.y + m[2] * a.z, m[3] * a.x + m[4] * a.y + m[5] * a.z,
               m[6] * a.x + m[7] * a.y + m[8] * a.z);
 }
-__device__ __forceinline__ V3f ld3v(const float* p, int i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
-__device__ __forceinline__ void st3v(float* p, int i, V3f a) { p[3 * i] = a.x; p[3 * i + 1] = a.y; p[3 * i + 2] = a.z; }
+template <class P> __device__ __forceinline__ V3f ld3v(P p, int i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+template <class P> __device__ __forceinline__ void st3v(P p, int i, V3f a) { p[3 * i] = a.x; p[3 * i + 1] = a.y; p[3 * i + 2] = a.z; }
 __device__ __forceinline__ V3f v3(float4 a) { return mk(a.x, a.y, a.z); }
+__device__ __forceinline__ V3f v3(f4n a) { return mk(a.x, a.y, a.z); }
+__device__ __forceinline__ f4n f4g(V3f a) { return f4n{a.x, a.y, a.z, 0.0f}; }
 __device__ __forceinline__ float4 f4(V3f a) { return make_float4(a.x, a.y, a.z, 0.0f); }
 
 __device__ __forceinline__ void acc3(float& y0, float& y1, float& y2, const float (&a)[9], V3f v) {
@@ -365,6 +367,31 @@ struct BCloth {
     PcgDev* sc;
 };
 
+// A BCloth's pointers in the global address space (loads and stores through them
+// compile to global instructions, not flat ones).
+struct BClothG {
+    int nn, nb;
+    const G1<int> *sliceOff, *sliceW, *cols, *cRowPtr, *cCol;
+    const G1<float> *vals, *cVal, *Minv, *b, *bc;
+    G1<float> *x, *r, *q;
+    G1<f4n> *z, *d0, *d1;
+    G1<float>* part;
+    G1<PcgDev>* sc;
+};
+__device__ __forceinline__ BClothG gview(const BCloth& c) {
+    BClothG g;
+    g.nn = c.nn;
+    g.nb = c.nb;
+    g.sliceOff = g1(c.sliceOff); g.sliceW = g1(c.sliceW); g.cols = g1(c.cols);
+    g.cRowPtr = g1(c.cRowPtr); g.cCol = g1(c.cCol);
+    g.vals = g1(c.vals); g.cVal = g1(c.cVal); g.Minv = g1(c.Minv); g.b = g1(c.b); g.bc = g1(c.bc);
+    g.x = g1(c.x); g.r = g1(c.r); g.q = g1(c.q);
+    g.z = (G1<f4n>*)c.z; g.d0 = (G1<f4n>*)c.d0; g.d1 = (G1<f4n>*)c.d1;
+    g.part = g1(c.part);
+    g.sc = g1(c.sc);
+    return g;
+}
+
 // Sum of NS values over a BP-thread block (every thread gets the sums).
 template <int NS, int BP>
 __device__ __forceinline__ void bsums(float (&v)[NS], float* lds) {
@@ -397,14 +424,14 @@ __device__ __forceinline__ void bsums(float (&v)[NS], float* lds) {
 template <int NS, int BP>
 struct BPartials {
     float raw[NS][4];
-    __device__ __forceinline__ void issue(const float* __restrict__ part, const int (&slot)[NS], int nb) {
+    template <class P> __device__ __forceinline__ void issue(P part, const int (&slot)[NS], int nb) {
 #pragma unroll
         for (int s = 0; s < NS; s++)
 #pragma unroll
             for (int u = 0; u < 4; u++) raw[s][u] = part[slot[s] * nb + min((int)threadIdx.x + u * BP, nb - 1)];
     }
-    __device__ __forceinline__ void sums(const float* __restrict__ part, const int (&slot)[NS], int nb, float* lds,
-                                         float (&out)[NS]) {
+    template <class P> __device__ __forceinline__ void sums(P part, const int (&slot)[NS], int nb, float* lds,
+                                                         float (&out)[NS]) {
 #pragma unroll
         for (int s = 0; s < NS; s++)
 #pragma unroll
@@ -438,7 +465,7 @@ __global__ void __launch_bounds__(BP) k_bpcg_init(const BCloth* __restrict__ cl,
     __shared__ float lds[2 * BP / 64];
     const int2 m = batch_block(bmap);
     if (m.x < 0) return;
-    const BCloth& c = cl[m.x];
+    const BClothG c = gview(cl[m.x]);
     const int lb = m.y;
     const int i = lb * BP + threadIdx.x;
     float acc[2] = {0.0f, 0.0f};
@@ -447,7 +474,7 @@ __global__ void __launch_bounds__(BP) k_bpcg_init(const BCloth* __restrict__ cl,
         st3v(c.x, i, mk(0.0f, 0.0f, 0.0f));
         st3v(c.r, i, bi);
         const V3f zi = mat3v(c.Minv + 9 * (size_t)i, bi);
-        c.z[i] = f4(zi);
+        c.z[i] = f4g(zi);
         acc[0] = dot(bi, zi);
         acc[1] = dot(bi, bi);
     }
@@ -462,7 +489,7 @@ __global__ void __launch_bounds__(BP) k_bpcg_init(const BCloth* __restrict__ cl,
 template <int BP>
 __global__ void __launch_bounds__(BP) k_bpcg_setup(const BCloth* __restrict__ cl, float tol2, int maxit) {
     __shared__ float lds[2 * BP / 64];
-    const BCloth c = cl[blockIdx.x];
+    const BClothG c = gview(cl[blockIdx.x]);
     const int slot[2] = {P_BB, P_RZ0};
     BPartials<2, BP> pr;
     pr.issue(c.part, slot, c.nb);
@@ -489,14 +516,14 @@ __global__ void __launch_bounds__(BP) k_bpcg_spmv(const BCloth* __restrict__ cl,
     __shared__ float lds[3 * BP / 64];
     const int2 m = batch_block(bmap);
     if (m.x < 0) return;
-    const BCloth& c = cl[m.x];
+    const BClothG c = gview(cl[m.x]);
     const int lb = m.y;
     const int p = it & 1;
     const int slot[3] = {P_RR, P_RZ0 + p, P_RZ0 + (1 - p)};
     BPartials<3, BP> pr;
     if (!FIRST) pr.issue(c.part, slot, c.nb);
     if (FIRST && c.sc->done) return;
-    const float4* dprev = p ? c.d0 : c.d1;
+    const G1<f4n>* dprev = p ? c.d0 : c.d1;
     const int i = lb * BP + threadIdx.x;
     const bool live = i < c.nn;
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
@@ -504,11 +531,11 @@ __global__ void __launch_bounds__(BP) k_bpcg_spmv(const BCloth* __restrict__ cl,
         const int sl = i / SLICE, lane = i % SLICE;
         const int W = c.sliceW[sl];
         const size_t off = (size_t)c.sliceOff[sl];
-        const int* cp = c.cols + off * SLICE + lane;
-        const float* vp = c.vals + off * 9 * SLICE + lane;
+        const G1<int>* cp = c.cols + off * SLICE + lane;
+        const G1<float>* vp = c.vals + off * 9 * SLICE + lane;
         for (int k = 0; k < W; k++) {
             const int j = cp[k * SLICE];
-            const float* ap = vp + (size_t)k * 9 * SLICE;
+            const G1<float>* ap = vp + (size_t)k * 9 * SLICE;
             float a[9];
 #pragma unroll
             for (int q = 0; q < 9; q++) a[q] = ap[q * SLICE];
@@ -540,18 +567,18 @@ __global__ void __launch_bounds__(BP) k_bpcg_spmv(const BCloth* __restrict__ cl,
     }
     float pdq = 0.0f;
     if (live) {
-        float4* dcur = FIRST ? c.d0 : (p ? c.d1 : c.d0);
-        const float4 zi = c.z[i];
+        G1<f4n>* dcur = FIRST ? c.d0 : (p ? c.d1 : c.d0);
+        const f4n zi = c.z[i];
         V3f di, qi;
         if (FIRST) {
             di = v3(zi);
             qi = mk(s0, s1, s2);
         } else {
-            const float4 dp = dprev[i];
+            const f4n dp = dprev[i];
             di = mk(zi.x + beta * dp.x, zi.y + beta * dp.y, zi.z + beta * dp.z);
             qi = mk(s0 + beta * t0, s1 + beta * t1, s2 + beta * t2);
         }
-        dcur[i] = f4(di);
+        dcur[i] = f4g(di);
         st3v(c.q, i, qi);
         pdq = dot(di, qi);
     }
@@ -571,13 +598,13 @@ __global__ void __launch_bounds__(BP) k_bpcg_update(const BCloth* __restrict__ c
     __shared__ float lds[2 * BP / 64];
     const int2 m = batch_block(bmap);
     if (m.x < 0) return;
-    const BCloth& c = cl[m.x];
+    const BClothG c = gview(cl[m.x]);
     const int lb = m.y;
     const int p = it & 1;
     const int slot[2] = {P_DQ, P_RZ0 + p};
     BPartials<2, BP> pr;
     pr.issue(c.part, slot, c.nb);
-    const float4* d = it == 0 ? c.d0 : (p ? c.d1 : c.d0);
+    const G1<f4n>* d = it == 0 ? c.d0 : (p ? c.d1 : c.d0);
     const int i = lb * BP + threadIdx.x;
     const bool live = i < c.nn;
     V3f di = mk(0.0f, 0.0f, 0.0f), xi = di, ri = di, qi = di;
@@ -598,7 +625,7 @@ __global__ void __launch_bounds__(BP) k_bpcg_update(const BCloth* __restrict__ c
         st3v(c.x, i, xi);
         st3v(c.r, i, ri);
         const V3f zi = mat3v(c.Minv + 9 * (size_t)i, ri);
-        c.z[i] = f4(zi);
+        c.z[i] = f4g(zi);
         acc[0] = dot(ri, zi);
         acc[1] = dot(ri, ri);
     }
