@@ -490,21 +490,32 @@ __device__ __forceinline__ Box node_box(const BNode& n) {
     return b;
 }
 
+// A tree node (32 B) read through a global-address-space pointer: two 16-byte loads.
+typedef int i2n __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ BNode load_node(const G1<f4n>* t, int i) {
+    const f4n a = t[2 * i], b = t[2 * i + 1];
+    BNode n;
+    n.lo[0] = a.x; n.lo[1] = a.y; n.lo[2] = a.z; n.maxIndex = __float_as_int(a.w);
+    n.hi[0] = b.x; n.hi[1] = b.y; n.hi[2] = b.z; n.face = __float_as_int(b.w);
+    return n;
+}
+
 // Leaf-vs-tree traversal for query leaf qi.  emit(face) for each reported leaf.
 // Returns false if the stack overflowed (the query is then incomplete).
 template <bool SELF, class Emit>
-__device__ __forceinline__ bool traverse(const BNode* __restrict__ tn, const int2* __restrict__ tchild, int tnf,
+__device__ __forceinline__ bool traverse(const G1<f4n>* __restrict__ tn, const G1<i2n>* __restrict__ tchild, int tnf,
                                          const Box& qb, int qi, float th, int* stk, Emit&& emit) {
     if (tnf == 1) {  // a one-face target has no internal node
-        if (!SELF && box_overlap(qb, node_box(tn[0]), th)) emit(tn[0].face);
+        const BNode t0 = load_node(tn, 0);
+        if (!SELF && box_overlap(qb, node_box(t0), th)) emit(t0.face);
         return true;
     }
     const int ni = tnf - 1;
     int top = 0;
     int node = 0;
     for (;;) {
-        const int2 ch = tchild[node];
-        const BNode L = tn[ch.x], R = tn[ch.y];
+        const i2n ch = tchild[node];
+        const BNode L = load_node(tn, ch.x), R = load_node(tn, ch.y);
         const bool lLeaf = ch.x >= ni, rLeaf = ch.y >= ni;
         const bool ovL = (!SELF || qi < L.maxIndex) && box_overlap(qb, node_box(L), th);
         const bool ovR = (!SELF || qi < R.maxIndex) && box_overlap(qb, node_box(R), th);
@@ -550,7 +561,7 @@ __global__ void __launch_bounds__(TB) k_traverse_batch(const TravJob* __restrict
     const TravJob J = jobs[lo];
     const int qi = ((int)blockIdx.x - J.blk0) * TB + threadIdx.x;
     if (qi >= J.qnf) return;
-    const BNode q = J.qn[J.qnf - 1 + qi];
+    const BNode q = load_node((const G1<f4n>*)J.qn, J.qnf - 1 + qi);
     const Box qb = node_box(q);
     const int qface = q.face;
     uint32_t cnt = 0;
@@ -559,8 +570,10 @@ __global__ void __launch_bounds__(TB) k_traverse_batch(const TravJob* __restrict
         if (MODE == 1) out[base + cnt] = make_int2(qface, f);
         cnt++;
     };
-    const bool ok = J.self ? traverse<true>(J.tn, J.tchild, J.tnf, qb, qi, th, stk + threadIdx.x, emit)
-                           : traverse<false>(J.tn, J.tchild, J.tnf, qb, qi, th, stk + threadIdx.x, emit);
+    const G1<f4n>* tn = (const G1<f4n>*)J.tn;
+    const G1<i2n>* tchild = (const G1<i2n>*)J.tchild;
+    const bool ok = J.self ? traverse<true>(tn, tchild, J.tnf, qb, qi, th, stk + threadIdx.x, emit)
+                           : traverse<false>(tn, tchild, J.tnf, qb, qi, th, stk + threadIdx.x, emit);
     if (MODE == 0) num[J.off + qi] = cnt;
     if (!ok) atomicOr(overflow, 1u);
 }
