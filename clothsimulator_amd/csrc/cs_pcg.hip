@@ -697,6 +697,41 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     // iteration (only the SpMV returns early), each poll one host round trip.
     const int chunk0 = (amg || amgl) ? 4 : learned ? 8 : 16;
     int chunk = chunk0;
+    // The multigrid cycle of an iteration (with the learned smoother: its two pre-smoothing
+    // sweeps too) is the same launch sequence every iteration up to the parity of the
+    // r.z partial slot; with CS_FLAG_GRAPH_CYCLES it is captured once per parity and solve
+    // into a HIP graph and replayed (about 15 launches become one).  Measured on the 513^2
+    // crumple it saves nothing: the two instantiations per solve cost what the replays
+    // save, so direct launches are the default.
+    hipGraphExec_t cycle[2] = {nullptr, nullptr};
+    const bool graphs = (amg || amgl) && (C->P.flags & CS_FLAG_GRAPH_CYCLES);
+    auto launch_cycle = [&](int par) {
+        if (amgl) lpc_smooth_pre(C, c, S.r.ptr<float>(), xs);
+        amg_apply(C, c, S.r.ptr<float>(), z, part + (P_RZ0 + (1 - par)) * PMAX, nb, true, amgl);
+    };
+    auto run_cycle = [&](int par) {
+        if (!graphs) {
+            launch_cycle(par);
+            return;
+        }
+        if (!cycle[par]) {
+            hipGraph_t g = nullptr;
+            CS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            launch_cycle(par);
+            CS_HIP(hipStreamEndCapture(s, &g));
+            const hipError_t e = hipGraphInstantiate(&cycle[par], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            CS_HIP(e);
+        }
+        CS_HIP(hipGraphLaunch(cycle[par], s));
+    };
+    struct GraphRelease {
+        hipGraphExec_t* e;
+        ~GraphRelease() {
+            for (int k = 0; k < 2; k++)
+                if (e[k]) (void)hipGraphExecDestroy(e[k]);
+        }
+    } release{cycle};
     int done = 0;
     PcgDev h{};
     int it = 0;
@@ -721,12 +756,11 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
             if (amgl) {
                 hipLaunchKernelGGL(k_pcg_update<PC_EXTERNAL>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
                                    S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
-                lpc_smooth_pre(C, c, S.r.ptr<float>(), xs);
-                amg_apply(C, c, S.r.ptr<float>(), z, part + (P_RZ0 + (1 - (it & 1))) * PMAX, nb, true, true);
+                run_cycle(it & 1);
             } else if (amg) {
                 hipLaunchKernelGGL(k_pcg_update<PC_SMOOTH>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
                                    S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
-                amg_apply(C, c, S.r.ptr<float>(), z, part + (P_RZ0 + (1 - (it & 1))) * PMAX, nb, true);
+                run_cycle(it & 1);
             } else if (learned) {
                 hipLaunchKernelGGL(k_pcg_update<PC_EXTERNAL>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
                                    S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);

@@ -64,8 +64,10 @@ enum {
                                  (the reference traverses every cloth pair, src/Simulator.cu:179-196, which finds
                                  nothing between instances placed apart), and with block-Jacobi PCG all cloths'
                                  solves advance together in batched launches (no counterpart) */
-    CS_FLAG_FIXED_TUNING = 16 /* batched solves keep their default launch shape instead of autotuning it
+    CS_FLAG_FIXED_TUNING = 16, /* batched solves keep their default launch shape instead of autotuning it
                                  (bit-reproducible runs; no counterpart) */
+    CS_FLAG_GRAPH_CYCLES = 32 /* multigrid cycles replayed from HIP graphs captured once per solve instead of
+                                 launched kernel by kernel (the same kernels in the same order; no counterpart) */
 };
 
 /* kernel ids for cs_kernel_time.  CS_KERNEL_EVENT_PAIR times an empty event pair recorded on

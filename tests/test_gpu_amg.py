@@ -132,3 +132,26 @@ def test_vcycle_matches_oracle_with_contacts():
         zo = o.amg_apply(r.astype(np.float64), 0)
         err = np.abs(zg - zo).max() / np.abs(zo).max()
         assert err < 2e-3, (trial, err)
+
+
+@pytest.mark.parametrize("precond", ["amg", "amg-learned"])
+def test_graph_replayed_cycles_equal_direct_launches(precond):
+    """The multigrid cycle replayed from captured HIP graphs (CS_FLAG_GRAPH_CYCLES) gives
+    bit-identical steps to the same kernels launched one by one, over several steps with
+    eager work between the replays (every step reassembles and re-solves)."""
+    from clothsimulator_amd._cabi import CS_FLAG_GRAPH_CYCLES, CS_PRECOND_AMG_LEARNED
+    runs = []
+    for flags in (CS_FLAG_GRAPH_CYCLES, 0):
+        spec = scenes.crumple(33, steps=8)
+        spec.params.precond = CS_PRECOND_AMG if precond == "amg" else CS_PRECOND_AMG_LEARNED
+        spec.params.flags |= flags
+        sim = Simulator(spec)
+        its = []
+        for _ in range(8):
+            s, st = sim.step()
+            assert s == 0, s
+            its.append(st.pcg_iters)
+        runs.append((sim.state(CS_MESH_CLOTH, 0)[0].copy(), its))
+    print("%s iterations: graphs %s, direct %s" % (precond, runs[0][1], runs[1][1]))
+    assert runs[0][1] == runs[1][1]
+    assert np.array_equal(runs[0][0], runs[1][0])
