@@ -119,6 +119,7 @@ def run(workload, precond, steps, warmup, timed):
         sim.set_state(CS_MESH_CLOTH, i, x, x0, v)
     for i, (x, x0, v) in enumerate(obs):
         sim.set_state(CS_MESH_OBSTACLE, i, x, x0, v)
+    sim.set_step_count(0)  # motions restart from t = 0 with the initial state
     if timed:
         sim.kernel_time(CS_KERNEL_PCG_SPMV, reset=True)
         sim.kernel_time(CS_KERNEL_EVENT_PAIR, reset=True)

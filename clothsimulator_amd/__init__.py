@@ -12,7 +12,7 @@ import numpy as np
 
 from ._cabi import (CS_E_NOT_CONVERGED, CS_MESH_CLOTH, CS_PRECOND_AMG_LEARNED, CS_PRECOND_LEARNED, CS_MESH_OBSTACLE, CS_OK, IMPACT_DTYPE, LIB_PATH, PROX_DTYPE,
                     STATUS_NAMES, ImpactRec, MaterialIn, MeshArrays, MeshIn, Params, ProximityRec, SceneInfo,
-                    Stats, default_params, fptr, iptr, material_struct)
+                    Stats, Transform, axis_angle, default_params, fptr, iptr, material_struct, transform_struct)
 
 __all__ = ["load_library", "Simulator", "ClothError", "default_params", "material_struct", "MeshArrays"]
 
@@ -24,7 +24,9 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_lbvh_pairs", "cs_find_proximities", "cs_assemble", "cs_solve", "cs_find_impacts",
             "cs_independent_impacts", "cs_kernel_time", "cs_load_scene", "cs_scene_error", "cs_precondition",
             "cs_lpc_weight_count", "cs_set_learned_weights", "cs_get_learned_weights", "cs_lpc_factor",
-            "cs_lpc_train_step", "cs_lpc_smoother_omega", "cs_batch_tuning"]
+            "cs_lpc_train_step", "cs_lpc_smoother_omega", "cs_batch_tuning", "cs_add_motion", "cs_motion_eval",
+            "cs_set_obstacle_motion", "cs_add_moving_handles", "cs_set_step_count", "cs_get_step_count",
+            "cs_scene_motion"]
 
 # trained weights of the learned preconditioner shipped with the package (float32, .npy)
 LPC_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lpc_weights.npy")
@@ -82,8 +84,45 @@ def load_library(path=LIB_PATH):
     L.cs_lpc_train_step.argtypes = [vp, C.c_int, C.c_float, C.c_float, F, F]
     L.cs_lpc_smoother_omega.argtypes = [vp, C.c_int, F]
     L.cs_batch_tuning.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    TP = C.POINTER(Transform)
+    L.cs_add_motion.argtypes = [vp, F, TP, C.c_int, C.POINTER(C.c_int)]
+    L.cs_motion_eval.argtypes = [F, TP, C.c_int, C.c_float, TP]
+    L.cs_set_obstacle_motion.argtypes = [vp, C.c_int, C.c_int]
+    L.cs_add_moving_handles.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int]
+    L.cs_set_step_count.argtypes = [vp, C.c_int]
+    L.cs_get_step_count.argtypes = [vp, C.POINTER(C.c_int)]
+    L.cs_scene_motion.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_float, TP]
     _lib = L
     return L
+
+
+def _keys(times, keys):
+    t = np.ascontiguousarray(times, dtype=np.float32)
+    k = (Transform * len(keys))(*[x if isinstance(x, Transform) else
+                                  transform_struct(x[0], x[1:4], x[4:8]) for x in keys])
+    return t, k
+
+
+def motion_eval(times, keys, time):
+    """Motion::computeTransformation of keyframes (times, keys) at `time` (cs_motion_eval; host only).
+    keys: Transform structs or 8 floats (scale, translate xyz, rotate s x y z).  Returns a Transform."""
+    L = load_library()
+    t, k = _keys(times, keys)
+    out = Transform()
+    st = L.cs_motion_eval(fptr(t), k, len(t), float(time), C.byref(out))
+    if st != CS_OK:
+        raise ClothError(st, "cs_motion_eval")
+    return out
+
+
+def scene_motion(config_path, motion_id, time, root_dir=None):
+    """Motion `motion_id` of a reference configuration file at `time` (cs_scene_motion; host only)."""
+    L = load_library()
+    out = Transform()
+    st = L.cs_scene_motion(config_path.encode(), (root_dir or "").encode(), motion_id, float(time), C.byref(out))
+    if st != CS_OK:
+        raise ClothError(st, "cs_scene_motion", L.cs_scene_error().decode())
+    return out
 
 
 def scene_info(config_path, root_dir=None):
@@ -127,12 +166,17 @@ class Simulator:
         self._keep = []
         self.n_cloths = 0
         if spec is not None:
-            for c in spec.cloths:
-                cid = self.add_cloth(c.mesh, c.material)
+            cids = [self.add_cloth(c.mesh, c.material) for c in spec.cloths]
+            for times, keys in getattr(spec, "motions", []):
+                self.add_motion(times, keys)
+            for cid, c in zip(cids, spec.cloths):
                 if c.handles:
-                    self.add_handles(cid, c.handles)
-            for o in spec.obstacles:
-                self.add_obstacle(o)
+                    self.add_handles(cid, c.handles, getattr(c, "handle_motion", -1))
+            for i, o in enumerate(spec.obstacles):
+                oid = self.add_obstacle(o)
+                m = spec.obstacle_motions[i] if getattr(spec, "obstacle_motions", None) else -1
+                if m >= 0:
+                    self.set_obstacle_motion(oid, m)
         if self.params.precond in (CS_PRECOND_LEARNED, CS_PRECOND_AMG_LEARNED) and os.path.exists(LPC_WEIGHTS):
             self.set_learned_weights(default_learned_weights())
 
@@ -191,9 +235,30 @@ class Simulator:
         self._check(self.L.cs_add_obstacle(self.h, C.byref(m), C.byref(oid)), "cs_add_obstacle")
         return oid.value
 
-    def add_handles(self, cloth, nodes):
+    def add_handles(self, cloth, nodes, motion=-1):
         a = np.ascontiguousarray(nodes, dtype=np.int32)
-        self._check(self.L.cs_add_handles(self.h, cloth, iptr(a), len(a)), "cs_add_handles")
+        if motion < 0:
+            self._check(self.L.cs_add_handles(self.h, cloth, iptr(a), len(a)), "cs_add_handles")
+        else:
+            self._check(self.L.cs_add_moving_handles(self.h, cloth, iptr(a), len(a), motion), "cs_add_moving_handles")
+
+    def add_motion(self, times, keys):
+        """Keyframes (times, keys) as a motion of this context; returns its index."""
+        t, k = _keys(times, keys)
+        mid = C.c_int()
+        self._check(self.L.cs_add_motion(self.h, fptr(t), k, len(t), C.byref(mid)), "cs_add_motion")
+        return mid.value
+
+    def set_obstacle_motion(self, obstacle, motion):
+        self._check(self.L.cs_set_obstacle_motion(self.h, obstacle, motion), "cs_set_obstacle_motion")
+
+    def step_count(self):
+        n = C.c_int()
+        self._check(self.L.cs_get_step_count(self.h, C.byref(n)), "cs_get_step_count")
+        return n.value
+
+    def set_step_count(self, n):
+        self._check(self.L.cs_set_step_count(self.h, n), "cs_set_step_count")
 
     def physics_step(self):
         st = Stats()

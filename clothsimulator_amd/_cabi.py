@@ -142,7 +142,45 @@ class SceneInfo(C.Structure):
         ("n_handles", C.c_int),
         ("n_cloth_nodes", C.c_int),
         ("n_cloth_faces", C.c_int),
+        ("n_motions", C.c_int),
+        ("n_moving_obstacles", C.c_int),
     ]
+
+
+CS_MAX_MOTIONS = 64
+
+
+class Transform(C.Structure):
+    """cs_transform: p -> scale * rotate(p) + translate, rotate = quaternion (s, x, y, z)."""
+    _fields_ = [
+        ("scale", C.c_float),
+        ("translate", C.c_float * 3),
+        ("rotate", C.c_float * 4),
+    ]
+
+    def as_array(self):
+        return np.array([self.scale] + list(self.translate) + list(self.rotate), dtype=np.float32)
+
+
+def transform_struct(scale=1.0, translate=(0.0, 0.0, 0.0), rotate=(1.0, 0.0, 0.0, 0.0)):
+    t = Transform()
+    t.scale = scale
+    t.translate[:] = [float(v) for v in translate]
+    t.rotate[:] = [float(v) for v in rotate]
+    return t
+
+
+def axis_angle(axis, degrees):
+    """The quaternion of Transformation(json) "rotate": [degrees, ax, ay, az] (float32 steps of
+    src/Transformation.cu:9-14 and src/Quaternion.cu:7-16)."""
+    ang = np.float32(np.float32(degrees) * np.pi / np.float32(180.0))
+    if ang == 0:
+        return (1.0, 0.0, 0.0, 0.0)
+    a = np.asarray(axis, dtype=np.float32)
+    a = a * (np.float32(1) / np.sqrt(np.float32(a @ a)))
+    h = np.float32(0.5) * ang
+    s = np.sin(h, dtype=np.float32)
+    return (float(np.cos(h, dtype=np.float32)), float(s * a[0]), float(s * a[1]), float(s * a[2]))
 
 
 PROX_DTYPE = np.dtype([("nodes", "<i4", 4), ("n", "<f4", 3), ("mu", "<f4"), ("stiffness", "<f4"), ("w", "<f4", 4)])

@@ -108,6 +108,7 @@ int physics_step(cs_ctx* C, cs_stats* st) {
     hipStream_t s = C->stream;
     StageTimer T;
     cs_stats local{};
+    motion_step(C);
     if (!(C->P.flags & CS_FLAG_NO_PROXIMITY)) find_proximities(C);
     else C->nProx = 0;
     local.n_proximities = C->nProx;
@@ -325,19 +326,81 @@ cs_status cs_add_obstacle(cs_ctx* C, const cs_mesh_in* mesh, int* obstacle_id) {
 }
 
 cs_status cs_add_handles(cs_ctx* C, int cloth_id, const int* nodes, int n) {
+    return cs_add_moving_handles(C, cloth_id, nodes, n, -1);
+}
+
+cs_status cs_add_moving_handles(cs_ctx* C, int cloth_id, const int* nodes, int n, int motion_id) {
     if (!C || n < 0 || (n > 0 && !nodes)) return CS_E_ARG;
     if (C->finalized) return CS_E_STATE;
     if (cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
+    if (motion_id < -1 || motion_id >= (int)C->motions.size()) return CS_E_ARG;
     const MeshInfo& M = C->meshes[cloth_id];
     for (int i = 0; i < n; i++)
         if (nodes[i] < 0 || nodes[i] >= M.nn) return CS_E_ARG;
+    // anchor = T^-1 (x) for a moving handle (src/ClothHelper.cu:12, src/Cloth.cu:663)
+    const Xf inv = motion_id >= 0 ? xf_inverse(motion_now(C, motion_id)) : xf_identity();
     for (int i = 0; i < n; i++) {
         cs_ctx::HandleH h;
         h.cloth = cloth_id;
         h.node = M.n0 + nodes[i];
-        for (int k = 0; k < 3; k++) h.p[k] = C->hx[3 * (size_t)h.node + k];
+        h.motion = motion_id;
+        const float* x = &C->hx[3 * (size_t)h.node];
+        if (motion_id >= 0) xf_point(inv, x[0], x[1], x[2], h.p[0], h.p[1], h.p[2]);
+        else for (int k = 0; k < 3; k++) h.p[k] = x[k];
         C->handles.push_back(h);
     }
+    return CS_OK;
+}
+
+cs_status cs_add_motion(cs_ctx* C, const float* times, const cs_transform* keys, int n, int* motion_id) {
+    if (!C) return CS_E_ARG;
+    if ((int)C->motions.size() >= CS_MAX_MOTIONS) {
+        C->err = "at most CS_MAX_MOTIONS motions per context";
+        return CS_E_CAPACITY;
+    }
+    Motion M;
+    const int st = motion_build(times, keys, n, M);
+    if (st != CS_OK) {
+        C->err = "motion keys need n >= 1 and finite, strictly increasing times";
+        return (cs_status)st;
+    }
+    C->motions.push_back(std::move(M));
+    if (motion_id) *motion_id = (int)C->motions.size() - 1;
+    return CS_OK;
+}
+
+cs_status cs_set_obstacle_motion(cs_ctx* C, int obstacle_id, int motion_id) {
+    if (!C) return CS_E_ARG;
+    if (C->finalized) return CS_E_STATE;
+    if (obstacle_id < 0 || obstacle_id >= C->nObstacles) return CS_E_ARG;
+    if (motion_id < 0 || motion_id >= (int)C->motions.size()) return CS_E_ARG;
+    MeshInfo& M = C->meshes[C->nCloths + obstacle_id];
+    if (M.motion >= 0) return CS_E_STATE;  // its base shape is already taken
+    const int first = C->meshes[C->nCloths].n0;  // obstacle nodes follow every cloth node
+    const size_t need = 3 * (size_t)(M.n0 + M.nn - first);
+    if (C->hObsBase.size() < need) C->hObsBase.resize(need, 0.0f);
+    // base = the loaded shape; x = x0 = T(base) (src/Obstacle.cu:21-64)
+    const Xf T = motion_now(C, motion_id);
+    for (int i = M.n0; i < M.n0 + M.nn; i++) {
+        float* b = &C->hObsBase[3 * (size_t)(i - first)];
+        float* x = &C->hx[3 * (size_t)i];
+        for (int k = 0; k < 3; k++) b[k] = x[k];
+        xf_point(T, b[0], b[1], b[2], x[0], x[1], x[2]);
+    }
+    M.motion = motion_id;
+    C->anyMovingObstacle = true;
+    return CS_OK;
+}
+
+cs_status cs_set_step_count(cs_ctx* C, int steps) {
+    if (!C || steps < 0) return CS_E_ARG;
+    C->steps = steps;
+    return CS_OK;
+}
+
+cs_status cs_get_step_count(const cs_ctx* C, int* steps) {
+    if (!C || !steps) return CS_E_ARG;
+    *steps = C->steps;
     return CS_OK;
 }
 

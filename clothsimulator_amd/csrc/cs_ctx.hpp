@@ -19,6 +19,7 @@
 
 #include "../../include/clothsim.h"
 #include "cs_common.hpp"
+#include "cs_motion.hpp"
 #include "cs_prims.hpp"
 
 namespace cs {
@@ -35,6 +36,7 @@ struct MeshInfo {
     bool cloth;
     int n0, nn, f0, nf, e0, ne, v0, nv;
     int material;
+    int motion = -1;  // obstacles: the motion moving it (-1: static)
 };
 
 // Per-mesh LBVH over faces.  Unified node array: internal i at i (i < nf-1),
@@ -119,6 +121,10 @@ struct ClothSys {
     DBuf inc;       // int2 per incidence: x = element (face id, or edge id | 1<<31), y = a | slots<<2 (6 bits each)
     DBuf hcount;    // float[nn]  handles on the node
     DBuf hpos;      // float4[nn] sum of handle anchors
+    // rows with a moving handle: all of the cloth's handled rows, re-anchored every step
+    int nHandleRows = 0;
+    DBuf hgPtr, hgRow;  // int[rows+1] into hAnchor, int[rows] row
+    DBuf hAnchor;       // float4 per handle: anchor, motion index bits in w (-1: fixed)
     DBuf b;         // float[3nn] static right-hand side
     DBuf bc;        // float[3nn] contact right-hand side
     DBuf Minv;      // float[9nn]
@@ -179,10 +185,18 @@ struct cs_ctx {
     std::vector<int> hfe;            // 3F edge ids
     std::vector<int> hen, heopp, headj;  // 2E each (eopp: uv vertex id)
     std::vector<cs::HostMaterial> materials;
-    struct HandleH { int cloth, node; float p[3]; };
+    struct HandleH { int cloth, node; float p[3]; int motion; };
     std::vector<HandleH> handles;
 
     int N = 0, F = 0, E = 0, NC = 0, FC = 0, EC = 0;  // totals and cloth totals
+
+    // motions (cs_motion.hip): keyframes, the step counter they are evaluated at, and the
+    // obstacles' base shapes (host: 3 floats per obstacle node, set by cs_set_obstacle_motion)
+    std::vector<cs::Motion> motions;
+    int steps = 0;
+    std::vector<float> hObsBase;
+    bool anyMovingObstacle = false;
+    cs::DBuf obsBase, obsMotion;  // per obstacle node: float4 base, int motion (-1: static)
 
     // device state
     cs::DBuf x, x0, x1, v, nrm, area, mass;  // nodes

@@ -281,12 +281,37 @@ static void build_cloth_system(cs_ctx* C, int c, ClothSys& S) {
     }
     std::vector<float> hcount(M.nn, 0.0f), hpos(4 * (size_t)M.nn, 0.0f);
     S.nHandles = 0;
-    for (auto& h : C->handles) {
+    bool moving = false;
+    std::vector<std::vector<int>> byRow(M.nn);  // handles of each row, in insertion order
+    for (size_t k = 0; k < C->handles.size(); k++) {
+        const auto& h = C->handles[k];
         if (h.cloth != c) continue;
         int i = h.node - M.n0;
         hcount[i] += 1.0f;
-        for (int k = 0; k < 3; k++) hpos[4 * i + k] += h.p[k];
+        for (int a = 0; a < 3; a++) hpos[4 * i + a] += h.p[a];
         S.nHandles++;
+        moving |= h.motion >= 0;
+        byRow[i].push_back((int)k);
+    }
+    S.nHandleRows = 0;
+    if (moving) {  // every handled row is re-anchored each step (src/PhysicsHelper.cu:386)
+        std::vector<int> ptr(1, 0), rows;
+        std::vector<float4> anchor;
+        for (int i = 0; i < M.nn; i++) {
+            if (byRow[i].empty()) continue;
+            for (int k : byRow[i]) {
+                const auto& h = C->handles[k];
+                float4 a = make_float4(h.p[0], h.p[1], h.p[2], 0.0f);
+                std::memcpy(&a.w, &h.motion, sizeof(int));
+                anchor.push_back(a);
+            }
+            rows.push_back(i);
+            ptr.push_back((int)anchor.size());
+        }
+        S.nHandleRows = (int)rows.size();
+        upload(S.hgPtr, ptr, C->stream);
+        upload(S.hgRow, rows, C->stream);
+        upload(S.hAnchor, anchor, C->stream);
     }
     hipStream_t st = C->stream;
     upload(S.sliceOff, S.hSliceOff, st);
@@ -334,6 +359,21 @@ void host_finalize(cs_ctx* C) {
     upload(C->x0, x4, st);
     upload(C->x1, x4, st);
     upload(C->v, v4, st);
+    if (C->anyMovingObstacle) {
+        const int no = N - C->NC;
+        std::vector<float4> base(no, make_float4(0, 0, 0, 0));
+        std::vector<int> mot(no, -1);
+        for (const MeshInfo& M : C->meshes) {
+            if (M.cloth || M.motion < 0) continue;
+            for (int i = M.n0; i < M.n0 + M.nn; i++) {
+                const float* b = &C->hObsBase[3 * (size_t)(i - C->NC)];
+                base[i - C->NC] = make_float4(b[0], b[1], b[2], 0.0f);
+                mot[i - C->NC] = M.motion;
+            }
+        }
+        upload(C->obsBase, base, st);
+        upload(C->obsMotion, mot, st);
+    }
     C->nrm.as<float4>(N);
     C->area.as<float>(N);
     C->mass.as<float>(N);

@@ -63,6 +63,10 @@ void lpc_factor_blocks(cs_ctx* C, int c, float* diag, float* lower);
 // when the system has been solved), gradient (optional host copy), Adam when lr > 0
 float lpc_train_step(cs_ctx* C, int c, float lr, float solWeight, float* grad);
 
+// motions (cs_motion.hip)
+void motion_step(cs_ctx* C);                // step counter + 1, obstacle velocities and handle anchors
+Xf motion_now(const cs_ctx* C, int motion);  // a motion at the context's current time
+
 // LBVH (cs_lbvh.hip)
 void bvh_build(cs_ctx* C, int mesh, bool ccd);
 void bvh_build_all(cs_ctx* C, bool ccd);  // every mesh; small meshes in one batched launch

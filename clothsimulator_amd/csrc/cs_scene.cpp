@@ -5,11 +5,11 @@
 // minimum angle, src/Mesh.cu:27-54), "transform" blocks
 // (src/Transformation.cu:9-14,82-88, quaternion rotation src/Quaternion.cu:7-16,
 // 76-78) and material JSON files (src/Material.cu:3-41), then creates a
-// context and adds the cloths, static obstacles and handles in the reference's
-// order.  Scene features outside the per-step hot path are reported, not
-// guessed: obstacle or handle motions and the "obstacles": "<pattern>"
-// sequence form give CS_E_ARG; "remeshing" blocks and the "disable" list are
-// read and ignored (this library never remeshes).
+// context and adds the cloths, motions (keyframe lists or a .mot file,
+// src/Simulator.cu:39-79), obstacles (listed, or the "obstacles": "<pattern>"
+// sequence, each moving with the motion of its index, src/Simulator.cu:90-101)
+// and handles in the reference's order.  "remeshing" blocks and the "disable"
+// list are read and ignored (this library never remeshes).
 #include <cmath>
 #include <climits>
 #include <cstring>
@@ -20,6 +20,7 @@
 
 #include "../../include/clothsim.h"
 #include "cs_json.hpp"
+#include "cs_motion.hpp"
 
 namespace {
 
@@ -79,6 +80,7 @@ struct Transform {
     F3 rotate(F3 x) const { return x * (qs * qs - dot(qv, qv)) + 2.0f * qv * dot(qv, x) + 2.0f * cross(qv, x) * qs; }
     F3 point(F3 p) const { return scaling * rotate(p) + translation; }
     F3 vector(F3 v) const { return rotate(v); }
+    cs::Xf xf() const { return cs::Xf{scaling, translation.x, translation.y, translation.z, qs, qv.x, qv.y, qv.z}; }
 };
 
 std::string read_file(const std::string& path, const char* what) {
@@ -237,12 +239,99 @@ cs_material_in material_in(const cs::json::Value& j, const std::string& root) {
     return mat;
 }
 
+struct MotionKeys {
+    std::vector<float> t;
+    std::vector<cs_transform> x;
+};
+
+// "motions" (src/Simulator.cu:39-79): a list of keyframe lists (src/Motion.cu:3-20), or a
+// {"motfile", "fps", "transform"} block naming a .mot file of per-body frames whose keys are
+// transform * (position, rotation quaternion w x y z) at i / fps (src/Motion.cu:22-37).
+std::vector<MotionKeys> read_motions(const cs::json::Value& j, const std::string& root) {
+    std::vector<MotionKeys> out;
+    const cs::json::Value& mj = j["motions"];
+    if (mj.is_null()) return out;
+    if (mj.is_array()) {
+        for (size_t m = 0; m < mj.size(); m++) {
+            MotionKeys K;
+            for (size_t k = 0; k < mj[m].size(); k++) {
+                K.t.push_back((float)mj[m][k]["time"].as_number("motion time"));
+                K.x.push_back(cs::xf_to(Transform(mj[m][k]["transform"]).xf()));
+            }
+            if (K.t.empty()) throw SceneError(CS_E_FORMAT, "motion " + std::to_string(m) + " has no keyframes");
+            out.push_back(std::move(K));
+        }
+        return out;
+    }
+    if (!mj.is_object() || !mj["motfile"].is_string())
+        throw SceneError(CS_E_FORMAT, "\"motions\" is neither a list nor a {\"motfile\"} block");
+    const float fps = num(mj["fps"], 30.0f, "motions.fps");
+    if (!(fps > 0.0f)) throw SceneError(CS_E_FORMAT, "motions.fps must be positive");
+    const cs::Xf T = Transform(mj["transform"]).xf();
+    const std::string path = resolve(root, mj["motfile"].as_string("motions.motfile"));
+    std::istringstream in(read_file(path, "motion"));
+    std::string a, b;
+    int frames = 0;
+    if (!(in >> a >> frames) || frames < 1) throw SceneError(CS_E_FORMAT, "motion file " + path + ": bad frame count");
+    const float dt = 1.0f / fps;
+    while (in >> a >> b) {  // "body[k] position", frames x (x y z), "body[k] rotation", frames x (w x y z)
+        std::vector<float> pos(3 * (size_t)frames), rot(4 * (size_t)frames);
+        for (float& f : pos)
+            if (!(in >> f)) throw SceneError(CS_E_FORMAT, "motion file " + path + ": truncated positions");
+        if (!(in >> a >> b)) throw SceneError(CS_E_FORMAT, "motion file " + path + ": missing rotations");
+        for (float& f : rot)
+            if (!(in >> f)) throw SceneError(CS_E_FORMAT, "motion file " + path + ": truncated rotations");
+        MotionKeys K;
+        for (int i = 0; i < frames; i++) {
+            const cs::Xf body{1.0f, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2],
+                              rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]};
+            K.t.push_back((float)i * dt);
+            K.x.push_back(cs::xf_to(cs::xf_compose(T, body)));
+        }
+        out.push_back(std::move(K));
+    }
+    if (out.empty()) throw SceneError(CS_E_FORMAT, "motion file " + path + " holds no bodies");
+    return out;
+}
+
+// The reference's stringFormat(pattern, i) (src/Simulator.cu:134-141) for the one integer
+// conversion a sequence pattern may hold ("%d", "%04d", ...); anything else is refused.
+std::string format_index(const std::string& pat, int i) {
+    std::string out;
+    int conversions = 0;
+    for (size_t k = 0; k < pat.size(); k++) {
+        if (pat[k] != '%') { out += pat[k]; continue; }
+        if (k + 1 < pat.size() && pat[k + 1] == '%') { out += '%'; k++; continue; }
+        size_t e = k + 1;
+        bool zero = false;
+        if (e < pat.size() && pat[e] == '0') { zero = true; e++; }
+        int width = 0;
+        while (e < pat.size() && pat[e] >= '0' && pat[e] <= '9' && width < 100) width = 10 * width + (pat[e++] - '0');
+        if (e >= pat.size() || pat[e] != 'd' || ++conversions > 1)
+            throw SceneError(CS_E_FORMAT, "obstacle pattern \"" + pat + "\" needs exactly one %d conversion");
+        std::string d = std::to_string(i);
+        while ((int)d.size() < width) d.insert(d.begin(), zero ? '0' : ' ');
+        out += d;
+        k = e;
+    }
+    if (conversions != 1) throw SceneError(CS_E_FORMAT, "obstacle pattern \"" + pat + "\" needs exactly one %d conversion");
+    return out;
+}
+
+bool file_exists(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    return (bool)f;
+}
+
 struct Scene {
     cs_scene_info info;
     cs_params params;
     std::vector<ObjMesh> cloths, obstacles;
+    std::vector<int> obstacleMotion;
+    std::vector<MotionKeys> motions;
     std::vector<cs_material_in> materials;
-    std::vector<std::pair<int, std::vector<int>>> handles;
+    struct Handles { int cloth, motion; std::vector<int> nodes; };
+    std::vector<Handles> handles;
 };
 
 Scene read_scene(const std::string& path, const std::string& root, const cs_params* base) {
@@ -282,22 +371,36 @@ Scene read_scene(const std::string& path, const std::string& root, const cs_para
         S.cloths.push_back(load_obj(resolve(root, c["mesh"].as_string("mesh")), T));
         S.materials.push_back(material_in(c["materials"].is_array() ? c["materials"][0] : c["materials"], root));
     }
+    S.motions = read_motions(j, root);
+    const int nm = (int)S.motions.size();
+    if (nm > CS_MAX_MOTIONS) throw SceneError(CS_E_CAPACITY, "more than CS_MAX_MOTIONS motions");
     const cs::json::Value& ob = j["obstacles"];
-    if (ob.is_string()) throw SceneError(CS_E_ARG, "obstacle sequences (\"obstacles\": \"<pattern>\") are not supported");
-    for (size_t i = 0; i < ob.size(); i++) {
-        const cs::json::Value& o = ob[i];
-        if (inum(o["motion"], -1, "obstacle motion") >= 0)
-            throw SceneError(CS_E_ARG, "obstacle " + std::to_string(i) + " has a motion: moving obstacles are not supported");
-        if (!o["mesh"].is_string()) throw SceneError(CS_E_FORMAT, "obstacle " + std::to_string(i) + " has no mesh");
-        S.obstacles.push_back(load_obj(resolve(root, o["mesh"].as_string("mesh")), Transform(o["transform"])));
+    if (ob.is_string()) {  // obstacle i = pattern % i, moving with motion i (src/Simulator.cu:95-101)
+        const std::string pat = ob.as_string("obstacles");
+        for (int i = 0;; i++) {
+            const std::string p = resolve(root, format_index(pat, i));
+            if (!file_exists(p)) break;
+            if (i >= nm) throw SceneError(CS_E_FORMAT, "obstacle sequence file " + p + " has no motion " + std::to_string(i));
+            S.obstacles.push_back(load_obj(p, Transform(cs::json::Value())));
+            S.obstacleMotion.push_back(i);
+        }
+    } else {
+        for (size_t i = 0; i < ob.size(); i++) {
+            const cs::json::Value& o = ob[i];
+            const int m = inum(o["motion"], -1, "obstacle motion");
+            if (m < -1 || m >= nm) throw SceneError(CS_E_FORMAT, "obstacle " + std::to_string(i) + " names a missing motion");
+            if (!o["mesh"].is_string()) throw SceneError(CS_E_FORMAT, "obstacle " + std::to_string(i) + " has no mesh");
+            S.obstacles.push_back(load_obj(resolve(root, o["mesh"].as_string("mesh")), Transform(o["transform"])));
+            S.obstacleMotion.push_back(m);
+        }
     }
     const cs::json::Value& hs = j["handles"];
     for (size_t i = 0; i < hs.size(); i++) {
         const cs::json::Value& h = hs[i];
         const int cloth = inum(h["cloth"], 0, "handle cloth");
         if (cloth < 0 || cloth >= (int)S.cloths.size()) throw SceneError(CS_E_FORMAT, "handle on a missing cloth");
-        if (inum(h["motion"], -1, "handle motion") >= 0)
-            throw SceneError(CS_E_ARG, "handle " + std::to_string(i) + " has a motion: moving handles are not supported");
+        const int motion = inum(h["motion"], -1, "handle motion");
+        if (motion < -1 || motion >= nm) throw SceneError(CS_E_FORMAT, "handle " + std::to_string(i) + " names a missing motion");
         std::vector<int> nodes;
         const int nn = (int)(S.cloths[cloth].x.size() / 3);
         for (size_t k = 0; k < h["nodes"].size(); k++) {
@@ -305,11 +408,13 @@ Scene read_scene(const std::string& path, const std::string& root, const cs_para
             if (n < 0 || n >= nn) throw SceneError(CS_E_FORMAT, "handle node " + std::to_string(n) + " out of range");
             nodes.push_back(n);
         }
-        S.handles.emplace_back(cloth, std::move(nodes));
+        S.handles.push_back(Scene::Handles{cloth, motion, std::move(nodes)});
     }
     S.info.n_cloths = (int)S.cloths.size();
     S.info.n_obstacles = (int)S.obstacles.size();
     S.info.n_handles = (int)S.handles.size();
+    S.info.n_motions = nm;
+    for (int m : S.obstacleMotion) S.info.n_moving_obstacles += m >= 0;
     for (const ObjMesh& m : S.cloths) {
         S.info.n_cloth_nodes += (int)(m.x.size() / 3);
         S.info.n_cloth_faces += (int)(m.fx.size() / 3);
@@ -341,13 +446,17 @@ extern "C" cs_status cs_load_scene(const char* config_path, const char* root_dir
             check(cs_add_cloth(C, &mi, &S.materials[i], &id), "cs_add_cloth");
             ids.push_back(id);
         }
-        for (const ObjMesh& m : S.obstacles) {
-            cs_mesh_in mi = mesh_in(m);
+        for (const MotionKeys& K : S.motions)
+            check(cs_add_motion(C, K.t.data(), K.x.data(), (int)K.t.size(), nullptr), "cs_add_motion");
+        for (size_t i = 0; i < S.obstacles.size(); i++) {
+            cs_mesh_in mi = mesh_in(S.obstacles[i]);
             int id = -1;
             check(cs_add_obstacle(C, &mi, &id), "cs_add_obstacle");
+            if (S.obstacleMotion[i] >= 0) check(cs_set_obstacle_motion(C, id, S.obstacleMotion[i]), "cs_set_obstacle_motion");
         }
         for (const auto& h : S.handles)
-            check(cs_add_handles(C, ids[h.first], h.second.data(), (int)h.second.size()), "cs_add_handles");
+            check(cs_add_moving_handles(C, ids[h.cloth], h.nodes.data(), (int)h.nodes.size(), h.motion),
+                  "cs_add_moving_handles");
         *ctx_out = C;
         return CS_OK;
     } catch (const SceneError& e) {
@@ -362,3 +471,21 @@ extern "C" cs_status cs_load_scene(const char* config_path, const char* root_dir
 }
 
 extern "C" const char* cs_scene_error(void) { return g_scene_error.c_str(); }
+
+extern "C" cs_status cs_scene_motion(const char* config_path, const char* root_dir, int motion_id, float time,
+                                     cs_transform* out) {
+    g_scene_error.clear();
+    if (!config_path || !out) return CS_E_ARG;
+    try {
+        cs::json::Value j = parse_file(config_path, "configuration");
+        std::vector<MotionKeys> ms = read_motions(j, root_dir ? root_dir : "");
+        if (motion_id < 0 || motion_id >= (int)ms.size()) throw SceneError(CS_E_ARG, "no motion " + std::to_string(motion_id));
+        return cs_motion_eval(ms[motion_id].t.data(), ms[motion_id].x.data(), (int)ms[motion_id].t.size(), time, out);
+    } catch (const SceneError& e) {
+        g_scene_error = e.what();
+        return (cs_status)e.status;
+    } catch (const std::exception& e) {
+        g_scene_error = e.what();
+        return CS_E_FORMAT;
+    }
+}

@@ -123,6 +123,7 @@ class ClothSpec:
     mesh: MeshArrays
     material: object
     handles: List[int] = field(default_factory=list)
+    handle_motion: int = -1  # the motion moving the handles (-1: fixed)
 
 
 @dataclass
@@ -132,6 +133,9 @@ class SceneSpec:
     cloths: List[ClothSpec]
     obstacles: List[MeshArrays]
     steps: int
+    # keyframe motions: (times, keys), a key = 8 floats (scale, translate xyz, rotate s x y z)
+    motions: list = field(default_factory=list)
+    obstacle_motions: List[int] = field(default_factory=list)  # per obstacle (-1: static)
 
     @property
     def n_cloth_nodes(self):
@@ -151,6 +155,24 @@ def drape(n=129, steps=400, sphere_level=3):
     sx, sf = icosphere(sphere_level, 0.3)
     return SceneSpec("drape%d" % n, default_params(dt=0.005), [ClothSpec(cloth, material_struct())],
                      [MeshArrays(sx, sf)], steps)
+
+
+def sphere_sweep(n=129, steps=800, sphere_level=3):
+    """A sheet hanging from two corners, pushed back and forth by a sphere: the motion and the
+    placement of the reference's conf/sphere.json (sphere scaled by 0.1 in every key, at rest
+    until t = 2 s, then swept along y), with an n x n grid in place of the remeshed square and
+    an icosphere of radius 1 in place of meshes/sphere.obj."""
+    x, f = grid_mesh(n, 1.0, origin=(-0.5, -1.0, 0.5))
+    handles = [(n - 1) * n + 0, (n - 1) * n + (n - 1)]  # the two corners at y = 0
+    sx, sf = icosphere(sphere_level, 1.0)
+    ys = [0.0, 0.0, 0.0, -1.5, 0.0, 1.5, 0.0, -0.4]
+    keys = [[0.1, 0.0, y, 0.0, 1.0, 0.0, 0.0, 0.0] for y in ys]
+    p = default_params(dt=0.005)
+    p.repulsion_thickness = 5e-3
+    p.collision_thickness = 5e-4
+    p.collision_stiffness = 1e6
+    return SceneSpec("sphere_sweep%d" % n, p, [ClothSpec(MeshArrays(x, f), material_struct(), handles)],
+                     [MeshArrays(sx, sf)], steps, motions=[(list(range(8)), keys)], obstacle_motions=[0])
 
 
 def ground_plane(half=2.0, z=0.0):
@@ -201,7 +223,8 @@ def by_name(name):
     if name in table:
         return table[name]()
     # parametrised small variants: hangN, drapeN, crumpleN
-    for prefix, fn in (("hang", hang), ("drape", drape), ("crumple", crumple), ("batch", instances)):
+    for prefix, fn in (("hang", hang), ("drape", drape), ("crumple", crumple), ("batch", instances),
+                       ("sphere_sweep", sphere_sweep)):
         if name.startswith(prefix) and name[len(prefix):].isdigit():
             return fn(int(name[len(prefix):]))
     raise KeyError(name)

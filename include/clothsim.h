@@ -164,10 +164,45 @@ cs_status cs_set_params(cs_ctx* ctx, const cs_params* params);
 /* ---- scene construction ---- */
 /* replaces Cloth::Cloth(json, pool) (src/Cloth.cu:3-23) + Mesh::load (src/Mesh.cu:306-452) */
 cs_status cs_add_cloth(cs_ctx* ctx, const cs_mesh_in* mesh, const cs_material_in* material, int* cloth_id);
-/* replaces Obstacle::Obstacle(json, transformations, pool) (src/Obstacle.cu:3-11); static obstacles only */
+/* replaces Obstacle::Obstacle(json, transformations, pool) (src/Obstacle.cu:3-11); static until
+ * cs_set_obstacle_motion gives it a motion */
 cs_status cs_add_obstacle(cs_ctx* ctx, const cs_mesh_in* mesh, int* obstacle_id);
 /* replaces Cloth::addHandles(nodeIndices, motionIndex=-1, ...) (src/Cloth.cu:649-673); anchors = current x */
 cs_status cs_add_handles(cs_ctx* ctx, int cloth_id, const int* nodes, int n);
+
+/* ---- motions (replaces Motion, src/Motion.cu:3-72, and its users: Obstacle::transform/step,
+ * src/Obstacle.cu:44-85, src/ObstacleHelper.cu:10-26; handle anchors, src/ClothHelper.cu:3-14,
+ * src/PhysicsHelper.cu:379-396; Simulator::updateTransformations, src/Simulator.cu:565-569) ---- */
+#define CS_MAX_MOTIONS 64
+/* Transformation (src/Transformation.cuh:12-16): p -> scale * rotate(p) + translate with the
+ * quaternion (s, x, y, z) = rotate[0..3] applied as x (s^2 - |v|^2) + 2 v (v.x) + 2 s v x x
+ * (src/Quaternion.cu:73-75; not renormalised, as in the reference). */
+typedef struct cs_transform {
+    float scale;
+    float translate[3];
+    float rotate[4];
+} cs_transform;
+/* Motion(json) (src/Motion.cu:3-20): n >= 1 keyframes at increasing times; a first key after
+ * t = 0 gets an identity key at t = 0 in front.  The transformation at time t is the cubic
+ * Hermite of each component between the bracketing keys, with tangents
+ * (x[i+1] - x[i-1]) / (t[i+1] - t[i-1]) and zero at both ends, and the last key after the last
+ * time (src/Motion.cu:45-72).  Fails with CS_E_CAPACITY beyond CS_MAX_MOTIONS. */
+cs_status cs_add_motion(cs_ctx* ctx, const float* times, const cs_transform* keys, int n, int* motion_id);
+/* Motion::computeTransformation for the keyframes (times, keys, n) at `time`, without a
+ * context (host code only; the steps evaluate their motions with the same routine). */
+cs_status cs_motion_eval(const float* times, const cs_transform* keys, int n, float time, cs_transform* out);
+/* Obstacle(json) with "motion" (src/Obstacle.cu:3-10,44-64): the obstacle's current positions
+ * become its base shape, and x = x0 = T(t)(base) now, T = the motion at the context's time
+ * t = step count * dt.  Every physics step then sets v = (T(t)(base) - x) / dt on its nodes
+ * before the proximity search (src/Obstacle.cu:66-85).  Before the first step only. */
+cs_status cs_set_obstacle_motion(cs_ctx* ctx, int obstacle_id, int motion_id);
+/* Cloth::addHandles with a motion (src/Cloth.cu:653-673): anchor = T(t)^-1 (x) now; every
+ * physics step pulls the node towards T(t)(anchor).  Before the first step only. */
+cs_status cs_add_moving_handles(cs_ctx* ctx, int cloth_id, const int* nodes, int n, int motion_id);
+/* Simulator::nSteps: each cs_physics_step first increments it and evaluates the motions at
+ * nSteps * dt (src/Simulator.cu:838-843,565-569).  Set it to resume at a later step. */
+cs_status cs_set_step_count(cs_ctx* ctx, int steps);
+cs_status cs_get_step_count(const cs_ctx* ctx, int* steps);
 
 /* ---- stepping ---- */
 /* replaces Simulator::physicsStep (src/Simulator.cu:571-600) incl. Cloth::physicsStep GPU (src/Cloth.cu:695-813) */
@@ -258,16 +293,25 @@ typedef struct cs_scene_info {
     int end_frame;         /* "end_frame", INT_MAX when absent */
     int n_cloths, n_obstacles, n_handles;
     int n_cloth_nodes, n_cloth_faces;
+    int n_motions;           /* "motions": keyframe lists, or the bodies of a "motfile" */
+    int n_moving_obstacles;  /* obstacles with a "motion" (every one of an "obstacles": "<pattern>" sequence) */
 } cs_scene_info;
 
 /* Reads the configuration JSON at config_path; relative mesh and material paths are resolved
  * against root_dir (the reference resolves them against its working directory).  base (or the
  * defaults when NULL) supplies what the file does not hold (PCG settings, flags).  With
- * ctx_out == NULL only parses and fills info.  Scenes with obstacle or handle motions, or the
- * "obstacles": "<pattern>" sequence form, give CS_E_ARG; unreadable or malformed files give
+ * ctx_out == NULL only parses and fills info.  "motions" may be a list of keyframe lists or a
+ * {"motfile", "fps", "transform"} block (src/Simulator.cu:39-79; the .mot file path is resolved
+ * against root_dir too); obstacles and handles take a "motion" index; "obstacles":
+ * "<printf pattern>" loads <pattern % 0>, <pattern % 1>, ... until a file is missing, obstacle i
+ * moving with motion i (src/Simulator.cu:90-101).  Unreadable or malformed files give
  * CS_E_ARG / CS_E_FORMAT.  The message of the last failure on this thread: cs_scene_error(). */
 cs_status cs_load_scene(const char* config_path, const char* root_dir, const cs_params* base, cs_ctx** ctx_out,
                         cs_scene_info* info);
+/* The transformation of motion `motion_id` of a configuration at `time` (host only: parses the
+ * file's motions, no meshes and no context). */
+cs_status cs_scene_motion(const char* config_path, const char* root_dir, int motion_id, float time,
+                          cs_transform* out);
 const char* cs_scene_error(void);
 
 #ifdef __cplusplus

@@ -64,6 +64,15 @@ def lib():
         L.or_cubic_roots_unit.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.or_amg.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                              C.POINTER(C.c_double)]
+        FP, DP = C.POINTER(C.c_float), C.POINTER(C.c_double)
+        L.or_add_moving_handles.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int]
+        L.or_add_motion.argtypes = [vp, FP, FP, C.c_int, C.POINTER(C.c_int)]
+        L.or_motion_eval.argtypes = [FP, FP, C.c_int, C.c_float, FP]
+        L.or_transform_points.argtypes = [FP, FP, C.c_int, DP, C.c_int, DP]
+        L.or_transform_inverse.argtypes = [FP, FP]
+        L.or_transform_compose.argtypes = [FP, FP, FP]
+        L.or_set_obstacle_motion.argtypes = [vp, C.c_int, C.c_int]
+        L.or_set_step_count.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -104,6 +113,48 @@ def solve_cubic(a, b, c, d):
     return [r[i] for i in range(n)]
 
 
+def _key_array(keys):
+    """keys: (n, 8) floats (scale, translate xyz, rotate s x y z) or objects with as_array()."""
+    return np.ascontiguousarray([k.as_array() if hasattr(k, "as_array") else k for k in keys],
+                                dtype=np.float32).reshape(-1, 8)
+
+
+def motion_eval(times, keys, time):
+    """Motion::computeTransformation restated: the 8 floats of the transformation at `time`."""
+    t = np.ascontiguousarray(times, dtype=np.float32)
+    k = _key_array(keys)
+    out = np.zeros(8, dtype=np.float32)
+    _check(lib().or_motion_eval(fptr(t), fptr(k), len(t), float(time), fptr(out)), "motion_eval")
+    return out
+
+
+def transform_points(a, p, op=0, b=None):
+    """op 0: A(p); 1: A^-1(p); 2: (A * B)(p), the composed transformation (FP32 keys, FP64 points)."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b if b is not None else a, dtype=np.float32)
+    p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros_like(p)
+    lib().or_transform_points(fptr(a), fptr(b), op, p.ctypes.data_as(C.POINTER(C.c_double)), len(p),
+                              out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def transform_compose(a, b):
+    """The keys of A * B (apply B, then A), src/Transformation.cu:58-64."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    out = np.zeros(8, dtype=np.float32)
+    lib().or_transform_compose(fptr(a), fptr(b), fptr(out))
+    return out
+
+
+def transform_inverse(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    out = np.zeros(8, dtype=np.float32)
+    lib().or_transform_inverse(fptr(a), fptr(out))
+    return out
+
+
 def cubic_roots_unit(a, b, c, d):
     """Roots on [0, 1] of the isolated root finder used by the CCD."""
     r = (C.c_double * 4)()
@@ -122,12 +173,17 @@ class OracleScene:
         self._keep = []
         self.cloth_nodes = []
         if spec is not None:
-            for c in spec.cloths:
-                cid = self.add_cloth(c.mesh, c.material)
+            cids = [self.add_cloth(c.mesh, c.material) for c in spec.cloths]
+            for times, keys in getattr(spec, "motions", []):
+                self.add_motion(times, keys)
+            for cid, c in zip(cids, spec.cloths):
                 if c.handles:
-                    self.add_handles(cid, c.handles)
-            for o in spec.obstacles:
-                self.add_obstacle(o)
+                    self.add_handles(cid, c.handles, getattr(c, "handle_motion", -1))
+            for i, o in enumerate(spec.obstacles):
+                oid = self.add_obstacle(o)
+                m = spec.obstacle_motions[i] if getattr(spec, "obstacle_motions", None) else -1
+                if m >= 0:
+                    self.set_obstacle_motion(oid, m)
 
     def close(self):
         if self.h:
@@ -159,9 +215,22 @@ class OracleScene:
         _check(self.L.or_add_obstacle(self.h, C.byref(m), C.byref(oid)), "add_obstacle")
         return oid.value
 
-    def add_handles(self, cloth, nodes):
+    def add_handles(self, cloth, nodes, motion=-1):
         a = np.ascontiguousarray(nodes, dtype=np.int32)
-        _check(self.L.or_add_handles(self.h, cloth, iptr(a), len(a)), "add_handles")
+        _check(self.L.or_add_moving_handles(self.h, cloth, iptr(a), len(a), motion), "add_handles")
+
+    def add_motion(self, times, keys):
+        t = np.ascontiguousarray(times, dtype=np.float32)
+        k = _key_array(keys)
+        mid = C.c_int()
+        _check(self.L.or_add_motion(self.h, fptr(t), fptr(k), len(t), C.byref(mid)), "add_motion")
+        return mid.value
+
+    def set_obstacle_motion(self, obstacle, motion):
+        _check(self.L.or_set_obstacle_motion(self.h, obstacle, motion), "set_obstacle_motion")
+
+    def set_step_count(self, n):
+        _check(self.L.or_set_step_count(self.h, n), "set_step_count")
 
     def physics_step(self, direct=False):
         st = Stats()

@@ -776,6 +776,18 @@ template <class T> int collision_optimize(Scene<T>& S, const vector<Impact>& imp
 // Simulator::physicsStep (src/Simulator.cu:571-600), one cloth system at a time.
 template <class T> int physics_step(Scene<T>& S, int directSolver, cs_stats* st) {
     S.started = true;
+    // Simulator::simulateStep: nSteps + 1, updateTransformations, Obstacle::step
+    // (src/Simulator.cu:838-843,565-573, src/Obstacle.cu:66-85)
+    S.steps++;
+    if (!S.motions.empty()) {
+        T invDt = T(1) / T(S.P.dt);
+        for (int o = S.nCloths; o < (int)S.meshes.size(); o++) {
+            const MeshRange& R = S.meshes[o];
+            if (R.motion < 0) continue;
+            Tf X = S.motion_now(R.motion);
+            for (int i = R.n0; i < R.n0 + R.nn; i++) S.v[i] = (tf_point(X, S.base[i]) - S.x[i]) * invDt;
+        }
+    }
     vector<Prox<T>> prox;
     if (!(S.P.flags & CS_FLAG_NO_PROXIMITY)) prox = find_proximities(S);
     if (st) { st->n_proximities = (int)prox.size(); st->pcg_iters = 0; st->pcg_rel_residual = 0; st->pcg_converged = 1; }

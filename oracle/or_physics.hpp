@@ -264,7 +264,9 @@ template <class T> void assemble(const Scene<T>& S, int cloth, const vector<Prox
         int i = h.node - R.n0;
         array<T, 9>& a = sys.blk(i, i);
         for (int j = 0; j < 3; j++) a[4 * j] += dt * dt * hk;
-        V3<T> fh = dt * ((h.pos - S.x[h.node]) - dt * S.v[h.node]) * hk;
+        // the anchor moves with its motion (src/PhysicsHelper.cu:386)
+        V3<T> pos = h.motion >= 0 ? tf_point(S.motion_now(h.motion), h.pos) : h.pos;
+        V3<T> fh = dt * ((pos - S.x[h.node]) - dt * S.v[h.node]) * hk;
         for (int j = 0; j < 3; j++) sys.b[3 * i + j] += fh[j];
     }
     // addProximityForces (src/Cloth.cu:175-192)

@@ -18,6 +18,7 @@
 
 #include "../include/clothsim.h"
 #include "or_math.hpp"
+#include "or_motion.hpp"
 
 namespace orc {
 
@@ -179,6 +180,7 @@ struct MeshRange {
     int f0, nf;  // faces
     int e0, ne;  // edges
     int material;
+    int motion = -1;  // obstacles: the motion moving it (src/Obstacle.cuh:19)
 };
 
 template <class T> struct Scene {
@@ -203,9 +205,15 @@ template <class T> struct Scene {
     // edges: nodes (min,max), per side: uv vertices (2), opposite uv vertex, adjacent face
     std::vector<std::array<int, 2>> en, eopp, eadj;
     std::vector<std::array<std::array<int, 2>, 2>> ev;
-    // handles (src/Handle.cuh:9-16), motion index always -1
-    struct Handle { int cloth; int node; V3<T> pos; };
+    // handles (src/Handle.cuh:9-16): anchor in the motion's frame (motion -1: world)
+    struct Handle { int cloth; int node; V3<T> pos; int motion; };
     std::vector<Handle> handles;
+    // motions, the step counter they are evaluated at (src/Simulator.cu:565-569) and the
+    // base shape of every moving obstacle's nodes (src/Obstacle.cu:21-36)
+    std::vector<Motion> motions;
+    int steps = 0;
+    std::vector<V3<T>> base;
+    Tf motion_now(int m) const { return motions[m].at(float(steps) * P.dt); }
     std::vector<Material> materials;
 
     bool nodeFree(int i) const { return isFree[i] != 0; }

@@ -104,20 +104,98 @@ int or_add_obstacle(void* h, const cs_mesh_in* m, int* id) {
     return dispatch(h, [&](auto& S) { return add_mesh(S, m, nullptr, id); });
 }
 
-int or_add_handles(void* h, int cloth, const int* nodes, int n) {
+int or_add_moving_handles(void* h, int cloth, const int* nodes, int n, int motion) {
     return dispatch(h, [&](auto& S) {
-        using T = std::decay_t<decltype(S.x[0][0])>;
         if (cloth < 0 || cloth >= S.nCloths || n < 0) return (int)CS_E_ARG;
+        if (motion < -1 || motion >= (int)S.motions.size()) return (int)CS_E_ARG;
         const MeshRange& R = S.meshes[cloth];
-        for (int i = 0; i < n; i++) {
+        for (int i = 0; i < n; i++)
             if (nodes[i] < 0 || nodes[i] >= R.nn) return (int)CS_E_ARG;
+        // anchor = T^-1 (x) for a moving handle (src/Cloth.cu:653-665)
+        Tf inv = motion >= 0 ? tf_inverse(S.motion_now(motion)) : tf_identity();
+        for (int i = 0; i < n; i++) {
             typename std::decay_t<decltype(S)>::Handle hd;
-            hd.cloth = cloth; hd.node = R.n0 + nodes[i]; hd.pos = S.x[hd.node];
+            hd.cloth = cloth; hd.node = R.n0 + nodes[i]; hd.motion = motion;
+            hd.pos = motion >= 0 ? tf_point(inv, S.x[hd.node]) : S.x[hd.node];
             S.handles.push_back(hd);
         }
-        (void)sizeof(T);
         return (int)CS_OK;
     });
+}
+
+int or_add_handles(void* h, int cloth, const int* nodes, int n) { return or_add_moving_handles(h, cloth, nodes, n, -1); }
+
+static Tf tf_in(const float* k) { Tf t; for (int i = 0; i < 8; i++) t[i] = k[i]; return t; }
+
+// keys: 8 floats per key (scale, translate xyz, rotate s x y z)
+int or_add_motion(void* h, const float* times, const float* keys, int n, int* id) {
+    std::vector<Tf> k(n > 0 ? n : 0);
+    for (int i = 0; i < n; i++) k[i] = tf_in(keys + 8 * i);
+    return dispatch(h, [&](auto& S) {
+        Motion M;
+        if (!M.build(times, k.data(), n)) return (int)CS_E_ARG;
+        S.motions.push_back(M);
+        if (id) *id = (int)S.motions.size() - 1;
+        return (int)CS_OK;
+    });
+}
+
+int or_motion_eval(const float* times, const float* keys, int n, float time, float* out) {
+    std::vector<Tf> k(n > 0 ? n : 0);
+    for (int i = 0; i < n; i++) k[i] = tf_in(keys + 8 * i);
+    Motion M;
+    if (!M.build(times, k.data(), n)) return CS_E_ARG;
+    Tf r = M.at(time);
+    for (int i = 0; i < 8; i++) out[i] = r[i];
+    return CS_OK;
+}
+
+// op 0: T(p); 1: T^-1 (p); 2: A(B(p)) through the composed transformation A * B (b = B)
+int or_transform_points(const float* a, const float* b, int op, const double* p, int n, double* out) {
+    Tf A = tf_in(a);
+    if (op == 1) A = tf_inverse(A);
+    if (op == 2) A = tf_compose(A, tf_in(b));
+    for (int i = 0; i < n; i++) {
+        V3<double> q = tf_point(A, mk3<double>(p[3 * i], p[3 * i + 1], p[3 * i + 2]));
+        for (int c = 0; c < 3; c++) out[3 * i + c] = q[c];
+    }
+    return CS_OK;
+}
+
+int or_transform_compose(const float* a, const float* b, float* out) {
+    Tf r = tf_compose(tf_in(a), tf_in(b));
+    for (int i = 0; i < 8; i++) out[i] = r[i];
+    return CS_OK;
+}
+
+int or_transform_inverse(const float* a, float* out) {
+    Tf r = tf_inverse(tf_in(a));
+    for (int i = 0; i < 8; i++) out[i] = r[i];
+    return CS_OK;
+}
+
+int or_set_obstacle_motion(void* h, int obstacle, int motion) {
+    return dispatch(h, [&](auto& S) {
+        if (S.started || obstacle < 0 || obstacle >= S.nObstacles) return (int)CS_E_ARG;
+        if (motion < 0 || motion >= (int)S.motions.size()) return (int)CS_E_ARG;
+        MeshRange& R = S.meshes[S.nCloths + obstacle];
+        if (R.motion >= 0) return (int)CS_E_STATE;
+        if (S.base.size() < S.x.size()) S.base.resize(S.x.size());
+        // Obstacle::initialize + transform (src/Obstacle.cu:21-64)
+        Tf T = S.motion_now(motion);
+        for (int i = R.n0; i < R.n0 + R.nn; i++) {
+            S.base[i] = S.x[i];
+            S.x[i] = S.x0[i] = tf_point(T, S.base[i]);
+        }
+        R.motion = motion;
+        update_face_geometry(S, R);
+        update_node_geometry(S, R);
+        return (int)CS_OK;
+    });
+}
+
+int or_set_step_count(void* h, int steps) {
+    return dispatch(h, [&](auto& S) { S.steps = steps; return (int)CS_OK; });
 }
 
 int or_physics_step(void* h, int direct, cs_stats* st) {

@@ -3,6 +3,7 @@ through cs_load_scene.  CPU tests parse only (no device); the GPU test checks
 the placed meshes against an explicit construction of the same scene."""
 import math
 import os
+import re
 
 import numpy as np
 import pytest
@@ -87,12 +88,12 @@ def test_scene_info_of_written_scene(tmp_path):
 
 
 @pytest.mark.parametrize("bad, status", [
-    ('"obstacles": [{"mesh": "tri.obj", "motion": 0}]', "CS_E_ARG"),
-    ('"obstacles": "obs_%d.obj"', "CS_E_ARG"),
+    ('"obstacles": [{"mesh": "tri.obj", "motion": 0}]', "CS_E_FORMAT.*missing motion"),
+    ('"obstacles": "obs_%s.obj"', "CS_E_FORMAT.*%d"),
     ('"handles": [{"nodes": [99]}]', "CS_E_FORMAT"),
-    ('"handles": [{"nodes": [0], "motion": 1}]', "CS_E_ARG"),
+    ('"handles": [{"nodes": [0], "motion": 1}]', "CS_E_FORMAT.*missing motion"),
 ])
-def test_unsupported_or_malformed_scenes(tmp_path, bad, status):
+def test_malformed_scenes(tmp_path, bad, status):
     cfg = CONFIG.replace('"obstacles": [{"mesh": "tri.obj", "transform": {"scale": 0.5}}]', '"obstacles": []')
     cfg = cfg.replace('"handles": [{"nodes": [0, 2]}]', '"handles": []')
     key = bad.split(":")[0]
@@ -116,17 +117,22 @@ def test_missing_mesh_file(tmp_path):
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="reference tree not present")
 def test_reference_configurations_parse():
-    """Every configuration of the reference either loads or is refused for a named
-    unsupported feature (motions / obstacle sequences), never for a parse error."""
-    loaded = 0
+    """Every configuration of the reference loads (with its motions, .mot files and obstacle
+    sequences), except where a file it names is not in the reference tree: conf/karate.json's
+    motions/karate-smooth.mot is absent, and that is reported by name."""
+    loaded = {}
     for name in sorted(os.listdir(os.path.join(REF, "conf"))):
         try:
             info = cs.scene_info(os.path.join(REF, "conf", name), REF)
             assert info.n_cloths >= 1 and info.n_cloth_faces > 0
-            loaded += 1
+            loaded[name] = info
         except ClothError as e:
-            assert "CS_E_ARG" in str(e) and ("motion" in str(e) or "sequences" in str(e)), (name, str(e))
-    assert loaded >= 3
+            missing = re.search(r"failed to open \w+ file: (\S+)", str(e))
+            assert "CS_E_ARG" in str(e) and missing and not os.path.exists(missing.group(1)), (name, str(e))
+    assert len(loaded) >= 11
+    assert (loaded["funnel.json"].n_motions, loaded["funnel.json"].n_moving_obstacles) == (1, 1)
+    assert (loaded["dress-blue.json"].n_motions, loaded["dress-blue.json"].n_moving_obstacles) == (16, 16)
+    assert loaded["sphere.json"].n_moving_obstacles == 1 and loaded["sphere.json"].n_handles == 1
 
 
 def expected_scene():
