@@ -653,6 +653,23 @@ void bvh_build(cs_ctx* C, int mesh, bool ccd) {
 }
 
 constexpr int SMALL_BVH = 1 << 15;  // meshes up to this many faces build and refit in one batched launch
+// One workgroup builds a tree in time growing with its size (2 ms at 32,768 faces, latency-bound
+// on one CU); the per-mesh kernels spread a tree over the chip for about 35 launches.  Trees
+// above ALONE_FACES take the per-mesh path when at most ALONE_MAX of them are in the stage.
+constexpr int ALONE_FACES = 1 << 12;
+constexpr int ALONE_MAX = 4;
+
+// Meshes of [m0, m1) that build and refit with the per-mesh kernels.
+static std::vector<bool> per_mesh(const cs_ctx* C, int m0, int m1) {
+    int big = 0;
+    for (int m = m0; m < m1; m++) big += C->meshes[m].nf > ALONE_FACES && C->meshes[m].nf <= SMALL_BVH;
+    std::vector<bool> alone(m1 - m0);
+    for (int m = m0; m < m1; m++) {
+        const int nf = C->meshes[m].nf;
+        alone[m - m0] = nf > SMALL_BVH || (nf > ALONE_FACES && big <= ALONE_MAX);
+    }
+    return alone;
+}
 
 static BuildJob small_job(const MeshInfo& M, Bvh& B) {
     BuildJob j;
@@ -692,9 +709,10 @@ static void launch_jobs(cs_ctx* C, const std::vector<BuildJob>& jobs, K kern, A.
 void bvh_build_all(cs_ctx* C, bool ccd) {
     std::vector<BuildJob> jobs;
     size_t tmpLen = 0;
+    const std::vector<bool> alone = per_mesh(C, 0, (int)C->meshes.size());
     for (size_t m = 0; m < C->meshes.size(); m++) {
         const MeshInfo& M = C->meshes[m];
-        if (M.nf > SMALL_BVH || M.nf < 1) {
+        if (alone[m] || M.nf < 1) {
             bvh_build(C, (int)m, ccd);
             continue;
         }
@@ -717,10 +735,11 @@ void bvh_build_all(cs_ctx* C, bool ccd) {
 // Refit of meshes [m0, m1) (each built before): small meshes batched.
 void bvh_refit_range(cs_ctx* C, int m0, int m1) {
     std::vector<BuildJob> jobs;
+    const std::vector<bool> alone = per_mesh(C, 0, (int)C->meshes.size());
     for (int m = m0; m < m1; m++) {
         Bvh& B = C->bvh[m];
         if (B.nf < 1) continue;
-        if (B.nf > SMALL_BVH) {
+        if (alone[m]) {
             bvh_refit(C, m);
             continue;
         }
