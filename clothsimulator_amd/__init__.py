@@ -26,7 +26,7 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_lpc_weight_count", "cs_set_learned_weights", "cs_get_learned_weights", "cs_lpc_factor",
             "cs_lpc_train_step", "cs_lpc_smoother_omega", "cs_batch_tuning", "cs_add_motion", "cs_motion_eval",
             "cs_set_obstacle_motion", "cs_add_moving_handles", "cs_set_step_count", "cs_get_step_count",
-            "cs_scene_motion"]
+            "cs_scene_motion", "cs_get_topology", "cs_write_config"]
 
 # trained weights of the learned preconditioner shipped with the package (float32, .npy)
 LPC_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lpc_weights.npy")
@@ -92,6 +92,9 @@ def load_library(path=LIB_PATH):
     L.cs_set_step_count.argtypes = [vp, C.c_int]
     L.cs_get_step_count.argtypes = [vp, C.POINTER(C.c_int)]
     L.cs_scene_motion.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_float, TP]
+    IP = C.POINTER(C.c_int)
+    L.cs_get_topology.argtypes = [vp, C.c_int, C.c_int, IP, IP, F, IP]
+    L.cs_write_config.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p), C.c_int]
     _lib = L
     return L
 
@@ -123,6 +126,15 @@ def scene_motion(config_path, motion_id, time, root_dir=None):
     if st != CS_OK:
         raise ClothError(st, "cs_scene_motion", L.cs_scene_error().decode())
     return out
+
+
+def write_config(config_path, out_path, cloth_meshes):
+    """Simulator::save's configuration (cs_write_config): cloth i's "mesh" = cloth_meshes[i]."""
+    L = load_library()
+    arr = (C.c_char_p * len(cloth_meshes))(*[m.encode() for m in cloth_meshes])
+    st = L.cs_write_config(config_path.encode(), out_path.encode(), arr, len(cloth_meshes))
+    if st != CS_OK:
+        raise ClothError(st, "cs_write_config", L.cs_scene_error().decode())
 
 
 def scene_info(config_path, root_dir=None):
@@ -251,6 +263,18 @@ class Simulator:
 
     def set_obstacle_motion(self, obstacle, motion):
         self._check(self.L.cs_set_obstacle_motion(self.h, obstacle, motion), "cs_set_obstacle_motion")
+
+    def topology(self, kind=CS_MESH_CLOTH, idx=0):
+        """(face node ids (nf, 3), face uv-vertex ids (nf, 3), uv (nv, 2)), mesh-local (cs_get_topology)."""
+        _, nf, _ = self.counts(kind, idx)
+        nv = C.c_int()
+        self._check(self.L.cs_get_topology(self.h, kind, idx, None, None, None, C.byref(nv)), "cs_get_topology")
+        fn = np.zeros((nf, 3), dtype=np.int32)
+        fv = np.zeros((nf, 3), dtype=np.int32)
+        uv = np.zeros((nv.value, 2), dtype=np.float32)
+        self._check(self.L.cs_get_topology(self.h, kind, idx, iptr(fn), iptr(fv), fptr(uv), C.byref(nv)),
+                    "cs_get_topology")
+        return fn, fv, uv
 
     def step_count(self):
         n = C.c_int()

@@ -394,7 +394,25 @@ cs_status cs_set_obstacle_motion(cs_ctx* C, int obstacle_id, int motion_id) {
 
 cs_status cs_set_step_count(cs_ctx* C, int steps) {
     if (!C || steps < 0) return CS_E_ARG;
-    C->steps = steps;
+    return guard(C, [&]() {
+        C->steps = steps;
+        motion_place(C);
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_get_topology(cs_ctx* C, int kind, int id, int* face_nodes, int* face_verts, float* uv, int* n_verts) {
+    if (!C) return CS_E_ARG;
+    const MeshInfo* M = mesh_of(C, kind, id);
+    if (!M) return CS_E_ARG;
+    if (n_verts) *n_verts = M->nv;
+    for (int f = 0; f < M->nf; f++)
+        for (int k = 0; k < 3; k++) {
+            const size_t g = 3 * (size_t)(M->f0 + f) + k;
+            if (face_nodes) face_nodes[3 * f + k] = C->hfn[g] - M->n0;
+            if (face_verts) face_verts[3 * f + k] = C->hfv[g] - M->v0;
+        }
+    if (uv) std::copy(C->huv.begin() + 2 * (size_t)M->v0, C->huv.begin() + 2 * (size_t)(M->v0 + M->nv), uv);
     return CS_OK;
 }
 

@@ -8,6 +8,7 @@
 #pragma once
 #include <cstdlib>
 #include <stdexcept>
+#include <cstdio>
 #include <string>
 #include <utility>
 #include <vector>
@@ -184,6 +185,65 @@ class Parser {
 };
 
 inline Value parse(const std::string& text) { return Parser(text).parse(); }
+
+// Serialises a value (keys in their parsed order, numbers with 17 significant digits so they
+// read back exactly, two-space indentation).
+inline void dump_to(const Value& v, std::string& out, int indent) {
+    const std::string pad(2 * (indent + 1), ' '), end(2 * indent, ' ');
+    switch (v.kind) {
+        case Value::Null: out += "null"; break;
+        case Value::Bool: out += v.b ? "true" : "false"; break;
+        case Value::Number: {
+            char buf[32];
+            std::snprintf(buf, sizeof(buf), "%.17g", v.num);
+            out += buf;
+            break;
+        }
+        case Value::String: {
+            out += '"';
+            for (char c : v.str) {
+                if (c == '"' || c == '\\') { out += '\\'; out += c; }
+                else if ((unsigned char)c < 0x20) {
+                    char buf[8];
+                    std::snprintf(buf, sizeof(buf), "\\u%04x", (unsigned)(unsigned char)c);
+                    out += buf;
+                } else out += c;
+            }
+            out += '"';
+            break;
+        }
+        case Value::Array:
+            out += "[";
+            for (size_t i = 0; i < v.arr.size(); i++) {
+                out += i ? ",\n" : "\n";
+                out += pad;
+                dump_to(v.arr[i], out, indent + 1);
+            }
+            out += v.arr.empty() ? "]" : "\n" + end + "]";
+            break;
+        case Value::Object:
+            out += "{";
+            for (size_t i = 0; i < v.obj.size(); i++) {
+                out += i ? ",\n" : "\n";
+                out += pad;
+                Value key;
+                key.kind = Value::String;
+                key.str = v.obj[i].first;
+                dump_to(key, out, indent + 1);
+                out += ": ";
+                dump_to(v.obj[i].second, out, indent + 1);
+            }
+            out += v.obj.empty() ? "}" : "\n" + end + "}";
+            break;
+    }
+}
+
+inline std::string dump(const Value& v) {
+    std::string out;
+    dump_to(v, out, 0);
+    out += "\n";
+    return out;
+}
 
 }  // namespace json
 }  // namespace cs

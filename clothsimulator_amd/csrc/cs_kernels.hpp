@@ -66,6 +66,7 @@ float lpc_train_step(cs_ctx* C, int c, float lr, float solWeight, float* grad);
 // motions (cs_motion.hip)
 void motion_step(cs_ctx* C);                // step counter + 1, obstacle velocities and handle anchors
 Xf motion_now(const cs_ctx* C, int motion);  // a motion at the context's current time
+void motion_place(cs_ctx* C);               // moving obstacles to x = x0 = T(now)(base)
 
 // LBVH (cs_lbvh.hip)
 void bvh_build(cs_ctx* C, int mesh, bool ccd);

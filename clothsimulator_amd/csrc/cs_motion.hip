@@ -48,6 +48,22 @@ __global__ void __launch_bounds__(256) k_obstacle_motion(int nc, int n, const fl
     v[i] = make_float4((qx - p.x) * invDt, (qy - p.y) * invDt, (qz - p.z) * invDt, 0.0f);
 }
 
+// Obstacle::transform (src/Obstacle.cu:44-64, src/ObstacleHelper.cu:10-17): x = x0 = T(base).
+__global__ void __launch_bounds__(256) k_obstacle_place(int nc, int n, const float4* __restrict__ base,
+                                                        const int* __restrict__ motion, XfTable T,
+                                                        float4* __restrict__ x, float4* __restrict__ x0) {
+    const int i = nc + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = motion[i - nc];
+    if (m < 0) return;
+    const float4 b = base[i - nc];
+    float qx, qy, qz;
+    xf_point(T.m[m], b.x, b.y, b.z, qx, qy, qz);
+    const float4 p = make_float4(qx, qy, qz, 0.0f);
+    x[i] = p;
+    x0[i] = p;
+}
+
 // One thread per handled row: the sum of its handles' current anchors, in insertion order.
 __global__ void __launch_bounds__(256) k_handle_anchors(int ng, const int* __restrict__ ptr,
                                                         const int* __restrict__ row,
@@ -129,6 +145,37 @@ void motion_step(cs_ctx* C) {
                            (const float4*)S.hAnchor.ptr<float4>(), T, S.hpos.ptr<float4>());
         CS_LAUNCH_CHECK();
     }
+}
+
+void motion_place(cs_ctx* C) {
+    if (!C->anyMovingObstacle) return;
+    if (!C->finalized) {  // host state: re-place from the stored bases
+        const int first = C->meshes[C->nCloths].n0;
+        for (const MeshInfo& M : C->meshes) {
+            if (M.cloth || M.motion < 0) continue;
+            const Xf T = motion_now(C, M.motion);
+            for (int i = M.n0; i < M.n0 + M.nn; i++) {
+                const float* b = &C->hObsBase[3 * (size_t)(i - first)];
+                float* x = &C->hx[3 * (size_t)i];
+                xf_point(T, b[0], b[1], b[2], x[0], x[1], x[2]);
+            }
+        }
+        return;
+    }
+    const float time = (float)C->steps * C->P.dt;
+    XfTable T;
+    for (size_t m = 0; m < C->motions.size(); m++) T.m[m] = motion_at(C->motions[m], time);
+    const int n = C->N - C->NC;
+    hipLaunchKernelGGL(k_obstacle_place, dim3(cdiv(n, 256)), dim3(256), 0, C->stream, C->NC, C->N,
+                       (const float4*)C->obsBase.ptr<float4>(), (const int*)C->obsMotion.ptr<int>(), T,
+                       C->x.ptr<float4>(), C->x0.ptr<float4>());
+    CS_LAUNCH_CHECK();
+    for (const MeshInfo& M : C->meshes) {
+        if (M.cloth || M.motion < 0) continue;
+        launch_face_geometry(C, M.f0, M.nf);
+        launch_node_geometry(C, M.n0, M.nn);
+    }
+    CS_HIP(hipStreamSynchronize(C->stream));
 }
 
 Xf motion_now(const cs_ctx* C, int motion) { return motion_at(C->motions[motion], (float)C->steps * C->P.dt); }

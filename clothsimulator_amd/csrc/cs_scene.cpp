@@ -472,6 +472,51 @@ extern "C" cs_status cs_load_scene(const char* config_path, const char* root_dir
 
 extern "C" const char* cs_scene_error(void) { return g_scene_error.c_str(); }
 
+extern "C" cs_status cs_write_config(const char* config_path, const char* out_path, const char* const* cloth_meshes,
+                                     int n_cloths) {
+    g_scene_error.clear();
+    if (!config_path || !out_path || n_cloths < 0 || (n_cloths > 0 && !cloth_meshes)) return CS_E_ARG;
+    try {
+        cs::json::Value j = parse_file(config_path, "configuration");
+        if (!j.is_object()) throw SceneError(CS_E_FORMAT, std::string("configuration ") + config_path + " is not a JSON object");
+        cs::json::Value* cloths = nullptr;
+        for (auto& kv : j.obj)
+            if (kv.first == "cloths") cloths = &kv.second;
+        if (!cloths || !cloths->is_array() || (int)cloths->arr.size() != n_cloths)
+            throw SceneError(CS_E_ARG, "the configuration's \"cloths\" do not match the cloth count");
+        for (int i = 0; i < n_cloths; i++) {
+            cs::json::Value& c = cloths->arr[i];
+            if (!c.is_object() || !cloth_meshes[i]) throw SceneError(CS_E_ARG, "cloth " + std::to_string(i));
+            std::vector<std::pair<std::string, cs::json::Value>> kept;
+            bool mesh = false;
+            for (auto& kv : c.obj) {
+                if (kv.first == "transform") continue;
+                if (kv.first == "mesh") { kv.second = cs::json::Value(); kv.second.kind = cs::json::Value::String;
+                                          kv.second.str = cloth_meshes[i]; mesh = true; }
+                kept.push_back(kv);
+            }
+            if (!mesh) {
+                cs::json::Value m;
+                m.kind = cs::json::Value::String;
+                m.str = cloth_meshes[i];
+                kept.emplace_back("mesh", m);
+            }
+            c.obj = std::move(kept);
+        }
+        std::ofstream out(out_path, std::ios::binary);
+        if (!out) throw SceneError(CS_E_ARG, std::string("cannot write ") + out_path);
+        out << cs::json::dump(j);
+        if (!out) throw SceneError(CS_E_ARG, std::string("cannot write ") + out_path);
+        return CS_OK;
+    } catch (const SceneError& e) {
+        g_scene_error = e.what();
+        return (cs_status)e.status;
+    } catch (const std::exception& e) {
+        g_scene_error = e.what();
+        return CS_E_FORMAT;
+    }
+}
+
 extern "C" cs_status cs_scene_motion(const char* config_path, const char* root_dir, int motion_id, float time,
                                      cs_transform* out) {
     g_scene_error.clear();

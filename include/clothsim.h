@@ -200,7 +200,8 @@ cs_status cs_set_obstacle_motion(cs_ctx* ctx, int obstacle_id, int motion_id);
  * physics step pulls the node towards T(t)(anchor).  Before the first step only. */
 cs_status cs_add_moving_handles(cs_ctx* ctx, int cloth_id, const int* nodes, int n, int motion_id);
 /* Simulator::nSteps: each cs_physics_step first increments it and evaluates the motions at
- * nSteps * dt (src/Simulator.cu:838-843,565-569).  Set it to resume at a later step. */
+ * nSteps * dt (src/Simulator.cu:838-843,565-569).  Setting it also places every moving
+ * obstacle at T(nSteps * dt)(base), x = x0, as Simulator::resume does (src/Simulator.cu:1045-1053). */
 cs_status cs_set_step_count(cs_ctx* ctx, int steps);
 cs_status cs_get_step_count(const cs_ctx* ctx, int* steps);
 
@@ -218,6 +219,10 @@ cs_status cs_run(cs_ctx* ctx, int steps, cs_stats* stats_accum);
 cs_status cs_get_counts(cs_ctx* ctx, int kind, int id, int* n_nodes, int* n_faces, int* n_edges);
 cs_status cs_get_state(cs_ctx* ctx, int kind, int id, float* x, float* x0, float* v);
 cs_status cs_set_state(cs_ctx* ctx, int kind, int id, const float* x, const float* x0, const float* v);
+/* The faces and material-space vertices Mesh::save writes (src/Mesh.cu:454-507): face node ids
+ * (3 nf) and face uv-vertex ids (3 nf), both mesh-local, and the uv vertices (2 n_verts).  NULL
+ * buffers query n_verts only.  Host data: needs no device. */
+cs_status cs_get_topology(cs_ctx* ctx, int kind, int id, int* face_nodes, int* face_verts, float* uv, int* n_verts);
 /* node normal (3n), area (n), mass (n) as maintained by Mesh::updateNodeGeometries (src/Mesh.cu:182-214) */
 cs_status cs_get_geometry(cs_ctx* ctx, int kind, int id, float* normals, float* area, float* mass);
 
@@ -308,6 +313,12 @@ typedef struct cs_scene_info {
  * CS_E_ARG / CS_E_FORMAT.  The message of the last failure on this thread: cs_scene_error(). */
 cs_status cs_load_scene(const char* config_path, const char* root_dir, const cs_params* base, cs_ctx** ctx_out,
                         cs_scene_info* info);
+/* Simulator::save's configuration file (src/Simulator.cu:950-957, src/Cloth.cu:865-872): the
+ * configuration at config_path with cloth i's "mesh" set to cloth_meshes[i] and its "transform"
+ * removed, written as JSON to out_path (comments are not kept).  A later cs_load_scene of out_path
+ * resumes from those meshes; handle node ids stay valid (this library never remeshes). */
+cs_status cs_write_config(const char* config_path, const char* out_path, const char* const* cloth_meshes,
+                          int n_cloths);
 /* The transformation of motion `motion_id` of a configuration at `time` (host only: parses the
  * file's motions, no meshes and no context). */
 cs_status cs_scene_motion(const char* config_path, const char* root_dir, int motion_id, float time,

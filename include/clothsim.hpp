@@ -6,7 +6,12 @@
 //   Simulator::collisionStep()    src/Simulator.cu:602-684
 //   Simulator::simulateStep()     src/Simulator.cu:838-897 (remeshing and separation disabled)
 //   Simulator::getCloths() / getObstacles(), Cloth::getMesh(), Obstacle::getMesh()
-//   Mesh::save(path)              OBJ with "v" and "nv" lines at precision 20 (src/Mesh.cu:454-470)
+//   Mesh::save(path)              OBJ with "v"/"nv" per node, "vt" per uv vertex and "f a/b" faces at
+//                                 precision 20 (src/Mesh.cu:454-507), readable by cs_load_scene
+//   Simulator::save(directory)    frame<N>_cloth<i>.obj + config.json (src/Simulator.cu:950-957)
+//   Simulator::simulateOffline    src/Simulator.cu:1018-1043 (a save per frame until finished())
+//   Simulator(Resume, directory)  Resume mode: directory/config.json, nFrames = lastFrame(),
+//   Simulator::resumeOffline      nSteps = nFrames * frameSteps (src/Simulator.cu:1045-1083)
 // Errors throw clothsim::Error (the reference prints and exit(1)s instead).
 #pragma once
 #include <cstdio>
@@ -30,6 +35,8 @@ struct Error : std::runtime_error {
 struct Mesh {
     std::vector<float> x, x0, v;  // 3 per node
     std::vector<int> faces;       // 3 node ids per face (mesh-local)
+    std::vector<int> faceVerts;   // 3 uv-vertex ids per face (mesh-local)
+    std::vector<float> uv;        // 2 per uv vertex (material space)
     int nodes() const { return (int)(x.size() / 3); }
     void save(const std::string& path) const {
         std::ofstream out(path);
@@ -39,8 +46,13 @@ struct Mesh {
             out << "v " << x[3 * i] << " " << x[3 * i + 1] << " " << x[3 * i + 2] << "\n";
             out << "nv " << v[3 * i] << " " << v[3 * i + 1] << " " << v[3 * i + 2] << "\n";
         }
-        for (size_t f = 0; f + 2 < faces.size(); f += 3)
-            out << "f " << faces[f] + 1 << " " << faces[f + 1] + 1 << " " << faces[f + 2] + 1 << "\n";
+        for (size_t k = 0; k + 1 < uv.size(); k += 2) out << "vt " << uv[k] << " " << uv[k + 1] << "\n";
+        for (size_t f = 0; f + 2 < faces.size(); f += 3) {
+            out << "f";
+            for (int c = 0; c < 3; c++) out << " " << faces[f + c] + 1 << "/" << faceVerts[f + c] + 1;
+            out << "\n";
+        }
+        if (!out) throw Error(CS_E_ARG, "cannot write " + path);
     }
 };
 
@@ -58,6 +70,13 @@ class MeshHandle {
         m.x0.resize(3 * (size_t)nn);
         m.v.resize(3 * (size_t)nn);
         check(cs_get_state(ctx_, kind_, id_, m.x.data(), m.x0.data(), m.v.data()), "cs_get_state");
+        int nv = 0;
+        check(cs_get_topology(ctx_, kind_, id_, nullptr, nullptr, nullptr, &nv), "cs_get_topology");
+        m.faces.resize(3 * (size_t)nf);
+        m.faceVerts.resize(3 * (size_t)nf);
+        m.uv.resize(2 * (size_t)nv);
+        check(cs_get_topology(ctx_, kind_, id_, m.faces.data(), m.faceVerts.data(), m.uv.data(), &nv),
+              "cs_get_topology");
         return m;
     }
     void setState(const Mesh& m) {
@@ -82,14 +101,24 @@ struct Obstacle : MeshHandle {
     Obstacle(cs_ctx* ctx, int id) : MeshHandle(ctx, CS_MESH_OBSTACLE, id) {}
 };
 
+struct Resume {};  // tag of the resuming constructor
+
 class Simulator {
   public:
     // Loads a reference configuration file; relative mesh / material paths resolve against rootDir.
-    explicit Simulator(const std::string& path, const std::string& rootDir = "", const cs_params* base = nullptr) {
-        cs_status s = cs_load_scene(path.c_str(), rootDir.c_str(), base, &ctx_, &info_);
-        if (s != CS_OK) throw Error(s, std::string("cs_load_scene: ") + cs_scene_error());
-        for (int i = 0; i < info_.n_cloths; i++) cloths_.emplace_back(ctx_, i);
-        for (int i = 0; i < info_.n_obstacles; i++) obstacles_.emplace_back(ctx_, i);
+    explicit Simulator(const std::string& path, const std::string& rootDir = "", const cs_params* base = nullptr)
+        : path_(path), root_(rootDir) {
+        load(base);
+    }
+    // Resume mode: the configuration a save wrote into `directory` (its cloths are the last saved
+    // frame), continued from that frame (src/Simulator.cu:3-16,1045-1053).
+    Simulator(Resume, const std::string& directory, const std::string& rootDir = "", const cs_params* base = nullptr)
+        : path_(directory + "/config.json"), root_(rootDir) {
+        load(base);
+        nFrames_ = lastFrame(directory, info_.n_cloths);
+        nSteps_ = nFrames_ * info_.frame_steps;
+        cs_status s = cs_set_step_count(ctx_, nSteps_);
+        if (s != CS_OK) throw Error(s, std::string("cs_set_step_count: ") + cs_last_error(ctx_));
     }
     ~Simulator() { cs_destroy(ctx_); }
     Simulator(const Simulator&) = delete;
@@ -103,22 +132,50 @@ class Simulator {
         if (nSteps_ % info_.frame_steps == 0) nFrames_++;
         return st;
     }
-    // Steps until the configuration's end_time / end_frame, saving every cloth once per frame
-    // as <directory>/frame<NNNNN>_<cloth>.obj (frame 0 is the initial state).
-    void simulateOffline(const std::string& directory) {
-        saveFrame(directory);
+    // Simulator::finished (src/Simulator.cu:979-981)
+    bool finished() const {
         const float dt = info_.frame_time / info_.frame_steps;
-        while ((nSteps_ + 1) * dt <= info_.end_time + 0.5f * dt && nFrames_ < info_.end_frame) {
-            const int frames = nFrames_;
+        return nSteps_ * dt >= info_.end_time || nFrames_ >= info_.end_frame;
+    }
+    // simulateOffline: save, then steps until finished() with a save after every frame
+    // (src/Simulator.cu:1018-1043; remeshing and separation are not run)
+    void simulateOffline(const std::string& directory) {
+        save(directory);
+        resumeOffline(directory);
+    }
+    // resumeOffline's loop (src/Simulator.cu:1073-1083), for a context made with Resume
+    void resumeOffline(const std::string& directory) {
+        while (!finished()) {
             simulateStep();
-            if (nFrames_ != frames) saveFrame(directory);
+            if (nSteps_ % info_.frame_steps == 0) save(directory);
         }
     }
-    void saveFrame(const std::string& directory) const {
+    // Simulator::save (src/Simulator.cu:950-957, src/Cloth.cu:865-887): every cloth as
+    // <directory>/frame<nFrames>_cloth<i>.obj and <directory>/config.json naming those meshes.
+    void save(const std::string& directory) const {
+        std::vector<std::string> paths;
         for (size_t c = 0; c < cloths_.size(); c++) {
-            char name[64];
-            std::snprintf(name, sizeof(name), "/frame%05d_%02d.obj", nFrames_, (int)c);
-            cloths_[c].getMesh().save(directory + name);
+            paths.push_back(directory + "/frame" + std::to_string(nFrames_) + "_cloth" + std::to_string(c) + ".obj");
+            cloths_[c].getMesh().save(paths.back());
+        }
+        std::vector<const char*> p;
+        for (const std::string& s : paths) p.push_back(s.c_str());
+        const std::string cfg = directory + "/config.json";
+        const std::string tmp = cfg + ".tmp";
+        cs_status s = cs_write_config(path_.c_str(), tmp.c_str(), p.data(), (int)p.size());
+        if (s != CS_OK) throw Error(s, std::string("cs_write_config: ") + cs_scene_error());
+        if (std::rename(tmp.c_str(), cfg.c_str()) != 0) throw Error(CS_E_ARG, "cannot write " + cfg);
+    }
+    // Simulator::lastFrame (src/Simulator.cu:962-977): the last frame whose every cloth file exists
+    static int lastFrame(const std::string& directory, int nCloths) {
+        int f = 0;
+        for (;; f++) {
+            bool all = true;
+            for (int c = 0; c < nCloths && all; c++) {
+                std::ifstream in(directory + "/frame" + std::to_string(f + 1) + "_cloth" + std::to_string(c) + ".obj");
+                all = (bool)in;
+            }
+            if (!all) return f;
         }
     }
 
@@ -130,12 +187,19 @@ class Simulator {
     cs_ctx* handle() { return ctx_; }
 
   private:
+    void load(const cs_params* base) {
+        cs_status s = cs_load_scene(path_.c_str(), root_.c_str(), base, &ctx_, &info_);
+        if (s != CS_OK) throw Error(s, std::string("cs_load_scene: ") + cs_scene_error());
+        for (int i = 0; i < info_.n_cloths; i++) cloths_.emplace_back(ctx_, i);
+        for (int i = 0; i < info_.n_obstacles; i++) obstacles_.emplace_back(ctx_, i);
+    }
     cs_stats run(cs_status (*f)(cs_ctx*, cs_stats*), const char* what) {
         cs_stats st{};
         cs_status s = f(ctx_, &st);
         if (s != CS_OK && s != CS_E_NOT_CONVERGED) throw Error(s, std::string(what) + ": " + cs_last_error(ctx_));
         return st;
     }
+    std::string path_, root_;
     cs_ctx* ctx_ = nullptr;
     cs_scene_info info_{};
     std::vector<Cloth> cloths_;

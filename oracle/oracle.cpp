@@ -194,8 +194,20 @@ int or_set_obstacle_motion(void* h, int obstacle, int motion) {
     });
 }
 
+// Simulator::resume: nSteps, then Obstacle::transform at the new time (src/Simulator.cu:1045-1053)
 int or_set_step_count(void* h, int steps) {
-    return dispatch(h, [&](auto& S) { S.steps = steps; return (int)CS_OK; });
+    return dispatch(h, [&](auto& S) {
+        S.steps = steps;
+        for (int o = S.nCloths; o < (int)S.meshes.size(); o++) {
+            const MeshRange& R = S.meshes[o];
+            if (R.motion < 0) continue;
+            Tf T = S.motion_now(R.motion);
+            for (int i = R.n0; i < R.n0 + R.nn; i++) S.x[i] = S.x0[i] = tf_point(T, S.base[i]);
+            update_face_geometry(S, R);
+            update_node_geometry(S, R);
+        }
+        return (int)CS_OK;
+    });
 }
 
 int or_physics_step(void* h, int direct, cs_stats* st) {

@@ -127,11 +127,16 @@ def test_resumed_step_count_continues_the_motion(tmp_path):
     _, key_path = write_motion_scene(d)
     root = os.path.join(d, "keys")
     sim = Simulator.from_config(key_path, root)
+    base = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], dtype=np.float64)
+    dt = np.float32(0.04) / np.float32(8)
+    sim.state(CS_MESH_OBSTACLE, 0)  # the device state exists before the counter moves
     sim.set_step_count(12)
+    # placed at T(12 dt)(base) with x0 = x, as the reference's resume transforms its obstacles
+    T12 = cs.scene_motion(key_path, 0, np.float32(12) * dt, root).as_array()
+    x, x0, _ = sim.state(CS_MESH_OBSTACLE, 0)
+    assert np.abs(x - oracle.transform_points(T12, base)).max() <= 1e-6 and np.array_equal(x, x0)
     s, _ = sim.physics_step()
     assert s in (CS_OK, CS_E_NOT_CONVERGED)
     assert sim.step_count() == 13
-    base = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], dtype=np.float64)
-    # the obstacle was placed at T(0)(base) and is now carried to T(13 dt)(base) in one step
-    T = cs.scene_motion(key_path, 0, np.float32(13) * (np.float32(0.04) / np.float32(8)), root).as_array()
+    T = cs.scene_motion(key_path, 0, np.float32(13) * dt, root).as_array()
     assert np.abs(sim.state(CS_MESH_OBSTACLE, 0)[0] - oracle.transform_points(T, base)).max() <= 1e-5

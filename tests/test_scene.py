@@ -174,6 +174,26 @@ def build_host_mirror(out_dir):
     return exe
 
 
+def test_write_config_round_trip(tmp_path):
+    """cs_write_config keeps the configuration, points cloth 0 at a new mesh and drops its
+    transform; the written file loads to the same scene."""
+    import json
+    d = str(tmp_path)
+    path = write_scene(d)
+    with open(os.path.join(d, "frame7_cloth0.obj"), "w") as f:
+        f.write(open(os.path.join(d, "sheet.obj")).read())
+    out = os.path.join(d, "saved.json")
+    cs.write_config(path, out, [os.path.join(d, "frame7_cloth0.obj")])
+    j = json.load(open(out))
+    assert j["cloths"][0]["mesh"].endswith("frame7_cloth0.obj") and "transform" not in j["cloths"][0]
+    assert j["frame_steps"] == 8 and j["obstacles"][0]["transform"]["scale"] == 0.5
+    a, b = cs.scene_info(path, d), cs.scene_info(out, d)
+    for f in ("n_cloths", "n_obstacles", "n_handles", "n_cloth_nodes", "n_cloth_faces", "frame_steps"):
+        assert getattr(a, f) == getattr(b, f), f
+    with pytest.raises(ClothError, match="CS_E_ARG"):
+        cs.write_config(path, out, ["a.obj", "b.obj"])
+
+
 def test_cpp_mirror_compiles(tmp_path):
     cs.load_library()
     assert os.path.exists(build_host_mirror(str(tmp_path)))
@@ -189,13 +209,45 @@ def test_cpp_mirror_runs_frames(tmp_path):
     r = subprocess.run([exe, path, str(tmp_path), str(out), "2"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     files = sorted(os.listdir(out))
-    assert files == ["frame00000_00.obj", "frame00001_00.obj", "frame00002_00.obj"]
-    v = [l for l in open(out / "frame00000_00.obj") if l.startswith("v ")]
-    x0 = np.array([[float(t) for t in l.split()[1:]] for l in v])
+    assert files == ["config.json", "frame0_cloth0.obj", "frame1_cloth0.obj", "frame2_cloth0.obj"]
+    x0 = obj_rows(out / "frame0_cloth0.obj", "v")
     assert np.allclose(x0, expected_scene()[0], atol=1e-6)
-    v2 = [l for l in open(out / "frame00002_00.obj") if l.startswith("v ")]
-    x2 = np.array([[float(t) for t in l.split()[1:]] for l in v2])
+    x2 = obj_rows(out / "frame2_cloth0.obj", "v")
     assert np.all(np.isfinite(x2)) and not np.allclose(x2, x0)  # 16 steps of gravity moved it
+    # the reference's OBJ: v / nv per node, vt per material-space vertex, f a/b faces
+    assert len(obj_rows(out / "frame2_cloth0.obj", "nv")) == 6
+    # the sheet has no vt lines: one material-space vertex per polygon corner at scale * (x, y)
+    vt, raw = obj_rows(out / "frame2_cloth0.obj", "vt"), obj_rows(tmp_path / "sheet.obj", "v")
+    assert len(vt) == 8 and all(np.any(np.all(2 * raw[:, :2] == u, axis=1)) for u in vt)
+    assert all("/" in l for l in open(out / "frame2_cloth0.obj") if l.startswith("f "))
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_resume_continues_the_run(tmp_path):
+    """Two frames saved, resumed for a third (Resume mode reads the saved config.json and frame
+    files), against three frames straight through.  Without handles (a resumed run anchors its
+    handles at the loaded positions, as the reference's does) the two agree."""
+    import subprocess
+    exe = build_host_mirror(str(tmp_path))
+    path = write_scene(str(tmp_path), CONFIG.replace('"handles": [{"nodes": [0, 2]}]', '"handles": []'))
+    out, straight = tmp_path / "frames", tmp_path / "straight"
+    out.mkdir()
+    straight.mkdir()
+    r = subprocess.run([exe, path, str(tmp_path), str(out), "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe, "--resume", str(out), str(tmp_path), "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "resumed at frame 2 step 16" in r.stdout
+    r = subprocess.run([exe, path, str(tmp_path), str(straight), "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for tag in ("v", "nv"):
+        a, b = obj_rows(out / "frame3_cloth0.obj", tag), obj_rows(straight / "frame3_cloth0.obj", tag)
+        print("resumed vs straight frame 3 %s: max diff %.3g" % (tag, np.abs(a - b).max()))
+        assert np.abs(a - b).max() <= 1e-6
+
+
+def obj_rows(path, tag):
+    return np.array([[float(t) for t in l.split()[1:]] for l in open(path) if l.split() and l.split()[0] == tag])
 
 
 def test_instances_scene_generator():
