@@ -533,7 +533,31 @@ __global__ void __launch_bounds__(BP) k_bpcg_spmv(const BCloth* __restrict__ cl,
         const size_t off = (size_t)c.sliceOff[sl];
         const G1<int>* cp = c.cols + off * SLICE + lane;
         const G1<float>* vp = c.vals + off * 9 * SLICE + lane;
-        for (int k = 0; k < W; k++) {
+        // Two slots per step: both columns, then both slots' values and gathers, so a wave
+        // has two slots' loads in flight behind one column latency (the batch's matrix
+        // streams from HBM).  The sums still run slot by slot.
+        int k = 0;
+        for (; k + 1 < W; k += 2) {
+            const int j0 = cp[k * SLICE], j1 = cp[(k + 1) * SLICE];
+            const G1<float>* ap = vp + (size_t)k * 9 * SLICE;
+            float a0[9], a1[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) {
+                a0[q] = ap[q * SLICE];
+                a1[q] = ap[(9 + q) * SLICE];
+            }
+            const f4n z0 = c.z[j0], z1 = c.z[j1];
+            f4n e0, e1;
+            if (!FIRST) {
+                e0 = dprev[j0];
+                e1 = dprev[j1];
+            }
+            acc3(s0, s1, s2, a0, v3(z0));
+            if (!FIRST) acc3(t0, t1, t2, a0, v3(e0));
+            acc3(s0, s1, s2, a1, v3(z1));
+            if (!FIRST) acc3(t0, t1, t2, a1, v3(e1));
+        }
+        if (k < W) {
             const int j = cp[k * SLICE];
             const G1<float>* ap = vp + (size_t)k * 9 * SLICE;
             float a[9];
