@@ -230,10 +230,14 @@ def main():
     marker_s = (pair_ms / pair_n) * 1e-3 if pair_n else 0.0
     avg_s = (spmv_ms / spmv_n) * 1e-3 - marker_s if spmv_n else float("nan")
     achieved = bytes_per_launch / avg_s / 1e9 if spmv_n else None
+    from clothsimulator_amd._cabi import CS_FLAG_INSTANCES, CS_PRECOND_BLOCK_JACOBI
+    batched = bool(spec.params.flags & CS_FLAG_INSTANCES) and spec.params.precond == CS_PRECOND_BLOCK_JACOBI \
+        and len(spec.cloths) > 1  # the batched solve of independent instances
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                 "traffic": read_traffic(a.workload, a.precond),
-                "kernel": "k_pcg_spmv", "avg_us": avg_s * 1e6, "event_marker_us": marker_s * 1e6,
+                "kernel": "k_bpcg_spmv" if batched else "k_pcg_spmv",
+                "avg_us": avg_s * 1e6, "event_marker_us": marker_s * 1e6,
                 "launches_sampled": spmv_n, "bytes_per_launch": bytes_per_launch}
     ms = summary(main_run)
     out = {
