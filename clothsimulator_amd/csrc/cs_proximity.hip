@@ -2,8 +2,7 @@
 // src/Simulator.cu:300-349; tests src/PhysicsHelper.cu:3-150; records src/Proximity.cu).
 //
 // Candidate face pairs come from the LBVH at 2 x repulsion thickness.  One
-// thread per (pair, test) runs one of the 15 tests (6 vertex-face, 9
-// edge-edge).  The nearest candidate per (primitive, side) is kept with a
+// thread per pair runs its 15 tests (6 vertex-face, 9 edge-edge).  The nearest candidate per (primitive, side) is kept with a
 // 64-bit atomicMin on (distance bits << 32 | other id): distances are >= 0, so
 // the unsigned order of the packed word is the reference's
 // Pair<float, X*>::operator< with pointer order = global id order, and the
@@ -46,13 +45,13 @@ struct ProxArgs {
     uint64_t *bestNode, *bestEdge, *bestFace;
 };
 
-// checkVertexFaceProximityGpu (src/PhysicsHelper.cu:20-49)
-__device__ void vf_test(const ProxArgs& A, int node, int f) {
-    int4 fn = A.fnodes[f];
+// checkVertexFaceProximityGpu (src/PhysicsHelper.cu:20-49): node (at xn) against face f
+// (nodes fn at xf)
+__device__ __forceinline__ void vf_test(const ProxArgs& A, int node, V3f xn, int f, int4 fn, const V3f (&xf)[3]) {
     if (node == fn.x || node == fn.y || node == fn.z) return;
     V3f n;
     float w[4], h;
-    if (!signed_vf(ld3(A.x[node]), ld3(A.x[fn.x]), ld3(A.x[fn.y]), ld3(A.x[fn.z]), n, w, h)) return;
+    if (!signed_vf(xn, xf[0], xf[1], xf[2], n, w, h)) return;
     const float d = fabsf(h);
     if (!(tmin(-w[1], tmin(-w[2], -w[3])) >= 1e-6f)) return;
     if (node < A.NC) {
@@ -65,13 +64,14 @@ __device__ void vf_test(const ProxArgs& A, int node, int f) {
     }
 }
 
-// checkEdgeEdgeProximityGpu (src/PhysicsHelper.cu:51-81)
-__device__ void ee_test(const ProxArgs& A, int e0, int e1) {
-    int2 a = A.enodes[e0], b = A.enodes[e1];
+// checkEdgeEdgeProximityGpu (src/PhysicsHelper.cu:51-81): edge e0 (nodes a at xa) against
+// edge e1 (nodes b at xb)
+__device__ __forceinline__ void ee_test(const ProxArgs& A, int e0, int2 a, V3f xa0, V3f xa1, int e1, int2 b, V3f xb0,
+                                        V3f xb1) {
     if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) return;
     V3f n;
     float w[4], h;
-    if (!signed_ee(ld3(A.x[a.x]), ld3(A.x[a.y]), ld3(A.x[b.x]), ld3(A.x[b.y]), n, w, h)) return;
+    if (!signed_ee(xa0, xa1, xb0, xb1, n, w, h)) return;
     const float d = fabsf(h);
     if (!(tmin(tmin(w[0], w[1]), tmin(-w[2], -w[3])) >= 1e-6f)) return;
     if (!in_edge(w[1], a, b, A.eadj[e1], A.x, A.fnrm)) return;
@@ -86,24 +86,41 @@ __device__ void ee_test(const ProxArgs& A, int e0, int e1) {
     }
 }
 
-// checkProximitiesGpu (src/PhysicsHelper.cu:88-111), one thread per (pair, test)
+// position of node id among a face's three nodes (every edge node of the face is one of them)
+__device__ __forceinline__ V3f pick(int id, int4 fn, const V3f (&xf)[3]) {
+    return id == fn.x ? xf[0] : (id == fn.y ? xf[1] : xf[2]);
+}
+
+// checkProximitiesGpu (src/PhysicsHelper.cu:88-111), one thread per face pair: the two faces'
+// six node positions are loaded once for all 15 tests (the 3 + 3 vertex-face tests, then the
+// 3 x 3 edge-edge tests).  The atomicMin results do not depend on which thread runs a test.
 __global__ void __launch_bounds__(256) k_prox_tests(ProxArgs A) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 15 * A.nPairs) return;
-    const int k = (int)(t / A.nPairs);  // test-major: one kind of test per wavefront
-    const long p = t - (long)k * A.nPairs;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.nPairs) return;
     const int2 pr = A.pairs[p];
-    if (k < 3) {
-        int4 fn = A.fnodes[pr.x];
-        vf_test(A, k == 0 ? fn.x : (k == 1 ? fn.y : fn.z), pr.y);
-    } else if (k < 6) {
-        int4 fn = A.fnodes[pr.y];
-        vf_test(A, k == 3 ? fn.x : (k == 4 ? fn.y : fn.z), pr.x);
-    } else {
-        const int j = (k - 6) / 3, l = (k - 6) % 3;
-        int4 e0 = A.fedges[pr.x], e1 = A.fedges[pr.y];
-        ee_test(A, j == 0 ? e0.x : (j == 1 ? e0.y : e0.z), l == 0 ? e1.x : (l == 1 ? e1.y : e1.z));
+    const int4 fa = A.fnodes[pr.x], fb = A.fnodes[pr.y];
+    const V3f xa[3] = {ld3(A.x[fa.x]), ld3(A.x[fa.y]), ld3(A.x[fa.z])};
+    const V3f xb[3] = {ld3(A.x[fb.x]), ld3(A.x[fb.y]), ld3(A.x[fb.z])};
+    vf_test(A, fa.x, xa[0], pr.y, fb, xb);
+    vf_test(A, fa.y, xa[1], pr.y, fb, xb);
+    vf_test(A, fa.z, xa[2], pr.y, fb, xb);
+    vf_test(A, fb.x, xb[0], pr.x, fa, xa);
+    vf_test(A, fb.y, xb[1], pr.x, fa, xa);
+    vf_test(A, fb.z, xb[2], pr.x, fa, xa);
+    const int4 ea = A.fedges[pr.x], eb = A.fedges[pr.y];
+    const int ia[3] = {ea.x, ea.y, ea.z}, ib[3] = {eb.x, eb.y, eb.z};
+    int2 na[3], nb[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        na[j] = A.enodes[ia[j]];
+        nb[j] = A.enodes[ib[j]];
     }
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+            ee_test(A, ia[j], na[j], pick(na[j].x, fa, xa), pick(na[j].y, fa, xa), ib[l], nb[l],
+                    pick(nb[l].x, fb, xb), pick(nb[l].y, fb, xb));
 }
 
 __global__ void k_prox_flags(long nTotal, long nNodeSlots, long nEdgeSlots, const uint64_t* __restrict__ bestNode,
@@ -202,8 +219,7 @@ int find_proximities(cs_ctx* C) {
         A.enodes = C->enodes.ptr<int2>(); A.eadj = C->eadj.ptr<int2>();
         A.bestNode = C->bestNode.ptr<uint64_t>(); A.bestEdge = C->bestEdge.ptr<uint64_t>();
         A.bestFace = C->bestFace.ptr<uint64_t>();
-        long nt = 15 * nPairs;
-        hipLaunchKernelGGL(k_prox_tests, dim3(cdiv(nt, 256)), dim3(256), 0, s, A);
+        hipLaunchKernelGGL(k_prox_tests, dim3(cdiv(nPairs, 256)), dim3(256), 0, s, A);
         CS_LAUNCH_CHECK();
     }
     const long nNode = 2L * C->N, nEdge = 2L * C->E, nFace = 2L * C->F;
