@@ -455,7 +455,11 @@ struct SmallCycle {
     int ns;  // levels in the sub-cycle; the last is the coarsest
     int m;   // 3 n of the coarsest
     const float* inv;
-    const float* b0;  // right-hand side of the first level
+    // the first level's right-hand side is the restriction of the finer level's residual:
+    // bc_I = sum of rt over its members rmidx[rmptr[I] .. rmptr[I + 1]) (ascending), nFine nodes
+    int nFine;
+    const int *rmptr, *rmidx;
+    const float* rt;
     float* out0;      // its result (the coarsest's solution when it is the only level)
     SmallLevel L[MAXSMALL];
 };
@@ -505,6 +509,9 @@ __global__ void __launch_bounds__(SB) k_amg_small_cycle(SmallCycle Q) {
     // come back from HBM in one round of independent loads instead of one round per phase.
     {
         int acc = touch(Q.inv, sizeof(float) * (size_t)Q.m * Q.m, tid);
+        acc ^= touch(Q.rmptr, sizeof(int) * (size_t)(Q.L[0].n + 1), tid);
+        acc ^= touch(Q.rmidx, sizeof(int) * (size_t)Q.nFine, tid);
+        acc ^= touch(Q.rt, sizeof(float) * 3 * (size_t)Q.nFine, tid);
         for (int l = 0; l + 1 < Q.ns; l++) {
             const SmallLevel& L = Q.L[l];
             acc ^= touch(L.rp, sizeof(int) * (size_t)(L.n + 1), tid);
@@ -517,7 +524,23 @@ __global__ void __launch_bounds__(SB) k_amg_small_cycle(SmallCycle Q) {
         }
         asm volatile("" ::"v"(acc));
     }
-    for (int e = tid; e < 3 * Q.L[0].n; e += SB) vb[3 * Q.L[0].off + e] = Q.b0[e];
+    // restriction into the first level (as k_amg_restrict: members in order)
+    for (int I = tid; I < Q.L[0].n; I += SB) {
+        V3f sum = mk(0.0f, 0.0f, 0.0f);
+        const int k0 = Q.rmptr[I], k1 = Q.rmptr[I + 1];
+        for (int k = k0; k < k1; k += 8) {
+            int j[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) j[u] = Q.rmidx[min(k + u, k1 - 1)];
+            V3f m[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) m[u] = ld3v(Q.rt, j[u]);
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (k + u < k1) sum = sum + m[u];
+        }
+        st3s(vb + 3 * Q.L[0].off, I, sum);
+    }
     __syncthreads();
     for (int l = 0; l + 1 < Q.ns; l++) {
         const SmallLevel& L = Q.L[l];
@@ -934,14 +957,18 @@ void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int n
                                L.val.ptr<float>(), Dinv, b, L.x.ptr<float>(), L.t.ptr<float>());
         }
         AmgLevel& N = G.L[l + 1];
-        hipLaunchKernelGGL(k_amg_restrict, dim3(cdiv(N.n, AB)), dim3(AB), 0, s, N.n, L.mptr.ptr<int>(),
-                           L.midx.ptr<int>(), L.t.ptr<float>(), N.b.ptr<float>());
+        if (l + 1 < l0)  // the small-level cycle restricts into level l0 itself
+            hipLaunchKernelGGL(k_amg_restrict, dim3(cdiv(N.n, AB)), dim3(AB), 0, s, N.n, L.mptr.ptr<int>(),
+                               L.midx.ptr<int>(), L.t.ptr<float>(), N.b.ptr<float>());
     }
     SmallCycle Q;
     Q.ns = G.nl - l0;
     Q.m = 3 * G.L[G.nl - 1].n;
     Q.inv = G.inv.ptr<float>();
-    Q.b0 = G.L[l0].b.ptr<float>();
+    Q.nFine = G.L[l0 - 1].n;
+    Q.rmptr = G.L[l0 - 1].mptr.ptr<int>();
+    Q.rmidx = G.L[l0 - 1].midx.ptr<int>();
+    Q.rt = G.L[l0 - 1].t.ptr<float>();
     Q.out0 = l0 == G.nl - 1 ? G.L[l0].x.ptr<float>() : G.L[l0].t.ptr<float>();
     int off = 0;
     for (int k = 0; k < Q.ns; k++) {
