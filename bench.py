@@ -199,7 +199,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the cloth step has no "
-              "multi-GPU path here; see DESIGN.md for how it would shard)" % a.gpus, file=sys.stderr)
+              "multi-GPU path here: it runs on one GPU's share of the work)" % a.gpus, file=sys.stderr)
         sys.exit(2)
 
     if a.copy_probe:
