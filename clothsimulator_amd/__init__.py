@@ -26,7 +26,7 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_lpc_weight_count", "cs_set_learned_weights", "cs_get_learned_weights", "cs_lpc_factor",
             "cs_lpc_train_step", "cs_lpc_smoother_omega", "cs_batch_tuning", "cs_add_motion", "cs_motion_eval",
             "cs_set_obstacle_motion", "cs_add_moving_handles", "cs_set_step_count", "cs_get_step_count",
-            "cs_scene_motion", "cs_get_topology", "cs_write_config"]
+            "cs_scene_motion", "cs_get_topology", "cs_write_config", "cs_impact_response"]
 
 # trained weights of the learned preconditioner shipped with the package (float32, .npy)
 LPC_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lpc_weights.npy")
@@ -73,6 +73,7 @@ def load_library(path=LIB_PATH):
     L.cs_find_impacts.argtypes = [vp, C.c_float, C.POINTER(ImpactRec), C.c_int, C.POINTER(C.c_int)]
     L.cs_independent_impacts.argtypes = [vp, C.POINTER(ImpactRec), C.c_int, C.c_int, C.POINTER(ImpactRec),
                                          C.POINTER(C.c_int)]
+    L.cs_impact_response.argtypes = [vp, C.POINTER(ImpactRec), C.c_int, C.c_int, C.c_float, C.POINTER(C.c_int)]
     L.cs_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]
     L.cs_load_scene.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Params), C.POINTER(vp), C.POINTER(SceneInfo)]
     L.cs_scene_error.restype = C.c_char_p
@@ -331,11 +332,15 @@ class Simulator:
         self._check(self.L.cs_get_geometry(self.h, kind, idx, fptr(n), fptr(a), fptr(m)), "cs_get_geometry")
         return n, a, m
 
-    def lbvh_pairs(self, ccd, thickness):
+    def lbvh_pairs(self, ccd, thickness, refit=False):
+        """Candidate face pairs (cs_lbvh_pairs).  refit: refit the trees of the previous call (same
+        ccd) to the current positions instead of building new ones (both calls below refit: a second
+        refit on unchanged positions gives the same boxes)."""
         n = C.c_int()
-        self._check(self.L.cs_lbvh_pairs(self.h, int(ccd), thickness, None, 0, C.byref(n)), "cs_lbvh_pairs")
+        mode = int(bool(ccd)) | (2 if refit else 0)
+        self._check(self.L.cs_lbvh_pairs(self.h, mode, thickness, None, 0, C.byref(n)), "cs_lbvh_pairs")
         pairs = np.zeros((max(n.value, 1), 2), dtype=np.int32)
-        self._check(self.L.cs_lbvh_pairs(self.h, int(ccd), thickness, iptr(pairs), n.value, C.byref(n)),
+        self._check(self.L.cs_lbvh_pairs(self.h, mode, thickness, iptr(pairs), n.value, C.byref(n)),
                     "cs_lbvh_pairs")
         return pairs[:n.value]
 
@@ -434,6 +439,15 @@ class Simulator:
         cnt = C.c_int()
         self._check(self.L.cs_kernel_time(self.h, kernel_id, C.byref(tot), C.byref(cnt), int(reset)), "cs_kernel_time")
         return tot.value, cnt.value
+
+    def impact_response(self, imps, deform, obstacle_mass=1e3):
+        """The impact response (cs_impact_response) on the given impacts from the current state;
+        returns its iteration count."""
+        a = np.ascontiguousarray(imps, dtype=IMPACT_DTYPE)
+        it = C.c_int()
+        self._check(self.L.cs_impact_response(self.h, a.ctypes.data_as(C.POINTER(ImpactRec)), len(a), deform,
+                                              obstacle_mass, C.byref(it)), "cs_impact_response")
+        return it.value
 
     def independent_impacts(self, imps, deform):
         a = np.ascontiguousarray(imps, dtype=IMPACT_DTYPE)

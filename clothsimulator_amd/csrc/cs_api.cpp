@@ -109,8 +109,13 @@ int physics_step(cs_ctx* C, cs_stats* st) {
     StageTimer T;
     cs_stats local{};
     motion_step(C);
-    if (!(C->P.flags & CS_FLAG_NO_PROXIMITY)) find_proximities(C);
-    else C->nProx = 0;
+    if (!(C->P.flags & CS_FLAG_NO_PROXIMITY)) {
+        bvh_build_all(C, false);
+        local.ms_bvh = T.lap(s);
+        find_proximities(C, false);
+    } else {
+        C->nProx = 0;
+    }
     local.n_proximities = C->nProx;
     local.ms_proximity = T.lap(s);
     int status = CS_OK;
@@ -156,6 +161,7 @@ int physics_step(cs_ctx* C, cs_stats* st) {
         st->pcg_converged = local.pcg_converged;
         st->n_proximities = local.n_proximities;
         st->n_contact_blocks = local.n_contact_blocks;
+        st->ms_bvh = local.ms_bvh;
         st->ms_proximity = local.ms_proximity;
         st->ms_assemble = local.ms_assemble;
         st->ms_solve = local.ms_solve;
@@ -169,6 +175,7 @@ int collision_step(cs_ctx* C, cs_stats* st) {
     hipStream_t s = C->stream;
     StageTimer T;
     bvh_build_all(C, true);
+    const float msBvh = T.lap(s);
     const float th = C->P.collision_thickness;
     float obstacleMass = 1e3f;
     bool success = false;
@@ -197,6 +204,7 @@ int collision_step(cs_ctx* C, cs_stats* st) {
         st->collision_iters = iters;
         st->collision_deform = deform;
         st->n_impacts = C->nImpAcc;
+        st->ms_bvh += msBvh;
     }
     if (!success) {
         C->err = "Collision step failed: impacts remain after max_collision_iters in both deform passes";
@@ -424,18 +432,21 @@ cs_status cs_get_step_count(const cs_ctx* C, int* steps) {
 
 cs_status cs_physics_step(cs_ctx* C, cs_stats* st) {
     if (!C) return CS_E_ARG;
+    if (st) *st = cs_stats{};
     if (C->nCloths == 0) return CS_E_STATE;
     return guard(C, [&]() { return physics_step(C, st); });
 }
 
 cs_status cs_collision_step(cs_ctx* C, cs_stats* st) {
     if (!C) return CS_E_ARG;
+    if (st) *st = cs_stats{};
     if (C->nCloths == 0) return CS_E_STATE;
     return guard(C, [&]() { return collision_step(C, st); });
 }
 
 cs_status cs_step(cs_ctx* C, cs_stats* st) {
     if (!C) return CS_E_ARG;
+    if (st) *st = cs_stats{};
     if (C->nCloths == 0) return CS_E_STATE;
     return guard(C, [&]() { return step(C, st); });
 }
@@ -460,6 +471,7 @@ cs_status cs_run(cs_ctx* C, int steps, cs_stats* acc) {
             sum.collision_iters += one.collision_iters;
             sum.collision_deform = std::max(sum.collision_deform, one.collision_deform);
             sum.n_impacts += one.n_impacts;
+            sum.ms_bvh += one.ms_bvh;
             sum.ms_proximity += one.ms_proximity;
             sum.ms_assemble += one.ms_assemble;
             sum.ms_solve += one.ms_solve;
@@ -562,12 +574,23 @@ cs_status cs_material_table(const cs_material_in* material, float* table, float*
     return CS_OK;
 }
 
-cs_status cs_lbvh_pairs(cs_ctx* C, int ccd, float thickness, int* pairs, int cap, int* n_out) {
-    if (!C || !n_out || cap < 0) return CS_E_ARG;
+cs_status cs_lbvh_pairs(cs_ctx* C, int mode, float thickness, int* pairs, int cap, int* n_out) {
+    if (!C || !n_out || cap < 0 || (mode & ~(CS_LBVH_CCD | CS_LBVH_REFIT))) return CS_E_ARG;
     if (C->nCloths == 0) return CS_E_STATE;
+    const bool ccd = (mode & CS_LBVH_CCD) != 0;
+    if (mode & CS_LBVH_REFIT)
+        for (size_t m = 0; m < C->meshes.size(); m++)
+            if (!C->finalized || C->bvh.size() != C->meshes.size() || C->bvh[m].mesh != (int)m ||
+                C->bvh[m].nf != C->meshes[m].nf || C->bvh[m].ccd != ccd) {
+                C->err = "cs_lbvh_pairs: CS_LBVH_REFIT needs trees built by a previous call in the same mode";
+                return CS_E_STATE;
+            }
     return guard(C, [&]() {
         host_finalize(C);
-        bvh_build_all(C, ccd != 0);
+        if (mode & CS_LBVH_REFIT)
+            bvh_refit_range(C, 0, (int)C->meshes.size());
+        else
+            bvh_build_all(C, ccd);
         long n = bvh_pairs_ordered(C, thickness, pairs, pairs ? cap : 0);
         *n_out = (int)n;
         return (pairs && n > cap) ? (int)CS_E_CAPACITY : (int)CS_OK;
@@ -836,6 +859,29 @@ cs_status cs_independent_impacts(cs_ctx* C, const cs_impact_rec* in, int n, int 
         CS_HIP(hipMemcpyAsync(out, dout, sizeof(cs_impact_rec) * m, hipMemcpyDeviceToHost, s));
         CS_HIP(hipStreamSynchronize(s));
         *n_out = m;
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_impact_response(cs_ctx* C, const cs_impact_rec* in, int n, int deform, float obstacle_mass, int* iters) {
+    if (!C || n < 0 || (n > 0 && !in) || (deform != 0 && deform != 1) || !(obstacle_mass > 0.0f)) return CS_E_ARG;
+    return guard(C, [&]() {
+        host_finalize(C);
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < 4; k++)
+                if (in[i].nodes[k] < 0 || in[i].nodes[k] >= C->N) return (int)CS_E_ARG;
+        if (iters) *iters = 0;
+        if (n == 0) return (int)CS_OK;
+        hipStream_t s = C->stream;
+        cs_impact_rec* d = C->impAcc.as<cs_impact_rec>(n);
+        CS_HIP(hipMemcpyAsync(d, in, sizeof(cs_impact_rec) * n, hipMemcpyHostToDevice, s));
+        fill_u32(C->optScal.as<uint32_t>(4), 0u, 4, s);
+        collision_response(C, n, deform, obstacle_mass);
+        int it = 0;
+        CS_HIP(hipMemcpyAsync(C->hCount, C->optScal.ptr<uint32_t>(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipStreamSynchronize(s));
+        it = (int)C->hCount[0];
+        if (iters) *iters = it;
         return (int)CS_OK;
     });
 }

@@ -80,7 +80,7 @@ long bvh_all_pairs(cs_ctx* C, float thickness);
 long bvh_pairs_ordered(cs_ctx* C, float thickness, int* host_out, long cap);
 
 // proximities (cs_proximity.hip)
-int find_proximities(cs_ctx* C);
+int find_proximities(cs_ctx* C, bool build_trees = true);  // build_trees false: the stage's trees are built
 
 // CCD and impact response (cs_ccd.hip, cs_response.hip)
 int find_impacts(cs_ctx* C, float thickness);  // sorted, unique impacts in C->impSorted (cs_impact_rec)

@@ -201,10 +201,10 @@ __global__ void k_prox_records(long nTotal, long nNodeSlots, long nEdgeSlots, co
 
 }  // namespace
 
-int find_proximities(cs_ctx* C) {
+int find_proximities(cs_ctx* C, bool build_trees) {
     hipStream_t s = C->stream;
     const float rep = C->P.repulsion_thickness;
-    bvh_build_all(C, false);
+    if (build_trees) bvh_build_all(C, false);
     long nPairs = bvh_all_pairs(C, 2.0f * rep);
     fill_u64(C->bestNode.ptr<uint64_t>(), ~0ull, 2L * C->N, s);
     fill_u64(C->bestEdge.ptr<uint64_t>(), ~0ull, 2L * C->E, s);

@@ -190,6 +190,28 @@ def crumple(n=513, steps=300, seed=0, height=0.5, sigma=1e-3):
                      [ground_plane()], steps)
 
 
+def fold(n=65, gap=1.5e-3, steps=50, seed=0, noise=2e-4):
+    """An n x n sheet (1 m, uv = the flat grid) folded over itself: rows j < n // 2 lie flat at
+    z = 0; rows j >= n // 2 are laid back over them, reversed, offset by half a cell in x and y,
+    at height `gap` (+ U[-noise, noise]), joined to the lower half by one strip of slanted faces.
+    Every upper node starts within `gap` of a lower face: a self-contact state from the first
+    step, over a ground plane 5 cm below."""
+    x, f = grid_mesh(n, 1.0)
+    uv = x[:, :2].copy()
+    h = 1.0 / (n - 1)
+    m = n // 2
+    rng = np.random.default_rng(seed)
+    X = x.copy()
+    for j in range(m, n):
+        row = slice(j * n, (j + 1) * n)
+        X[row, 0] = x[row, 0] + 0.5 * h
+        X[row, 1] = (n - 1 - j) * h + 0.5 * h
+        X[row, 2] = gap + rng.uniform(-noise, noise, n)
+    mesh = MeshArrays(X, f, uv=uv, face_verts=f)
+    return SceneSpec("fold%d" % n, default_params(dt=0.002), [ClothSpec(mesh, material_struct())],
+                     [ground_plane(z=-0.05)], steps)
+
+
 def instances(count=256, seed=0, side_lo=17, side_hi=129, steps=200, spacing=3.0, gap=None):
     """Independent drapes in one context (CS_FLAG_INSTANCES): instance k is a 1 m square
     grid with side drawn uniformly from {side_lo..side_hi} (uniform over the side, not the node

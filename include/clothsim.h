@@ -132,6 +132,9 @@ typedef struct cs_stats {
     int collision_iters;      /* impact-response iterations of the last collision step */
     int collision_deform;     /* 0 or 1: the deform pass that succeeded */
     int n_impacts;            /* impacts resolved in the last collision step */
+    /* host wall time (ms) of the stages: tree builds of both stages (refits inside the collision
+     * loop count to ms_collision), proximity search, assembly, solve, position/geometry update,
+     * collision loop.  Every step entry point zeroes the whole struct first. */
     float ms_bvh, ms_proximity, ms_assemble, ms_solve, ms_update, ms_collision;
 } cs_stats;
 
@@ -230,8 +233,12 @@ cs_status cs_get_geometry(cs_ctx* ctx, int kind, int id, float* normals, float* 
 /* Material::Material stretching table (30^3 x 4) and bending table (3x5) (src/Material.cu:3-85) */
 cs_status cs_material_table(const cs_material_in* material, float* table, float* density, float* bending);
 /* BVH(mesh, ccd) + Simulator::traverse (src/BVH.cu:3-51, src/Simulator.cu:179-196): candidate face pairs
- * (global face ids, first = query face) over cloth self, cloth x earlier cloth, cloth x obstacle. */
-cs_status cs_lbvh_pairs(cs_ctx* ctx, int ccd, float thickness, int* pairs, int cap, int* n_out);
+ * (global face ids, first = query face) over cloth self, cloth x earlier cloth, cloth x obstacle.
+ * mode: CS_LBVH_CCD builds the trees over the swept boxes of x0 -> x; with CS_LBVH_REFIT the trees of
+ * the previous call (same CS_LBVH_CCD bit) are refit to the current positions instead of rebuilt
+ * (BVH::update, src/BVH.cu:167-177; CS_E_STATE when there are none). */
+enum { CS_LBVH_CCD = 1, CS_LBVH_REFIT = 2 };
+cs_status cs_lbvh_pairs(cs_ctx* ctx, int mode, float thickness, int* pairs, int cap, int* n_out);
 /* Simulator::findProximitiesGpu (src/Simulator.cu:300-349) */
 cs_status cs_find_proximities(cs_ctx* ctx, cs_proximity_rec* out, int cap, int* n_out);
 /* Assembled implicit-Euler system of one cloth after proximity detection (src/Cloth.cu:695-764):
@@ -279,6 +286,15 @@ cs_status cs_find_impacts(cs_ctx* ctx, float thickness, cs_impact_rec* out, int 
  * (sorted by t, as cs_find_impacts returns them); writes the selected subset in order. */
 cs_status cs_independent_impacts(cs_ctx* ctx, const cs_impact_rec* in, int n, int deform,
                                  cs_impact_rec* out, int* n_out);
+
+/* CollisionOptimization(impacts, collision_thickness, deform, obstacleMass) + Optimization::solve
+ * (src/optimization/CollisionOptimization.cu:37-145, src/optimization/Optimization.cu:137-181) on
+ * caller-provided impacts: moves the impacts' nodes (free nodes only when deform == 0) from their
+ * current positions x, minimising 1/2 invMass sum m |x - x1|^2 (x1: the positions of the last
+ * geometry update) subject to sum_j w_j n . x_j >= collision_thickness per impact.  iters (may be
+ * NULL): the solve's gradient iterations. */
+cs_status cs_impact_response(cs_ctx* ctx, const cs_impact_rec* impacts, int n, int deform, float obstacle_mass,
+                             int* iters);
 
 /* ---- instrumentation (no counterpart in the reference, which only prints wall-clock stage
  * times, src/Simulator.cu:845-896) ----

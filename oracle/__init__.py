@@ -60,6 +60,8 @@ def lib():
         L.or_find_impacts.argtypes = [vp, C.c_float, C.POINTER(ImpactRec), C.c_int, C.POINTER(C.c_int)]
         L.or_independent_impacts.argtypes = [vp, C.POINTER(ImpactRec), C.c_int, C.c_int, C.POINTER(ImpactRec),
                                              C.POINTER(C.c_int)]
+        L.or_collision_optimize.argtypes = [vp, C.POINTER(ImpactRec), C.c_int, C.c_float, C.c_int, C.c_float,
+                                            C.POINTER(C.c_int)]
         L.or_solve_cubic.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.or_cubic_roots_unit.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.or_amg.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
@@ -364,6 +366,16 @@ class OracleScene:
         _check(self.L.or_find_impacts(self.h, thickness, out.ctypes.data_as(C.POINTER(ImpactRec)), cap, C.byref(n)),
                "find_impacts")
         return out[:n.value].copy()
+
+    def impact_response(self, imps, deform, obstacle_mass=1e3, thickness=None):
+        """The augmented-Lagrangian impact response on the given impacts from the current state
+        (x: start, x1: anchor of the objective); returns its iteration count."""
+        a = np.ascontiguousarray(imps, dtype=IMPACT_DTYPE)
+        it = C.c_int()
+        th = self.params.collision_thickness if thickness is None else thickness
+        _check(self.L.or_collision_optimize(self.h, a.ctypes.data_as(C.POINTER(ImpactRec)), len(a), th, deform,
+                                            obstacle_mass, C.byref(it)), "collision_optimize")
+        return it.value
 
     def independent_impacts(self, imps, deform):
         a = np.ascontiguousarray(imps, dtype=IMPACT_DTYPE)

@@ -511,6 +511,29 @@ int or_independent_impacts(void* h, const cs_impact_rec* in, int n, int deform, 
     });
 }
 
+// CollisionOptimization(impacts, thickness, deform, obstacleMass) + Optimization::solve on the
+// scene's current x (start) and x1 (objective anchor): src/optimization/CollisionOptimization.cu:37-145,
+// src/optimization/Optimization.cu:98-145.
+int or_collision_optimize(void* h, const cs_impact_rec* in, int n, float thickness, int deform, float obstacleMass,
+                          int* iters) {
+    return dispatch(h, [&](auto& S) {
+        using T = std::decay_t<decltype(S.x[0][0])>;
+        vector<Impact> v(n);
+        for (int i = 0; i < n; i++) {
+            v[i].type = in[i].type; v[i].t = in[i].t;
+            for (int k = 0; k < 4; k++) {
+                if (in[i].nodes[k] < 0 || in[i].nodes[k] >= (int)S.x.size()) return (int)CS_E_ARG;
+                v[i].nodes[k] = in[i].nodes[k]; v[i].w[k] = in[i].w[k];
+            }
+            for (int c = 0; c < 3; c++) v[i].n[c] = in[i].n[c];
+            v[i].key = 0;
+        }
+        int it = n > 0 ? collision_optimize(S, v, (T)thickness, deform, (T)obstacleMass) : 0;
+        if (iters) *iters = it;
+        return (int)CS_OK;
+    });
+}
+
 // The reference's whole-line cubic solver restated in FP64 (src/CollisionHelper.cu:3-60).
 int or_solve_cubic(double a, double b, double c, double d, double* roots) { return solve_cubic(a, b, c, d, roots); }
 
