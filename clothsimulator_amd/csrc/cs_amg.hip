@@ -86,23 +86,6 @@ template <class V> void upload_vec(DBuf& d, const std::vector<V>& h, hipStream_t
 
 // ---------------------------------------------------------------- numeric setup kernels
 
-__device__ __forceinline__ void inv3f(const float a[9], float o[9]) {
-    float c00 = a[4] * a[8] - a[5] * a[7];
-    float c01 = a[5] * a[6] - a[3] * a[8];
-    float c02 = a[3] * a[7] - a[4] * a[6];
-    float det = a[0] * c00 + a[1] * c01 + a[2] * c02;
-    float id = 1.0f / det;
-    o[0] = c00 * id;
-    o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
-    o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
-    o[3] = c01 * id;
-    o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
-    o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
-    o[6] = c02 * id;
-    o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
-    o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
-}
-
 // coarse block k = sum of its contributors; FROM_ELL: contributor ids are value
 // offsets in the level-0 sliced ELL (component c at id + 64 c), else block ids of
 // the finer block CSR (component c at 9 id + c)
@@ -157,7 +140,7 @@ __global__ void __launch_bounds__(AB) k_amg_dinv(int n, const int* __restrict__ 
     float a[9], o[9];
 #pragma unroll
     for (int c = 0; c < 9; c++) a[c] = val[(size_t)diag[i] * 9 + c];
-    inv3f(a, o);
+    inv3(a, o);
 #pragma unroll
     for (int c = 0; c < 9; c++) Dinv[(size_t)i * 9 + c] = o[c];
 }

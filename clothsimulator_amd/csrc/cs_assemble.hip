@@ -515,24 +515,6 @@ __global__ void k_contact_rowptr(int nn, int nu, const uint64_t* __restrict__ uk
     rowPtr[r] = lo;
 }
 
-// 3x3 block inverse by the adjugate
-__device__ __forceinline__ void inv3(const float a[9], float o[9]) {
-    float c00 = a[4] * a[8] - a[5] * a[7];
-    float c01 = a[5] * a[6] - a[3] * a[8];
-    float c02 = a[3] * a[7] - a[4] * a[6];
-    float det = a[0] * c00 + a[1] * c01 + a[2] * c02;
-    float id = 1.0f / det;
-    o[0] = c00 * id;
-    o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
-    o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
-    o[3] = c01 * id;
-    o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
-    o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
-    o[6] = c02 * id;
-    o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
-    o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
-}
-
 __device__ __forceinline__ void block_jacobi_row(int row, const int* __restrict__ sliceOff,
                                                  const int* __restrict__ diagSlot, const float* __restrict__ vals,
                                                  const float* __restrict__ cblk, float* __restrict__ Minv) {

@@ -119,4 +119,22 @@ CS_HD bool box_overlap(const Box& a, const Box& b, float th) {
     return true;
 }
 
+// 3x3 inverse by the adjugate (row-major blocks)
+__device__ __forceinline__ void inv3(const float a[9], float o[9]) {
+    float c00 = a[4] * a[8] - a[5] * a[7];
+    float c01 = a[5] * a[6] - a[3] * a[8];
+    float c02 = a[3] * a[7] - a[4] * a[6];
+    float det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    float id = 1.0f / det;
+    o[0] = c00 * id;
+    o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    o[3] = c01 * id;
+    o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    o[6] = c02 * id;
+    o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+
 }  // namespace cs

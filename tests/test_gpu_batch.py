@@ -103,3 +103,55 @@ def test_batched_solve_autotuner_settles():
         s, st = fixed.step()
         assert s == 0, s
     assert fixed.batch_tuning() == (256, 16, 0)
+
+
+def test_batch32_up_to_side129_matches_separate_runs_and_oracle():
+    """32 instances with sides drawn from 17..129 (most above 4,096 faces): the batch builds every
+    tree with the one-workgroup kernel, each separate run builds its large tree with the
+    per-mesh kernels.  The smallest and the largest instance, resynchronised with the oracle
+    before every step, are within 1e-5 m of the oracle's PCG step (1e-7 relative) after each of
+    5 steps; then every instance, run 3 more steps alone from the batch's state, is within
+    1e-5 m of the batch's own 3 steps."""
+    import oracle
+    spec = scenes.instances(count=32, seed=11, side_lo=17, side_hi=129)
+    spec.params.flags |= CS_FLAG_FIXED_TUNING
+    sides = [int(round(np.sqrt(c.mesh.n_nodes))) for c in spec.cloths]
+    print("sides: min %d max %d, %d above 4,096 faces" % (min(sides), max(sides),
+                                                         sum(2 * (n - 1) ** 2 > 4096 for n in sides)))
+    assert max(sides) >= 120 and sum(2 * (n - 1) ** 2 > 4096 for n in sides) > 4
+    batch = Simulator(spec)
+    picks = [int(np.argmin(sides)), int(np.argmax(sides))]
+    refs = {}
+    for k in picks:
+        one = single(spec, k)
+        one.params.pcg_rel_tol = 1e-7
+        refs[k] = oracle.OracleScene(one)
+    worst_o = 0.0
+    for step in range(5):
+        for k in picks:
+            batch.set_state(CS_MESH_CLOTH, k, *refs[k].state())
+        s, st = batch.step()
+        assert s == 0, s
+        for k in picks:
+            r, _ = refs[k].step(direct=False)
+            assert r == 0
+            d = float(np.abs(batch.state(CS_MESH_CLOTH, k)[0] - refs[k].state()[0]).max())
+            worst_o = max(worst_o, d)
+            print("step %d instance %d (side %d): max|dx| vs oracle %.3g" % (step, k, sides[k], d), flush=True)
+    assert worst_o <= 1e-5
+    # separate runs of every instance from the batch's current state
+    state = [batch.state(CS_MESH_CLOTH, k) for k in range(32)]
+    for _ in range(3):
+        s, st = batch.step()
+        assert s == 0, s
+    worst = 0.0
+    for k in range(32):
+        sim = Simulator(single(spec, k))
+        sim.set_state(CS_MESH_CLOTH, 0, *state[k])
+        for _ in range(3):
+            s, st = sim.step()
+            assert s == 0, s
+        worst = max(worst, float(np.abs(batch.state(CS_MESH_CLOTH, k)[0] - sim.state()[0]).max()))
+        sim.close()
+    print("32 instances: max|dx| batch vs separate %.3g" % worst)
+    assert worst <= 1e-5
