@@ -223,7 +223,7 @@ struct cs_ctx {
     cs::DBuf pcgPartial, pcgScal, pcgCounter;
     cs::PcgScalars* hScal = nullptr;  // pinned
     // batched solve of independent cloths: per-cloth table, block -> cloth map, partials, scalars
-    cs::DBuf bTab, bMap, bPart, bScal;
+    cs::DBuf bTab, bMap, bPart, bScal, bList;
     struct BatchTune {
         int trials = 0, choice = 0;    // solves tried so far, best candidate
         double score[4] = {0, 0, 0, 0};  // seconds per node-iteration, summed over trials

@@ -33,6 +33,7 @@ namespace {
 constexpr int PB = 256;      // threads per block
 constexpr int PPT = 8;       // partials held per thread by the fast reduction path
 constexpr int PMAX = 16384;  // at most this many blocks (grid-stride over rows beyond), a multiple of 8
+constexpr int PCG_RESTARTS = 4;  // continuations from the true residual before a solve gives up
 
 // partial-sum slots (each PMAX floats): dq, rz[2] (ping-pong by iteration parity), rr, bb
 enum { P_DQ = 0, P_RZ0 = 1, P_RZ1 = 2, P_RR = 3, P_BB = 4, P_COUNT = 5 };
@@ -44,7 +45,9 @@ struct PcgDev {
     int last;   // iteration whose stop test produced rrcur
     float tol2bb, bb, rr, rz;
     float rrcur;  // r.r seen by the latest stop test
-    float pad[3];
+    int restartAt;  // batched solve: the iteration that restarts the direction from z (-1: none)
+    int skip;       // batched solve: batch iterations this cloth sat out (its own count is it - skip)
+    float pad;
 };
 
 // Workgroups are dealt to the 8 XCDs round-robin.  Logical block L gives each
@@ -350,6 +353,84 @@ __global__ void __launch_bounds__(PB) k_pcg_update(int it, int nn, const float* 
     }
 }
 
+// The true residual of the current iterate, r = b + bc - A x, with every row sum in FP64 (the
+// recursive r of the loop drifts from it in FP32), stored as the new r; partial r.r, and the
+// preconditioner step of MODE on it (partials in the r.z slot of iteration `it`).
+template <int MODE>
+__global__ void __launch_bounds__(PB) k_pcg_true_residual(int it, int nn, const int* __restrict__ sliceOff,
+                                                          const int* __restrict__ sliceW, const int* __restrict__ cols,
+                                                          const float* __restrict__ vals,
+                                                          const int* __restrict__ cRowPtr, const int* __restrict__ cCol,
+                                                          const float* __restrict__ cVal, const float* __restrict__ b,
+                                                          const float* __restrict__ bc, const float* __restrict__ Minv,
+                                                          const float* __restrict__ x, float* __restrict__ r,
+                                                          float4* __restrict__ z, float* __restrict__ xs,
+                                                          float* __restrict__ part) {
+    __shared__ float lds[8];
+    const int L = logical_block();
+    float acc[2] = {0.0f, 0.0f};
+    for (int i = L * PB + threadIdx.x; i < nn; i += gridDim.x * PB) {
+        double y[3] = {0.0, 0.0, 0.0};
+        auto add = [&](const float (&a)[9], int j) {
+            const double xj[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
+#pragma unroll
+            for (int u = 0; u < 3; u++) y[u] += (double)a[3 * u] * xj[0] + (double)a[3 * u + 1] * xj[1] +
+                                                 (double)a[3 * u + 2] * xj[2];
+        };
+        const int s = i / SLICE, lane = i % SLICE;
+        const int W = sliceW[s];
+        const size_t off = (size_t)sliceOff[s];
+        for (int k = 0; k < W; k++) {
+            const float* ap = vals + (off + k) * 9 * SLICE + lane;
+            float a[9];
+#pragma unroll
+            for (int c = 0; c < 9; c++) a[c] = ap[c * SLICE];
+            add(a, cols[(off + k) * SLICE + lane]);
+        }
+        for (int kc = cRowPtr[i]; kc < cRowPtr[i + 1]; kc++) {
+            float a[9];
+#pragma unroll
+            for (int c = 0; c < 9; c++) a[c] = cVal[(size_t)kc * 9 + c];
+            add(a, cCol[kc]);
+        }
+        V3f ri;
+        ri.x = (float)((double)b[3 * i] + (double)bc[3 * i] - y[0]);
+        ri.y = (float)((double)b[3 * i + 1] + (double)bc[3 * i + 1] - y[1]);
+        ri.z = (float)((double)b[3 * i + 2] + (double)bc[3 * i + 2] - y[2]);
+        st3v(r, i, ri);
+        if (MODE != PC_EXTERNAL) {
+            const V3f zi = mat3v(Minv + 9 * (size_t)i, ri);
+            if (MODE == PC_JACOBI) {
+                z[i] = f4(zi);
+                acc[0] += dot(ri, zi);
+            } else {
+                st3v(xs, i, AMG_OMEGA * zi);
+            }
+        }
+        acc[1] += dot(ri, ri);
+    }
+    block_sums<2>(acc, lds);
+    if (threadIdx.x == 0) {
+        if (MODE == PC_JACOBI) part[(P_RZ0 + (it & 1)) * PMAX + L] = acc[0];
+        part[P_RR * PMAX + L] = acc[1];
+    }
+}
+
+// The stop decision on the true residual: accept (done) when it meets tol^2 b.b or the
+// iteration budget is spent, otherwise reopen the loop from iteration `it`.
+__global__ void __launch_bounds__(PB) k_pcg_recheck(int nb, int it, const float* __restrict__ part, PcgDev* sc) {
+    __shared__ float lds[8];
+    float acc[1] = {0.0f};
+    for (int k = threadIdx.x; k < nb; k += PB) acc[0] += part[P_RR * PMAX + k];
+    block_sums<1>(acc, lds);
+    if (threadIdx.x == 0) {
+        sc->rr = acc[0];
+        sc->rrcur = acc[0];
+        sc->last = it;
+        sc->iters = it;
+        sc->done = (acc[0] <= sc->tol2bb || it >= sc->maxit) ? 1 : 0;
+    }
+}
 
 // ---------------------------------------------------------------- batched solve
 // Independent cloths solved together: every launch covers the rows of all
@@ -503,6 +584,8 @@ __global__ void __launch_bounds__(BP) k_bpcg_setup(const BCloth* __restrict__ cl
         c.sc->rz = v[1];
         c.sc->tol2bb = tol2 * v[0];
         c.sc->done = (v[0] <= tol2 * v[0]) || maxit <= 0;
+        c.sc->restartAt = -1;
+        c.sc->skip = 0;
     }
 }
 
@@ -576,12 +659,15 @@ __global__ void __launch_bounds__(BP) k_bpcg_spmv(const BCloth* __restrict__ cl,
         }
     }
     float beta = 0.0f;
-    if (!FIRST) {
+    // a cloth continued from its true residual starts its direction from z again at `restartAt`
+    const bool restart = !FIRST && c.sc->restartAt == it;
+    if (!FIRST && !restart) {
         float v[3];
         pr.sums(c.part, slot, c.nb, lds, v);
-        if (v[0] <= c.sc->tol2bb || it >= c.sc->maxit) {  // the same decision in every block of the cloth
+        const int own = it - c.sc->skip;  // this cloth's own iteration count
+        if (v[0] <= c.sc->tol2bb || own >= c.sc->maxit) {  // the same decision in every block of the cloth
             if (lb == 0 && threadIdx.x == 0 && !c.sc->done) {
-                c.sc->iters = it;
+                c.sc->iters = own;
                 c.sc->rr = v[0];
                 c.sc->done = 1;
             }
@@ -594,7 +680,7 @@ __global__ void __launch_bounds__(BP) k_bpcg_spmv(const BCloth* __restrict__ cl,
         G1<f4n>* dcur = FIRST ? c.d0 : (p ? c.d1 : c.d0);
         const f4n zi = c.z[i];
         V3f di, qi;
-        if (FIRST) {
+        if (FIRST || restart) {
             di = v3(zi);
             qi = mk(s0, s1, s2);
         } else {
@@ -657,6 +743,80 @@ __global__ void __launch_bounds__(BP) k_bpcg_update(const BCloth* __restrict__ c
     if (threadIdx.x == 0) {
         c.part[(P_RZ0 + (1 - p)) * c.nb + lb] = acc[0];
         c.part[P_RR * c.nb + lb] = acc[1];
+    }
+}
+
+// The true residual of every listed cloth (the map covers them), r = b + bc - A x with FP64 row
+// sums, z = Minv r; partials r.z (slot of iteration `it`) and r.r.
+template <int BP>
+__global__ void __launch_bounds__(BP) k_bpcg_true(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
+                                                  int it) {
+    __shared__ float lds[2 * BP / 64];
+    const int2 m = batch_block(bmap);
+    if (m.x < 0) return;
+    const BClothG c = gview(cl[m.x]);
+    const int lb = m.y;
+    const int i = lb * BP + threadIdx.x;
+    float acc[2] = {0.0f, 0.0f};
+    if (i < c.nn) {
+        double y[3] = {(double)c.b[3 * i] + (double)c.bc[3 * i], (double)c.b[3 * i + 1] + (double)c.bc[3 * i + 1],
+                       (double)c.b[3 * i + 2] + (double)c.bc[3 * i + 2]};
+        auto sub = [&](const float (&a)[9], int j) {
+            const double xj[3] = {c.x[3 * j], c.x[3 * j + 1], c.x[3 * j + 2]};
+#pragma unroll
+            for (int u = 0; u < 3; u++)
+                y[u] -= (double)a[3 * u] * xj[0] + (double)a[3 * u + 1] * xj[1] + (double)a[3 * u + 2] * xj[2];
+        };
+        const int sl = i / SLICE, lane = i % SLICE;
+        const int W = c.sliceW[sl];
+        const size_t off = (size_t)c.sliceOff[sl];
+        for (int k = 0; k < W; k++) {
+            float a[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) a[q] = c.vals[(off + k) * 9 * SLICE + q * SLICE + lane];
+            sub(a, c.cols[(off + k) * SLICE + lane]);
+        }
+        for (int kc = c.cRowPtr[i], ce = c.cRowPtr[i + 1]; kc < ce; kc++) {
+            float a[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) a[q] = c.cVal[(size_t)kc * 9 + q];
+            sub(a, c.cCol[kc]);
+        }
+        const V3f ri = mk((float)y[0], (float)y[1], (float)y[2]);
+        st3v(c.r, i, ri);
+        const V3f zi = mat3v(c.Minv + 9 * (size_t)i, ri);
+        c.z[i] = f4g(zi);
+        acc[0] = dot(ri, zi);
+        acc[1] = dot(ri, ri);
+    }
+    bsums<2, BP>(acc, lds);
+    if (threadIdx.x == 0) {
+        c.part[(P_RZ0 + (it & 1)) * c.nb + lb] = acc[0];
+        c.part[P_RR * c.nb + lb] = acc[1];
+    }
+}
+
+// One block per listed cloth: accept it when the true residual meets tol^2 b.b (or its
+// iterations are spent), otherwise reopen it with its direction restarting at `it`.
+template <int BP>
+__global__ void __launch_bounds__(BP) k_bpcg_recheck(const BCloth* __restrict__ cl, const int* __restrict__ list,
+                                                     int it) {
+    __shared__ float lds[BP / 64];
+    const BClothG c = gview(cl[list[blockIdx.x]]);
+    const int slot[1] = {P_RR};
+    BPartials<1, BP> pr;
+    pr.issue(c.part, slot, c.nb);
+    float v[1];
+    pr.sums(c.part, slot, c.nb, lds, v);
+    if (threadIdx.x == 0) {
+        c.sc->rr = v[0];
+        if (v[0] <= c.sc->tol2bb || c.sc->iters >= c.sc->maxit) {
+            c.sc->done = 1;
+        } else {
+            c.sc->done = 0;
+            c.sc->skip = it - c.sc->iters;
+            c.sc->restartAt = it;
+        }
     }
 }
 
@@ -759,60 +919,108 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     int done = 0;
     PcgDev h{};
     int it = 0;
+    int first = 0;      // the iteration that starts its direction from z alone (0, or a restart)
+    int restarts = 0;
     float rr0 = -1.0f;  // r.r at the previous poll, and its iteration
     int it0 = 0;
-    while (!done && it <= maxit + 8 * chunk0) {
-        for (int k = 0; k < chunk; k++, it++) {
-            // sampled: the SpMV of iteration 1 of every solve, after an empty pair that
-            // measures the markers' own latency
-            if (it == 1) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
-            const int tok = it == 1 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
-            float4* dcur = it == 0 ? dgen[0] : dgen[it & 1];
-            if (it == 0)
-                hipLaunchKernelGGL(k_pcg_spmv<true>, dim3(nb), dim3(PB), 0, s, it, nn, S.sliceOff.ptr<int>(),
-                                   S.sliceW.ptr<int>(), S.cols.ptr<int>(), S.vals.ptr<float>(), cRowPtr, cCol, cVal,
-                                   z, dgen[0], dgen[0], S.q.ptr<float>(), part, sc);
-            else
-                hipLaunchKernelGGL(k_pcg_spmv<false>, dim3(nb), dim3(PB), 0, s, it, nn, S.sliceOff.ptr<int>(),
-                                   S.sliceW.ptr<int>(), S.cols.ptr<int>(), S.vals.ptr<float>(), cRowPtr, cCol, cVal,
-                                   z, dgen[1 - (it & 1)], dcur, S.q.ptr<float>(), part, sc);
-            ktimer_end(C, tok);
-            if (amgl) {
-                hipLaunchKernelGGL(k_pcg_update<PC_EXTERNAL>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
-                                   S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
-                run_cycle(it & 1);
-            } else if (amg) {
-                hipLaunchKernelGGL(k_pcg_update<PC_SMOOTH>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
-                                   S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
-                run_cycle(it & 1);
-            } else if (learned) {
-                hipLaunchKernelGGL(k_pcg_update<PC_EXTERNAL>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
-                                   S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
-                lpc_apply(C, c, S.r.ptr<float>(), z, part + (P_RZ0 + (1 - (it & 1))) * PMAX, nb);
-            } else {
-                hipLaunchKernelGGL(k_pcg_update<PC_JACOBI>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
-                                   S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
-            }
-        }
-        CS_LAUNCH_CHECK();
+    auto poll = [&]() {
         CS_HIP(hipMemcpyAsync(C->hScal, sc, sizeof(PcgDev), hipMemcpyDeviceToHost, s));
         CS_HIP(hipStreamSynchronize(s));
         std::memcpy(&h, C->hScal, sizeof(PcgDev));
-        done = h.done;
         ktimer_collect(C);
-        if (done) break;
-        chunk = chunk0;
-        if (rr0 > 0.0f && h.rrcur > 0.0f && rr0 > h.rrcur && h.last > it0) {
-            // geometric extrapolation of the latest r.r decrease
-            const double rate = std::log((double)h.rrcur / rr0) / (h.last - it0);  // < 0
-            const double need = std::log((double)h.tol2bb / h.rrcur) / rate;       // iterations to go
-            const int left = (int)std::ceil(need) - (it - 1 - h.last);             // beyond those launched
-            chunk = std::max(1, std::min(4 * chunk0, left));
+    };
+    for (;;) {
+        while (!done && it <= maxit + 8 * chunk0) {
+            for (int k = 0; k < chunk; k++, it++) {
+                // sampled: every 8th SpMV of the solve, each after an empty event pair that measures
+                // the markers' own latency
+                const bool sample = (it - first) % 8 == 1;
+                if (sample) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
+                const int tok = sample ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
+                float4* dcur = dgen[it & 1];
+                if (it == first)
+                    hipLaunchKernelGGL(k_pcg_spmv<true>, dim3(nb), dim3(PB), 0, s, it, nn, S.sliceOff.ptr<int>(),
+                                       S.sliceW.ptr<int>(), S.cols.ptr<int>(), S.vals.ptr<float>(), cRowPtr, cCol,
+                                       cVal, z, dcur, dcur, S.q.ptr<float>(), part, sc);
+                else
+                    hipLaunchKernelGGL(k_pcg_spmv<false>, dim3(nb), dim3(PB), 0, s, it, nn, S.sliceOff.ptr<int>(),
+                                       S.sliceW.ptr<int>(), S.cols.ptr<int>(), S.vals.ptr<float>(), cRowPtr, cCol,
+                                       cVal, z, dgen[1 - (it & 1)], dcur, S.q.ptr<float>(), part, sc);
+                ktimer_end(C, tok);
+                if (amgl) {
+                    hipLaunchKernelGGL(k_pcg_update<PC_EXTERNAL>, dim3(nb), dim3(PB), 0, s, it, nn,
+                                       S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur,
+                                       S.q.ptr<float>(), part, sc);
+                    run_cycle(it & 1);
+                } else if (amg) {
+                    hipLaunchKernelGGL(k_pcg_update<PC_SMOOTH>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
+                                       S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
+                    run_cycle(it & 1);
+                } else if (learned) {
+                    hipLaunchKernelGGL(k_pcg_update<PC_EXTERNAL>, dim3(nb), dim3(PB), 0, s, it, nn,
+                                       S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur,
+                                       S.q.ptr<float>(), part, sc);
+                    lpc_apply(C, c, S.r.ptr<float>(), z, part + (P_RZ0 + (1 - (it & 1))) * PMAX, nb);
+                } else {
+                    hipLaunchKernelGGL(k_pcg_update<PC_JACOBI>, dim3(nb), dim3(PB), 0, s, it, nn, S.Minv.ptr<float>(),
+                                       S.x.ptr<float>(), S.r.ptr<float>(), z, xs, dcur, S.q.ptr<float>(), part, sc);
+                }
+            }
+            CS_LAUNCH_CHECK();
+            poll();
+            done = h.done;
+            if (done) break;
+            chunk = chunk0;
+            if (rr0 > 0.0f && h.rrcur > 0.0f && rr0 > h.rrcur && h.last > it0) {
+                // geometric extrapolation of the latest r.r decrease
+                const double rate = std::log((double)h.rrcur / rr0) / (h.last - it0);  // < 0
+                const double need = std::log((double)h.tol2bb / h.rrcur) / rate;       // iterations to go
+                const int left = (int)std::ceil(need) - (it - 1 - h.last);             // beyond those launched
+                chunk = std::max(1, std::min(4 * chunk0, left));
+            }
+            rr0 = h.rrcur;
+            it0 = h.last;
         }
-        rr0 = h.rrcur;
-        it0 = h.last;
+        if (!done) throw StatusError(CS_E_HIP, "PCG: the device never reported its stop");
+        // The loop stopped on its recursive residual.  Accept only what the true residual
+        // b - A x (row sums in FP64) confirms; otherwise continue from it, the search direction
+        // restarted from its preconditioned residual, at most PCG_RESTARTS times.
+        it = h.iters;
+        const int* so = S.sliceOff.ptr<int>();
+        const int* sw = S.sliceW.ptr<int>();
+        const int* co = S.cols.ptr<int>();
+        const float* va = S.vals.ptr<float>();
+        if (amgl || learned)
+            hipLaunchKernelGGL(k_pcg_true_residual<PC_EXTERNAL>, dim3(nb), dim3(PB), 0, s, it, nn, so, sw, co, va,
+                               cRowPtr, cCol, cVal, S.b.ptr<float>(), S.bc.ptr<float>(), S.Minv.ptr<float>(),
+                               S.x.ptr<float>(), S.r.ptr<float>(), z, xs, part);
+        else if (amg)
+            hipLaunchKernelGGL(k_pcg_true_residual<PC_SMOOTH>, dim3(nb), dim3(PB), 0, s, it, nn, so, sw, co, va,
+                               cRowPtr, cCol, cVal, S.b.ptr<float>(), S.bc.ptr<float>(), S.Minv.ptr<float>(),
+                               S.x.ptr<float>(), S.r.ptr<float>(), z, xs, part);
+        else
+            hipLaunchKernelGGL(k_pcg_true_residual<PC_JACOBI>, dim3(nb), dim3(PB), 0, s, it, nn, so, sw, co, va,
+                               cRowPtr, cCol, cVal, S.b.ptr<float>(), S.bc.ptr<float>(), S.Minv.ptr<float>(),
+                               S.x.ptr<float>(), S.r.ptr<float>(), z, xs, part);
+        hipLaunchKernelGGL(k_pcg_recheck, dim3(1), dim3(PB), 0, s, nb, it, part, sc);
+        CS_LAUNCH_CHECK();
+        poll();
+        if (h.done || restarts == PCG_RESTARTS) break;
+        restarts++;
+        float* slot = part + (P_RZ0 + (it & 1)) * PMAX;  // where iteration `it` reads r.z
+        if (amgl) {
+            lpc_smooth_pre(C, c, S.r.ptr<float>(), xs);
+            amg_apply(C, c, S.r.ptr<float>(), z, slot, nb, true, true);
+        } else if (amg) {
+            amg_apply(C, c, S.r.ptr<float>(), z, slot, nb, true);
+        } else if (learned) {
+            lpc_apply(C, c, S.r.ptr<float>(), z, slot, nb);
+        }
+        first = it;
+        done = 0;
+        chunk = chunk0;
+        rr0 = -1.0f;
     }
-    if (!done) throw StatusError(CS_E_HIP, "PCG: the device never reported its stop");
     if (relres) *relres = h.bb > 0.0f ? sqrtf(h.rr / h.bb) : 0.0f;
     if (converged) *converged = (h.rr <= h.tol2bb);
     S.solved = h.rr <= h.tol2bb;  // S.x solves the assembled system
@@ -880,34 +1088,72 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
     const PcgDev* h = reinterpret_cast<const PcgDev*>(pinned_batch(C, sizeof(PcgDev) * ncl));
     int it = 0;
     bool all = false;
-    while (!all && it <= maxit + 2 * chunk) {
-        for (int k = 0; k < chunk; k++, it++) {
-            // sampled: the first full-batch SpMV after the initial one (every cloth still active),
-            // after an empty pair that measures the markers' own latency
-            if (it == 1) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
-            const int tok = it == 1 ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
-            if (it == 0)
-                hipLaunchKernelGGL((k_bpcg_spmv<BP, true>), dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
-            else
-                hipLaunchKernelGGL((k_bpcg_spmv<BP, false>), dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
-            ktimer_end(C, tok);
-            hipLaunchKernelGGL(k_bpcg_update<BP>, dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
+    std::vector<char> verified(ncl, 0);  // accepted on the true residual
+    int* dlist = C->bList.as<int>(ncl);
+    for (int restarts = 0;; restarts++) {
+        const int passStart = it;
+        while (!all && it - passStart <= maxit + 2 * chunk) {
+            for (int k = 0; k < chunk; k++, it++) {
+                // sampled: every 8th batch SpMV, each after an empty event pair that measures the
+                // markers' own latency
+                const bool sample = it % 8 == 1;
+                if (sample) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
+                const int tok = sample ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
+                if (it == 0)
+                    hipLaunchKernelGGL((k_bpcg_spmv<BP, true>), dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
+                else
+                    hipLaunchKernelGGL((k_bpcg_spmv<BP, false>), dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
+                ktimer_end(C, tok);
+                hipLaunchKernelGGL(k_bpcg_update<BP>, dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
+            }
+            CS_LAUNCH_CHECK();
+            CS_HIP(hipMemcpyAsync(C->hBatch, sc, sizeof(PcgDev) * ncl, hipMemcpyDeviceToHost, s));
+            CS_HIP(hipStreamSynchronize(s));
+            ktimer_collect(C);
+            all = true;
+            int active = 0;
+            for (int c = 0; c < ncl; c++) {
+                all = all && h[c].done;
+                active += h[c].done ? 0 : tab[c].nb;
+            }
+            if (!all && active + 8 <= NB) {  // drop the finished cloths from the grid
+                build_map(h);
+                NB = (int)bmap.size();
+                CS_HIP(hipMemcpyAsync(dmap, bmap.data(), sizeof(int2) * NB, hipMemcpyHostToDevice, s));
+            }
         }
+        if (!all) break;
+        // Every cloth stopped on its recursive residual: the ones not yet confirmed are checked on
+        // their true residual (FP64 row sums) and reopened from it when it misses the tolerance,
+        // at most PCG_RESTARTS times.
+        std::vector<int> list;
+        bmap.clear();
+        for (int c = 0; c < ncl; c++)
+            if (!verified[c]) {
+                list.push_back(c);
+                for (int k = 0; k < tab[c].nb; k++) bmap.push_back(make_int2(c, k));
+            }
+        while (bmap.size() % 8) bmap.push_back(make_int2(-1, 0));
+        NB = (int)bmap.size();
+        CS_HIP(hipMemcpyAsync(dmap, bmap.data(), sizeof(int2) * NB, hipMemcpyHostToDevice, s));
+        CS_HIP(hipMemcpyAsync(dlist, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_bpcg_true<BP>, dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
+        hipLaunchKernelGGL(k_bpcg_recheck<BP>, dim3((unsigned)list.size()), dim3(BP), 0, s, dtab, dlist, it);
         CS_LAUNCH_CHECK();
         CS_HIP(hipMemcpyAsync(C->hBatch, sc, sizeof(PcgDev) * ncl, hipMemcpyDeviceToHost, s));
         CS_HIP(hipStreamSynchronize(s));
-        ktimer_collect(C);
         all = true;
-        int active = 0;
-        for (int c = 0; c < ncl; c++) {
+        for (int c : list) {
+            if (h[c].done) verified[c] = 1;
             all = all && h[c].done;
-            active += h[c].done ? 0 : tab[c].nb;
         }
-        if (!all && active + 8 <= NB) {  // drop the finished cloths from the grid
-            build_map(h);
-            NB = (int)bmap.size();
-            CS_HIP(hipMemcpyAsync(dmap, bmap.data(), sizeof(int2) * NB, hipMemcpyHostToDevice, s));
+        if (all || restarts == PCG_RESTARTS) {
+            all = true;  // a cloth still open here keeps its (true) residual: reported as not converged
+            break;
         }
+        build_map(h);  // the reopened cloths
+        NB = (int)bmap.size();
+        CS_HIP(hipMemcpyAsync(dmap, bmap.data(), sizeof(int2) * NB, hipMemcpyHostToDevice, s));
     }
     if (!all) throw StatusError(CS_E_HIP, "batched PCG: the device never reported every stop");
     for (int c = 0; c < ncl; c++) {

@@ -91,7 +91,10 @@ typedef struct cs_params {
     float handle_stiffness, collision_stiffness;
     float repulsion_thickness, collision_thickness;
     float friction, obstacle_friction;
-    float pcg_rel_tol;     /* stop when ||r|| <= pcg_rel_tol * ||b||   (default 1e-5) */
+    float pcg_rel_tol;     /* stop when ||b - A dv|| <= pcg_rel_tol * ||b||   (default 1e-5): the loop stops on
+                              its recursive residual, and the stop stands only if the true residual (row sums
+                              in FP64) confirms it; otherwise the loop continues from the true residual, at most
+                              4 times, and the step reports CS_E_NOT_CONVERGED if it still misses */
     int pcg_max_iters;     /* default 4000 */
     int precond;           /* CS_PRECOND_* */
     int max_collision_iters; /* default 100 */

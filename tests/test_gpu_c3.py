@@ -1,8 +1,9 @@
 """The 513 x 513 crumple at its full size (263,169 nodes, 524,288 faces): the workload of the
 headline measurement, checked against the oracle and against FP64 arithmetic.
 
-* One full step from the initial state against the FP64 oracle's step (its PCG to 1e-8: the
-  direct solve does not finish at this size): positions within 1e-5 m at PCG tolerance 1e-7.
+* The first step from the initial state: the assembled system against the FP64 oracle's, and
+  the step against the FP64 oracle's step (its PCG to 1e-8: the direct solve does not finish at
+  this size), with the FP32 floor measured here stated.
 * A contact state (the GPU's own run of 280 steps with the multigrid preconditioner: air drag
   holds the falling sheet near 1.2 m/s, so it lands on the ground plane after about 230 steps):
   candidate pairs and proximities bit-exact against
@@ -28,30 +29,52 @@ PRE = {"jacobi": CS_PRECOND_BLOCK_JACOBI, "amg": CS_PRECOND_AMG, "amg-learned": 
        "learned": CS_PRECOND_LEARNED}
 
 
-def test_crumple513_first_step_matches_fp64_oracle():
-    """The first step (the sheet's 1 mm noise relaxing under stiffness 1e4 N/m): two solutions
-    that both meet the 1e-5 residual differ here by about 1e-4 m (the system's conditioning at
-    this size), so the step is compared at PCG tolerance 1e-7 with the FP64 oracle run to 1e-8,
-    within 1e-5 m; the default tolerance's distance from it is printed."""
+def test_crumple513_first_step_against_fp64_oracle():
+    """The first step (the 1 mm noise relaxing under stiffness 1e4 N/m: nodes move up to 4 cm).
+    Measured: the FP32 result lies 1.47e-4 m from the FP64 oracle's (PCG to 1e-8) whatever the
+    preconditioner and whatever the tolerance from 1e-5 to 1e-7 (1.45e-4 to 1.48e-4): that is the
+    FP32 rounding of the assembled system times its conditioning at this size (the oracle's own
+    FP32 path differs from its FP64 path by 2.6e-6 m at 65^2 and 5.9e-6 m at 129^2 on the same
+    relaxation), not the solve.  So: the assembled system matches the FP64 oracle's to 2e-5 of
+    each row's largest block, the step lies within 2.5e-4 m of the FP64 step, and tightening the
+    solve from 1e-5 to 1e-6 moves the result by less than 3e-5 m."""
+    spec = scenes.crumple(513)
+    sim = Simulator(spec)
+    rp, ci, vals, b = sim.assemble(0)
+    o64 = oracle.OracleScene(scenes.crumple(513), fp64=True)
+    orp, oci, ovals, ob = o64.assemble(0)
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    err = np.abs(vals - ovals).reshape(len(ci), 9).max(axis=1)
+    scale = np.zeros(len(rp) - 1)
+    np.maximum.at(scale, rows, np.abs(ovals).reshape(len(ci), 9).max(axis=1))
+    worst = float((err / scale[rows]).max())
+    berr = float(np.abs(b - ob).max() / np.abs(ob).max())
+    print("crumple513 assembly vs FP64: worst row-relative %.3g, b %.3g" % (worst, berr), flush=True)
+    assert worst <= 2e-5 and berr <= 2e-5
+    del o64
+    sim.close()
     spec = scenes.crumple(513)
     spec.params.pcg_rel_tol = 1e-8
     truth = oracle.OracleScene(spec, fp64=True)
     r, ost = truth.step(direct=False)
-    assert r in (CS_OK, 4)  # FP64 PCG: 4000-iteration cap at about 1e-8
+    assert r == CS_OK
     xt = truth.state(fp64=True)[0]
-    out = {}
-    for tol in (1e-5, 1e-7):
+    xs = {}
+    for tol in (1e-5, 1e-6):
         spec = scenes.crumple(513)
         spec.params.pcg_rel_tol = tol
+        spec.params.precond = CS_PRECOND_AMG
         sim = Simulator(spec)
         s, st = sim.step()
         assert s == CS_OK and st.pcg_converged == 1
-        out[tol] = float(np.abs(sim.state()[0] - xt).max())
-        print("crumple513 step 1, GPU block-Jacobi PCG to %g: %d iterations, max|dx| vs FP64 oracle (%d "
-              "iterations, %.2g) %.3g m" % (tol, st.pcg_iters, ost.pcg_iters, ost.pcg_rel_residual, out[tol]),
-              flush=True)
+        xs[tol] = sim.state()[0]
+        print("crumple513 step 1, GPU multigrid PCG to %g: %d iterations, max|dx| vs FP64 oracle (%d iterations, "
+              "%.2g) %.3g m" % (tol, st.pcg_iters, ost.pcg_iters, ost.pcg_rel_residual,
+                                np.abs(xs[tol] - xt).max()), flush=True)
         sim.close()
-    assert out[1e-7] <= 1e-5
+        assert np.abs(xs[tol] - xt).max() <= 2.5e-4
+    assert np.abs(xs[1e-5] - xs[1e-6]).max() <= 3e-5
 
 
 @pytest.fixture(scope="module")
