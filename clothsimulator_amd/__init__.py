@@ -26,7 +26,8 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_lpc_weight_count", "cs_set_learned_weights", "cs_get_learned_weights", "cs_lpc_factor",
             "cs_lpc_train_step", "cs_lpc_smoother_omega", "cs_batch_tuning", "cs_add_motion", "cs_motion_eval",
             "cs_set_obstacle_motion", "cs_add_moving_handles", "cs_set_step_count", "cs_get_step_count",
-            "cs_scene_motion", "cs_get_topology", "cs_write_config", "cs_impact_response"]
+            "cs_scene_motion", "cs_get_topology", "cs_write_config", "cs_impact_response",
+            "cs_solve_bsr_noalloc"]
 
 # trained weights of the learned preconditioner shipped with the package (float32, .npy)
 LPC_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lpc_weights.npy")
@@ -73,6 +74,8 @@ def load_library(path=LIB_PATH):
     L.cs_find_impacts.argtypes = [vp, C.c_float, C.POINTER(ImpactRec), C.c_int, C.POINTER(C.c_int)]
     L.cs_independent_impacts.argtypes = [vp, C.POINTER(ImpactRec), C.c_int, C.c_int, C.POINTER(ImpactRec),
                                          C.POINTER(C.c_int)]
+    L.cs_solve_bsr_noalloc.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, C.c_float, C.c_int, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_float)]
     L.cs_impact_response.argtypes = [vp, C.POINTER(ImpactRec), C.c_int, C.c_int, C.c_float, C.POINTER(C.c_int)]
     L.cs_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]
     L.cs_load_scene.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Params), C.POINTER(vp), C.POINTER(SceneInfo)]

@@ -251,6 +251,23 @@ cs_status cs_assemble(cs_ctx* ctx, int cloth_id, int* row_ptr, int* col_idx, flo
                       int* nnzb_out, float* b);
 /* Solve the last assembled system of one cloth (src/Cloth.cu:766-809): dv (3n). */
 cs_status cs_solve(cs_ctx* ctx, int cloth_id, float* dv, cs_stats* stats);
+/* ---- solve without allocation.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The block-Jacobi PCG of cs_solve (x = 0, same iteration, stop when
+ * ||r|| <= rel_tol ||b|| or after max_iters) on a block-CSR system in the caller's DEVICE memory:
+ * row_ptr (n_nodes + 1), col_idx (nnzb), vals (9 per block, 3x3 row-major; every row holds its
+ * diagonal block), b (3 n_nodes) in, x (3 n_nodes) out; iters and rel_residual (the recursive
+ * ||r|| / ||b||) are host pointers and may be NULL.  Runs on `stream` (a hipStream_t, NULL for
+ * the default stream) and returns after it has completed.  Allocates nothing: every temporary
+ * lives in one fixed device array of the library (about 5.5 MB per GPU, held until the process
+ * ends), shared by every call on that GPU, so calls that overlap on one GPU race.
+ * CS_E_CAPACITY above CS_NOALLOC_MAX_NODES nodes; CS_E_ARG for a row without its diagonal
+ * block or with a column outside [0, n_nodes); CS_E_NOT_CONVERGED when max_iters ends the
+ * solve. */
+#define CS_NOALLOC_MAX_NODES 65536
+cs_status cs_solve_bsr_noalloc(void* stream, int n_nodes, const int* row_ptr, const int* col_idx, const float* vals,
+                               const float* b, float* x, float rel_tol, int max_iters, int* iters,
+                               float* rel_residual);
 /* The CS_PRECOND_AMG, CS_PRECOND_LEARNED or CS_PRECOND_AMG_LEARNED preconditioner of the last
  * assembled system (no counterpart in the reference, whose PCG uses the Jacobi diagonal): the
  * level sizes of the multigrid hierarchy (n_levels = 0 when the cloth cannot use it; always 0
