@@ -8,9 +8,12 @@ A step is one full simulation step (physics: LBVH + proximities + implicit
 assembly + preconditioned CG to 1e-5 relative residual + position/geometry
 update; collision: CCD LBVH + FP64 impacts + impact response + velocity
 update).  The default workload is the 513 x 513 crumple (263,169 nodes,
-524,288 faces, dt = 2 ms, ground plane), timed over its full 300 steps from the
-initial state with every input resident in HBM.  W warmup steps run first on
-the same context, which is then reset to the initial state.
+524,288 faces, dt = 2 ms, ground plane), run over its full 300 steps from the
+initial state with every input resident in HBM: with --steps K below 300, K of
+the 300 steps, spread evenly over the run (free fall, landing and contact alike),
+are timed, each between two completed streams, and value = N K / their summed
+time; the other steps run untimed.  W warmup steps run first on the same
+context, which is then reset to the initial state.
 
 The measured run uses the learned preconditioner in its fastest form: the
 aggregation-multigrid V-cycle whose finest-level smoother is the network's
@@ -98,6 +101,14 @@ PRECONDS = {"learned": "learned_spai", "amg-learned": "amg_v11_learned_smoother"
             "amg": "amg_v11"}
 
 
+def timed_steps(steps, span):
+    """Which of the workload's `span` steps are timed: all of them, or, for fewer, `steps` steps
+    spread evenly over the run (step floor(i span / steps) for i < steps)."""
+    if steps >= span:
+        return list(range(steps))
+    return sorted(set(int(i * span // steps) for i in range(steps)))
+
+
 def run(workload, precond, steps, warmup, timed):
     """Steps `workload` from its initial state with one preconditioner; returns the timings."""
     from clothsimulator_amd import CS_MESH_CLOTH, CS_MESH_OBSTACLE, Simulator, scenes
@@ -123,20 +134,29 @@ def run(workload, precond, steps, warmup, timed):
     if timed:
         sim.kernel_time(CS_KERNEL_PCG_SPMV, reset=True)
         sim.kernel_time(CS_KERNEL_EVENT_PAIR, reset=True)
+    span = max(steps, spec.steps)
+    sel = set(timed_steps(steps, span))
     iters, contacts, failed = [], [], 0
-    t0 = time.perf_counter()  # every library call returns after its stream has completed
-    for k in range(steps):
+    elapsed = 0.0
+    t_all = time.perf_counter()
+    for k in range(span):
+        # every library call returns after its stream has completed: a step's wall time is
+        # bracketed by two completed streams
+        t0 = time.perf_counter()
         s, st = sim.step()
-        if s != 0:  # a PCG that hit its iteration cap (status 4) counts as a failed step too
-            failed += 1
-        iters.append(st.pcg_iters)
-        contacts.append(st.n_contact_blocks)
+        dt = time.perf_counter() - t0
+        if k in sel:
+            elapsed += dt
+            if s != 0:  # a PCG that hit its iteration cap (status 4) counts as a failed step too
+                failed += 1
+            iters.append(st.pcg_iters)
+            contacts.append(st.n_contact_blocks)
         if (k + 1) % 50 == 0:
-            print("%s step %d/%d  %.1f s  cg %d" % (precond, k + 1, steps, time.perf_counter() - t0, st.pcg_iters),
+            print("%s step %d/%d  %.1f s  cg %d" % (precond, k + 1, span, time.perf_counter() - t_all, st.pcg_iters),
                   file=sys.stderr, flush=True)
-    elapsed = time.perf_counter() - t0
     out = {"spec": spec, "elapsed": elapsed, "iters": iters, "contacts": contacts, "failed": failed,
-           "value": spec.n_cloth_nodes * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps}
+           "value": spec.n_cloth_nodes * len(sel) / elapsed, "ms_per_step": 1e3 * elapsed / len(sel),
+           "span": span, "timed": len(sel)}
     if timed:
         out["spmv"] = sim.kernel_time(CS_KERNEL_PCG_SPMV)
         out["pair"] = sim.kernel_time(CS_KERNEL_EVENT_PAIR)
@@ -254,7 +274,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": spec.name, "nodes": n_nodes, "faces": int(c0.mesh.n_faces), "dt": spec.params.dt,
-                   "precond": PRECONDS[a.precond], "rel_tol": 1e-5, "obstacles": len(spec.obstacles)},
+                   "precond": PRECONDS[a.precond], "rel_tol": 1e-5, "obstacles": len(spec.obstacles),
+                   "steps_in_run": main_run["span"],
+                   "timed_steps": "all" if main_run["timed"] >= main_run["span"] else
+                   "%d of the run's %d steps, spread evenly (every %g-th)" % (
+                       main_run["timed"], main_run["span"], main_run["span"] / main_run["timed"])},
         "cg_iters_mean": ms["cg_iters_mean"],
         "cg_iters_max": ms["cg_iters_max"],
         "cg_iters_learned": ms["cg_iters_mean"] if a.precond == "learned" else (

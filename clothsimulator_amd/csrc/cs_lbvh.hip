@@ -16,6 +16,9 @@
 //   stack in LDS; a leaf pair is reported when the query box overlaps the
 //   target leaf box dilated by `thickness`; for a mesh against itself only
 //   leaves with a larger sorted index are reported.
+#include <cstring>
+#include <string>
+
 #include "cs_kernels.hpp"
 #include "cs_math.hpp"
 
@@ -501,13 +504,15 @@ __device__ __forceinline__ BNode load_node(const G1<f4n>* t, int i) {
 }
 
 // Leaf-vs-tree traversal for query leaf qi.  emit(face) for each reported leaf.
-// Returns false if the stack overflowed (the query is then incomplete).
+constexpr uint32_t LEAF_PAIRS = 1u << 16;  // candidate pairs of one query face at most
+
+// Returns false if the stack overflowed or emit() refused a leaf (the query is then incomplete).
 template <bool SELF, class Emit>
 __device__ __forceinline__ bool traverse(const G1<f4n>* __restrict__ tn, const G1<i2n>* __restrict__ tchild, int tnf,
                                          const Box& qb, int qi, float th, int* stk, Emit&& emit) {
     if (tnf == 1) {  // a one-face target has no internal node
         const BNode t0 = load_node(tn, 0);
-        if (!SELF && box_overlap(qb, node_box(t0), th)) emit(t0.face);
+        if (!SELF && box_overlap(qb, node_box(t0), th)) return emit(t0.face);
         return true;
     }
     const int ni = tnf - 1;
@@ -519,8 +524,8 @@ __device__ __forceinline__ bool traverse(const G1<f4n>* __restrict__ tn, const G
         const bool lLeaf = ch.x >= ni, rLeaf = ch.y >= ni;
         const bool ovL = (!SELF || qi < L.maxIndex) && box_overlap(qb, node_box(L), th);
         const bool ovR = (!SELF || qi < R.maxIndex) && box_overlap(qb, node_box(R), th);
-        if (ovL && lLeaf) emit(L.face);
-        if (ovR && rLeaf) emit(R.face);
+        if (ovL && lLeaf && !emit(L.face)) return false;
+        if (ovR && rLeaf && !emit(R.face)) return false;
         const bool tL = ovL && !lLeaf, tR = ovR && !rLeaf;
         if (!tL && !tR) {
             if (top == 0) break;
@@ -567,15 +572,32 @@ __global__ void __launch_bounds__(TB) k_traverse_batch(const TravJob* __restrict
     uint32_t cnt = 0;
     const uint32_t base = MODE == 1 ? offs[J.off + qi] : 0;
     auto emit = [&](int f) {
+        if (cnt >= LEAF_PAIRS) return false;  // far beyond any real contact: positions not finite
         if (MODE == 1) out[base + cnt] = make_int2(qface, f);
         cnt++;
+        return true;
     };
     const G1<f4n>* tn = (const G1<f4n>*)J.tn;
     const G1<i2n>* tchild = (const G1<i2n>*)J.tchild;
     const bool ok = J.self ? traverse<true>(tn, tchild, J.tnf, qb, qi, th, stk + threadIdx.x, emit)
                            : traverse<false>(tn, tchild, J.tnf, qb, qi, th, stk + threadIdx.x, emit);
     if (MODE == 0) num[J.off + qi] = cnt;
-    if (!ok) atomicOr(overflow, 1u);
+    if (!ok) atomicOr(overflow, cnt >= LEAF_PAIRS ? 2u : 1u);
+}
+
+// The exact 64-bit sum of the per-leaf pair counts (one atomic per block: an integer sum, so the
+// order does not matter).  The exclusive scan that places the pairs is 32-bit: a total at or
+// above 2^32 would wrap it, so the host checks this sum before the emit pass.
+__global__ void __launch_bounds__(256) k_count_total(const uint32_t* __restrict__ num, long n,
+                                                     unsigned long long* __restrict__ total) {
+    __shared__ unsigned long long part[4];
+    unsigned long long v = 0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) v += num[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(total, part[0] + part[1] + part[2] + part[3]);
 }
 
 static int floor_log2_host(int v) {
@@ -652,6 +674,7 @@ void bvh_build(cs_ctx* C, int mesh, bool ccd) {
     }
 }
 
+constexpr long MAX_PAIRS = 1L << 28;  // candidate pairs per traversal pass at most
 constexpr int SMALL_BVH = 1 << 15;  // meshes up to this many faces build and refit in one batched launch
 // One workgroup builds a tree in time growing with its size (2 ms at 32,768 faces, latency-bound
 // on one CU); the per-mesh kernels spread a tree over the chip for about 35 launches.  Trees
@@ -785,7 +808,7 @@ template <class F> static void for_each_traversal(cs_ctx* C, F&& f) {
 // then tree order.  One host synchronisation per call, however many meshes.
 long bvh_all_pairs(cs_ctx* C, float th) {
     hipStream_t s = C->stream;
-    unsigned* flag = C->pairCount.as<unsigned>(4);
+    unsigned* flag = C->pairCount.as<unsigned>(4);  // [1] stack overflow, [2..3] 64-bit pair total
     fill_u32(flag, 0u, 4, s);
     std::vector<TravJob> jobs;
     long len = 0;
@@ -816,11 +839,25 @@ long bvh_all_pairs(cs_ctx* C, float th) {
     hipLaunchKernelGGL(k_traverse_batch<0>, dim3(blocks), dim3(TB), 0, s, (const TravJob*)dj, nj, th, num,
                        (const uint32_t*)nullptr, (int2*)nullptr, flag + 1);
     CS_LAUNCH_CHECK();
+    unsigned long long* total64 = reinterpret_cast<unsigned long long*>(flag + 2);
+    hipLaunchKernelGGL(k_count_total, dim3(std::min(cdiv(len, 256), 1024)), dim3(256), 0, s, num, len, total64);
+    CS_LAUNCH_CHECK();
     scan_exclusive(num, offs, (int)len, C->ws, s);
     CS_HIP(hipMemcpyAsync(C->hCount, offs + len, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    CS_HIP(hipMemcpyAsync(C->hCount + 1, flag + 1, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    CS_HIP(hipMemcpyAsync(C->hCount + 1, flag + 1, 3 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
     CS_HIP(hipStreamSynchronize(s));  // also ends the job table's host-to-device copy
+    if (C->hCount[1] & 2u)
+        throw StatusError(CS_E_CAPACITY, "LBVH traversal: a face has more than 65536 candidate pairs "
+                                         "(positions not finite?)");
     if (C->hCount[1]) throw StatusError(CS_E_CAPACITY, "LBVH traversal stack overflow (tree deeper than 64)");
+    unsigned long long pairs64 = 0;
+    std::memcpy(&pairs64, C->hCount + 2, sizeof(pairs64));
+    // Every box overlaps every other once a position is not finite (NaN compares false), which
+    // puts the pair count in the tens of billions at 513^2; the cap keeps the pair list, the
+    // 32-bit scans over it and the CCD's 15 tests per pair in range.
+    if (pairs64 > (unsigned long long)MAX_PAIRS)
+        throw StatusError(CS_E_CAPACITY, "LBVH traversal: " + std::to_string(pairs64) +
+                                             " candidate pairs exceed the limit of 2^28 (positions not finite?)");
     const long total = C->hCount[0];
     if (total == 0) return 0;
     C->pairs.grow_keep(sizeof(int2) * (size_t)total, 0, s);

@@ -10,6 +10,7 @@
 // `scratch`, one file-scope device array of fixed size: each GPU holds its own copy for the
 // life of the process, and every call on that GPU reuses it, so calls that overlap on one GPU
 // race.  The host reads back one scalar record every NA_CHUNK iterations.
+#include <cfloat>
 #include <mutex>
 
 #include "cs_common.hpp"
@@ -156,7 +157,7 @@ __global__ void __launch_bounds__(NA_TB) k_na_reduce(int nb, int mode, float tol
     } else {
         sc->rr = tot[0];
         sc->iters += 1;
-        if (tot[0] <= sc->tol2bb || sc->iters >= sc->maxit) {
+        if (tot[0] <= sc->tol2bb || sc->iters >= sc->maxit || !(tot[0] <= FLT_MAX)) {  // NaN: breakdown
             sc->done = 1;
             return;
         }
@@ -268,7 +269,7 @@ __global__ void __launch_bounds__(NA_TB) k_na_recheck(int nb, NaPtr P) {
     NaScal* sc = P.sc;
     sc->rr = tot[0];
     sc->rz = tot[1];
-    sc->done = (tot[0] <= sc->tol2bb || sc->iters >= sc->maxit) ? 1 : 0;
+    sc->done = (tot[0] <= sc->tol2bb || sc->iters >= sc->maxit || !(tot[0] <= FLT_MAX)) ? 1 : 0;
 }
 
 __global__ void k_na_clear(NaScal* sc) {

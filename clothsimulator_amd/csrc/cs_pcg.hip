@@ -19,6 +19,7 @@
 // at kernel entry and reduced after the kernel's memory sweep (SpMV) or after
 // its row loads (update), so their latency hides behind the sweep.
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -262,7 +263,8 @@ __global__ void __launch_bounds__(PB) k_pcg_spmv(int it, int nn, const int* __re
                 sc->rrcur = tot[0];
                 sc->last = it;
             }
-            if (tot[0] <= sc->tol2bb || it >= sc->maxit) {  // the same decision in every block
+            // stop on the tolerance, the budget, or a non-finite r.r (NaN fails every comparison)
+            if (tot[0] <= sc->tol2bb || it >= sc->maxit || !(tot[0] <= FLT_MAX)) {  // the same in every block
                 if (L == 0 && threadIdx.x == 0 && !sc->done) {
                     sc->iters = it;
                     sc->rr = tot[0];
@@ -428,7 +430,7 @@ __global__ void __launch_bounds__(PB) k_pcg_recheck(int nb, int it, const float*
         sc->rrcur = acc[0];
         sc->last = it;
         sc->iters = it;
-        sc->done = (acc[0] <= sc->tol2bb || it >= sc->maxit) ? 1 : 0;
+        sc->done = (acc[0] <= sc->tol2bb || it >= sc->maxit || !(acc[0] <= FLT_MAX)) ? 1 : 0;
     }
 }
 
@@ -665,7 +667,7 @@ __global__ void __launch_bounds__(BP) k_bpcg_spmv(const BCloth* __restrict__ cl,
         float v[3];
         pr.sums(c.part, slot, c.nb, lds, v);
         const int own = it - c.sc->skip;  // this cloth's own iteration count
-        if (v[0] <= c.sc->tol2bb || own >= c.sc->maxit) {  // the same decision in every block of the cloth
+        if (v[0] <= c.sc->tol2bb || own >= c.sc->maxit || !(v[0] <= FLT_MAX)) {  // the same in every block
             if (lb == 0 && threadIdx.x == 0 && !c.sc->done) {
                 c.sc->iters = own;
                 c.sc->rr = v[0];
@@ -810,7 +812,7 @@ __global__ void __launch_bounds__(BP) k_bpcg_recheck(const BCloth* __restrict__ 
     pr.sums(c.part, slot, c.nb, lds, v);
     if (threadIdx.x == 0) {
         c.sc->rr = v[0];
-        if (v[0] <= c.sc->tol2bb || c.sc->iters >= c.sc->maxit) {
+        if (v[0] <= c.sc->tol2bb || c.sc->iters >= c.sc->maxit || !(v[0] <= FLT_MAX)) {
             c.sc->done = 1;
         } else {
             c.sc->done = 0;
