@@ -444,6 +444,11 @@ struct SmallCycle {
     const int *rmptr, *rmidx;
     const float* rt;
     float* out0;      // its result (the coarsest's solution when it is the only level)
+    // unit mode (op != nullptr): workgroup j runs the sub-cycle on the unit right-hand side e_j
+    // of the first level instead of the restriction, and writes its result to column j of the
+    // row-major dense operator op (dim x dim, dim = 3 n of the first level)
+    float* op;
+    int dim;
     SmallLevel L[MAXSMALL];
 };
 
@@ -488,9 +493,10 @@ __device__ __forceinline__ int touch(const void* p, size_t bytes, int tid) {
 __global__ void __launch_bounds__(SB) k_amg_small_cycle(SmallCycle Q) {
     __shared__ float vb[3 * SMAXN], vx[3 * SMAXN], vt[3 * SMAXN];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool unit = Q.op != nullptr;
     // The sub-cycle's levels (about 0.25 MB) were last read an iteration ago; their lines
     // come back from HBM in one round of independent loads instead of one round per phase.
-    {
+    if (!unit) {
         int acc = touch(Q.inv, sizeof(float) * (size_t)Q.m * Q.m, tid);
         acc ^= touch(Q.rmptr, sizeof(int) * (size_t)(Q.L[0].n + 1), tid);
         acc ^= touch(Q.rmidx, sizeof(int) * (size_t)Q.nFine, tid);
@@ -507,8 +513,11 @@ __global__ void __launch_bounds__(SB) k_amg_small_cycle(SmallCycle Q) {
         }
         asm volatile("" ::"v"(acc));
     }
+    if (unit) {
+        for (int e = tid; e < 3 * Q.L[0].n; e += SB) vb[3 * Q.L[0].off + e] = e == (int)blockIdx.x ? 1.0f : 0.0f;
+    }
     // restriction into the first level (as k_amg_restrict: members in order)
-    for (int I = tid; I < Q.L[0].n; I += SB) {
+    for (int I = tid; !unit && I < Q.L[0].n; I += SB) {
         V3f sum = mk(0.0f, 0.0f, 0.0f);
         const int k0 = Q.rmptr[I], k1 = Q.rmptr[I + 1];
         for (int k = k0; k < k1; k += 8) {
@@ -585,7 +594,56 @@ __global__ void __launch_bounds__(SB) k_amg_small_cycle(SmallCycle Q) {
         }
         __syncthreads();
     }
+    if (unit) {
+        for (int e = tid; e < Q.dim; e += SB) Q.op[(size_t)e * Q.dim + blockIdx.x] = vx[3 * Q.L[0].off + e];
+        return;
+    }
     for (int e = tid; e < 3 * Q.L[0].n; e += SB) Q.out0[e] = vx[3 * Q.L[0].off + e];
+}
+
+// The small levels' sub-cycle applied as its dense operator: every workgroup forms the first
+// small level's right-hand side (the restriction of the finer residual, members summed in
+// order, as the sub-cycle does) in LDS, then each wave produces one row of op . b, its lanes
+// striding along the row and the row sum reduced over the wave.
+constexpr int DB = 256;  // threads per workgroup, one row per wave
+__global__ void __launch_bounds__(DB) k_amg_small_dense(int n0, int dim, const int* __restrict__ rmptr,
+                                                        const int* __restrict__ rmidx, const float* __restrict__ rt,
+                                                        const float* __restrict__ op, float* __restrict__ out) {
+    __shared__ float b[3 * SMAXN];
+    constexpr int NK = SMAXN * 3 / 64;  // row entries per lane at most
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = blockIdx.x * (DB / 64) + wave;
+    const int nk = (dim + 63) / 64;
+    // the row's loads go out first: they do not depend on the right-hand side
+    float a[NK];
+    const float* row = op + (size_t)min(r, dim - 1) * dim;
+#pragma unroll
+    for (int u = 0; u < NK; u++) a[u] = u < nk && lane + 64 * u < dim ? row[lane + 64 * u] : 0.0f;
+    for (int I = tid; I < n0; I += DB) {  // members' ids, then their values, eight at a time
+        V3f sum = mk(0.0f, 0.0f, 0.0f);
+        const int k0 = rmptr[I], k1 = rmptr[I + 1];
+        for (int k = k0; k < k1; k += 8) {
+            int j[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) j[u] = rmidx[min(k + u, k1 - 1)];
+            V3f m[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) m[u] = ld3v(rt, j[u]);
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (k + u < k1) sum = sum + m[u];
+        }
+        st3s(b, I, sum);
+    }
+    __syncthreads();
+    if (r >= dim) return;
+    float acc = 0.0f;
+#pragma unroll
+    for (int u = 0; u < NK; u++)
+        if (u < nk && lane + 64 * u < dim) acc += a[u] * b[lane + 64 * u];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) out[r] = acc;
 }
 
 // AP = A_static P of level 0, per step: each AP slot sums the level-0 blocks of
@@ -864,8 +922,61 @@ int amg_levels(cs_ctx* C, int c, int* sizes, int cap) {
     return G.nl;
 }
 
-// Per-solve numeric setup: Galerkin operators of the static system, their
-// diagonal inverses, and the coarsest dense inverse.
+// The first of the small levels (at most SMALL nodes, their vectors within SMAXN nodes, at
+// most MAXSMALL of them), which run as one sub-cycle.
+static int small_first(const AmgSys& G) {
+    int l0 = G.nl - 1;
+    for (int l = 1; l < G.nl; l++)
+        if (G.L[l].n <= SMALL) { l0 = l; break; }
+    l0 = std::max(l0, G.nl - MAXSMALL);
+    for (;;) {  // the small levels' vectors must fit the workgroup's LDS
+        int tot = 0;
+        for (int l = l0; l < G.nl; l++) tot += G.L[l].n;
+        if (tot <= SMAXN || l0 == G.nl - 1) break;
+        l0++;
+    }
+    return l0;
+}
+
+// The sub-cycle's arguments for the small levels l0.. (the first restricts level l0 - 1's residual).
+static SmallCycle small_args(AmgSys& G, int l0) {
+    SmallCycle Q;
+    Q.ns = G.nl - l0;
+    Q.m = 3 * G.L[G.nl - 1].n;
+    Q.inv = G.inv.ptr<float>();
+    Q.nFine = G.L[l0 - 1].n;
+    Q.rmptr = G.L[l0 - 1].mptr.ptr<int>();
+    Q.rmidx = G.L[l0 - 1].midx.ptr<int>();
+    Q.rt = G.L[l0 - 1].t.ptr<float>();
+    Q.out0 = l0 == G.nl - 1 ? G.L[l0].x.ptr<float>() : G.L[l0].t.ptr<float>();
+    Q.op = nullptr;
+    Q.dim = 3 * G.L[l0].n;
+    int off = 0;
+    for (int k = 0; k < Q.ns; k++) {
+        AmgLevel& L = G.L[l0 + k];
+        const bool last = l0 + k == G.nl - 1;
+        SmallLevel& q = Q.L[k];
+        q.n = L.n;
+        q.off = off;
+        q.nnzb = L.nnzb;
+        q.nNext = last ? 0 : G.L[l0 + k + 1].n;
+        off += L.n;
+        q.rp = L.rp.ptr<int>();
+        q.ci = L.ci.ptr<int>();
+        q.val = L.val.ptr<float>();
+        q.Dinv = L.Dinv.ptr<float>();
+        q.agg = last ? nullptr : L.agg.ptr<int>();
+        q.mptr = last ? nullptr : L.mptr.ptr<int>();
+        q.midx = last ? nullptr : L.midx.ptr<int>();
+    }
+    return Q;
+}
+
+// Per-solve numeric setup: Galerkin operators of the static system, their diagonal inverses,
+// the coarsest dense inverse, and the small levels' sub-cycle as a dense operator (the
+// sub-cycle run on every unit right-hand side at once, one workgroup each): the cycles of the
+// solve then apply it as one matrix-vector product spread over the chip instead of running the
+// sub-cycle's dependent phases in one workgroup.
 void amg_prepare(cs_ctx* C, int c) {
     ClothSys& S = C->sys[c];
     AmgSys& G = S.amg;
@@ -896,6 +1007,13 @@ void amg_prepare(cs_ctx* C, int c) {
     hipLaunchKernelGGL(k_amg_coarse_inverse, dim3(1), dim3(CB), 0, s, Lc.n, Lc.rp.ptr<int>(), Lc.ci.ptr<int>(),
                        Lc.val.ptr<float>(), G.work.ptr<double>(), G.inv.ptr<float>());
     CS_LAUNCH_CHECK();
+    const int l0 = small_first(G);
+    SmallCycle Q = small_args(G, l0);
+    G.smallL0 = l0;
+    G.smallDim = Q.dim;
+    Q.op = G.smallOp.as<float>((size_t)Q.dim * Q.dim);
+    hipLaunchKernelGGL(k_amg_small_cycle, dim3(Q.dim), dim3(SB), 0, s, Q);
+    CS_LAUNCH_CHECK();
 }
 
 // z = M^-1 r (one V(1,1) cycle); partRZ[b] = block b's share of r.z for the
@@ -915,17 +1033,9 @@ void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int n
     A0.cCol = S.nContact > 0 ? S.cCol.ptr<int>() : S.cRowPtr.ptr<int>();
     A0.cVal = S.nContact > 0 ? S.cVal.ptr<float>() : S.vals.ptr<float>();
     A0.Minv = S.Minv.ptr<float>();
-    // levels l0.. (the first with at most SMALL nodes, l0 >= 1) run in one workgroup
-    int l0 = G.nl - 1;
-    for (int l = 1; l < G.nl; l++)
-        if (G.L[l].n <= SMALL) { l0 = l; break; }
-    l0 = std::max(l0, G.nl - MAXSMALL);
-    for (;;) {  // the small levels' vectors must fit the workgroup's LDS
-        int tot = 0;
-        for (int l = l0; l < G.nl; l++) tot += G.L[l].n;
-        if (tot <= SMAXN || l0 == G.nl - 1) break;
-        l0++;
-    }
+    // levels l0.. (the first with at most SMALL nodes, l0 >= 1): their sub-cycle is applied as
+    // the dense operator amg_prepare built
+    const int l0 = G.smallL0;
     // down: smooth, residual, restrict
     for (int l = 0; l < l0; l++) {
         AmgLevel& L = G.L[l];
@@ -944,34 +1054,12 @@ void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int n
             hipLaunchKernelGGL(k_amg_restrict, dim3(cdiv(N.n, AB)), dim3(AB), 0, s, N.n, L.mptr.ptr<int>(),
                                L.midx.ptr<int>(), L.t.ptr<float>(), N.b.ptr<float>());
     }
-    SmallCycle Q;
-    Q.ns = G.nl - l0;
-    Q.m = 3 * G.L[G.nl - 1].n;
-    Q.inv = G.inv.ptr<float>();
-    Q.nFine = G.L[l0 - 1].n;
-    Q.rmptr = G.L[l0 - 1].mptr.ptr<int>();
-    Q.rmidx = G.L[l0 - 1].midx.ptr<int>();
-    Q.rt = G.L[l0 - 1].t.ptr<float>();
-    Q.out0 = l0 == G.nl - 1 ? G.L[l0].x.ptr<float>() : G.L[l0].t.ptr<float>();
-    int off = 0;
-    for (int k = 0; k < Q.ns; k++) {
-        AmgLevel& L = G.L[l0 + k];
-        const bool last = l0 + k == G.nl - 1;
-        SmallLevel& q = Q.L[k];
-        q.n = L.n;
-        q.off = off;
-        q.nnzb = L.nnzb;
-        q.nNext = last ? 0 : G.L[l0 + k + 1].n;
-        off += L.n;
-        q.rp = L.rp.ptr<int>();
-        q.ci = L.ci.ptr<int>();
-        q.val = L.val.ptr<float>();
-        q.Dinv = L.Dinv.ptr<float>();
-        q.agg = last ? nullptr : L.agg.ptr<int>();
-        q.mptr = last ? nullptr : L.mptr.ptr<int>();
-        q.midx = last ? nullptr : L.midx.ptr<int>();
+    {
+        const AmgLevel& F = G.L[l0 - 1];
+        float* out0 = l0 == G.nl - 1 ? G.L[l0].x.ptr<float>() : G.L[l0].t.ptr<float>();
+        hipLaunchKernelGGL(k_amg_small_dense, dim3(cdiv(G.smallDim, DB / 64)), dim3(DB), 0, s, G.L[l0].n, G.smallDim,
+                           F.mptr.ptr<int>(), F.midx.ptr<int>(), F.t.ptr<float>(), G.smallOp.ptr<float>(), out0);
     }
-    hipLaunchKernelGGL(k_amg_small_cycle, dim3(1), dim3(SB), 0, s, Q);
     // up: prolongate and post-smooth; the result of level l is in L[l].t (coarsest: x)
     for (int l = l0 - 1; l >= 0; l--) {
         AmgLevel& L = G.L[l];

@@ -83,6 +83,11 @@ struct AmgSys {
     std::vector<AmgLevel> L;
     DBuf work;  // double 2 m^2: Gauss-Jordan of the coarsest operator (m = 3 n of the last level)
     DBuf inv;   // float m^2: its inverse
+    // The small levels' sub-cycle (levels l0.. of at most 512 nodes, from the restricted
+    // residual of level l0 to level l0's result) as one dense matrix, rebuilt per solve:
+    // row-major, dim = 3 n(l0) rows and columns
+    DBuf smallOp;
+    int smallDim = 0, smallL0 = 0;
     // A_static P of level 0 in sliced ELL (row i, one slot per distinct aggregate of
     // its columns): apTarget[level-0 slot] = the AP slot it sums into (255: none)
     DBuf apTarget, apSliceOff, apSliceW, apCols, apVals;
