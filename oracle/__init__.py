@@ -58,6 +58,7 @@ def lib():
                                    C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_double),
                                    C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.or_find_impacts.argtypes = [vp, C.c_float, C.POINTER(ImpactRec), C.c_int, C.POINTER(C.c_int)]
+        L.or_find_impacts_ref32.argtypes = [vp, C.c_float, C.POINTER(ImpactRec), C.c_int, C.POINTER(C.c_int)]
         L.or_independent_impacts.argtypes = [vp, C.POINTER(ImpactRec), C.c_int, C.c_int, C.POINTER(ImpactRec),
                                              C.POINTER(C.c_int)]
         L.or_collision_optimize.argtypes = [vp, C.POINTER(ImpactRec), C.c_int, C.c_float, C.c_int, C.c_float,
@@ -376,6 +377,15 @@ class OracleScene:
         _check(self.L.or_collision_optimize(self.h, a.ctypes.data_as(C.POINTER(ImpactRec)), len(a), th, deform,
                                             obstacle_mass, C.byref(it)), "collision_optimize")
         return it.value
+
+    def impacts_ref32(self, thickness, cap=1 << 20):
+        """Impacts of the reference's own FP32 CCD (float arithmetic and its Newton root finder) on
+        the same candidates, deduplicated and sorted like impacts()."""
+        out = np.zeros(cap, dtype=IMPACT_DTYPE)
+        n = C.c_int()
+        _check(self.L.or_find_impacts_ref32(self.h, thickness, out.ctypes.data_as(C.POINTER(ImpactRec)), cap,
+                                            C.byref(n)), "find_impacts_ref32")
+        return out[:n.value].copy()
 
     def independent_impacts(self, imps, deform):
         a = np.ascontiguousarray(imps, dtype=IMPACT_DTYPE)

@@ -602,6 +602,90 @@ inline bool check_impact(int type, const V3<double> X0[4], const V3<double> X1[4
     return false;
 }
 
+// ---- the reference's own FP32 CCD (src/CollisionHelper.cu:3-109), kept in float.  The
+// product and the checks above run the cubic in FP64 with isolated roots; this restatement only
+// measures how often the FP32 reference path decides differently (a divergence statistic).
+inline float newton_root_f(float a, float b, float c, float d, float x0, int dir) {
+    if (dir != 0) {
+        float y0 = d + x0 * (c + x0 * (b + x0 * a));
+        float ddy0 = 2.0f * b + x0 * (6.0f * a);
+        x0 += dir * std::sqrt(tabs(2.0f * y0 / ddy0));
+    }
+    for (int iter = 0; iter < 100; iter++) {
+        float y = d + x0 * (c + x0 * (b + x0 * a));
+        float dy = c + x0 * (2.0f * b + 3.0f * x0 * a);
+        if (dy == 0) return x0;
+        float x1 = x0 - y / dy;
+        if (tabs(x0 - x1) < 1e-6f) return x0;
+        x0 = x1;
+    }
+    return x0;
+}
+inline int solve_quadratic_f(float a, float b, float c, float x[2]) {
+    float d = b * b - 4.0f * a * c;
+    if (d < 0.0f) { x[0] = -b / (2.0f * a); return 0; }
+    float q = -(b + tsign(b) * std::sqrt(d)) * 0.5f;
+    int i = 0;
+    if (tabs(a) > 1e-12f * tabs(q)) x[i++] = q / a;
+    if (tabs(q) > 1e-12f * tabs(c)) x[i++] = c / q;
+    if (i == 2 && x[0] > x[1]) std::swap(x[0], x[1]);
+    return i;
+}
+inline int solve_cubic_f(float a, float b, float c, float d, float x[3]) {
+    float xc[2] = {0.0f, 0.0f};
+    int n = solve_quadratic_f(3.0f * a, 2.0f * b, c, xc);
+    if (n == 0) { x[0] = newton_root_f(a, b, c, d, xc[0], 0); return 1; }
+    if (n == 1) return solve_quadratic_f(b, c, d, x);
+    float yc[2] = {d + xc[0] * (c + xc[0] * (b + xc[0] * a)), d + xc[1] * (c + xc[1] * (b + xc[1] * a))};
+    int i = 0;
+    if (yc[0] * a >= 0.0f) x[i++] = newton_root_f(a, b, c, d, xc[0], -1);
+    if (yc[0] * yc[1] <= 0.0f) {
+        int closer = tabs(yc[0]) < tabs(yc[1]) ? 0 : 1;
+        x[i++] = newton_root_f(a, b, c, d, xc[closer], closer == 0 ? 1 : -1);
+    }
+    if (yc[1] * a <= 0.0f) x[i++] = newton_root_f(a, b, c, d, xc[1], 1);
+    return i;
+}
+inline bool check_impact_ref32(int type, const V3<float> X0[4], const V3<float> X1[4], Impact& imp) {
+    V3<float> x0 = X0[0];
+    V3<float> v0 = X1[0] - x0;
+    V3<float> x1 = X0[1] - x0, x2 = X0[2] - x0, x3 = X0[3] - x0;
+    V3<float> v1 = (X1[1] - X0[1]) - v0, v2 = (X1[2] - X0[2]) - v0, v3 = (X1[3] - X0[3]) - v0;
+    float a0 = mixed(x1, x2, x3);
+    float a1 = mixed(v1, x2, x3) + mixed(x1, v2, x3) + mixed(x1, x2, v3);
+    float a2 = mixed(x1, v2, v3) + mixed(v1, x2, v3) + mixed(v1, v2, x3);
+    float a3 = mixed(v1, v2, v3);
+    float t[3];
+    int ns = solve_cubic_f(a3, a2, a1, a0, t);
+    for (int i = 0; i < ns; i++) {
+        if (t[i] < 0.0f || t[i] > 1.0f) continue;
+        V3<float> p[4];
+        for (int k = 0; k < 4; k++) p[k] = X0[k] + t[i] * (X1[k] - X0[k]);
+        V3<float> n; float w[4], d;
+        bool ok, inside;
+        if (type == 0) {
+            ok = signed_vf_distance(p[0], p[1], p[2], p[3], n, w, d);
+            inside = ok && (tmin3(-w[1], -w[2], -w[3]) >= -1e-6f);
+        } else {
+            ok = signed_ee_distance(p[0], p[1], p[2], p[3], n, w, d);
+            inside = ok && (tmin4(w[0], w[1], -w[2], -w[3]) >= -1e-6f);
+        }
+        if (!ok) continue;
+        if (n.dot(w[1] * v1 + w[2] * v2 + w[3] * v3) > 0.0f) n = -n;
+        if (tabs(d) < 1e-6f && inside) {
+            imp.t = t[i];
+            for (int k = 0; k < 4; k++) imp.w[k] = w[k];
+            for (int k = 0; k < 3; k++) imp.n[k] = n[k];
+            return true;
+        }
+    }
+    return false;
+}
+template <class T> void load4f(const Scene<T>& S, const int nd[4], V3<float> X0[4], V3<float> X1[4]) {
+    for (int k = 0; k < 4; k++)
+        for (int c = 0; c < 3; c++) { X0[k][c] = (float)S.x0[nd[k]][c]; X1[k][c] = (float)S.x[nd[k]][c]; }
+}
+
 template <class T> void load4(const Scene<T>& S, const int nd[4], V3<double> X0[4], V3<double> X1[4]) {
     for (int k = 0; k < 4; k++)
         for (int c = 0; c < 3; c++) { X0[k][c] = (double)S.x0[nd[k]][c]; X1[k][c] = (double)S.x[nd[k]][c]; }
@@ -610,35 +694,52 @@ template <class T> void load4(const Scene<T>& S, const int nd[4], V3<double> X0[
 inline int64_t impact_key(int type, int a, int b) { return ((int64_t)type << 62) | ((int64_t)a << 31) | (int64_t)b; }
 
 // checkVertexFaceImpact (src/CollisionHelper.cu:111-122)
-template <class T> void vf_impact(const Scene<T>& S, int node, int f, T th, vector<Impact>& out) {
+template <class T> void vf_impact(const Scene<T>& S, int node, int f, T th, vector<Impact>& out, bool ref32 = false) {
     int n0 = S.fn[f][0], n1 = S.fn[f][1], n2 = S.fn[f][2];
     if (node == n0 || node == n1 || node == n2) return;
     if (!node_box(S, node, true).overlap(face_box(S, f, true), th)) return;
     Impact imp; imp.type = 0;
     imp.nodes[0] = node; imp.nodes[1] = n0; imp.nodes[2] = n1; imp.nodes[3] = n2;
-    V3<double> X0[4], X1[4];
-    load4(S, imp.nodes, X0, X1);
-    if (check_impact(0, X0, X1, imp)) { imp.key = impact_key(0, node, f); out.push_back(imp); }
+    bool hit;
+    if (ref32) {
+        V3<float> X0[4], X1[4];
+        load4f(S, imp.nodes, X0, X1);
+        hit = check_impact_ref32(0, X0, X1, imp);
+    } else {
+        V3<double> X0[4], X1[4];
+        load4(S, imp.nodes, X0, X1);
+        hit = check_impact(0, X0, X1, imp);
+    }
+    if (hit) { imp.key = impact_key(0, node, f); out.push_back(imp); }
 }
 // checkEdgeEdgeImpact (src/CollisionHelper.cu:124-135); the pair is evaluated in
 // canonical order (lower global edge id first) so that one edge pair has one result.
-template <class T> void ee_impact(const Scene<T>& S, int e0, int e1, T th, vector<Impact>& out) {
+template <class T> void ee_impact(const Scene<T>& S, int e0, int e1, T th, vector<Impact>& out, bool ref32 = false) {
     if (e1 < e0) std::swap(e0, e1);
     int a = S.en[e0][0], b = S.en[e0][1], c = S.en[e1][0], d = S.en[e1][1];
     if (a == c || a == d || b == c || b == d) return;
     if (!edge_box(S, e0, true).overlap(edge_box(S, e1, true), th)) return;
     Impact imp; imp.type = 1;
     imp.nodes[0] = a; imp.nodes[1] = b; imp.nodes[2] = c; imp.nodes[3] = d;
-    V3<double> X0[4], X1[4];
-    load4(S, imp.nodes, X0, X1);
-    if (check_impact(1, X0, X1, imp)) { imp.key = impact_key(1, e0, e1); out.push_back(imp); }
+    bool hit;
+    if (ref32) {
+        V3<float> X0[4], X1[4];
+        load4f(S, imp.nodes, X0, X1);
+        hit = check_impact_ref32(1, X0, X1, imp);
+    } else {
+        V3<double> X0[4], X1[4];
+        load4(S, imp.nodes, X0, X1);
+        hit = check_impact(1, X0, X1, imp);
+    }
+    if (hit) { imp.key = impact_key(1, e0, e1); out.push_back(imp); }
 }
 
 inline bool impact_less(const Impact& p, const Impact& q) { return p.t < q.t || (p.t == q.t && p.key < q.key); }
 
 // Simulator::findImpacts (src/Simulator.cu:383-392) + checkImpactsGpu
 // (src/CollisionHelper.cu:137-162), deduplicated and sorted by (t, key).
-template <class T> vector<Impact> find_impacts_with(const Scene<T>& S, const vector<BVH<T>>& bvhs, T th) {
+template <class T> vector<Impact> find_impacts_with(const Scene<T>& S, const vector<BVH<T>>& bvhs, T th,
+                                                    bool ref32 = false) {
     vector<std::pair<int, int>> pairs = collect_pairs(S, bvhs, th);
     const long np = (long)pairs.size(), CH = 4096, nch = (np + CH - 1) / CH;
     vector<vector<Impact>> parts(nch);
@@ -646,10 +747,10 @@ template <class T> vector<Impact> find_impacts_with(const Scene<T>& S, const vec
     for (long ch = 0; ch < nch; ch++)
         for (long p = ch * CH; p < std::min(np, (ch + 1) * CH); p++) {
             int f0 = pairs[p].first, f1 = pairs[p].second;
-            for (int j = 0; j < 3; j++) vf_impact(S, S.fn[f0][j], f1, th, parts[ch]);
-            for (int j = 0; j < 3; j++) vf_impact(S, S.fn[f1][j], f0, th, parts[ch]);
+            for (int j = 0; j < 3; j++) vf_impact(S, S.fn[f0][j], f1, th, parts[ch], ref32);
+            for (int j = 0; j < 3; j++) vf_impact(S, S.fn[f1][j], f0, th, parts[ch], ref32);
             for (int j = 0; j < 3; j++)
-                for (int k = 0; k < 3; k++) ee_impact(S, S.fe[f0][j], S.fe[f1][k], th, parts[ch]);
+                for (int k = 0; k < 3; k++) ee_impact(S, S.fe[f0][j], S.fe[f1][k], th, parts[ch], ref32);
         }
     vector<Impact> all;
     for (auto& part : parts) all.insert(all.end(), part.begin(), part.end());

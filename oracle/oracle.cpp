@@ -495,6 +495,21 @@ int or_find_impacts(void* h, float thickness, cs_impact_rec* out, int cap, int* 
     });
 }
 
+// The impacts of the reference's FP32 CCD (float arithmetic, its whole-line Newton root finder,
+// src/CollisionHelper.cu:3-109) on the same candidates: for the FP32-vs-FP64 divergence measure.
+int or_find_impacts_ref32(void* h, float thickness, cs_impact_rec* out, int cap, int* n) {
+    return dispatch(h, [&](auto& S) {
+        using T = std::decay_t<decltype(S.x[0][0])>;
+        vector<BVH<T>> bvhs;
+        build_all(S, true, bvhs);
+        auto imps = find_impacts_with(S, bvhs, (T)thickness, true);
+        *n = (int)imps.size();
+        if ((int)imps.size() > cap) return (int)CS_E_CAPACITY;
+        for (size_t i = 0; i < imps.size(); i++) impact_out(imps[i], out[i]);
+        return (int)CS_OK;
+    });
+}
+
 int or_independent_impacts(void* h, const cs_impact_rec* in, int n, int deform, cs_impact_rec* out, int* nout) {
     return dispatch(h, [&](auto& S) {
         vector<Impact> v(n);
