@@ -71,6 +71,26 @@ def read_traffic(workload, precond):
     return None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def omp_threads(n):
+    """Sets the OpenMP thread count of this process (the oracle's runtime) and returns the old one."""
+    import ctypes
+    gomp = ctypes.CDLL("libgomp.so.1")
+    old = gomp.omp_get_max_threads()
+    gomp.omp_set_num_threads(int(n))
+    return old
+
+
 def cpu_baseline(spec, steps):
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
@@ -81,8 +101,31 @@ def cpu_baseline(spec, steps):
     dt = time.perf_counter() - t0
     n = spec.n_cloth_nodes
     return {"value": n * steps / dt, "unit": "node-steps/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": "first %d step(s) of %s from its initial state: oracle FP32 restatement (block-Jacobi PCG "
                       "to 1e-5, FP64 CCD), OpenMP on %d threads, %.1f s" % (steps, spec.name, threads, dt)}
+
+
+def cpu_reference_direct(steps=100):
+    """The reference's own CPU solve (one thread, a direct sparse LL^T per step) on the hanging
+    33^2 sheet for its full 100 steps: the oracle's FP32 step with its direct solver (RCM-ordered
+    banded Cholesky in FP64) on one OpenMP thread."""
+    import oracle
+    from clothsimulator_amd import scenes
+    spec = scenes.hang(33)
+    old = omp_threads(1)
+    try:
+        o = oracle.OracleScene(spec)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o.step(direct=True)
+        dt = time.perf_counter() - t0
+    finally:
+        omp_threads(old)
+    n = spec.n_cloth_nodes
+    return {"value": n * steps / dt, "unit": "node-steps/s", "cores": 1, "kind": "port", "cpu_model": cpu_model(),
+            "sample": "%s, all %d steps: oracle FP32 step with a direct sparse Cholesky (RCM + banded, FP64) on "
+                      "one thread, %.1f s" % (spec.name, steps, dt)}
 
 
 def copy_probe(mib):
@@ -136,7 +179,7 @@ def run(workload, precond, steps, warmup, timed):
         sim.kernel_time(CS_KERNEL_EVENT_PAIR, reset=True)
     span = max(steps, spec.steps)
     sel = set(timed_steps(steps, span))
-    iters, contacts, failed = [], [], 0
+    iters, contacts, failed, failures = [], [], 0, []
     elapsed = 0.0
     t_all = time.perf_counter()
     for k in range(span):
@@ -149,12 +192,17 @@ def run(workload, precond, steps, warmup, timed):
             elapsed += dt
             if s != 0:  # a PCG that hit its iteration cap (status 4) counts as a failed step too
                 failed += 1
+                failures.append({"step": k, "status": int(s), "pcg_converged": int(st.pcg_converged),
+                                 "pcg_rel_residual": float(st.pcg_rel_residual), "pcg_iters": int(st.pcg_iters)})
+                print("%s step %d: status %d (pcg converged %d, relative residual %.3g)" % (
+                    precond, k, s, st.pcg_converged, st.pcg_rel_residual), file=sys.stderr, flush=True)
             iters.append(st.pcg_iters)
             contacts.append(st.n_contact_blocks)
         if (k + 1) % 50 == 0:
             print("%s step %d/%d  %.1f s  cg %d" % (precond, k + 1, span, time.perf_counter() - t_all, st.pcg_iters),
                   file=sys.stderr, flush=True)
     out = {"spec": spec, "elapsed": elapsed, "iters": iters, "contacts": contacts, "failed": failed,
+           "failures": failures,
            "value": spec.n_cloth_nodes * len(sel) / elapsed, "ms_per_step": 1e3 * elapsed / len(sel),
            "span": span, "timed": len(sel)}
     if timed:
@@ -171,13 +219,14 @@ def sequential_sample(spec, precond, steps, k, batched_value):
     the same steps: their summed node-steps over their summed wall time is the throughput of
     running the instances one after another."""
     from clothsimulator_amd import Simulator, scenes
-    from clothsimulator_amd._cabi import CS_PRECOND_BLOCK_JACOBI
+    from clothsimulator_amd._cabi import CS_PRECOND_AMG, CS_PRECOND_BLOCK_JACOBI
     n = len(spec.cloths)
+    code = CS_PRECOND_AMG if precond == "amg" else CS_PRECOND_BLOCK_JACOBI
     picks = sorted(set(int(i * n / k) for i in range(k)))
     nodes, secs = 0, 0.0
     for i in picks:
         p = scenes.default_params(dt=spec.params.dt)
-        p.precond = CS_PRECOND_BLOCK_JACOBI
+        p.precond = code
         one = scenes.SceneSpec("instance%d" % i, p, [spec.cloths[i]], [spec.obstacles[i]], steps)
         sim = Simulator(one)
         sim.step()   # first-step allocations outside the timing
@@ -190,12 +239,13 @@ def sequential_sample(spec, precond, steps, k, batched_value):
         print("sequential sample: instance %d (%d nodes) done" % (i, one.n_cloth_nodes), file=sys.stderr, flush=True)
     seq = nodes / secs
     return {"instances_sampled": len(picks), "node_steps_per_s": seq, "batched_speedup": batched_value / seq,
-            "note": "instances run one after another, each on its own context, block-Jacobi PCG"}
+            "note": "instances run one after another, each on its own context, %s PCG" % (
+                "multigrid" if code == CS_PRECOND_AMG else "block-Jacobi")}
 
 
 def summary(r):
     return {"value": r["value"], "ms_per_step": r["ms_per_step"], "cg_iters_mean": float(np.mean(r["iters"])),
-            "cg_iters_max": int(np.max(r["iters"])), "failed_steps": r["failed"]}
+            "cg_iters_max": int(np.max(r["iters"])), "failed_steps": r["failed"], "failures": r["failures"]}
 
 
 def main():
@@ -250,9 +300,9 @@ def main():
     marker_s = (pair_ms / pair_n) * 1e-3 if pair_n else 0.0
     avg_s = (spmv_ms / spmv_n) * 1e-3 - marker_s if spmv_n else float("nan")
     achieved = bytes_per_launch / avg_s / 1e9 if spmv_n else None
-    from clothsimulator_amd._cabi import CS_FLAG_INSTANCES, CS_PRECOND_BLOCK_JACOBI
-    batched = bool(spec.params.flags & CS_FLAG_INSTANCES) and spec.params.precond == CS_PRECOND_BLOCK_JACOBI \
-        and len(spec.cloths) > 1  # the batched solve of independent instances
+    from clothsimulator_amd._cabi import CS_FLAG_INSTANCES, CS_PRECOND_AMG, CS_PRECOND_BLOCK_JACOBI
+    batched = bool(spec.params.flags & CS_FLAG_INSTANCES) and len(spec.cloths) > 1 \
+        and spec.params.precond in (CS_PRECOND_BLOCK_JACOBI, CS_PRECOND_AMG)  # the batched solve of instances
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                 "traffic": read_traffic(a.workload, a.precond),
@@ -286,6 +336,7 @@ def main():
         "cg_iters_jacobi": ms["cg_iters_mean"] if a.precond == "jacobi" else (
             summary(others["jacobi"])["cg_iters_mean"] if "jacobi" in others else None),
         "failed_steps": ms["failed_steps"],
+        "failures": ms["failures"],
         "roofline": roofline,
     }
     for p, r in others.items():
@@ -300,6 +351,8 @@ def main():
     if not a.no_cpu:
         print("cpu baseline: %d step(s) of %s on the host" % (a.cpu_steps, a.workload), file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(scenes.by_name(a.workload), a.cpu_steps)
+        print("cpu reference column: hang33, 100 steps, direct solve on one thread", file=sys.stderr, flush=True)
+        out["cpu_reference_direct"] = cpu_reference_direct()
     print(json.dumps(out))
 
 

@@ -90,11 +90,8 @@ template <class V> void upload_vec(DBuf& d, const std::vector<V>& h, hipStream_t
 // offsets in the level-0 sliced ELL (component c at id + 64 c), else block ids of
 // the finer block CSR (component c at 9 id + c)
 template <bool FROM_ELL>
-__global__ void __launch_bounds__(AB) k_amg_galerkin(int nnzb, const int* __restrict__ cptr,
-                                                     const int* __restrict__ cidx, const float* __restrict__ fine,
-                                                     float* __restrict__ out) {
-    const int k = blockIdx.x * AB + threadIdx.x;
-    if (k >= nnzb) return;
+__device__ __forceinline__ void galerkin_block(int k, const int* __restrict__ cptr, const int* __restrict__ cidx,
+                                               const float* __restrict__ fine, float* __restrict__ out) {
     float s[9];
 #pragma unroll
     for (int c = 0; c < 9; c++) s[c] = 0.0f;
@@ -107,15 +104,20 @@ __global__ void __launch_bounds__(AB) k_amg_galerkin(int nnzb, const int* __rest
     for (int c = 0; c < 9; c++) out[(size_t)k * 9 + c] = s[c];
 }
 
+template <bool FROM_ELL>
+__global__ void __launch_bounds__(AB) k_amg_galerkin(int nnzb, const int* __restrict__ cptr,
+                                                     const int* __restrict__ cidx, const float* __restrict__ fine,
+                                                     float* __restrict__ out) {
+    const int k = blockIdx.x * AB + threadIdx.x;
+    if (k < nnzb) galerkin_block<FROM_ELL>(k, cptr, cidx, fine, out);
+}
+
 // Level-1 diagonal blocks += the contact diagonal blocks of their level-0 nodes (in
 // ascending node order): obstacle contacts and each node's share of a self-contact stiffen
 // its aggregate, so the coarse correction does not move what the contacts hold.
-__global__ void __launch_bounds__(AB) k_amg_contact_diag(int n1, const int* __restrict__ mptr,
-                                                         const int* __restrict__ midx, const int* __restrict__ diag1,
-                                                         const int* __restrict__ cDiag, const float* __restrict__ cVal,
-                                                         float* __restrict__ val1) {
-    const int a = blockIdx.x * AB + threadIdx.x;
-    if (a >= n1) return;
+__device__ __forceinline__ void contact_diag_row(int a, const int* __restrict__ mptr, const int* __restrict__ midx,
+                                                 const int* __restrict__ diag1, const int* __restrict__ cDiag,
+                                                 const float* __restrict__ cVal, float* __restrict__ val1) {
     float s[9];
 #pragma unroll
     for (int c = 0; c < 9; c++) s[c] = 0.0f;
@@ -133,16 +135,28 @@ __global__ void __launch_bounds__(AB) k_amg_contact_diag(int n1, const int* __re
     for (int c = 0; c < 9; c++) d[c] += s[c];
 }
 
-__global__ void __launch_bounds__(AB) k_amg_dinv(int n, const int* __restrict__ diag, const float* __restrict__ val,
-                                                 float* __restrict__ Dinv) {
-    const int i = blockIdx.x * AB + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(AB) k_amg_contact_diag(int n1, const int* __restrict__ mptr,
+                                                         const int* __restrict__ midx, const int* __restrict__ diag1,
+                                                         const int* __restrict__ cDiag, const float* __restrict__ cVal,
+                                                         float* __restrict__ val1) {
+    const int a = blockIdx.x * AB + threadIdx.x;
+    if (a < n1) contact_diag_row(a, mptr, midx, diag1, cDiag, cVal, val1);
+}
+
+__device__ __forceinline__ void dinv_row(int i, const int* __restrict__ diag, const float* __restrict__ val,
+                                         float* __restrict__ Dinv) {
     float a[9], o[9];
 #pragma unroll
     for (int c = 0; c < 9; c++) a[c] = val[(size_t)diag[i] * 9 + c];
     inv3(a, o);
 #pragma unroll
     for (int c = 0; c < 9; c++) Dinv[(size_t)i * 9 + c] = o[c];
+}
+
+__global__ void __launch_bounds__(AB) k_amg_dinv(int n, const int* __restrict__ diag, const float* __restrict__ val,
+                                                 float* __restrict__ Dinv) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i < n) dinv_row(i, diag, val, Dinv);
 }
 
 // Dense inverse of the coarsest operator (m = 3n <= 192, symmetric positive
@@ -155,9 +169,9 @@ __global__ void __launch_bounds__(AB) k_amg_dinv(int n, const int* __restrict__ 
 // pivots are positive Schur-complement diagonals.
 constexpr int CB = 1024;
 constexpr int CT = (3 * AMG_COARSEST * (3 * AMG_COARSEST + 1) / 2 + CB - 1) / CB;  // triangle slots per thread
-__global__ void __launch_bounds__(CB) k_amg_coarse_inverse(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                           const float* __restrict__ val, double* __restrict__ work,
-                                                           float* __restrict__ inv) {
+__device__ __forceinline__ void coarse_inverse(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                               const float* __restrict__ val, double* __restrict__ work,
+                                               float* __restrict__ inv) {
     __shared__ double colc[3 * AMG_COARSEST];
     const int m = 3 * n, tid = threadIdx.x;
     const int T = m * (m + 1) / 2;
@@ -217,6 +231,12 @@ __global__ void __launch_bounds__(CB) k_amg_coarse_inverse(int n, const int* __r
         inv[(size_t)r * m + k] = x;
         inv[(size_t)k * m + r] = x;
     }
+}
+
+__global__ void __launch_bounds__(CB) k_amg_coarse_inverse(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                           const float* __restrict__ val, double* __restrict__ work,
+                                                           float* __restrict__ inv) {
+    coarse_inverse(n, rp, ci, val, work, inv);
 }
 
 // ---------------------------------------------------------------- cycle kernels
@@ -337,12 +357,9 @@ __global__ void __launch_bounds__(AB) k_amg_resid(int n, const int* __restrict__
 // (stored for the own row, formed on the fly for the gathered ones) and
 // t = b - A x.  Coarse levels are small, so the extra gathers hit the cache and
 // one launch is saved.
-__global__ void __launch_bounds__(AB) k_amg_presid(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                   const float* __restrict__ val, const float* __restrict__ Dinv,
-                                                   const float* __restrict__ b, float* __restrict__ x,
-                                                   float* __restrict__ t) {
-    const int i = blockIdx.x * AB + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void presid_row(int i, const int* __restrict__ rp, const int* __restrict__ ci,
+                                           const float* __restrict__ val, const float* __restrict__ Dinv,
+                                           const float* __restrict__ b, float* __restrict__ x, float* __restrict__ t) {
     float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
     const int k0 = rp[i], k1 = rp[i + 1];
     for (int k = k0; k < k1; k += 4) {  // four blocks' loads at a time, summed in order
@@ -371,11 +388,17 @@ __global__ void __launch_bounds__(AB) k_amg_presid(int n, const int* __restrict_
     st3v(t, i, bi - mk(y0, y1, y2));
 }
 
+__global__ void __launch_bounds__(AB) k_amg_presid(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                   const float* __restrict__ val, const float* __restrict__ Dinv,
+                                                   const float* __restrict__ b, float* __restrict__ x,
+                                                   float* __restrict__ t) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i < n) presid_row(i, rp, ci, val, Dinv, b, x, t);
+}
+
 // bc_I = sum of t over the members of aggregate I (ascending)
-__global__ void __launch_bounds__(AB) k_amg_restrict(int nc, const int* __restrict__ mptr, const int* __restrict__ midx,
-                                                     const float* __restrict__ t, float* __restrict__ bc) {
-    const int I = blockIdx.x * AB + threadIdx.x;
-    if (I >= nc) return;
+__device__ __forceinline__ void restrict_row(int I, const int* __restrict__ mptr, const int* __restrict__ midx,
+                                             const float* __restrict__ t, float* __restrict__ bc) {
     V3f s = mk(0.0f, 0.0f, 0.0f);
     const int k0 = mptr[I], k1 = mptr[I + 1];
     for (int k = k0; k < k1; k += 4) {  // four members' loads at a time, summed in order
@@ -389,11 +412,17 @@ __global__ void __launch_bounds__(AB) k_amg_restrict(int nc, const int* __restri
     st3v(bc, I, s);
 }
 
+__global__ void __launch_bounds__(AB) k_amg_restrict(int nc, const int* __restrict__ mptr, const int* __restrict__ midx,
+                                                     const float* __restrict__ t, float* __restrict__ bc) {
+    const int I = blockIdx.x * AB + threadIdx.x;
+    if (I < nc) restrict_row(I, mptr, midx, t, bc);
+}
+
 // x = inv b on the coarsest level: one workgroup of 16 waves, a wave per row,
 // the lanes of a wave over the row (coalesced) and a shuffle reduction
 constexpr int SB = 1024;
-__global__ void __launch_bounds__(SB) k_amg_coarse_solve(int m, const float* __restrict__ inv,
-                                                         const float* __restrict__ b, float* __restrict__ x) {
+__device__ __forceinline__ void coarse_solve(int m, const float* __restrict__ inv, const float* __restrict__ b,
+                                             float* __restrict__ x) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int r = wave; r < m; r += SB / 64) {
         float s = 0.0f;
@@ -404,17 +433,29 @@ __global__ void __launch_bounds__(SB) k_amg_coarse_solve(int m, const float* __r
     }
 }
 
+__global__ void __launch_bounds__(SB) k_amg_coarse_solve(int m, const float* __restrict__ inv,
+                                                         const float* __restrict__ b, float* __restrict__ x) {
+    coarse_solve(m, inv, b, x);
+}
+
 // out = x' + w D^-1 (b - A x'), x' = x + e_agg, on a block-CSR level
+__device__ __forceinline__ void post_row(int i, const int* __restrict__ rp, const int* __restrict__ ci,
+                                         const float* __restrict__ val, const float* __restrict__ Dinv,
+                                         const int* __restrict__ agg, const float* __restrict__ e,
+                                         const float* __restrict__ b, const float* __restrict__ x,
+                                         float* __restrict__ out) {
+    const V3f ax = rowc<true>(i, rp, ci, val, x, agg, e);
+    const V3f xi = ld3v(x, i) + ld3v(e, agg[i]);
+    st3v(out, i, xi + AMG_OMEGA * mat3v(Dinv + 9 * (size_t)i, ld3v(b, i) - ax));
+}
+
 __global__ void __launch_bounds__(AB) k_amg_post(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                                  const float* __restrict__ val, const float* __restrict__ Dinv,
                                                  const int* __restrict__ agg, const float* __restrict__ e,
                                                  const float* __restrict__ b, const float* __restrict__ x,
                                                  float* __restrict__ out) {
     const int i = blockIdx.x * AB + threadIdx.x;
-    if (i >= n) return;
-    const V3f ax = rowc<true>(i, rp, ci, val, x, agg, e);
-    const V3f xi = ld3v(x, i) + ld3v(e, agg[i]);
-    st3v(out, i, xi + AMG_OMEGA * mat3v(Dinv + 9 * (size_t)i, ld3v(b, i) - ax));
+    if (i < n) post_row(i, rp, ci, val, Dinv, agg, e, b, x, out);
 }
 
 // ---- the small levels in one workgroup ----
@@ -648,13 +689,10 @@ __global__ void __launch_bounds__(DB) k_amg_small_dense(int n0, int dim, const i
 
 // AP = A_static P of level 0, per step: each AP slot sums the level-0 blocks of
 // its row whose column lies in its aggregate (slot order, fixed).
-__global__ void __launch_bounds__(AB) k_amg_ap_build(int nn, const int* __restrict__ sliceOff,
-                                                     const int* __restrict__ sliceW, const float* __restrict__ vals,
-                                                     const uint8_t* __restrict__ target,
-                                                     const int* __restrict__ apSliceOff,
-                                                     const int* __restrict__ apSliceW, float* __restrict__ apVals) {
-    const int i = blockIdx.x * AB + threadIdx.x;
-    if (i >= nn) return;
+__device__ __forceinline__ void ap_build_row(int i, const int* __restrict__ sliceOff, const int* __restrict__ sliceW,
+                                             const float* __restrict__ vals, const uint8_t* __restrict__ target,
+                                             const int* __restrict__ apSliceOff, const int* __restrict__ apSliceW,
+                                             float* __restrict__ apVals) {
     const int s = i / SLICE, lane = i % SLICE;
     const int W = sliceW[s], WA = apSliceW[s];
     const size_t off = (size_t)sliceOff[s], aoff = (size_t)apSliceOff[s];
@@ -672,6 +710,15 @@ __global__ void __launch_bounds__(AB) k_amg_ap_build(int nn, const int* __restri
 #pragma unroll
         for (int c = 0; c < 9; c++) o[c * SLICE] = a[c];
     }
+}
+
+__global__ void __launch_bounds__(AB) k_amg_ap_build(int nn, const int* __restrict__ sliceOff,
+                                                     const int* __restrict__ sliceW, const float* __restrict__ vals,
+                                                     const uint8_t* __restrict__ target,
+                                                     const int* __restrict__ apSliceOff,
+                                                     const int* __restrict__ apSliceW, float* __restrict__ apVals) {
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i < nn) ap_build_row(i, sliceOff, sliceW, vals, target, apSliceOff, apSliceW, apVals);
 }
 
 // Level 0 post-smoothing from the stored residual t = r - A x of the same cycle:
@@ -1079,6 +1126,284 @@ void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int n
                                L.val.ptr<float>(), L.Dinv.ptr<float>(), L.agg.ptr<int>(), e, L.b.ptr<float>(),
                                L.x.ptr<float>(), L.t.ptr<float>());
     }
+    CS_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- batched hierarchies
+// Independent instances (CS_FLAG_INSTANCES) solve together.  Every cloth keeps its own
+// hierarchy (amg_build); one launch covers level l of every cloth that has one, so a cycle
+// costs a few launches per level whatever the number of cloths.  Job = (cloth, level l >= 1);
+// a block map gives each workgroup its (job, block).  Each cloth's coarsest level is solved
+// with its own dense inverse (one workgroup per cloth), and every level's result, the
+// coarsest's too, is left in its t.  Level 0 (the cloth's own system, sliced ELL) belongs to
+// the batched PCG kernels.
+
+namespace {
+
+struct BLev {
+    int n, nnzb, coarsest, pad;
+    const int *rp, *ci, *diag, *cptr, *cidx;
+    const float* fine;         // Galerkin source: the level-0 sliced-ELL values (l = 1) or level l-1's blocks
+    float *val, *Dinv;
+    const int *fmptr, *fmidx;  // level l-1's members of each node here
+    const float* ft;           // level l-1's residual (the restriction's source)
+    float *b, *x, *t;          // right-hand side, pre-smoothed iterate, residual / result
+    const int* agg;            // here -> level l+1 (not coarsest)
+    const float* e;            // level l+1's result (not coarsest)
+    const int* cDiag;          // level 1: each level-0 node's contact diagonal block (-1: none)
+    const float* cVal;         // level 1: the cloth's contact blocks (null: none)
+    double* work;              // coarsest: Gauss-Jordan workspace
+    float* inv;                // coarsest: dense inverse
+};
+
+struct BAp {  // level 0 of one cloth: A_static P
+    int nn, pad;
+    const int *sliceOff, *sliceW, *apSliceOff, *apSliceW;
+    const float* vals;
+    const uint8_t* target;
+    float* apVals;
+};
+
+__device__ __forceinline__ int2 job_block(const int2* __restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    return make_int2(__builtin_amdgcn_readfirstlane(m.x), __builtin_amdgcn_readfirstlane(m.y));
+}
+
+template <bool FROM_ELL>
+__global__ void __launch_bounds__(AB) k_bamg_galerkin(const BLev* __restrict__ jobs, const int2* __restrict__ map) {
+    const int2 m = job_block(map);
+    const BLev& L = jobs[m.x];
+    const int k = m.y * AB + threadIdx.x;
+    if (k < L.nnzb) galerkin_block<FROM_ELL>(k, L.cptr, L.cidx, L.fine, L.val);
+}
+
+__global__ void __launch_bounds__(AB) k_bamg_contact_diag(const BLev* __restrict__ jobs, const int2* __restrict__ map) {
+    const int2 m = job_block(map);
+    const BLev& L = jobs[m.x];
+    const int a = m.y * AB + threadIdx.x;
+    if (a < L.n && L.cVal) contact_diag_row(a, L.fmptr, L.fmidx, L.diag, L.cDiag, L.cVal, L.val);
+}
+
+__global__ void __launch_bounds__(AB) k_bamg_dinv(const BLev* __restrict__ jobs, const int2* __restrict__ map) {
+    const int2 m = job_block(map);
+    const BLev& L = jobs[m.x];
+    const int i = m.y * AB + threadIdx.x;
+    if (i < L.n) dinv_row(i, L.diag, L.val, L.Dinv);
+}
+
+__global__ void __launch_bounds__(AB) k_bamg_ap_build(const BAp* __restrict__ jobs, const int2* __restrict__ map) {
+    const int2 m = job_block(map);
+    const BAp& J = jobs[m.x];
+    const int i = m.y * AB + threadIdx.x;
+    if (i < J.nn) ap_build_row(i, J.sliceOff, J.sliceW, J.vals, J.target, J.apSliceOff, J.apSliceW, J.apVals);
+}
+
+__global__ void __launch_bounds__(CB) k_bamg_coarse_inverse(const BLev* __restrict__ jobs,
+                                                            const int2* __restrict__ map) {
+    const BLev& L = jobs[job_block(map).x];
+    coarse_inverse(L.n, L.rp, L.ci, L.val, L.work, L.inv);
+}
+
+__global__ void __launch_bounds__(AB) k_bamg_restrict(const BLev* __restrict__ jobs, const int2* __restrict__ map) {
+    const int2 m = job_block(map);
+    const BLev& L = jobs[m.x];
+    const int I = m.y * AB + threadIdx.x;
+    if (I < L.n) restrict_row(I, L.fmptr, L.fmidx, L.ft, L.b);
+}
+
+__global__ void __launch_bounds__(AB) k_bamg_presid(const BLev* __restrict__ jobs, const int2* __restrict__ map) {
+    const int2 m = job_block(map);
+    const BLev& L = jobs[m.x];
+    const int i = m.y * AB + threadIdx.x;
+    if (i < L.n) presid_row(i, L.rp, L.ci, L.val, L.Dinv, L.b, L.x, L.t);
+}
+
+__global__ void __launch_bounds__(SB) k_bamg_coarse_solve(const BLev* __restrict__ jobs, const int2* __restrict__ map) {
+    const BLev& L = jobs[job_block(map).x];
+    coarse_solve(3 * L.n, L.inv, L.b, L.t);
+}
+
+__global__ void __launch_bounds__(AB) k_bamg_post(const BLev* __restrict__ jobs, const int2* __restrict__ map) {
+    const int2 m = job_block(map);
+    const BLev& L = jobs[m.x];
+    const int i = m.y * AB + threadIdx.x;
+    if (i < L.n) post_row(i, L.rp, L.ci, L.val, L.Dinv, L.agg, L.e, L.b, L.x, L.t);
+}
+
+}  // namespace
+
+// The job maps of the cloths' hierarchies (once per context: the meshes never change).
+const std::vector<int>& bamg_build(cs_ctx* C) {
+    auto& B = C->bamg;
+    if (B.ncl == C->nCloths) return B.usable;
+    const int ncl = C->nCloths;
+    B.usable.assign(ncl, 0);
+    B.nl = 0;
+    int sizes[32];
+    for (int c = 0; c < ncl; c++) {
+        const int nl = amg_levels(C, c, sizes, 32);
+        B.usable[c] = nl > 1;
+        if (nl > 1) B.nl = std::max(B.nl, nl);
+    }
+    std::vector<int2> map;
+    auto seg = [&](auto&& emit) {
+        cs_ctx::BatchAmg::Seg g;
+        g.off = (int)map.size();
+        emit();
+        g.n = (int)map.size() - g.off;
+        return g;
+    };
+    B.jobOff.assign(std::max(B.nl, 1) + 1, 0);
+    B.rows.assign(B.nl, {});
+    B.sparse.assign(B.nl, {});
+    B.blocks.assign(B.nl, {});
+    int job = 0;
+    std::vector<int> coarsestJob(ncl, -1);
+    for (int l = 1; l < B.nl; l++) {
+        B.jobOff[l] = job;
+        std::vector<int> cl;  // cloths with a level l, in order: job jobOff[l] + k is cl[k]
+        for (int c = 0; c < ncl; c++)
+            if (B.usable[c] && C->sys[c].amg.nl > l) cl.push_back(c);
+        B.rows[l] = seg([&] {
+            for (int k = 0; k < (int)cl.size(); k++)
+                for (int b = 0; b < cdiv(C->sys[cl[k]].amg.L[l].n, AB); b++) map.push_back(make_int2(job + k, b));
+        });
+        B.sparse[l] = seg([&] {
+            for (int k = 0; k < (int)cl.size(); k++)
+                if (C->sys[cl[k]].amg.nl - 1 > l)
+                    for (int b = 0; b < cdiv(C->sys[cl[k]].amg.L[l].n, AB); b++) map.push_back(make_int2(job + k, b));
+        });
+        B.blocks[l] = seg([&] {
+            for (int k = 0; k < (int)cl.size(); k++)
+                for (int b = 0; b < cdiv(C->sys[cl[k]].amg.L[l].nnzb, AB); b++) map.push_back(make_int2(job + k, b));
+        });
+        for (int k = 0; k < (int)cl.size(); k++)
+            if (C->sys[cl[k]].amg.nl - 1 == l) coarsestJob[cl[k]] = job + k;
+        job += (int)cl.size();
+    }
+    B.jobOff[std::max(B.nl, 1)] = job;
+    if (B.nl >= 1) B.jobOff[0] = 0;
+    int u = 0;
+    B.ap = seg([&] {
+        for (int c = 0; c < ncl; c++)
+            if (B.usable[c]) {
+                for (int b = 0; b < cdiv(C->sys[c].nn, AB); b++) map.push_back(make_int2(u, b));
+                u++;
+            }
+    });
+    B.coarse = seg([&] {
+        for (int c = 0; c < ncl; c++)
+            if (B.usable[c]) map.push_back(make_int2(coarsestJob[c], 0));
+    });
+    int2* dm = B.maps.as<int2>(std::max<size_t>(1, map.size()));
+    if (!map.empty())
+        CS_HIP(hipMemcpyAsync(dm, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice, C->stream));
+    CS_HIP(hipStreamSynchronize(C->stream));  // the host map is released on return
+    B.ncl = ncl;
+    return B.usable;
+}
+
+// Per solve: every usable cloth's Galerkin operators, their diagonal inverses, level 0's
+// A P and the coarsest inverses, as in amg_prepare, one launch per level for all cloths.
+void bamg_prepare(cs_ctx* C) {
+    auto& B = C->bamg;
+    bamg_build(C);
+    if (B.nl < 2) return;
+    hipStream_t s = C->stream;
+    const int ncl = C->nCloths;
+    // job tables, rebuilt per solve (the contact blocks move between steps)
+    std::vector<BLev> jobs(B.jobOff[B.nl]);
+    std::vector<BAp> ap;
+    for (int l = 1; l < B.nl; l++) {
+        int k = B.jobOff[l];
+        for (int c = 0; c < ncl; c++) {
+            ClothSys& S = C->sys[c];
+            AmgSys& G = S.amg;
+            if (!B.usable[c] || G.nl <= l) continue;
+            AmgLevel& L = G.L[l];
+            AmgLevel& F = G.L[l - 1];
+            BLev& J = jobs[k++];
+            std::memset(&J, 0, sizeof(J));
+            J.n = L.n;
+            J.nnzb = L.nnzb;
+            J.coarsest = l == G.nl - 1;
+            J.rp = L.rp.ptr<int>();
+            J.ci = L.ci.ptr<int>();
+            J.diag = L.diag.ptr<int>();
+            J.cptr = L.cptr.ptr<int>();
+            J.cidx = L.cidx.ptr<int>();
+            J.fine = l == 1 ? S.vals.ptr<float>() : F.val.ptr<float>();
+            J.val = L.val.ptr<float>();
+            J.Dinv = L.Dinv.ptr<float>();
+            J.fmptr = F.mptr.ptr<int>();
+            J.fmidx = F.midx.ptr<int>();
+            J.ft = F.t.ptr<float>();
+            J.b = L.b.ptr<float>();
+            J.x = L.x.ptr<float>();
+            J.t = L.t.ptr<float>();
+            J.agg = J.coarsest ? nullptr : L.agg.ptr<int>();
+            J.e = J.coarsest ? nullptr : G.L[l + 1].t.ptr<float>();
+            J.cDiag = S.cDiag.ptr<int>();
+            J.cVal = S.nContact > 0 ? S.cVal.ptr<float>() : nullptr;
+            J.work = G.work.ptr<double>();
+            J.inv = G.inv.ptr<float>();
+        }
+    }
+    for (int c = 0; c < ncl; c++) {
+        if (!B.usable[c]) continue;
+        ClothSys& S = C->sys[c];
+        AmgSys& G = S.amg;
+        BAp J;
+        std::memset(&J, 0, sizeof(J));
+        J.nn = S.nn;
+        J.sliceOff = S.sliceOff.ptr<int>();
+        J.sliceW = S.sliceW.ptr<int>();
+        J.apSliceOff = G.apSliceOff.ptr<int>();
+        J.apSliceW = G.apSliceW.ptr<int>();
+        J.vals = S.vals.ptr<float>();
+        J.target = G.apTarget.ptr<uint8_t>();
+        J.apVals = G.apVals.ptr<float>();
+        ap.push_back(J);
+    }
+    BLev* dj = B.jobs.as<BLev>(jobs.size());
+    BAp* da = B.apJobs.as<BAp>(ap.size());
+    CS_HIP(hipMemcpyAsync(dj, jobs.data(), sizeof(BLev) * jobs.size(), hipMemcpyHostToDevice, s));
+    CS_HIP(hipMemcpyAsync(da, ap.data(), sizeof(BAp) * ap.size(), hipMemcpyHostToDevice, s));
+    const int2* map = B.maps.ptr<int2>();
+    for (int l = 1; l < B.nl; l++) {
+        if (B.blocks[l].n) {
+            if (l == 1)
+                hipLaunchKernelGGL(k_bamg_galerkin<true>, dim3(B.blocks[l].n), dim3(AB), 0, s, dj, map + B.blocks[l].off);
+            else
+                hipLaunchKernelGGL(k_bamg_galerkin<false>, dim3(B.blocks[l].n), dim3(AB), 0, s, dj, map + B.blocks[l].off);
+        }
+        if (l == 1 && B.rows[l].n)
+            hipLaunchKernelGGL(k_bamg_contact_diag, dim3(B.rows[l].n), dim3(AB), 0, s, dj, map + B.rows[l].off);
+        if (B.rows[l].n) hipLaunchKernelGGL(k_bamg_dinv, dim3(B.rows[l].n), dim3(AB), 0, s, dj, map + B.rows[l].off);
+        CS_LAUNCH_CHECK();
+    }
+    if (B.ap.n) hipLaunchKernelGGL(k_bamg_ap_build, dim3(B.ap.n), dim3(AB), 0, s, da, map + B.ap.off);
+    if (B.coarse.n) hipLaunchKernelGGL(k_bamg_coarse_inverse, dim3(B.coarse.n), dim3(CB), 0, s, dj, map + B.coarse.off);
+    CS_LAUNCH_CHECK();
+    // the host tables are released on return
+    CS_HIP(hipStreamSynchronize(s));
+}
+
+// Levels >= 1 of one cycle for every usable cloth: from level 0's residual (its t, written
+// by the batched PCG) to level 1's result (its t), which level 0's post-smoothing prolongs.
+void bamg_cycle(cs_ctx* C) {
+    auto& B = C->bamg;
+    hipStream_t s = C->stream;
+    const BLev* dj = B.jobs.ptr<BLev>();
+    const int2* map = B.maps.ptr<int2>();
+    for (int l = 1; l < B.nl; l++) {
+        if (B.rows[l].n) hipLaunchKernelGGL(k_bamg_restrict, dim3(B.rows[l].n), dim3(AB), 0, s, dj, map + B.rows[l].off);
+        if (B.sparse[l].n)
+            hipLaunchKernelGGL(k_bamg_presid, dim3(B.sparse[l].n), dim3(AB), 0, s, dj, map + B.sparse[l].off);
+    }
+    if (B.coarse.n) hipLaunchKernelGGL(k_bamg_coarse_solve, dim3(B.coarse.n), dim3(SB), 0, s, dj, map + B.coarse.off);
+    for (int l = B.nl - 2; l >= 1; l--)
+        if (B.sparse[l].n) hipLaunchKernelGGL(k_bamg_post, dim3(B.sparse[l].n), dim3(AB), 0, s, dj, map + B.sparse[l].off);
     CS_LAUNCH_CHECK();
 }
 

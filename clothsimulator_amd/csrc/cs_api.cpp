@@ -120,13 +120,15 @@ int physics_step(cs_ctx* C, cs_stats* st) {
     local.ms_proximity = T.lap(s);
     int status = CS_OK;
     local.pcg_converged = 1;
-    if ((C->P.flags & CS_FLAG_INSTANCES) && C->P.precond == CS_PRECOND_BLOCK_JACOBI && C->nCloths > 1) {
+    const bool batched = (C->P.flags & CS_FLAG_INSTANCES) && C->nCloths > 1 &&
+                         (C->P.precond == CS_PRECOND_BLOCK_JACOBI || C->P.precond == CS_PRECOND_AMG);
+    if (batched) {
         // independent instances: assemble every cloth, then one batched solve
         local.n_contact_blocks += assemble_all(C);
         local.ms_assemble += T.lap(s);
         std::vector<int> its(C->nCloths), conv(C->nCloths);
         std::vector<float> rel(C->nCloths);
-        pcg_solve_batched(C, its.data(), rel.data(), conv.data());
+        pcg_solve_batched(C, its.data(), rel.data(), conv.data(), C->P.precond == CS_PRECOND_AMG);
         for (int c = 0; c < C->nCloths; c++) {
             local.pcg_iters += its[c];
             local.pcg_rel_residual = std::max(local.pcg_rel_residual, rel[c]);

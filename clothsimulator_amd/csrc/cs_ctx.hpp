@@ -236,6 +236,18 @@ struct cs_ctx {
     } batchTune;
     void* hBatch = nullptr;  // pinned readback of per-cloth data (scalars, contact bounds)
     size_t hBatchCap = 0;    // bytes
+    // batched multigrid of the independent cloths (cs_amg.hip): every cloth keeps its own
+    // hierarchy; one launch covers level l of every cloth that has one
+    struct BatchAmg {
+        int ncl = -1;                  // cloth count the maps were built for (-1: not built)
+        int nl = 0;                    // most levels of any cloth
+        std::vector<int> usable;       // per cloth: 1 if its hierarchy is used
+        std::vector<int> jobOff;       // first job of level l (l >= 1), jobOff[nl] = job count
+        struct Seg { int off = 0, n = 0; };
+        std::vector<Seg> rows, sparse, blocks;  // per level: block-map segments (all jobs / non-coarsest / Galerkin blocks)
+        Seg ap, coarse;                // level-0 A P of every cloth; coarsest job of every cloth
+        cs::DBuf jobs, apJobs, maps;   // BLev[jobOff[nl]], BAp[usable], int2 map entries
+    } bamg;
     cs::DBuf asmJobs;        // per-cloth table of the batched assembly
     cs::DBuf gcCol, gcVal, gcBc, gcDiag, gcU0;  // batch-wide contact reduction
     // CCD / impacts

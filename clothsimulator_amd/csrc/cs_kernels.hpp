@@ -31,10 +31,17 @@ void* pinned_batch(cs_ctx* C, size_t bytes);  // the context's pinned readback b
 
 // PCG (cs_pcg.hip)
 int pcg_solve(cs_ctx* C, int c, float* relres, int* converged);
-// block-Jacobi PCG of every cloth at once (independent instances); per-cloth results
-void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged);
+// PCG of every cloth at once (independent instances); per-cloth results.  amg: each cloth whose
+// hierarchy is usable is preconditioned by its multigrid cycle, the others by block Jacobi.
+void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged, bool amg = false);
 // aggregation multigrid preconditioner (cs_amg.hip)
 constexpr float AMG_OMEGA = 0.6f;  // damping of the block-Jacobi smoother
+// batched multigrid of the independent cloths: bamg_build returns, per cloth, 1 if its
+// hierarchy is used; bamg_prepare is the per-solve setup of all of them; bamg_cycle runs
+// levels >= 1 of one cycle (level 0's residual in, level 1's result out: amg.L[0].t, L[1].t)
+const std::vector<int>& bamg_build(cs_ctx* C);
+void bamg_prepare(cs_ctx* C);
+void bamg_cycle(cs_ctx* C);
 int amg_levels(cs_ctx* C, int c, int* sizes, int cap);  // 0: not usable for this cloth
 void amg_prepare(cs_ctx* C, int c);
 // z = M^-1 r.  presmoothed: the caller already wrote x = w Minv r into amg_level0_x.

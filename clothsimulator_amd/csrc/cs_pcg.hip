@@ -441,25 +441,37 @@ __global__ void __launch_bounds__(PB) k_pcg_recheck(int nb, int it, const float*
 // those of separate solves while a whole batch costs two launches per iteration.
 
 struct BCloth {
-    int nn, nb, pad0, pad1;  // rows, blocks
+    int nn, nb, amg, pad1;  // rows, blocks, preconditioned by its multigrid cycle
     const int *sliceOff, *sliceW, *cols, *cRowPtr, *cCol;
     const float *vals, *cVal, *Minv, *b, *bc;
     float *x, *r, *q;
     float4 *z, *d0, *d1;
     float* part;  // P_COUNT slots of nb partials: slot s of block k at part[s * nb + k]
     PcgDev* sc;
+    // multigrid (amg): level 0's pre-smoothed iterate and residual, its aggregates, level 1's
+    // result, and A_static P in sliced ELL
+    float *xs, *t0;
+    const int* agg0;
+    const float* e1;
+    const int *apSliceOff, *apSliceW, *apCols;
+    const float* apVals;
 };
 
 // A BCloth's pointers in the global address space (loads and stores through them
 // compile to global instructions, not flat ones).
 struct BClothG {
-    int nn, nb;
+    int nn, nb, amg;
     const G1<int> *sliceOff, *sliceW, *cols, *cRowPtr, *cCol;
     const G1<float> *vals, *cVal, *Minv, *b, *bc;
     G1<float> *x, *r, *q;
     G1<f4n> *z, *d0, *d1;
     G1<float>* part;
     G1<PcgDev>* sc;
+    G1<float> *xs, *t0;
+    const G1<int>* agg0;
+    const G1<float>* e1;
+    const G1<int> *apSliceOff, *apSliceW, *apCols;
+    const G1<float>* apVals;
 };
 __device__ __forceinline__ BClothG gview(const BCloth& c) {
     BClothG g;
@@ -472,6 +484,9 @@ __device__ __forceinline__ BClothG gview(const BCloth& c) {
     g.z = (G1<f4n>*)c.z; g.d0 = (G1<f4n>*)c.d0; g.d1 = (G1<f4n>*)c.d1;
     g.part = g1(c.part);
     g.sc = g1(c.sc);
+    g.amg = c.amg;
+    g.xs = g1(c.xs); g.t0 = g1(c.t0); g.agg0 = g1(c.agg0); g.e1 = g1(c.e1);
+    g.apSliceOff = g1(c.apSliceOff); g.apSliceW = g1(c.apSliceW); g.apCols = g1(c.apCols); g.apVals = g1(c.apVals);
     return g;
 }
 
@@ -557,13 +572,17 @@ __global__ void __launch_bounds__(BP) k_bpcg_init(const BCloth* __restrict__ cl,
         st3v(c.x, i, mk(0.0f, 0.0f, 0.0f));
         st3v(c.r, i, bi);
         const V3f zi = mat3v(c.Minv + 9 * (size_t)i, bi);
-        c.z[i] = f4g(zi);
-        acc[0] = dot(bi, zi);
+        if (c.amg) {
+            st3v(c.xs, i, AMG_OMEGA * zi);  // the cycle's pre-smoothing; it writes z and r.z
+        } else {
+            c.z[i] = f4g(zi);
+            acc[0] = dot(bi, zi);
+        }
         acc[1] = dot(bi, bi);
     }
     bsums<2, BP>(acc, lds);
     if (threadIdx.x == 0) {
-        c.part[P_RZ0 * c.nb + lb] = acc[0];
+        if (!c.amg) c.part[P_RZ0 * c.nb + lb] = acc[0];
         c.part[P_BB * c.nb + lb] = acc[1];
     }
 }
@@ -737,13 +756,17 @@ __global__ void __launch_bounds__(BP) k_bpcg_update(const BCloth* __restrict__ c
         st3v(c.x, i, xi);
         st3v(c.r, i, ri);
         const V3f zi = mat3v(c.Minv + 9 * (size_t)i, ri);
-        c.z[i] = f4g(zi);
-        acc[0] = dot(ri, zi);
+        if (c.amg) {
+            st3v(c.xs, i, AMG_OMEGA * zi);
+        } else {
+            c.z[i] = f4g(zi);
+            acc[0] = dot(ri, zi);
+        }
         acc[1] = dot(ri, ri);
     }
     bsums<2, BP>(acc, lds);
     if (threadIdx.x == 0) {
-        c.part[(P_RZ0 + (1 - p)) * c.nb + lb] = acc[0];
+        if (!c.amg) c.part[(P_RZ0 + (1 - p)) * c.nb + lb] = acc[0];
         c.part[P_RR * c.nb + lb] = acc[1];
     }
 }
@@ -787,15 +810,91 @@ __global__ void __launch_bounds__(BP) k_bpcg_true(const BCloth* __restrict__ cl,
         const V3f ri = mk((float)y[0], (float)y[1], (float)y[2]);
         st3v(c.r, i, ri);
         const V3f zi = mat3v(c.Minv + 9 * (size_t)i, ri);
-        c.z[i] = f4g(zi);
-        acc[0] = dot(ri, zi);
+        if (c.amg) {
+            st3v(c.xs, i, AMG_OMEGA * zi);
+        } else {
+            c.z[i] = f4g(zi);
+            acc[0] = dot(ri, zi);
+        }
         acc[1] = dot(ri, ri);
     }
     bsums<2, BP>(acc, lds);
     if (threadIdx.x == 0) {
-        c.part[(P_RZ0 + (it & 1)) * c.nb + lb] = acc[0];
+        if (!c.amg) c.part[(P_RZ0 + (it & 1)) * c.nb + lb] = acc[0];
         c.part[P_RR * c.nb + lb] = acc[1];
     }
+}
+
+// Level 0 of the multigrid cycle of every listed cloth that uses one: t = r - A xs (sliced
+// ELL, then the contact blocks, in slot order).  liveOnly: cloths already stopped are skipped.
+template <int BP>
+__global__ void __launch_bounds__(BP) k_bpcg_resid0(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
+                                                    int liveOnly) {
+    const int2 m = batch_block(bmap);
+    if (m.x < 0) return;
+    const BClothG c = gview(cl[m.x]);
+    if (!c.amg || (liveOnly && c.sc->done)) return;
+    const int i = m.y * BP + threadIdx.x;
+    if (i >= c.nn) return;
+    const int sl = i / SLICE, lane = i % SLICE;
+    const int W = c.sliceW[sl];
+    const size_t off = (size_t)c.sliceOff[sl];
+    float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+    for (int k = 0; k < W; k++) {
+        const int j = c.cols[(off + k) * SLICE + lane];
+        float a[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) a[q] = c.vals[(off + k) * 9 * SLICE + q * SLICE + lane];
+        acc3(y0, y1, y2, a, ld3v(c.xs, j));
+    }
+    for (int kc = c.cRowPtr[i], ce = c.cRowPtr[i + 1]; kc < ce; kc++) {
+        float a[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) a[q] = c.cVal[(size_t)kc * 9 + q];
+        acc3(y0, y1, y2, a, ld3v(c.xs, c.cCol[kc]));
+    }
+    st3v(c.t0, i, ld3v(c.r, i) - mk(y0, y1, y2));
+}
+
+// Level 0 post-smoothing from the stored residual t = r - A xs: with e = level 1's result,
+// r - A (xs + P e) = t - (A P) e (A P = the static A P plus the contact blocks gathered through
+// the aggregates), z = xs + P e + w Minv (t - (A P) e); the partial of r.z into `slot`.
+template <int BP>
+__global__ void __launch_bounds__(BP) k_bpcg_post0(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
+                                                   int liveOnly, int slot) {
+    __shared__ float lds[BP / 64];
+    const int2 m = batch_block(bmap);
+    if (m.x < 0) return;
+    const BClothG c = gview(cl[m.x]);
+    if (!c.amg || (liveOnly && c.sc->done)) return;
+    const int lb = m.y;
+    const int i = lb * BP + threadIdx.x;
+    float prz = 0.0f;
+    if (i < c.nn) {
+        const int sl = i / SLICE, lane = i % SLICE;
+        const int WA = c.apSliceW[sl];
+        const size_t aoff = (size_t)c.apSliceOff[sl];
+        float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+        for (int k = 0; k < WA; k++) {
+            float a[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) a[q] = c.apVals[(aoff + k) * 9 * SLICE + q * SLICE + lane];
+            acc3(y0, y1, y2, a, ld3v(c.e1, c.apCols[(aoff + k) * SLICE + lane]));
+        }
+        for (int kc = c.cRowPtr[i], ce = c.cRowPtr[i + 1]; kc < ce; kc++) {
+            float a[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) a[q] = c.cVal[(size_t)kc * 9 + q];
+            acc3(y0, y1, y2, a, ld3v(c.e1, c.agg0[c.cCol[kc]]));
+        }
+        const V3f xi = ld3v(c.xs, i) + ld3v(c.e1, c.agg0[i]);
+        const V3f zi = xi + AMG_OMEGA * mat3v(c.Minv + 9 * (size_t)i, ld3v(c.t0, i) - mk(y0, y1, y2));
+        c.z[i] = f4g(zi);
+        prz = dot(ld3v(c.r, i), zi);
+    }
+    float v[1] = {prz};
+    bsums<1, BP>(v, lds);
+    if (threadIdx.x == 0) c.part[slot * c.nb + lb] = v[0];
 }
 
 // One block per listed cloth: accept it when the true residual meets tol^2 b.b (or its
@@ -1029,11 +1128,18 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     return h.iters;
 }
 
-// The batched solve with BP rows per block and `chunk` iterations per host poll.
+// The batched solve with BP rows per block and `chunk` iterations per host poll.  amg: the
+// cloths with a usable hierarchy run one multigrid cycle per iteration (level 0 in these
+// kernels, the coarser levels of all cloths in bamg_cycle's launches).
 template <int BP>
-static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* converged) {
+static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* converged, bool amg) {
     hipStream_t s = C->stream;
     const int ncl = C->nCloths;
+    std::vector<int> useAmg(ncl, 0);
+    if (amg) {
+        useAmg = bamg_build(C);
+        bamg_prepare(C);
+    }
     std::vector<BCloth> tab(ncl);
     size_t partTotal = 0;
     for (int c = 0; c < ncl; c++) {
@@ -1066,7 +1172,33 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
         b.part = part + po;
         po += (size_t)P_COUNT * b.nb;
         b.sc = sc + c;
+        b.amg = useAmg[c];
+        if (b.amg) {
+            AmgSys& G = S.amg;
+            b.xs = G.L[0].x.ptr<float>();
+            b.t0 = G.L[0].t.ptr<float>();
+            b.agg0 = G.L[0].agg.ptr<int>();
+            b.e1 = G.L[1].t.ptr<float>();
+            b.apSliceOff = G.apSliceOff.ptr<int>();
+            b.apSliceW = G.apSliceW.ptr<int>();
+            b.apCols = G.apCols.ptr<int>();
+            b.apVals = G.apVals.ptr<float>();
+        } else {
+            b.xs = b.t0 = nullptr;
+            b.agg0 = b.apSliceOff = b.apSliceW = b.apCols = nullptr;
+            b.e1 = b.apVals = nullptr;
+        }
     }
+    const bool anyAmg = std::find(useAmg.begin(), useAmg.end(), 1) != useAmg.end();
+    // one multigrid cycle for the cloths of the map (their r and xs written): z and the partial
+    // r.z into `slot`
+    auto cycle = [&](const int2* map, int nblk, int liveOnly, int slot) {
+        if (!anyAmg) return;
+        hipLaunchKernelGGL(k_bpcg_resid0<BP>, dim3(nblk), dim3(BP), 0, s, (const BCloth*)C->bTab.p, map, liveOnly);
+        bamg_cycle(C);
+        hipLaunchKernelGGL(k_bpcg_post0<BP>, dim3(nblk), dim3(BP), 0, s, (const BCloth*)C->bTab.p, map, liveOnly,
+                           slot);
+    };
     // (cloth, block) of every logical block of the unfinished cloths, padded to a multiple of 8
     std::vector<int2> bmap;
     auto build_map = [&](const PcgDev* done) {
@@ -1078,6 +1210,7 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
     };
     build_map(nullptr);
     int NB = (int)bmap.size();
+    const int NBfull = NB;
     BCloth* dtab = C->bTab.as<BCloth>(ncl);
     int2* dmap = C->bMap.as<int2>(NB);
     CS_HIP(hipMemcpyAsync(dtab, tab.data(), sizeof(BCloth) * ncl, hipMemcpyHostToDevice, s));
@@ -1085,6 +1218,7 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
     const float tol = C->P.pcg_rel_tol;
     const int maxit = C->P.pcg_max_iters;
     hipLaunchKernelGGL(k_bpcg_init<BP>, dim3(NB), dim3(BP), 0, s, dtab, dmap);
+    cycle(dmap, NB, 0, P_RZ0);
     hipLaunchKernelGGL(k_bpcg_setup<BP>, dim3(ncl), dim3(BP), 0, s, dtab, tol * tol, maxit);
     CS_LAUNCH_CHECK();
     const PcgDev* h = reinterpret_cast<const PcgDev*>(pinned_batch(C, sizeof(PcgDev) * ncl));
@@ -1096,9 +1230,10 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
         const int passStart = it;
         while (!all && it - passStart <= maxit + 2 * chunk) {
             for (int k = 0; k < chunk; k++, it++) {
-                // sampled: every 8th batch SpMV, each after an empty event pair that measures the
+                // sampled: every 8th batch SpMV while every cloth is still iterating (the launch
+                // then covers the whole batch), each after an empty event pair that measures the
                 // markers' own latency
-                const bool sample = it % 8 == 1;
+                const bool sample = it % 8 == 1 && NB == NBfull;
                 if (sample) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
                 const int tok = sample ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
                 if (it == 0)
@@ -1107,6 +1242,7 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
                     hipLaunchKernelGGL((k_bpcg_spmv<BP, false>), dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
                 ktimer_end(C, tok);
                 hipLaunchKernelGGL(k_bpcg_update<BP>, dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
+                cycle(dmap, NB, 1, P_RZ0 + (1 - (it & 1)));
             }
             CS_LAUNCH_CHECK();
             CS_HIP(hipMemcpyAsync(C->hBatch, sc, sizeof(PcgDev) * ncl, hipMemcpyDeviceToHost, s));
@@ -1140,6 +1276,7 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
         CS_HIP(hipMemcpyAsync(dmap, bmap.data(), sizeof(int2) * NB, hipMemcpyHostToDevice, s));
         CS_HIP(hipMemcpyAsync(dlist, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_bpcg_true<BP>, dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
+        cycle(dmap, NB, 0, P_RZ0 + (it & 1));
         hipLaunchKernelGGL(k_bpcg_recheck<BP>, dim3((unsigned)list.size()), dim3(BP), 0, s, dtab, dlist, it);
         CS_LAUNCH_CHECK();
         CS_HIP(hipMemcpyAsync(C->hBatch, sc, sizeof(PcgDev) * ncl, hipMemcpyDeviceToHost, s));
@@ -1176,17 +1313,19 @@ static const int kTuneRows[] = {256, 128, 256, 128};
 static const int kTuneChunk[] = {16, 16, 32, 32};
 constexpr int kTuneN = 4, kTuneRounds = 2;
 
-// All cloths' systems (assembled, Minv current) solved together with block-Jacobi PCG.
-// Per cloth: iterations, relative residual and convergence, as pcg_solve.
-void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged) {
+// All cloths' systems (assembled, Minv current) solved together with PCG (block Jacobi, or
+// each cloth's multigrid cycle).  Per cloth: iterations, relative residual and convergence,
+// as pcg_solve.  A multigrid iteration costs several Jacobi ones, so it polls a quarter as often.
+void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged, bool amg) {
     auto& T = C->batchTune;
     int cand = T.choice;
     const bool tuning = !(C->P.flags & CS_FLAG_FIXED_TUNING) && T.trials < kTuneN * kTuneRounds;
     if (C->P.flags & CS_FLAG_FIXED_TUNING) cand = 0;
     else if (tuning) cand = T.trials % kTuneN;
     const auto t0 = std::chrono::steady_clock::now();
-    if (kTuneRows[cand] == 128) solve_batched<128>(C, kTuneChunk[cand], iters, relres, converged);
-    else solve_batched<256>(C, kTuneChunk[cand], iters, relres, converged);
+    const int chunk = amg ? kTuneChunk[cand] / 4 : kTuneChunk[cand];
+    if (kTuneRows[cand] == 128) solve_batched<128>(C, chunk, iters, relres, converged, amg);
+    else solve_batched<256>(C, chunk, iters, relres, converged, amg);
     if (tuning) {
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         double work = 0.0;
@@ -1200,7 +1339,7 @@ void pcg_solve_batched(cs_ctx* C, int* iters, float* relres, int* converged) {
         }
     }
     T.rows = kTuneRows[T.choice];
-    T.chunk = kTuneChunk[T.choice];
+    T.chunk = amg ? kTuneChunk[T.choice] / 4 : kTuneChunk[T.choice];
 }
 
 }  // namespace cs

@@ -62,8 +62,10 @@ enum {
     CS_FLAG_TIME_KERNELS = 4, /* sample HIP-event durations of the hot kernels (see cs_kernel_time) */
     CS_FLAG_INSTANCES = 8,    /* independent instances: cloth i collides only with itself and with obstacle i
                                  (the reference traverses every cloth pair, src/Simulator.cu:179-196, which finds
-                                 nothing between instances placed apart), and with block-Jacobi PCG all cloths'
-                                 solves advance together in batched launches (no counterpart) */
+                                 nothing between instances placed apart), and with block-Jacobi PCG or
+                                 CS_PRECOND_AMG all cloths' solves advance together in batched launches, each
+                                 cloth with its own multigrid hierarchy, one launch per level for all of them
+                                 (no counterpart) */
     CS_FLAG_FIXED_TUNING = 16, /* batched solves keep their default launch shape instead of autotuning it
                                  (bit-reproducible runs; no counterpart) */
     CS_FLAG_GRAPH_CYCLES = 32 /* multigrid cycles replayed from HIP graphs captured once per solve instead of
@@ -295,7 +297,7 @@ cs_status cs_lpc_train_step(cs_ctx* ctx, int cloth_id, float lr, float sol_weigh
 /* The damping w of the learned smoother (CS_PRECOND_AMG_LEARNED) of one cloth's last assembled
  * system: x += w G^T G (b - A x). */
 cs_status cs_lpc_smoother_omega(cs_ctx* ctx, int cloth_id, float* omega);
-/* The launch shape of the batched block-Jacobi solve of independent instances (CS_FLAG_INSTANCES):
+/* The launch shape of the batched solve of independent instances (CS_FLAG_INSTANCES):
  * rows per workgroup and iterations per host poll in use, and the number of solves its
  * autotuner has timed so far (it settles after 8).  No counterpart in the reference. */
 cs_status cs_batch_tuning(const cs_ctx* ctx, int* rows_per_block, int* chunk, int* trials);
