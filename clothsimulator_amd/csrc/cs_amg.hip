@@ -1283,6 +1283,12 @@ const std::vector<int>& bamg_build(cs_ctx* C) {
     }
     B.jobOff[std::max(B.nl, 1)] = job;
     if (B.nl >= 1) B.jobOff[0] = 0;
+    B.jobCloth.assign(job, -1);
+    for (int l = 1; l < B.nl; l++) {
+        int k = B.jobOff[l];
+        for (int c = 0; c < ncl; c++)
+            if (B.usable[c] && C->sys[c].amg.nl > l) B.jobCloth[k++] = c;
+    }
     int u = 0;
     B.ap = seg([&] {
         for (int c = 0; c < ncl; c++)
@@ -1298,9 +1304,42 @@ const std::vector<int>& bamg_build(cs_ctx* C) {
     int2* dm = B.maps.as<int2>(std::max<size_t>(1, map.size()));
     if (!map.empty())
         CS_HIP(hipMemcpyAsync(dm, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice, C->stream));
-    CS_HIP(hipStreamSynchronize(C->stream));  // the host map is released on return
+    CS_HIP(hipStreamSynchronize(C->stream));
+    B.hmap.swap(map);
+    B.selected = false;
     B.ncl = ncl;
     return B.usable;
+}
+
+// Restricts the cycle's launches to the cloths with live[c] != 0 (null: every cloth again).
+// The host copy of the selection must outlive its upload: call after a stream synchronisation.
+void bamg_select(cs_ctx* C, const std::vector<char>* live) {
+    auto& B = C->bamg;
+    if (B.nl < 2) return;
+    if (!live) {
+        B.selected = false;
+        return;
+    }
+    B.hsel.clear();
+    auto pick = [&](const cs_ctx::BatchAmg::Seg& g) {
+        cs_ctx::BatchAmg::Seg o;
+        o.off = (int)B.hsel.size();
+        for (int e = g.off; e < g.off + g.n; e++)
+            if ((*live)[B.jobCloth[B.hmap[e].x]]) B.hsel.push_back(B.hmap[e]);
+        o.n = (int)B.hsel.size() - o.off;
+        return o;
+    };
+    B.crows.assign(B.nl, {});
+    B.csparse.assign(B.nl, {});
+    for (int l = 1; l < B.nl; l++) {
+        B.crows[l] = pick(B.rows[l]);
+        B.csparse[l] = pick(B.sparse[l]);
+    }
+    B.ccoarse = pick(B.coarse);
+    int2* dm = B.cmaps.as<int2>(std::max<size_t>(1, B.hsel.size()));
+    if (!B.hsel.empty())
+        CS_HIP(hipMemcpyAsync(dm, B.hsel.data(), sizeof(int2) * B.hsel.size(), hipMemcpyHostToDevice, C->stream));
+    B.selected = true;
 }
 
 // Per solve: every usable cloth's Galerkin operators, their diagonal inverses, level 0's
@@ -1395,15 +1434,17 @@ void bamg_cycle(cs_ctx* C) {
     auto& B = C->bamg;
     hipStream_t s = C->stream;
     const BLev* dj = B.jobs.ptr<BLev>();
-    const int2* map = B.maps.ptr<int2>();
+    const int2* map = B.selected ? B.cmaps.ptr<int2>() : B.maps.ptr<int2>();
+    const auto& rows = B.selected ? B.crows : B.rows;
+    const auto& sparse = B.selected ? B.csparse : B.sparse;
+    const auto& coarse = B.selected ? B.ccoarse : B.coarse;
     for (int l = 1; l < B.nl; l++) {
-        if (B.rows[l].n) hipLaunchKernelGGL(k_bamg_restrict, dim3(B.rows[l].n), dim3(AB), 0, s, dj, map + B.rows[l].off);
-        if (B.sparse[l].n)
-            hipLaunchKernelGGL(k_bamg_presid, dim3(B.sparse[l].n), dim3(AB), 0, s, dj, map + B.sparse[l].off);
+        if (rows[l].n) hipLaunchKernelGGL(k_bamg_restrict, dim3(rows[l].n), dim3(AB), 0, s, dj, map + rows[l].off);
+        if (sparse[l].n) hipLaunchKernelGGL(k_bamg_presid, dim3(sparse[l].n), dim3(AB), 0, s, dj, map + sparse[l].off);
     }
-    if (B.coarse.n) hipLaunchKernelGGL(k_bamg_coarse_solve, dim3(B.coarse.n), dim3(SB), 0, s, dj, map + B.coarse.off);
+    if (coarse.n) hipLaunchKernelGGL(k_bamg_coarse_solve, dim3(coarse.n), dim3(SB), 0, s, dj, map + coarse.off);
     for (int l = B.nl - 2; l >= 1; l--)
-        if (B.sparse[l].n) hipLaunchKernelGGL(k_bamg_post, dim3(B.sparse[l].n), dim3(AB), 0, s, dj, map + B.sparse[l].off);
+        if (sparse[l].n) hipLaunchKernelGGL(k_bamg_post, dim3(sparse[l].n), dim3(AB), 0, s, dj, map + sparse[l].off);
     CS_LAUNCH_CHECK();
 }
 

@@ -247,6 +247,13 @@ struct cs_ctx {
         std::vector<Seg> rows, sparse, blocks;  // per level: block-map segments (all jobs / non-coarsest / Galerkin blocks)
         Seg ap, coarse;                // level-0 A P of every cloth; coarsest job of every cloth
         cs::DBuf jobs, apJobs, maps;   // BLev[jobOff[nl]], BAp[usable], int2 map entries
+        // the cycle's maps restricted to the cloths still iterating (bamg_select)
+        std::vector<int> jobCloth;     // cloth of each job
+        std::vector<int2> hmap, hsel;  // host copies: every map, the selected cycle maps
+        std::vector<Seg> crows, csparse;
+        Seg ccoarse;
+        bool selected = false;         // the cycle uses the selected maps
+        cs::DBuf cmaps;
     } bamg;
     cs::DBuf asmJobs;        // per-cloth table of the batched assembly
     cs::DBuf gcCol, gcVal, gcBc, gcDiag, gcU0;  // batch-wide contact reduction

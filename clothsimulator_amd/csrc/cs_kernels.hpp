@@ -42,6 +42,8 @@ constexpr float AMG_OMEGA = 0.6f;  // damping of the block-Jacobi smoother
 const std::vector<int>& bamg_build(cs_ctx* C);
 void bamg_prepare(cs_ctx* C);
 void bamg_cycle(cs_ctx* C);
+// restricts bamg_cycle to the cloths with live[c] != 0 (null: all); call after a stream sync
+void bamg_select(cs_ctx* C, const std::vector<char>* live);
 int amg_levels(cs_ctx* C, int c, int* sizes, int cap);  // 0: not usable for this cloth
 void amg_prepare(cs_ctx* C, int c);
 // z = M^-1 r.  presmoothed: the caller already wrote x = w Minv r into amg_level0_x.

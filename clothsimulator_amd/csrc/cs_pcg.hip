@@ -48,7 +48,7 @@ struct PcgDev {
     float rrcur;  // r.r seen by the latest stop test
     int restartAt;  // batched solve: the iteration that restarts the direction from z (-1: none)
     int skip;       // batched solve: batch iterations this cloth sat out (its own count is it - skip)
-    float pad;
+    float stop2bb;  // r.r that stops the recursion: tol^2 b.b, a quarter of it after a restart
 };
 
 // Workgroups are dealt to the 8 XCDs round-robin.  Logical block L gives each
@@ -187,6 +187,7 @@ __global__ void __launch_bounds__(PB) k_pcg_setup(int nb, const float* __restric
         sc->rr = bb;
         sc->rz = acc[1];
         sc->tol2bb = tol2 * bb;
+        sc->stop2bb = tol2 * bb;
         sc->done = (bb <= tol2 * bb) || maxit <= 0;
     }
 }
@@ -264,7 +265,7 @@ __global__ void __launch_bounds__(PB) k_pcg_spmv(int it, int nn, const int* __re
                 sc->last = it;
             }
             // stop on the tolerance, the budget, or a non-finite r.r (NaN fails every comparison)
-            if (tot[0] <= sc->tol2bb || it >= sc->maxit || !(tot[0] <= FLT_MAX)) {  // the same in every block
+            if (tot[0] <= sc->stop2bb || it >= sc->maxit || !(tot[0] <= FLT_MAX)) {  // the same in every block
                 if (L == 0 && threadIdx.x == 0 && !sc->done) {
                     sc->iters = it;
                     sc->rr = tot[0];
@@ -431,6 +432,9 @@ __global__ void __launch_bounds__(PB) k_pcg_recheck(int nb, int it, const float*
         sc->last = it;
         sc->iters = it;
         sc->done = (acc[0] <= sc->tol2bb || it >= sc->maxit || !(acc[0] <= FLT_MAX)) ? 1 : 0;
+        // a continuation aims at half the tolerance: stopping its recursion just under tol leaves
+        // the true residual, which trails the recursive one by rounding, just over it again
+        sc->stop2bb = 0.25f * sc->tol2bb;
     }
 }
 
@@ -604,6 +608,7 @@ __global__ void __launch_bounds__(BP) k_bpcg_setup(const BCloth* __restrict__ cl
         c.sc->rr = v[0];
         c.sc->rz = v[1];
         c.sc->tol2bb = tol2 * v[0];
+        c.sc->stop2bb = tol2 * v[0];
         c.sc->done = (v[0] <= tol2 * v[0]) || maxit <= 0;
         c.sc->restartAt = -1;
         c.sc->skip = 0;
@@ -686,7 +691,7 @@ __global__ void __launch_bounds__(BP) k_bpcg_spmv(const BCloth* __restrict__ cl,
         float v[3];
         pr.sums(c.part, slot, c.nb, lds, v);
         const int own = it - c.sc->skip;  // this cloth's own iteration count
-        if (v[0] <= c.sc->tol2bb || own >= c.sc->maxit || !(v[0] <= FLT_MAX)) {  // the same in every block
+        if (v[0] <= c.sc->stop2bb || own >= c.sc->maxit || !(v[0] <= FLT_MAX)) {  // the same in every block
             if (lb == 0 && threadIdx.x == 0 && !c.sc->done) {
                 c.sc->iters = own;
                 c.sc->rr = v[0];
@@ -913,10 +918,11 @@ __global__ void __launch_bounds__(BP) k_bpcg_recheck(const BCloth* __restrict__ 
         c.sc->rr = v[0];
         if (v[0] <= c.sc->tol2bb || c.sc->iters >= c.sc->maxit || !(v[0] <= FLT_MAX)) {
             c.sc->done = 1;
-        } else {
+        } else {  // reopened; the continuation aims at half the tolerance (as k_pcg_recheck)
             c.sc->done = 0;
             c.sc->skip = it - c.sc->iters;
             c.sc->restartAt = it;
+            c.sc->stop2bb = 0.25f * c.sc->tol2bb;
         }
     }
 }
@@ -1075,7 +1081,7 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
             if (rr0 > 0.0f && h.rrcur > 0.0f && rr0 > h.rrcur && h.last > it0) {
                 // geometric extrapolation of the latest r.r decrease
                 const double rate = std::log((double)h.rrcur / rr0) / (h.last - it0);  // < 0
-                const double need = std::log((double)h.tol2bb / h.rrcur) / rate;       // iterations to go
+                const double need = std::log((double)h.stop2bb / h.rrcur) / rate;      // iterations to go
                 const int left = (int)std::ceil(need) - (it - 1 - h.last);             // beyond those launched
                 chunk = std::max(1, std::min(4 * chunk0, left));
             }
@@ -1139,7 +1145,15 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
     if (amg) {
         useAmg = bamg_build(C);
         bamg_prepare(C);
+        bamg_select(C, nullptr);
     }
+    std::vector<char> live(ncl, 1);
+    // the coarse levels of the cycle follow the level-0 map: only the cloths still iterating
+    auto select_live = [&](const PcgDev* st) {
+        if (!amg) return;
+        for (int c = 0; c < ncl; c++) live[c] = !st[c].done;
+        bamg_select(C, &live);
+    };
     std::vector<BCloth> tab(ncl);
     size_t partTotal = 0;
     for (int c = 0; c < ncl; c++) {
@@ -1258,6 +1272,7 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
                 build_map(h);
                 NB = (int)bmap.size();
                 CS_HIP(hipMemcpyAsync(dmap, bmap.data(), sizeof(int2) * NB, hipMemcpyHostToDevice, s));
+                select_live(h);
             }
         }
         if (!all) break;
@@ -1275,6 +1290,10 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
         NB = (int)bmap.size();
         CS_HIP(hipMemcpyAsync(dmap, bmap.data(), sizeof(int2) * NB, hipMemcpyHostToDevice, s));
         CS_HIP(hipMemcpyAsync(dlist, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, s));
+        if (amg) {
+            for (int c = 0; c < ncl; c++) live[c] = !verified[c];
+            bamg_select(C, &live);
+        }
         hipLaunchKernelGGL(k_bpcg_true<BP>, dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
         cycle(dmap, NB, 0, P_RZ0 + (it & 1));
         hipLaunchKernelGGL(k_bpcg_recheck<BP>, dim3((unsigned)list.size()), dim3(BP), 0, s, dtab, dlist, it);
@@ -1293,6 +1312,7 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
         build_map(h);  // the reopened cloths
         NB = (int)bmap.size();
         CS_HIP(hipMemcpyAsync(dmap, bmap.data(), sizeof(int2) * NB, hipMemcpyHostToDevice, s));
+        select_live(h);
     }
     if (!all) throw StatusError(CS_E_HIP, "batched PCG: the device never reported every stop");
     for (int c = 0; c < ncl; c++) {
