@@ -48,7 +48,7 @@ struct PcgDev {
     float rrcur;  // r.r seen by the latest stop test
     int restartAt;  // batched solve: the iteration that restarts the direction from z (-1: none)
     int skip;       // batched solve: batch iterations this cloth sat out (its own count is it - skip)
-    float stop2bb;  // r.r that stops the recursion: tol^2 b.b, a quarter of it after a restart
+    float stop2bb;  // r.r that stops the recursion: tol^2 b.b, lowered after a missed true-residual check
 };
 
 // Workgroups are dealt to the 8 XCDs round-robin.  Logical block L gives each
@@ -419,6 +419,14 @@ __global__ void __launch_bounds__(PB) k_pcg_true_residual(int it, int nn, const 
     }
 }
 
+// The recursive stop of a continuation after the true residual missed the tolerance: the
+// previous stop scaled by the ratio recursive / true r.r just observed, with a 10 % margin, so
+// that the true residual, which trails the recursive one by about that ratio, lands under tol.
+__device__ __forceinline__ float continuation_stop(float stop, float rec, float tru) {
+    const float ratio = (rec > 0.0f && tru > 0.0f && tru <= FLT_MAX) ? fminf(1.0f, rec / tru) : 1.0f;
+    return 0.9f * ratio * stop;
+}
+
 // The stop decision on the true residual: accept (done) when it meets tol^2 b.b or the
 // iteration budget is spent, otherwise reopen the loop from iteration `it`.
 __global__ void __launch_bounds__(PB) k_pcg_recheck(int nb, int it, const float* __restrict__ part, PcgDev* sc) {
@@ -427,14 +435,13 @@ __global__ void __launch_bounds__(PB) k_pcg_recheck(int nb, int it, const float*
     for (int k = threadIdx.x; k < nb; k += PB) acc[0] += part[P_RR * PMAX + k];
     block_sums<1>(acc, lds);
     if (threadIdx.x == 0) {
+        const float rec = sc->rr;  // the recursive r.r the loop stopped on
         sc->rr = acc[0];
         sc->rrcur = acc[0];
         sc->last = it;
         sc->iters = it;
         sc->done = (acc[0] <= sc->tol2bb || it >= sc->maxit || !(acc[0] <= FLT_MAX)) ? 1 : 0;
-        // a continuation aims at half the tolerance: stopping its recursion just under tol leaves
-        // the true residual, which trails the recursive one by rounding, just over it again
-        sc->stop2bb = 0.25f * sc->tol2bb;
+        sc->stop2bb = continuation_stop(sc->stop2bb, rec, acc[0]);
     }
 }
 
@@ -915,14 +922,15 @@ __global__ void __launch_bounds__(BP) k_bpcg_recheck(const BCloth* __restrict__ 
     float v[1];
     pr.sums(c.part, slot, c.nb, lds, v);
     if (threadIdx.x == 0) {
+        const float rec = c.sc->rr;  // the recursive r.r the cloth stopped on
         c.sc->rr = v[0];
         if (v[0] <= c.sc->tol2bb || c.sc->iters >= c.sc->maxit || !(v[0] <= FLT_MAX)) {
             c.sc->done = 1;
-        } else {  // reopened; the continuation aims at half the tolerance (as k_pcg_recheck)
+        } else {  // reopened from its true residual
             c.sc->done = 0;
             c.sc->skip = it - c.sc->iters;
             c.sc->restartAt = it;
-            c.sc->stop2bb = 0.25f * c.sc->tol2bb;
+            c.sc->stop2bb = continuation_stop(c.sc->stop2bb, rec, v[0]);
         }
     }
 }

@@ -5,14 +5,15 @@
 Training systems are the implicit-Euler systems the simulator itself assembles:
 each round builds one context holding several cloths, far apart so they never
 interact, each a hanging sheet, a drape over its own sphere or a crumple onto
-its own ground patch, with a grid side drawn from {33, 65, 129}, the stretch
-stiffness scaled by U[0.5, 2] and dt drawn from U[2, 10] ms.  The round steps
-the scene (block-Jacobi PCG) and, after every step, takes one Adam step per
+its own ground patch, with a grid side drawn from {65, 129, 257, 513}, the
+stretch stiffness scaled by U[0.5, 2] and dt drawn from U[2, 10] ms.  The round
+steps the scene (multigrid PCG: the systems do not depend on the preconditioner)
+and, after every step, takes one Adam step per
 cloth on the Kaporin loss of that cloth's assembled system (cs_lpc_train_step:
 the network's forward and backward kernels on the device), optionally plus the
 solution term (--sol-weight).  Weights carry over from round to round; the
 optimiser restarts with each round.  The packaged weights come from
-`--minutes 6 --seed 0` (Kaporin loss only).
+`--minutes 8 --seed 0` (Kaporin loss only).
 """
 import argparse
 import math
@@ -22,10 +23,10 @@ import time
 import numpy as np
 
 from . import CS_MESH_CLOTH, Simulator, scenes
-from ._cabi import CS_PRECOND_BLOCK_JACOBI, CS_PRECOND_LEARNED, default_params, material_struct
+from ._cabi import CS_PRECOND_AMG, CS_PRECOND_BLOCK_JACOBI, CS_PRECOND_LEARNED, default_params, material_struct
 from .lpc_layout import init_weights
 
-SIDES = (33, 65, 129)
+SIDES = (65, 129, 257, 513)
 
 
 def training_scene(rng, slots):
@@ -53,7 +54,7 @@ def training_scene(rng, slots):
             px = np.array([[ox - 1.5, -1.5, 0.0], [ox + 1.5, -1.5, 0.0], [ox + 1.5, 1.5, 0.0], [ox - 1.5, 1.5, 0.0]])
             obstacles.append(scenes.MeshArrays(px, np.array([[0, 1, 2], [0, 2, 3]], dtype=np.int32)))
     p = default_params(dt=float(rng.uniform(0.002, 0.010)))
-    p.precond = CS_PRECOND_BLOCK_JACOBI
+    p.precond = CS_PRECOND_AMG
     return scenes.SceneSpec("train", p, cloths, obstacles, 0)
 
 

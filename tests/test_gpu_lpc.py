@@ -106,9 +106,8 @@ def test_loss_and_gradient_match_oracle(solved):
     lo, go = lpc.loss_and_grad(w, g, bf=True, b=b if solved else None, x=dv if solved else None)
     assert abs(loss - lo) <= 1e-2 * abs(lo) + 1e-4, (loss, lo)
     rel = np.linalg.norm(grad - go) / np.linalg.norm(go)
-    assert rel < 5e-2, rel
     # every parameter block (a wrong index shows up as a large error in one block)
-    off = 0
+    off, errs = 0, {}
     for name, shape in lpc.PARAMS:
         k = int(np.prod(shape))
         gg, oo = grad[off:off + k], go[off:off + k]
@@ -117,7 +116,14 @@ def test_loss_and_gradient_match_oracle(solved):
         if nrm < 1e-6 * np.linalg.norm(go):
             assert np.linalg.norm(gg) <= 1e-5 * np.linalg.norm(go) + 1e-12, name
             continue
-        e = np.linalg.norm(gg - oo) / nrm
+        errs[name] = (float(np.linalg.norm(gg - oo) / nrm), float(nrm / np.linalg.norm(go)))
+    worst = max(errs, key=lambda n: errs[n][0])
+    print("loss %.6g vs %.6g; gradient relative %.3g overall, worst block %s %.3g (%.3g of the norm)" % (
+        loss, lo, rel, worst, errs[worst][0], errs[worst][1]))
+    for name, (e, share) in sorted(errs.items(), key=lambda kv: -kv[1][0])[:6]:
+        print("  %-12s relative %.3g  share of |grad| %.3g" % (name, e, share))
+    assert rel < 5e-2, rel
+    for name, (e, share) in errs.items():
         assert e < 0.15, (name, e)
 
 

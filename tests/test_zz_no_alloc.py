@@ -79,9 +79,9 @@ def test_noalloc_solve_matches_cs_solve_on_two_calls():
         assert status == CS_OK
         assert np.isfinite(x).all()
         assert res <= 2e-5 and rel <= 1e-5
-        # cs_solve confirms its stop on the true residual and, when that misses, continues to half
-        # the tolerance; the no-alloc solve stops on its recursive residual
-        assert abs(iters - st.pcg_iters) <= max(2, 0.05 * st.pcg_iters)
+        # cs_solve confirms its stop on the true residual and, when that misses, continues a little
+        # below the tolerance; the no-alloc solve stops on its recursive residual
+        assert iters <= st.pcg_iters + 2 and st.pcg_iters <= max(iters + 2, 1.1 * iters)
         assert dx <= 1e-5
         sim.close()
 
