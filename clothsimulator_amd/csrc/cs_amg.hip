@@ -1024,7 +1024,7 @@ static SmallCycle small_args(AmgSys& G, int l0) {
 // sub-cycle run on every unit right-hand side at once, one workgroup each): the cycles of the
 // solve then apply it as one matrix-vector product spread over the chip instead of running the
 // sub-cycle's dependent phases in one workgroup.
-void amg_prepare(cs_ctx* C, int c) {
+void amg_prepare(cs_ctx* C, int c, bool fork) {
     ClothSys& S = C->sys[c];
     AmgSys& G = S.amg;
     if (!G.built) amg_build(C, c);
@@ -1050,6 +1050,15 @@ void amg_prepare(cs_ctx* C, int c) {
                        S.sliceW.ptr<int>(), S.vals.ptr<float>(), G.apTarget.ptr<uint8_t>(), G.apSliceOff.ptr<int>(),
                        G.apSliceW.ptr<int>(), G.apVals.ptr<float>());
     CS_LAUNCH_CHECK();
+    // The coarsest inverse and the small levels' operator occupy one workgroup and then one
+    // workgroup per unit vector for about 0.6 ms at 513^2; with `fork` they run on the side
+    // stream while the caller's next setup (the network's inference) runs on the main stream,
+    // and amg_join must precede the first cycle.
+    if (fork) {
+        CS_HIP(hipEventRecord(C->evFork, s));
+        CS_HIP(hipStreamWaitEvent(C->side, C->evFork, 0));
+        s = C->side;
+    }
     const AmgLevel& Lc = G.L[G.nl - 1];
     hipLaunchKernelGGL(k_amg_coarse_inverse, dim3(1), dim3(CB), 0, s, Lc.n, Lc.rp.ptr<int>(), Lc.ci.ptr<int>(),
                        Lc.val.ptr<float>(), G.work.ptr<double>(), G.inv.ptr<float>());
@@ -1061,7 +1070,11 @@ void amg_prepare(cs_ctx* C, int c) {
     Q.op = G.smallOp.as<float>((size_t)Q.dim * Q.dim);
     hipLaunchKernelGGL(k_amg_small_cycle, dim3(Q.dim), dim3(SB), 0, s, Q);
     CS_LAUNCH_CHECK();
+    if (fork) CS_HIP(hipEventRecord(C->evJoin, s));
 }
+
+// The main stream waits for the setup amg_prepare forked to the side stream.
+void amg_join(cs_ctx* C) { CS_HIP(hipStreamWaitEvent(C->stream, C->evJoin, 0)); }
 
 // z = M^-1 r (one V(1,1) cycle); partRZ[b] = block b's share of r.z for the
 // nb blocks of the PCG grid (nb * 256 threads, rows in grid-stride order).

@@ -275,6 +275,9 @@ cs_status cs_create(const cs_params* params, cs_ctx** ctx_out) {
         if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
             throw StatusError(CS_E_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", library built for gfx950");
         CS_HIP(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+        CS_HIP(hipStreamCreateWithFlags(&C->side, hipStreamNonBlocking));
+        CS_HIP(hipEventCreateWithFlags(&C->evFork, hipEventDisableTiming));
+        CS_HIP(hipEventCreateWithFlags(&C->evJoin, hipEventDisableTiming));
         CS_HIP(hipHostMalloc(reinterpret_cast<void**>(&C->hScal), sizeof(PcgScalars), hipHostMallocDefault));
         CS_HIP(hipHostMalloc(reinterpret_cast<void**>(&C->hCount), 64, hipHostMallocDefault));
         C->P = *params;
@@ -291,15 +294,19 @@ cs_status cs_create(const cs_params* params, cs_ctx** ctx_out) {
 void cs_destroy(cs_ctx* C) {
     if (!C) return;
     if (C->stream) (void)hipStreamSynchronize(C->stream);
+    if (C->side) (void)hipStreamSynchronize(C->side);
+    if (C->evFork) (void)hipEventDestroy(C->evFork);
+    if (C->evJoin) (void)hipEventDestroy(C->evJoin);
     if (C->hScal) (void)hipHostFree(C->hScal);
     if (C->hBatch) (void)hipHostFree(C->hBatch);
     if (C->hCount) (void)hipHostFree(C->hCount);
     for (hipEvent_t e : C->kt.events) (void)hipEventDestroy(e);
     cs_ctx* c = C;
     // device buffers are released by the DBuf destructors
-    hipStream_t s = c->stream;
+    hipStream_t s = c->stream, s2 = c->side;
     delete c;
     if (s) (void)hipStreamDestroy(s);
+    if (s2) (void)hipStreamDestroy(s2);
 }
 
 const char* cs_last_error(const cs_ctx* C) { return C ? C->err.c_str() : "null context"; }

@@ -177,6 +177,10 @@ struct KernelTimer {
 struct cs_ctx {
     cs_params P;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) for per-solve setup that one workgroup runs while the
+    // main stream is busy with other setup: forked from and joined to `stream` by events
+    hipStream_t side = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
     std::string err;
     bool finalized = false;
     int device = 0;

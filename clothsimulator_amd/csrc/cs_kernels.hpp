@@ -45,7 +45,10 @@ void bamg_cycle(cs_ctx* C);
 // restricts bamg_cycle to the cloths with live[c] != 0 (null: all); call after a stream sync
 void bamg_select(cs_ctx* C, const std::vector<char>* live);
 int amg_levels(cs_ctx* C, int c, int* sizes, int cap);  // 0: not usable for this cloth
-void amg_prepare(cs_ctx* C, int c);
+// fork: the coarsest inverse and the small-level operator run on the side stream; call
+// amg_join before the first cycle
+void amg_prepare(cs_ctx* C, int c, bool fork = false);
+void amg_join(cs_ctx* C);
 // z = M^-1 r.  presmoothed: the caller already wrote x = w Minv r into amg_level0_x.
 // learned: the level-0 smoother is the learned factor (lpc_smooth_*; the caller pre-smooths)
 void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int nb, bool presmoothed,

@@ -48,7 +48,7 @@ struct PcgDev {
     float rrcur;  // r.r seen by the latest stop test
     int restartAt;  // batched solve: the iteration that restarts the direction from z (-1: none)
     int skip;       // batched solve: batch iterations this cloth sat out (its own count is it - skip)
-    float stop2bb;  // r.r that stops the recursion: tol^2 b.b, lowered after a missed true-residual check
+    float stop2bb;  // r.r that stops the recursion: tol^2 b.b, a quarter of it after two missed checks
 };
 
 // Workgroups are dealt to the 8 XCDs round-robin.  Logical block L gives each
@@ -419,29 +419,29 @@ __global__ void __launch_bounds__(PB) k_pcg_true_residual(int it, int nn, const 
     }
 }
 
-// The recursive stop of a continuation after the true residual missed the tolerance: the
-// previous stop scaled by the ratio recursive / true r.r just observed, with a 10 % margin, so
-// that the true residual, which trails the recursive one by about that ratio, lands under tol.
-__device__ __forceinline__ float continuation_stop(float stop, float rec, float tru) {
-    const float ratio = (rec > 0.0f && tru > 0.0f && tru <= FLT_MAX) ? fminf(1.0f, rec / tru) : 1.0f;
-    return 0.9f * ratio * stop;
+// The recursive stop of a continuation after the true residual missed the tolerance: the first
+// continuation stops at tol again (a few iterations from the true residual usually land under
+// it); a later one aims at half the tolerance, for the systems whose true residual keeps
+// trailing the recursive one by more than the last few iterations gain.
+__device__ __forceinline__ float continuation_stop(float tol2bb, int restarts) {
+    return restarts >= 1 ? 0.25f * tol2bb : tol2bb;
 }
 
 // The stop decision on the true residual: accept (done) when it meets tol^2 b.b or the
 // iteration budget is spent, otherwise reopen the loop from iteration `it`.
-__global__ void __launch_bounds__(PB) k_pcg_recheck(int nb, int it, const float* __restrict__ part, PcgDev* sc) {
+__global__ void __launch_bounds__(PB) k_pcg_recheck(int nb, int it, int restarts, const float* __restrict__ part,
+                                                   PcgDev* sc) {
     __shared__ float lds[8];
     float acc[1] = {0.0f};
     for (int k = threadIdx.x; k < nb; k += PB) acc[0] += part[P_RR * PMAX + k];
     block_sums<1>(acc, lds);
     if (threadIdx.x == 0) {
-        const float rec = sc->rr;  // the recursive r.r the loop stopped on
         sc->rr = acc[0];
         sc->rrcur = acc[0];
         sc->last = it;
         sc->iters = it;
         sc->done = (acc[0] <= sc->tol2bb || it >= sc->maxit || !(acc[0] <= FLT_MAX)) ? 1 : 0;
-        sc->stop2bb = continuation_stop(sc->stop2bb, rec, acc[0]);
+        sc->stop2bb = continuation_stop(sc->tol2bb, restarts);
     }
 }
 
@@ -913,7 +913,7 @@ __global__ void __launch_bounds__(BP) k_bpcg_post0(const BCloth* __restrict__ cl
 // iterations are spent), otherwise reopen it with its direction restarting at `it`.
 template <int BP>
 __global__ void __launch_bounds__(BP) k_bpcg_recheck(const BCloth* __restrict__ cl, const int* __restrict__ list,
-                                                     int it) {
+                                                     int it, int restarts) {
     __shared__ float lds[BP / 64];
     const BClothG c = gview(cl[list[blockIdx.x]]);
     const int slot[1] = {P_RR};
@@ -922,7 +922,6 @@ __global__ void __launch_bounds__(BP) k_bpcg_recheck(const BCloth* __restrict__ 
     float v[1];
     pr.sums(c.part, slot, c.nb, lds, v);
     if (threadIdx.x == 0) {
-        const float rec = c.sc->rr;  // the recursive r.r the cloth stopped on
         c.sc->rr = v[0];
         if (v[0] <= c.sc->tol2bb || c.sc->iters >= c.sc->maxit || !(v[0] <= FLT_MAX)) {
             c.sc->done = 1;
@@ -930,7 +929,7 @@ __global__ void __launch_bounds__(BP) k_bpcg_recheck(const BCloth* __restrict__ 
             c.sc->done = 0;
             c.sc->skip = it - c.sc->iters;
             c.sc->restartAt = it;
-            c.sc->stop2bb = continuation_stop(c.sc->stop2bb, rec, v[0]);
+            c.sc->stop2bb = continuation_stop(c.sc->tol2bb, restarts);
         }
     }
 }
@@ -958,7 +957,7 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
     const bool learned = C->P.precond == CS_PRECOND_LEARNED;
     float* xs = nullptr;  // level-0 pre-smoothing iterate of the multigrid cycle
     if (amgl) {
-        amg_prepare(C, c);
+        amg_prepare(C, c, true);  // its single-workgroup part overlaps the network's inference
         lpc_prepare(C, c);
         lpc_smoother_prepare(C, c);
         xs = amg_level0_x(C, c);
@@ -966,6 +965,7 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
                            S.bc.ptr<float>(), S.Minv.ptr<float>(), S.x.ptr<float>(), S.r.ptr<float>(), z, xs, part);
         CS_LAUNCH_CHECK();
         lpc_smooth_pre(C, c, S.r.ptr<float>(), xs);
+        amg_join(C);
         amg_apply(C, c, S.r.ptr<float>(), z, part + P_RZ0 * PMAX, nb, true, true);
     } else if (amg) {
         amg_prepare(C, c);
@@ -1117,7 +1117,7 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
             hipLaunchKernelGGL(k_pcg_true_residual<PC_JACOBI>, dim3(nb), dim3(PB), 0, s, it, nn, so, sw, co, va,
                                cRowPtr, cCol, cVal, S.b.ptr<float>(), S.bc.ptr<float>(), S.Minv.ptr<float>(),
                                S.x.ptr<float>(), S.r.ptr<float>(), z, xs, part);
-        hipLaunchKernelGGL(k_pcg_recheck, dim3(1), dim3(PB), 0, s, nb, it, part, sc);
+        hipLaunchKernelGGL(k_pcg_recheck, dim3(1), dim3(PB), 0, s, nb, it, restarts, part, sc);
         CS_LAUNCH_CHECK();
         poll();
         if (h.done || restarts == PCG_RESTARTS) break;
@@ -1304,7 +1304,8 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
         }
         hipLaunchKernelGGL(k_bpcg_true<BP>, dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
         cycle(dmap, NB, 0, P_RZ0 + (it & 1));
-        hipLaunchKernelGGL(k_bpcg_recheck<BP>, dim3((unsigned)list.size()), dim3(BP), 0, s, dtab, dlist, it);
+        hipLaunchKernelGGL(k_bpcg_recheck<BP>, dim3((unsigned)list.size()), dim3(BP), 0, s, dtab, dlist, it,
+                           restarts);
         CS_LAUNCH_CHECK();
         CS_HIP(hipMemcpyAsync(C->hBatch, sc, sizeof(PcgDev) * ncl, hipMemcpyDeviceToHost, s));
         CS_HIP(hipStreamSynchronize(s));

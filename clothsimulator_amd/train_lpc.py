@@ -23,7 +23,8 @@ import time
 import numpy as np
 
 from . import CS_MESH_CLOTH, Simulator, scenes
-from ._cabi import CS_PRECOND_AMG, CS_PRECOND_BLOCK_JACOBI, CS_PRECOND_LEARNED, default_params, material_struct
+from ._cabi import (CS_E_NOT_CONVERGED, CS_OK, CS_PRECOND_AMG, CS_PRECOND_BLOCK_JACOBI, CS_PRECOND_LEARNED,
+                    default_params, material_struct)
 from .lpc_layout import init_weights
 
 SIDES = (65, 129, 257, 513)
@@ -82,7 +83,7 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--init", default=None, help="start from these weights (.npy) instead of a fresh network")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--slots", type=int, default=6)
     ap.add_argument("--steps-per-round", type=int, default=40)
     ap.add_argument("--sol-weight", type=float, default=0.0,
@@ -93,25 +94,36 @@ def main(argv=None):
     t0 = time.time()
     budget = 60.0 * a.minutes
     total = 0
+    skipped = 0
     rnd = 0
     while time.time() - t0 < budget:
         spec = training_scene(rng, a.slots)
         sim = Simulator(spec)
         sim.set_learned_weights(w)
         losses = []
+        good = w.copy()  # the weights after the last finite step
         for step in range(a.steps_per_round):
-            sim.step()
+            s, _ = sim.step()
+            if s not in (CS_OK, CS_E_NOT_CONVERGED):  # a failed step (an unresolved collision) ends the round
+                break
             frac = min(1.0, (time.time() - t0) / budget)
             lr = a.lr * 0.5 * (1.0 + math.cos(math.pi * frac))
             for c in range(len(spec.cloths)):
-                losses.append(sim.lpc_train_step(c, lr, sol_weight=a.sol_weight))
+                loss = sim.lpc_train_step(c, lr, sol_weight=a.sol_weight)
                 total += 1
+                cur = sim.learned_weights()
+                if not (np.isfinite(loss) and np.isfinite(cur).all()):
+                    sim.set_learned_weights(good)  # drop the update of a system whose loss overflowed
+                    skipped += 1
+                    continue
+                good = cur
+                losses.append(loss)
             if time.time() - t0 >= budget:
                 break
-        w = sim.learned_weights()
+        w = good
         sim.close()
-        print("round %d  systems %d  mean loss %.4f  %.0fs" % (rnd, total, float(np.mean(losses)), time.time() - t0),
-              flush=True)
+        print("round %d  systems %d (updates dropped %d)  mean loss %.4f  %.0fs" % (
+            rnd, total, skipped, float(np.mean(losses)) if losses else float("nan"), time.time() - t0), flush=True)
         rnd += 1
     if a.out:
         np.save(a.out, w.astype(np.float32), allow_pickle=False)

@@ -10,9 +10,10 @@ headline measurement, checked against the oracle and against FP64 arithmetic.
   the oracle on the per-mesh trees, and, for every preconditioner, the solution of the assembled
   system checked in FP64 on the host: ||b - A dv|| / ||b|| <= 1e-5 (the PCG stops on its
   recursive FP32 residual).
-* The learned preconditioner on this held-out system (its weights were trained on sheets of 33
-  to 129 nodes a side): fewer PCG iterations than block Jacobi, and the learned smoother fewer
-  than the block-Jacobi-smoothed multigrid cycle.
+* The learned preconditioner on this held-out system (its weights were trained on other
+  systems: random hanging, draped and crumpled sheets of 65 to 513 nodes a side): fewer PCG
+  iterations than block Jacobi, and the learned smoother fewer than the block-Jacobi-smoothed
+  multigrid cycle.
 Reference: src/Cloth.cu:695-813, src/Simulator.cu:571-684, BVH src/BVH.cu:18-50.
 """
 import numpy as np

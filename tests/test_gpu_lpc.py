@@ -122,9 +122,11 @@ def test_loss_and_gradient_match_oracle(solved):
         loss, lo, rel, worst, errs[worst][0], errs[worst][1]))
     for name, (e, share) in sorted(errs.items(), key=lambda kv: -kv[1][0])[:6]:
         print("  %-12s relative %.3g  share of |grad| %.3g" % (name, e, share))
-    assert rel < 5e-2, rel
+    # measured: 2.3e-3 / 2.7e-3 overall, 5.3e-3 in the worst block (bf16 roundings that flip
+    # between the device's and the oracle's accumulation orders)
+    assert rel < 1e-2, rel
     for name, (e, share) in errs.items():
-        assert e < 0.15, (name, e)
+        assert e < 2e-2, (name, e)
 
 
 def test_learned_pcg_steps_match_direct_solve():
