@@ -106,26 +106,31 @@ def cpu_baseline(spec, steps):
                       "to 1e-5, FP64 CCD), OpenMP on %d threads, %.1f s" % (steps, spec.name, threads, dt)}
 
 
-def cpu_reference_direct(steps=100):
-    """The reference's own CPU solve (one thread, a direct sparse LL^T per step) on the hanging
-    33^2 sheet for its full 100 steps: the oracle's FP32 step with its direct solver (RCM-ordered
-    banded Cholesky in FP64) on one OpenMP thread."""
+def cpu_reference_direct(workload="hang33", steps=100):
+    """The reference's own CPU solve (one thread, a direct sparse LL^T of every step's system,
+    src/Cloth.cu:676-693) on the first `steps` steps of a workload: the oracle's FP32 step with its
+    direct solver (nested-dissection multifrontal Cholesky in FP64) on one OpenMP thread."""
     import oracle
     from clothsimulator_amd import scenes
-    spec = scenes.hang(33)
+    spec = scenes.by_name(workload)
     old = omp_threads(1)
     try:
         o = oracle.OracleScene(spec)
         t0 = time.perf_counter()
-        for _ in range(steps):
-            o.step(direct=True)
+        for k in range(steps):
+            o.step(direct=2)
+            if time.perf_counter() - t0 > 30.0 and k + 1 < steps:
+                print("cpu reference column: %s step %d/%d  %.0f s" % (workload, k + 1, steps,
+                      time.perf_counter() - t0), file=sys.stderr, flush=True)
         dt = time.perf_counter() - t0
     finally:
         omp_threads(old)
     n = spec.n_cloth_nodes
+    span = "all %d steps" % steps if steps >= spec.steps else "the first %d of its %d steps" % (steps, spec.steps)
     return {"value": n * steps / dt, "unit": "node-steps/s", "cores": 1, "kind": "port", "cpu_model": cpu_model(),
-            "sample": "%s, all %d steps: oracle FP32 step with a direct sparse Cholesky (RCM + banded, FP64) on "
-                      "one thread, %.1f s" % (spec.name, steps, dt)}
+            "workload": workload, "ms_per_step": 1e3 * dt / steps,
+            "sample": "%s, %s: oracle FP32 step with a direct sparse Cholesky (nested dissection, multifrontal, "
+                      "FP64) on one thread, %.1f s" % (spec.name, span, dt)}
 
 
 def copy_probe(mib):
@@ -264,6 +269,9 @@ def main():
     ap.add_argument("--sequential-sample", type=int, default=16, metavar="K",
                     help="batch workloads: also run K of the instances one at a time, each on its own context, "
                          "and report the sequential throughput they imply")
+    ap.add_argument("--cpu-direct", default="", metavar="WORKLOAD:STEPS,...",
+                    help="also time the single-thread direct-solve CPU column on these windows "
+                         "(e.g. drape129:20,crumple513:3); hang33 over its 100 steps always runs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
     a = ap.parse_args()
@@ -353,6 +361,11 @@ def main():
         out["cpu_baseline"] = cpu_baseline(scenes.by_name(a.workload), a.cpu_steps)
         print("cpu reference column: hang33, 100 steps, direct solve on one thread", file=sys.stderr, flush=True)
         out["cpu_reference_direct"] = cpu_reference_direct()
+        for item in [w for w in a.cpu_direct.split(",") if w]:
+            wl, k = item.split(":")
+            print("cpu reference column: %s, %s steps, direct solve on one thread" % (wl, k), file=sys.stderr,
+                  flush=True)
+            out.setdefault("cpu_reference_direct_windows", []).append(cpu_reference_direct(wl, int(k)))
     print(json.dumps(out))
 
 

@@ -95,6 +95,8 @@ def material_table(mat):
 
 
 def solve_bsr(row_ptr, col, vals, b, direct=False, fp64=True, tol=1e-5, max_iters=4000):
+    """direct: False / 0 block-Jacobi PCG to tol, True / 1 the banded LL^T (RCM order), 2 the
+    nested-dissection multifrontal LL^T; returns (x, iterations, relative residual)."""
     L = lib()
     n = len(row_ptr) - 1
     rp = np.ascontiguousarray(row_ptr, dtype=np.int32)

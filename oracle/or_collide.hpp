@@ -899,7 +899,9 @@ template <class T> int physics_step(Scene<T>& S, int directSolver, cs_stats* st)
         assemble(S, c, prox, sys);
         vector<int> rp, ci; vector<T> val, dv;
         sys.to_bsr(rp, ci, val);
-        if (directSolver) {
+        if (directSolver == 2) {
+            if (!direct_solve_nd(rp, ci, val, sys.b, dv)) return CS_E_NOT_CONVERGED;
+        } else if (directSolver) {
             if (!direct_solve(rp, ci, val, sys.b, dv)) return CS_E_NOT_CONVERGED;
         } else {
             T rr = T(0);

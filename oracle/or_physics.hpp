@@ -6,6 +6,7 @@
 #include <map>
 #include <queue>
 
+#include "or_direct.hpp"
 #include "or_scene.hpp"
 
 namespace orc {
@@ -468,6 +469,16 @@ template <class T> bool direct_solve(const vector<int>& rowPtr, const vector<int
     }
     x.assign(m, T(0));
     for (int i = 0; i < n; i++) for (int a = 0; a < 3; a++) x[3 * i + a] = (T)y[3 * inv[i] + a];
+    return true;
+}
+
+// The same LL^T with the nested-dissection multifrontal factorisation of or_direct.hpp
+// (fill-reducing order: the large systems).
+template <class T> bool direct_solve_nd(const vector<int>& rowPtr, const vector<int>& col, const vector<T>& val,
+                                        const vector<T>& b, vector<T>& x) {
+    NdCholesky ch;
+    if (!ch.factor(rowPtr, col, val)) return false;
+    ch.solve(b, x);
     return true;
 }
 

@@ -462,7 +462,8 @@ int or_amg(void* h, int cloth, int* sizes, int cap, int* nlevels, const double* 
     });
 }
 
-// Block-Jacobi PCG (direct == 0) or the banded LL^T (direct == 1) on a caller BSR system.
+// Block-Jacobi PCG (direct == 0), the banded LL^T (direct == 1) or the nested-dissection
+// multifrontal LL^T (direct == 2) on a caller BSR system.
 int or_solve_bsr(int n, const int* rowPtr, const int* col, const double* vals, const double* b, int direct,
                  int fp64, double tol, int maxIters, double* x, int* iters, double* relres) {
     auto run = [&](auto zero) {
@@ -472,7 +473,8 @@ int or_solve_bsr(int n, const int* rowPtr, const int* col, const double* vals, c
         for (size_t i = 0; i < val.size(); i++) val[i] = (T)vals[i];
         for (size_t i = 0; i < bb.size(); i++) bb[i] = (T)b[i];
         int it = 0; T rr = T(0);
-        if (direct) { if (!direct_solve(rp, ci, val, bb, xx)) return (int)CS_E_NOT_CONVERGED; }
+        if (direct == 2) { if (!direct_solve_nd(rp, ci, val, bb, xx)) return (int)CS_E_NOT_CONVERGED; }
+        else if (direct) { if (!direct_solve(rp, ci, val, bb, xx)) return (int)CS_E_NOT_CONVERGED; }
         else it = pcg_block_jacobi(rp, ci, val, bb, (T)tol, maxIters, xx, &rr);
         for (size_t i = 0; i < xx.size(); i++) x[i] = (double)xx[i];
         if (iters) *iters = it;
