@@ -1029,6 +1029,9 @@ void amg_prepare(cs_ctx* C, int c, bool fork) {
     AmgSys& G = S.amg;
     if (!G.built) amg_build(C, c);
     hipStream_t s = C->stream;
+    // a previous fork whose caller failed before its join may still be running on the side
+    // stream: the operators below are rewritten only after it has finished
+    CS_HIP(hipStreamWaitEvent(s, C->evJoin, 0));
     for (int l = 1; l < G.nl; l++) {
         AmgLevel& L = G.L[l];
         if (l == 1)
