@@ -345,14 +345,6 @@ __global__ void __launch_bounds__(AB) k_amg_resid0(int nn, Level0 A, const float
     st3v(t, i, ld3v(b, i) - ax);
 }
 
-__global__ void __launch_bounds__(AB) k_amg_resid(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                  const float* __restrict__ val, const float* __restrict__ b,
-                                                  const float* __restrict__ x, float* __restrict__ t) {
-    const int i = blockIdx.x * AB + threadIdx.x;
-    if (i >= n) return;
-    st3v(t, i, ld3v(b, i) - rowc<false>(i, rp, ci, val, x, nullptr, nullptr));
-}
-
 // Pre-smoothing and residual of a block-CSR level in one pass: x = w D^-1 b
 // (stored for the own row, formed on the fly for the gathered ones) and
 // t = b - A x.  Coarse levels are small, so the extra gathers hit the cache and
@@ -431,11 +423,6 @@ __device__ __forceinline__ void coarse_solve(int m, const float* __restrict__ in
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
         if (lane == 0) x[r] = s;
     }
-}
-
-__global__ void __launch_bounds__(SB) k_amg_coarse_solve(int m, const float* __restrict__ inv,
-                                                         const float* __restrict__ b, float* __restrict__ x) {
-    coarse_solve(m, inv, b, x);
 }
 
 // out = x' + w D^-1 (b - A x'), x' = x + e_agg, on a block-CSR level
@@ -794,28 +781,6 @@ __global__ void __launch_bounds__(AB) k_amg_post0ap_res(int nn, Level0 A, const 
     }
     st3v(x, i, ld3v(x, i) + ld3v(e, agg[i]));
     st3v(t, i, ld3v(t, i) - mk(y0, y1, y2));
-}
-
-// Level 0 post-smoothing: z = x' + w Minv (r - A x'); also one partial of r.z per block.
-__global__ void __launch_bounds__(AB) k_amg_post0(int nn, Level0 A, const int* __restrict__ agg,
-                                                  const float* __restrict__ e, const float* __restrict__ r,
-                                                  const float* __restrict__ x, float4* __restrict__ z,
-                                                  float* __restrict__ partRZ) {
-    __shared__ float lds[AB / 64];
-    float prz = 0.0f;
-    for (int i = blockIdx.x * AB + threadIdx.x; i < nn; i += gridDim.x * AB) {
-        const V3f ax = row0<true>(i, A.sliceOff, A.sliceW, A.cols, A.vals, A.cRowPtr, A.cCol, A.cVal, x, agg, e);
-        const V3f xi = ld3v(x, i) + ld3v(e, agg[i]);
-        const V3f ri = ld3v(r, i);
-        const V3f zi = xi + AMG_OMEGA * mat3v(A.Minv + 9 * (size_t)i, ri - ax);
-        z[i] = make_float4(zi.x, zi.y, zi.z, 0.0f);
-        prz += dot(ri, zi);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) prz += __shfl_xor(prz, o, 64);
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = prz;
-    __syncthreads();
-    if (threadIdx.x == 0) partRZ[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
 }
 
 }  // namespace
