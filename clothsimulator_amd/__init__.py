@@ -27,7 +27,7 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_lpc_train_step", "cs_lpc_smoother_omega", "cs_batch_tuning", "cs_add_motion", "cs_motion_eval",
             "cs_set_obstacle_motion", "cs_add_moving_handles", "cs_set_step_count", "cs_get_step_count",
             "cs_scene_motion", "cs_get_topology", "cs_write_config", "cs_impact_response",
-            "cs_solve_bsr_noalloc"]
+            "cs_solve_bsr_noalloc", "cs_lpc_train_step_smooth"]
 
 # trained weights of the learned preconditioner shipped with the package (float32, .npy)
 LPC_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lpc_weights.npy")
@@ -86,6 +86,7 @@ def load_library(path=LIB_PATH):
     L.cs_get_learned_weights.argtypes = [vp, F, C.c_int]
     L.cs_lpc_factor.argtypes = [vp, C.c_int, F, F, C.c_int, C.POINTER(C.c_int)]
     L.cs_lpc_train_step.argtypes = [vp, C.c_int, C.c_float, C.c_float, F, F]
+    L.cs_lpc_train_step_smooth.argtypes = [vp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_uint, F, F]
     L.cs_lpc_smoother_omega.argtypes = [vp, C.c_int, F]
     L.cs_batch_tuning.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     TP = C.POINTER(Transform)
@@ -414,14 +415,19 @@ class Simulator:
                     "cs_lpc_factor")
         return diag, lower[:nl.value]
 
-    def lpc_train_step(self, cloth=0, lr=0.0, grad=False, sol_weight=0.0):
+    def lpc_train_step(self, cloth=0, lr=0.0, grad=False, sol_weight=0.0, smooth_weight=0.0, seed=0):
         """Kaporin loss of the network's factor on the last assembled system (plus sol_weight times
-        the solution term once the system is solved), its gradient (when grad) and one Adam step
-        when lr > 0."""
+        the solution term once the system is solved, plus smooth_weight times the smoothing term on
+        the probe of `seed`), its gradient (when grad) and one Adam step when lr > 0."""
         loss = C.c_float()
         g = np.zeros(self.L.cs_lpc_weight_count(), dtype=np.float32) if grad else None
-        self._check(self.L.cs_lpc_train_step(self.h, cloth, lr, sol_weight, C.byref(loss), fptr(g) if grad else None),
-                    "cs_lpc_train_step")
+        gp = fptr(g) if grad else None
+        if smooth_weight > 0.0:
+            self._check(self.L.cs_lpc_train_step_smooth(self.h, cloth, lr, sol_weight, smooth_weight, seed,
+                                                        C.byref(loss), gp), "cs_lpc_train_step_smooth")
+        else:
+            self._check(self.L.cs_lpc_train_step(self.h, cloth, lr, sol_weight, C.byref(loss), gp),
+                        "cs_lpc_train_step")
         return (loss.value, g) if grad else loss.value
 
     def smoother_omega(self, cloth=0):

@@ -807,7 +807,19 @@ cs_status cs_lpc_train_step(cs_ctx* C, int cloth_id, float lr, float sol_weight,
     if (!C->finalized) return CS_E_STATE;
     if (!C->lpcNet.loaded) return CS_E_STATE;
     return guard(C, [&]() {
-        *loss = lpc_train_step(C, cloth_id, lr, sol_weight, grad);
+        *loss = lpc_train_step(C, cloth_id, lr, sol_weight, 0.0f, 0u, grad);
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_lpc_train_step_smooth(cs_ctx* C, int cloth_id, float lr, float sol_weight, float smooth_weight,
+                                   unsigned int seed, float* loss, float* grad) {
+    if (!C || !loss || !(lr >= 0.0f) || !(sol_weight >= 0.0f) || !(smooth_weight >= 0.0f)) return CS_E_ARG;
+    if (cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
+    if (!C->finalized) return CS_E_STATE;
+    if (!C->lpcNet.loaded) return CS_E_STATE;
+    return guard(C, [&]() {
+        *loss = lpc_train_step(C, cloth_id, lr, sol_weight, smooth_weight, seed, grad);
         return (int)CS_OK;
     });
 }

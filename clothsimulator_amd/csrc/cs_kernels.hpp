@@ -73,7 +73,8 @@ float lpc_smoother_omega(cs_ctx* C, int c);
 void lpc_factor_blocks(cs_ctx* C, int c, float* diag, float* lower);
 // one training step on cloth c's system: loss (Kaporin, plus solWeight times the solution term
 // when the system has been solved), gradient (optional host copy), Adam when lr > 0
-float lpc_train_step(cs_ctx* C, int c, float lr, float solWeight, float* grad);
+float lpc_train_step(cs_ctx* C, int c, float lr, float solWeight, float smoothWeight, uint32_t seed,
+                     float* grad);
 
 // motions (cs_motion.hip)
 void motion_step(cs_ctx* C);                // step counter + 1, obstacle velocities and handle anchors

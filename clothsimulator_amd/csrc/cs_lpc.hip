@@ -885,6 +885,90 @@ __global__ void __launch_bounds__(256) k_lpc_sol_final(int nblk, const float* __
     }
 }
 
+// ---------------------------------------------------------------- training: smoothing term
+// L_sm = |(I - w G^T G A) v|_A^2 / |v|_A^2 for a probe v: how much of the probe's energy-norm
+// error one smoothing sweep leaves.  Its gradient dL/dG_ij = c2 (y_i p_j^T + w_i u_j^T) with
+// u = A v, y = G u, e = v - w G^T y, p = A e, w = G p, c2 = -2 w / (v.u) goes through
+// k_lpc_sol_grad (c1 = 0).
+
+// v_i = (+-1, +-1, +-1) from a hash of (i, seed)
+__global__ void k_lpc_probe(int nn, uint32_t seed, float4* __restrict__ v) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    uint32_t h = (uint32_t)i * 2654435761u + seed * 2246822519u;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    v[i] = make_float4((h & 1) ? 1.0f : -1.0f, (h & 2) ? 1.0f : -1.0f, (h & 4) ? 1.0f : -1.0f, 0.0f);
+}
+
+// e = v - w s
+__global__ void k_lpc_sub(int nn, const float4* __restrict__ v, const float4* __restrict__ s, float w,
+                          float4* __restrict__ e) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    const float4 a = v[i], b = s[i];
+    e[i] = make_float4(a.x - w * b.x, a.y - w * b.y, a.z - w * b.z, 0.0f);
+}
+
+// partials of e.p and v.u (e, v float4; p, u 3 per node)
+__global__ void __launch_bounds__(256) k_lpc_smooth_dots(int nn, const float4* __restrict__ e,
+                                                         const float* __restrict__ p, const float4* __restrict__ v,
+                                                         const float* __restrict__ u, float* __restrict__ part) {
+    __shared__ float lds[2][4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    float a0 = 0.0f, a1 = 0.0f;
+    if (i < nn) {
+        const float4 ei = e[i], vi = v[i];
+        a0 = ei.x * p[3 * i] + ei.y * p[3 * i + 1] + ei.z * p[3 * i + 2];
+        a1 = vi.x * u[3 * i] + vi.y * u[3 * i + 1] + vi.z * u[3 * i + 2];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, 64);
+        a1 += __shfl_xor(a1, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        lds[0][threadIdx.x >> 6] = a0;
+        lds[1][threadIdx.x >> 6] = a1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = (lds[0][0] + lds[0][1]) + (lds[0][2] + lds[0][3]);
+        part[2 * blockIdx.x + 1] = (lds[1][0] + lds[1][1]) + (lds[1][2] + lds[1][3]);
+    }
+}
+
+// L_sm = (e.p) / (v.u); sc = [0, -2 w / (v.u), L_sm]; adds lam L_sm to the loss
+__global__ void __launch_bounds__(256) k_lpc_smooth_final(int nblk, const float* __restrict__ pdots, float lam,
+                                                          float w, float* __restrict__ scal, float* __restrict__ sc) {
+    __shared__ double lds[2][256];
+    double num = 0.0, den = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += 256) {
+        num += pdots[2 * k];
+        den += pdots[2 * k + 1];
+    }
+    lds[0][threadIdx.x] = num;
+    lds[1][threadIdx.x] = den;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            lds[0][threadIdx.x] += lds[0][threadIdx.x + h];
+            lds[1][threadIdx.x] += lds[1][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double n = lds[0][0], d = lds[1][0];
+        const bool ok = d > 0.0;
+        const double L = ok ? n / d : 0.0;
+        sc[0] = 0.0f;
+        sc[1] = ok ? (float)(-2.0 * w / d) : 0.0f;
+        sc[2] = (float)L;
+        scal[0] += lam * (float)L;
+    }
+}
+
 // Z_ij += lam (T / 2) dL_sol/dGhat_ij for the lower and diagonal slots of row i, where
 // dL_sol/dGhat_ij = (c1 y_i b_j^T + c2 (y_i p_j^T + w_i b_j^T)) S_j^T; the decoders' backward
 // multiplies Z by 2 / T.
@@ -1579,6 +1663,30 @@ size_t carve_sol(char* base, size_t rows, int nblk, SolBufs& sb) {
     return c.used;
 }
 
+struct SmoothBufs {
+    float4 *v, *y, *s, *e, *w;  // probe, G u, G^T y, v - w s, G p
+    float *u, *p;               // A v, A e (3 per node row)
+    float* part;                // nblk partials of the G^T sweep (unused)
+    float* pdots;               // 2 nblk partials of e.p and v.u
+    float* sc;                  // c1, c2, L_sm
+};
+
+size_t carve_smooth(char* base, size_t rows, int nblk, SmoothBufs& m) {
+    Carver c;
+    c.base = base;
+    m.v = c.take<float4>(rows);
+    m.y = c.take<float4>(rows);
+    m.s = c.take<float4>(rows);
+    m.e = c.take<float4>(rows);
+    m.w = c.take<float4>(rows);
+    m.u = c.take<float>(3 * rows);
+    m.p = c.take<float>(3 * rows);
+    m.part = c.take<float>(nblk);
+    m.pdots = c.take<float>(2 * (size_t)nblk);
+    m.sc = c.take<float>(4);
+    return c.used;
+}
+
 template <int KB, int MB>
 void launch_wgrad(cs_ctx* C, const std::vector<WJob>& jobs, WJob* djobs, long maxRows) {
     if (jobs.empty()) return;
@@ -1589,7 +1697,8 @@ void launch_wgrad(cs_ctx* C, const std::vector<WJob>& jobs, WJob* djobs, long ma
 
 }  // namespace
 
-float lpc_train_step(cs_ctx* C, int c, float lr, float solWeight, float* gradOut) {
+float lpc_train_step(cs_ctx* C, int c, float lr, float solWeight, float smoothWeight, uint32_t seed,
+                     float* gradOut) {
     LpcNet& N = C->lpcNet;
     if (!N.loaded) throw StatusError(CS_E_STATE, "training needs initial weights (cs_set_learned_weights)");
     lpc_alloc(C, c);
@@ -1603,11 +1712,14 @@ float lpc_train_step(cs_ctx* C, int c, float lr, float solWeight, float* gradOut
     double* part = nullptr;
     float* scal = nullptr;
     SolBufs sb{};
+    SmoothBufs mb{};
     const size_t need1 = carve_all(nullptr, slots, rows, nblk, st, bb, part, scal);
-    const size_t need = need1 + carve_sol(nullptr, rows, nblk, sb);
+    const size_t need2 = need1 + carve_sol(nullptr, rows, nblk, sb);
+    const size_t need = need2 + carve_smooth(nullptr, rows, nblk, mb);
     char* base = N.arena.as<char>(need);
     carve_all(base, slots, rows, nblk, st, bb, part, scal);
     carve_sol(base + need1, rows, nblk, sb);
+    carve_smooth(base + need2, rows, nblk, mb);
     lpc_forward<true>(C, c, st);
     Topo t = topo_of(S);
     LossArgs la{t, st.gh, st.Ah, st.o, bb.Z, part};
@@ -1636,6 +1748,33 @@ float lpc_train_step(cs_ctx* C, int c, float lr, float solWeight, float* gradOut
         hipLaunchKernelGGL(k_lpc_sol_final, dim3(1), dim3(256), 0, s, nblk, sb.palpha, sb.pdots, lam, scal, sb.sc);
         hipLaunchKernelGGL(k_lpc_sol_grad, dim3(nblk), dim3(256), 0, s, t, sb.y, sb.w, sb.bt, sb.p, L.S.ptr<float>(),
                            scal, sb.sc, lam, bb.Z);
+        CS_LAUNCH_CHECK();
+    }
+    if (smoothWeight > 0.0f) {
+        // the smoothing term, on the probe of `seed`, for one sweep at damping 1
+        const float lam = smoothWeight, w = 1.0f;
+        LpcSys& L = S.lpc;
+        const int* cCol = S.nContact > 0 ? S.cCol.ptr<int>() : S.cRowPtr.ptr<int>();
+        const float* cVal = S.nContact > 0 ? S.cVal.ptr<float>() : S.vals.ptr<float>();
+        hipLaunchKernelGGL(k_lpc_probe, dim3(nblk), dim3(256), 0, s, S.nn, seed, mb.v);
+        hipLaunchKernelGGL(k_lpc_pow_spmv, dim3(nblk), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(),
+                           S.sliceW.ptr<int>(), S.cols.ptr<int>(), S.vals.ptr<float>(), S.cRowPtr.ptr<int>(), cCol,
+                           cVal, mb.v, nullptr, 0, mb.u);
+        hipLaunchKernelGGL(k_lpc_lower, dim3(nblk), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(), S.cols.ptr<int>(),
+                           S.diagSlot.ptr<int>(), L.G.ptr<bf16_t>(), mb.u, mb.y);
+        hipLaunchKernelGGL(k_lpc_upper<UP_PCG>, dim3(nblk), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(),
+                           S.cols.ptr<int>(), S.diagSlot.ptr<int>(), S.rowLen.ptr<int>(), L.G.ptr<bf16_t>(), mb.y,
+                           mb.u, nullptr, nullptr, mb.s, nullptr, mb.part);
+        hipLaunchKernelGGL(k_lpc_sub, dim3(nblk), dim3(256), 0, s, S.nn, mb.v, mb.s, w, mb.e);
+        hipLaunchKernelGGL(k_lpc_pow_spmv, dim3(nblk), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(),
+                           S.sliceW.ptr<int>(), S.cols.ptr<int>(), S.vals.ptr<float>(), S.cRowPtr.ptr<int>(), cCol,
+                           cVal, mb.e, nullptr, 0, mb.p);
+        hipLaunchKernelGGL(k_lpc_lower, dim3(nblk), dim3(256), 0, s, S.nn, S.sliceOff.ptr<int>(), S.cols.ptr<int>(),
+                           S.diagSlot.ptr<int>(), L.G.ptr<bf16_t>(), mb.p, mb.w);
+        hipLaunchKernelGGL(k_lpc_smooth_dots, dim3(nblk), dim3(256), 0, s, S.nn, mb.e, mb.p, mb.v, mb.u, mb.pdots);
+        hipLaunchKernelGGL(k_lpc_smooth_final, dim3(1), dim3(256), 0, s, nblk, mb.pdots, lam, w, scal, mb.sc);
+        hipLaunchKernelGGL(k_lpc_sol_grad, dim3(nblk), dim3(256), 0, s, t, mb.y, mb.w, mb.u, mb.p, L.S.ptr<float>(),
+                           scal, mb.sc, lam, bb.Z);
         CS_LAUNCH_CHECK();
     }
     N.grad.as<float>(W_COUNT);

@@ -11,7 +11,9 @@ steps the scene (multigrid PCG: the systems do not depend on the preconditioner)
 and, after every step, takes one Adam step per
 cloth on the Kaporin loss of that cloth's assembled system (cs_lpc_train_step:
 the network's forward and backward kernels on the device), optionally plus the
-solution term (--sol-weight).  Weights carry over from round to round; the
+solution term (--sol-weight) and the smoothing term (--smooth-weight: the
+A-energy one sweep of the learned smoother leaves of a random probe, the figure
+that sets its worth inside the multigrid cycle).  Weights carry over from round to round; the
 optimiser restarts with each round.  The packaged weights come from
 `--minutes 8 --seed 0` (Kaporin loss only).
 """
@@ -23,8 +25,8 @@ import time
 import numpy as np
 
 from . import CS_MESH_CLOTH, Simulator, scenes
-from ._cabi import (CS_E_NOT_CONVERGED, CS_OK, CS_PRECOND_AMG, CS_PRECOND_BLOCK_JACOBI, CS_PRECOND_LEARNED,
-                    default_params, material_struct)
+from ._cabi import (CS_E_NOT_CONVERGED, CS_OK, CS_PRECOND_AMG, CS_PRECOND_AMG_LEARNED, CS_PRECOND_BLOCK_JACOBI,
+                    CS_PRECOND_LEARNED, default_params, material_struct)
 from .lpc_layout import init_weights
 
 SIDES = (65, 129, 257, 513)
@@ -59,22 +61,26 @@ def training_scene(rng, slots):
     return scenes.SceneSpec("train", p, cloths, obstacles, 0)
 
 
+HELD_OUT = {"block Jacobi": CS_PRECOND_BLOCK_JACOBI, "learned": CS_PRECOND_LEARNED,
+            "multigrid": CS_PRECOND_AMG, "multigrid + learned smoother": CS_PRECOND_AMG_LEARNED}
+
+
 def held_out_iterations(w, n=129, steps=6):
-    """PCG iterations of a held-out crumple system: block Jacobi vs the learned preconditioner."""
+    """PCG iterations of a held-out crumple system under each preconditioner."""
     out = {}
-    for pre in (CS_PRECOND_BLOCK_JACOBI, CS_PRECOND_LEARNED):
+    for name, pre in HELD_OUT.items():
         spec = scenes.crumple(n, seed=123)
         spec.params.precond = pre
         sim = Simulator(spec)
-        if pre == CS_PRECOND_LEARNED:
+        if pre in (CS_PRECOND_LEARNED, CS_PRECOND_AMG_LEARNED):
             sim.set_learned_weights(w)
         its = []
         for _ in range(steps):
             s, st = sim.step()
             its.append(st.pcg_iters)
-        out[pre] = its
+        out[name] = its
         sim.close()
-    return out[CS_PRECOND_BLOCK_JACOBI], out[CS_PRECOND_LEARNED]
+    return out
 
 
 def main(argv=None):
@@ -88,6 +94,8 @@ def main(argv=None):
     ap.add_argument("--steps-per-round", type=int, default=40)
     ap.add_argument("--sol-weight", type=float, default=0.0,
                     help="weight of the solution term (1 - cos^2_A between M^-1 b and the step's solution)")
+    ap.add_argument("--smooth-weight", type=float, default=0.0,
+                    help="weight of the smoothing term (|(I - G^T G A) v|_A^2 / |v|_A^2, v a random probe)")
     a = ap.parse_args(argv)
     rng = np.random.default_rng(a.seed)
     w = np.load(a.init, allow_pickle=False).astype(np.float32) if a.init else init_weights(a.seed)
@@ -109,7 +117,8 @@ def main(argv=None):
             frac = min(1.0, (time.time() - t0) / budget)
             lr = a.lr * 0.5 * (1.0 + math.cos(math.pi * frac))
             for c in range(len(spec.cloths)):
-                loss = sim.lpc_train_step(c, lr, sol_weight=a.sol_weight)
+                loss = sim.lpc_train_step(c, lr, sol_weight=a.sol_weight, smooth_weight=a.smooth_weight,
+                                          seed=total)
                 total += 1
                 cur = sim.learned_weights()
                 if not (np.isfinite(loss) and np.isfinite(cur).all()):
@@ -128,8 +137,8 @@ def main(argv=None):
     if a.out:
         np.save(a.out, w.astype(np.float32), allow_pickle=False)
         print("wrote", a.out, flush=True)
-    jac, lrn = held_out_iterations(w)
-    print("held-out crumple129 PCG iterations: block Jacobi %s  learned %s" % (jac, lrn), flush=True)
+    for name, its in held_out_iterations(w).items():
+        print("held-out crumple129 PCG iterations, %s: %s (sum %d)" % (name, its, sum(its)), flush=True)
     return 0
 
 

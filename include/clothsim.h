@@ -294,6 +294,12 @@ cs_status cs_lpc_factor(cs_ctx* ctx, int cloth_id, float* diag, float* lower, in
  * cs_lpc_weight_count() floats, may be NULL) and, when lr > 0, one Adam step (beta 0.9 / 0.999)
  * on the context's weights. */
 cs_status cs_lpc_train_step(cs_ctx* ctx, int cloth_id, float lr, float sol_weight, float* loss, float* grad);
+/* cs_lpc_train_step plus smooth_weight |(I - G^T G A) v|_A^2 / |v|_A^2 for the probe v of
+ * `seed` (v_i = (+-1, +-1, +-1) from a hash of (i, seed)): the energy one sweep of the learned
+ * smoother leaves, the figure a multigrid smoother is judged by (no counterpart in the
+ * reference). */
+cs_status cs_lpc_train_step_smooth(cs_ctx* ctx, int cloth_id, float lr, float sol_weight, float smooth_weight,
+                                   unsigned int seed, float* loss, float* grad);
 /* The damping w of the learned smoother (CS_PRECOND_AMG_LEARNED) of one cloth's last assembled
  * system: x += w G^T G (b - A x). */
 cs_status cs_lpc_smoother_omega(cs_ctx* ctx, int cloth_id, float* omega);

@@ -310,6 +310,66 @@ def solution_term(g, o, q, S, b, x):
     return L, do, dq
 
 
+def probe(n, seed):
+    """The training probe of the smoothing term: v_i = (+-1, +-1, +-1) from a hash of (i, seed),
+    as the device forms it (uint32 arithmetic)."""
+    i = np.arange(n, dtype=np.uint32)
+    with np.errstate(over="ignore"):
+        h = i * np.uint32(2654435761) + np.uint32(seed) * np.uint32(2246822519)
+        h = h ^ (h >> np.uint32(15))
+        h = h * np.uint32(2246822519)
+        h = h ^ (h >> np.uint32(13))
+    return np.stack([np.where(h & np.uint32(k), 1.0, -1.0) for k in (1, 2, 4)], axis=1)
+
+
+def smoothing_term(g, o, q, S, v, omega=1.0):
+    """L_sm = |(I - omega M A) v|_A^2 / |v|_A^2, M = G^T G: how much of a probe's error in the
+    energy norm one smoothing sweep leaves (for a white probe the energy is mostly in the high
+    frequencies, the ones a smoother has to remove); and its gradients d/do, d/dq through Ghat."""
+    Gd, Gl = factor(g, o, q, S)
+    A = _csr(g, g.vals)
+    v = np.asarray(v, dtype=np.float64).reshape(-1, 3)
+    i_l, j_l = g.src[g.low], g.dst[g.low]
+
+    def Gmul(x):
+        y = np.einsum("nab,nb->na", Gd, x)
+        np.add.at(y, i_l, np.einsum("eab,eb->ea", Gl, x[j_l]))
+        return y
+
+    def GTmul(y):
+        z = np.einsum("nba,nb->na", Gd, y)
+        np.add.at(z, j_l, np.einsum("eba,eb->ea", Gl, y[i_l]))
+        return z
+
+    u = (A @ v.ravel()).reshape(-1, 3)
+    y = Gmul(u)
+    e = v - omega * GTmul(y)
+    p = (A @ e.ravel()).reshape(-1, 3)
+    w = Gmul(p)
+    num = float((e * p).sum())
+    den = float((v * u).sum())
+    L = num / den
+    # dL/ds = -2 omega A e / den with s = G^T G u;  ds = dG^T y + G^T dG u, so
+    # dL/dG_ij = c2 (y_i p_j^T + w_i u_j^T), c2 = -2 omega / den
+    c2 = -2.0 * omega / den
+
+    def dG(i, j):
+        return c2 * (np.outer(y[i], p[j]) + np.outer(w[i], u[j]))
+    do = np.zeros_like(o)
+    dq = np.zeros_like(q)
+    for i in range(g.n):
+        D = dG(i, i) @ S[i].T
+        do[i, 0] = D[0, 0] * np.exp(o[i, 0])
+        do[i, 1] = D[1, 1] * np.exp(o[i, 1])
+        do[i, 2] = D[2, 2] * np.exp(o[i, 2])
+        do[i, 3] = D[1, 0]
+        do[i, 4] = D[2, 0]
+        do[i, 5] = D[2, 1]
+    for t in range(len(g.low)):
+        dq[t, :9] = (dG(i_l[t], j_l[t]) @ S[j_l[t]].T).reshape(9)
+    return L, do, dq
+
+
 def backward(flat, g, c, do, dq):
     """Gradient of the loss w.r.t. the flat weight vector, given d loss / d o and d loss / d q
     (the same rounding points as the forward when it ran with bf16, treated as identity)."""
@@ -366,7 +426,7 @@ def backward(flat, g, c, do, dq):
     return pack(gr)
 
 
-def loss_and_grad(flat, g, bf=False, b=None, x=None, lam=1.0):
+def loss_and_grad(flat, g, bf=False, b=None, x=None, lam=1.0, v=None, lam_s=1.0, omega=1.0):
     o, q, c = forward(flat, g, bf)
     loss, T, do, dq = kaporin(g, o, q, c["Ah"])
     if b is not None:
@@ -374,4 +434,9 @@ def loss_and_grad(flat, g, bf=False, b=None, x=None, lam=1.0):
         loss = loss + lam * ls
         do = do + lam * dos
         dq = dq + lam * dqs
+    if v is not None:
+        lsm, dos, dqs = smoothing_term(g, o, q, c["S"], v, omega)
+        loss = loss + lam_s * lsm
+        do = do + lam_s * dos
+        dq = dq + lam_s * dqs
     return loss, backward(flat, g, c, do, dq)

@@ -94,16 +94,24 @@ def test_preconditioner_apply_and_symmetry():
     assert float((r1 * z1).sum()) > 0
 
 
-@pytest.mark.parametrize("solved", [False, True])
-def test_loss_and_gradient_match_oracle(solved):
-    """The Kaporin loss alone (assembled system) and with the solution term (after a solve)."""
+@pytest.mark.parametrize("mode", ["kaporin", "solution", "smoothing"])
+def test_loss_and_gradient_match_oracle(mode):
+    """The Kaporin loss alone (assembled system), with the solution term (after a solve) and
+    with the smoothing term (probe of seed 7, weight 0.5)."""
     sim, g, w = system(seed=1)
+    solved = mode == "solution"
     if solved:
         b = sim.assemble(0)[3]   # the same state: the same system
         dv, st = sim.solve(0)
         assert st.pcg_converged
-    loss, grad = sim.lpc_train_step(0, lr=0.0, grad=True, sol_weight=1.0)
-    lo, go = lpc.loss_and_grad(w, g, bf=True, b=b if solved else None, x=dv if solved else None)
+    if mode == "smoothing":
+        loss, grad = sim.lpc_train_step(0, lr=0.0, grad=True, smooth_weight=0.5, seed=7)
+        lo, go = lpc.loss_and_grad(w, g, bf=True, v=lpc.probe(g.n, 7), lam_s=0.5)
+        lk, _ = lpc.loss_and_grad(w, g, bf=True)
+        print("smoothing term: oracle %.6g (one sweep keeps that share of the probe's A-energy)" % ((lo - lk) / 0.5))
+    else:
+        loss, grad = sim.lpc_train_step(0, lr=0.0, grad=True, sol_weight=1.0)
+        lo, go = lpc.loss_and_grad(w, g, bf=True, b=b if solved else None, x=dv if solved else None)
     assert abs(loss - lo) <= 1e-2 * abs(lo) + 1e-4, (loss, lo)
     rel = np.linalg.norm(grad - go) / np.linalg.norm(go)
     # every parameter block (a wrong index shows up as a large error in one block)

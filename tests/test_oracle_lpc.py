@@ -189,3 +189,36 @@ def test_solution_term_gradient_and_zero_at_eigenvectors():
     x = V[:, np.argmax(lam.real)].real
     L, _, _ = lpc.solution_term(g, o, q, c["S"], A @ x, x)
     assert abs(L) < 1e-10
+
+
+def test_smoothing_term_gradient_and_dense_value():
+    """L_sm = |(I - w G^T G A) v|_A^2 / |v|_A^2 against dense algebra, and its gradient against
+    central finite differences; the device's probe hash restated (+-1 entries, reproducible)."""
+    g, vals, rp, ci = small_system(seed=12)
+    A = dense(rp, ci, vals, g.n)
+    v = lpc.probe(g.n, 7)
+    assert set(np.unique(v)) == {-1.0, 1.0} and np.array_equal(v, lpc.probe(g.n, 7))
+    assert not np.array_equal(v, lpc.probe(g.n, 8))
+    w = weights_with_outputs(seed=4, scale=0.1)
+    o, q, c = lpc.forward(w, g)
+    Gd, Gl = lpc.factor(g, o, q, c["S"])
+    G = np.zeros_like(A)
+    for i in range(g.n):
+        G[3 * i:3 * i + 3, 3 * i:3 * i + 3] = Gd[i]
+    for t, (i, j) in enumerate(zip(g.src[g.low], g.dst[g.low])):
+        G[3 * i:3 * i + 3, 3 * j:3 * j + 3] = Gl[t]
+    for omega in (1.0, 0.7):
+        e = v.ravel() - omega * G.T @ G @ A @ v.ravel()
+        dense_L = (e @ A @ e) / (v.ravel() @ A @ v.ravel())
+        L, _, _ = lpc.smoothing_term(g, o, q, c["S"], v, omega)
+        assert abs(L - dense_L) <= 1e-12 * max(1.0, abs(dense_L)), (omega, L, dense_L)
+    base = lpc.loss_and_grad(w, g)[0]
+    loss, grad = lpc.loss_and_grad(w, g, v=v, lam_s=0.5)
+    assert abs(loss - base - 0.5 * lpc.smoothing_term(g, o, q, c["S"], v)[0]) <= 1e-12 * max(1.0, abs(loss))
+    rng = np.random.default_rng(2)
+    for t in rng.choice(lpc.COUNT, 25, replace=False):
+        wp, wm = w.copy(), w.copy()
+        wp[t] += 1e-6
+        wm[t] -= 1e-6
+        fd = (lpc.loss_and_grad(wp, g, v=v, lam_s=0.5)[0] - lpc.loss_and_grad(wm, g, v=v, lam_s=0.5)[0]) / 2e-6
+        assert abs(fd - grad[t]) <= 1e-6 * max(1.0, abs(fd)) + 2e-9, (t, fd, grad[t])
