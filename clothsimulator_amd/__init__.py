@@ -12,7 +12,8 @@ import numpy as np
 
 from ._cabi import (CS_E_NOT_CONVERGED, CS_MESH_CLOTH, CS_PRECOND_AMG_LEARNED, CS_PRECOND_LEARNED, CS_MESH_OBSTACLE, CS_OK, IMPACT_DTYPE, LIB_PATH, PROX_DTYPE,
                     STATUS_NAMES, ImpactRec, MaterialIn, MeshArrays, MeshIn, Params, ProximityRec, SceneInfo,
-                    Stats, Transform, axis_angle, default_params, fptr, iptr, material_struct, transform_struct)
+                    Remeshing, Stats, Transform, axis_angle, default_params, fptr, iptr, material_struct,
+                    remeshing_struct, transform_struct)
 
 __all__ = ["load_library", "Simulator", "ClothError", "default_params", "material_struct", "MeshArrays"]
 
@@ -27,7 +28,8 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_lpc_train_step", "cs_lpc_smoother_omega", "cs_batch_tuning", "cs_add_motion", "cs_motion_eval",
             "cs_set_obstacle_motion", "cs_add_moving_handles", "cs_set_step_count", "cs_get_step_count",
             "cs_scene_motion", "cs_get_topology", "cs_write_config", "cs_impact_response",
-            "cs_solve_bsr_noalloc", "cs_lpc_train_step_smooth"]
+            "cs_solve_bsr_noalloc", "cs_lpc_train_step_smooth", "cs_default_remeshing", "cs_remesh_mesh",
+            "cs_remesh_result_counts", "cs_remesh_result_get", "cs_remesh_result_free"]
 
 # trained weights of the learned preconditioner shipped with the package (float32, .npy)
 LPC_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lpc_weights.npy")
@@ -100,8 +102,51 @@ def load_library(path=LIB_PATH):
     IP = C.POINTER(C.c_int)
     L.cs_get_topology.argtypes = [vp, C.c_int, C.c_int, IP, IP, F, IP]
     L.cs_write_config.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p), C.c_int]
+    RP = C.POINTER(Remeshing)
+    UP = C.POINTER(C.c_ubyte)
+    L.cs_default_remeshing.argtypes = [RP]
+    L.cs_default_remeshing.restype = None
+    L.cs_remesh_mesh.argtypes = [RP, C.c_int, F, F, F, UP, C.c_int, F, IP, C.c_int, IP, C.c_int, IP, F, IP,
+                                 C.POINTER(vp)]
+    L.cs_remesh_result_counts.argtypes = [vp, IP, IP, IP, IP]
+    L.cs_remesh_result_get.argtypes = [vp, F, F, F, UP, F, IP, IP, IP, F, IP]
+    L.cs_remesh_result_free.argtypes = [vp]
+    L.cs_remesh_result_free.restype = None
     _lib = L
     return L
+
+
+def remesh_mesh(r, x, x0, v, preserve, uv, vert_node, face_verts, edge_nodes, face_sizing, flip_count=0):
+    """The remeshing operators on one mesh in host arrays (cs_remesh_mesh; needs no device).
+    Returns (dict of x, x0, v, preserve, uv, vnode, fv, en, sizing, node_origin; flip_count)."""
+    L = load_library()
+    f32 = lambda a, k: np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1, k))
+    i32 = lambda a, k: np.ascontiguousarray(np.asarray(a, np.int32).reshape(-1, k))
+    x, x0, v, uv = f32(x, 3), f32(x0, 3), f32(v, 3), f32(uv, 2)
+    pres = np.ascontiguousarray(np.asarray(preserve, np.uint8))
+    vnode, fv, en, fs = i32(vert_node, 1).ravel(), i32(face_verts, 3), i32(edge_nodes, 2), f32(face_sizing, 4)
+    fc = C.c_int(flip_count)
+    res = C.c_void_p()
+    up = lambda a: a.ctypes.data_as(C.POINTER(C.c_ubyte))
+    st = L.cs_remesh_mesh(C.byref(r), len(x), fptr(x), fptr(x0), fptr(v), up(pres), len(uv), fptr(uv), iptr(vnode),
+                          len(fv), iptr(fv), len(en), iptr(en), fptr(fs), C.byref(fc), C.byref(res))
+    if st != CS_OK:
+        raise ClothError(st, "cs_remesh_mesh")
+    try:
+        n = [C.c_int() for _ in range(4)]
+        L.cs_remesh_result_counts(res, *[C.byref(k) for k in n])
+        nn, nv, nf, ne = (k.value for k in n)
+        o = dict(x=np.zeros((nn, 3), np.float32), x0=np.zeros((nn, 3), np.float32), v=np.zeros((nn, 3), np.float32),
+                 preserve=np.zeros(nn, np.uint8), uv=np.zeros((nv, 2), np.float32), vnode=np.zeros(nv, np.int32),
+                 fv=np.zeros((nf, 3), np.int32), en=np.zeros((ne, 2), np.int32), sizing=np.zeros((nv, 4), np.float32),
+                 node_origin=np.zeros(nn, np.int32))
+        L.cs_remesh_result_get(res, fptr(o["x"]), fptr(o["x0"]), fptr(o["v"]), up(o["preserve"]), fptr(o["uv"]),
+                               iptr(o["vnode"]), iptr(o["fv"]), iptr(o["en"]), fptr(o["sizing"]),
+                               iptr(o["node_origin"]))
+    finally:
+        L.cs_remesh_result_free(res)
+    o["preserve"] = o["preserve"].astype(bool)
+    return o, fc.value
 
 
 def _keys(times, keys):

@@ -185,6 +185,23 @@ def axis_angle(axis, degrees):
     return (float(np.cos(h, dtype=np.float32)), float(s * a[0]), float(s * a[1]), float(s * a[2]))
 
 
+class Remeshing(C.Structure):
+    """cs_remeshing: a cloth's "remeshing" block."""
+    _fields_ = [(n, C.c_float) for n in ("refine_angle", "refine_velocity", "refine_compression", "rib_stiffening",
+                                          "size_min", "size_max", "aspect_min", "flip_threshold")]
+
+
+def remeshing_struct(refine_angle=np.inf, refine_velocity=np.inf, refine_compression=np.inf, rib_stiffening=1.0,
+                     size=None, aspect_min=-np.inf, flip_threshold=1e-2):
+    """The block's fields with the reference's defaults (src/Remeshing.cu:3-19)."""
+    r = Remeshing()
+    r.refine_angle, r.refine_velocity, r.refine_compression = refine_angle, refine_velocity, refine_compression
+    r.rib_stiffening = rib_stiffening
+    r.size_min, r.size_max = (-np.inf, np.inf) if size is None else size
+    r.aspect_min, r.flip_threshold = aspect_min, flip_threshold
+    return r
+
+
 PROX_DTYPE = np.dtype([("nodes", "<i4", 4), ("n", "<f4", 3), ("mu", "<f4"), ("stiffness", "<f4"), ("w", "<f4", 4)])
 IMPACT_DTYPE = np.dtype({"names": ["type", "nodes", "t", "w", "n"],
                          "formats": ["<i4", ("<i4", 4), "<f8", ("<f8", 4), ("<f8", 3)],

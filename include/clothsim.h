@@ -369,6 +369,39 @@ cs_status cs_scene_motion(const char* config_path, const char* root_dir, int mot
                           cs_transform* out);
 const char* cs_scene_error(void);
 
+/* ---- remeshing: Cloth::remeshingStep (src/Cloth.cu:815-833), run once per frame by
+ * Simulator::remeshingStep (src/Simulator.cu:686-695) when the scene does not disable it ---- */
+typedef struct cs_remeshing { /* a cloth's "remeshing" block (src/Remeshing.cu:3-19) */
+    float refine_angle, refine_velocity, refine_compression, rib_stiffening;
+    float size_min, size_max, aspect_min, flip_threshold;
+} cs_remeshing;
+/* The block's defaults: refine_* = +inf, rib_stiffening 1, size (-inf, +inf), aspect_min -inf,
+ * flip_threshold 1e-2. */
+void cs_default_remeshing(cs_remeshing* r);
+/* The edge operators of the remeshing step (flip, split, collapse; src/Cloth.cu:268-646,
+ * src/Operator.cu:7-217) on one mesh held in host arrays, with the sizing given per face
+ * (face_sizing: 4 floats per face, a row-major 2x2 tensor).  Nodes: x, x0, v (3 each), preserve
+ * (1: never collapsed away: boundary, seam and handle nodes); uv vertices with their node;
+ * faces as uv-vertex triples; edges as node pairs, in list order (a freshly loaded mesh lists its
+ * distinct (min, max) node pairs ascending).  flip_count carries the size of the last flip pass
+ * between calls (the reference keeps it in a function-level static, src/Cloth.cu:321); start it
+ * at 0.  Host only: needs no device.  The result is read with cs_remesh_result_* and freed with
+ * cs_remesh_result_free.  CS_E_FORMAT when a face edge is missing from edge_nodes or an index
+ * is out of range. */
+typedef struct cs_remesh_result cs_remesh_result;
+cs_status cs_remesh_mesh(const cs_remeshing* r, int n_nodes, const float* x, const float* x0, const float* v,
+                         const unsigned char* preserve, int n_verts, const float* uv, const int* vert_node,
+                         int n_faces, const int* face_verts, int n_edges, const int* edge_nodes,
+                         const float* face_sizing, int* flip_count, cs_remesh_result** out);
+cs_status cs_remesh_result_counts(const cs_remesh_result* res, int* n_nodes, int* n_verts, int* n_faces,
+                                  int* n_edges);
+/* Any pointer may be NULL.  vert_sizing: 4 per vertex; node_origin: the input node of each
+ * output node, -1 for a node a split made. */
+cs_status cs_remesh_result_get(const cs_remesh_result* res, float* x, float* x0, float* v, unsigned char* preserve,
+                               float* uv, int* vert_node, int* face_verts, int* edge_nodes, float* vert_sizing,
+                               int* node_origin);
+void cs_remesh_result_free(cs_remesh_result* res);
+
 #ifdef __cplusplus
 }
 #endif
