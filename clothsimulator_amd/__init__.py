@@ -29,7 +29,8 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_set_obstacle_motion", "cs_add_moving_handles", "cs_set_step_count", "cs_get_step_count",
             "cs_scene_motion", "cs_get_topology", "cs_write_config", "cs_impact_response",
             "cs_solve_bsr_noalloc", "cs_lpc_train_step_smooth", "cs_default_remeshing", "cs_remesh_mesh",
-            "cs_remesh_result_counts", "cs_remesh_result_get", "cs_remesh_result_free"]
+            "cs_remesh_result_counts", "cs_remesh_result_get", "cs_remesh_result_free", "cs_set_remeshing",
+            "cs_remesh", "cs_remesh_sizing"]
 
 # trained weights of the learned preconditioner shipped with the package (float32, .npy)
 LPC_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lpc_weights.npy")
@@ -110,6 +111,9 @@ def load_library(path=LIB_PATH):
                                  C.POINTER(vp)]
     L.cs_remesh_result_counts.argtypes = [vp, IP, IP, IP, IP]
     L.cs_remesh_result_get.argtypes = [vp, F, F, F, UP, F, IP, IP, IP, F, IP]
+    L.cs_set_remeshing.argtypes = [vp, C.c_int, RP]
+    L.cs_remesh.argtypes = [vp]
+    L.cs_remesh_sizing.argtypes = [vp, C.c_int, RP, F, F]
     L.cs_remesh_result_free.argtypes = [vp]
     L.cs_remesh_result_free.restype = None
     _lib = L
@@ -474,6 +478,23 @@ class Simulator:
             self._check(self.L.cs_lpc_train_step(self.h, cloth, lr, sol_weight, C.byref(loss), gp),
                         "cs_lpc_train_step")
         return (loss.value, g) if grad else loss.value
+
+    def set_remeshing(self, cloth, r):
+        """Remeshing parameters of one cloth (a Remeshing struct, or None to clear them)."""
+        self._check(self.L.cs_set_remeshing(self.h, cloth, C.byref(r) if r is not None else None),
+                    "cs_set_remeshing")
+
+    def remesh(self):
+        """The remeshing step of every cloth with remeshing parameters (cs_remesh)."""
+        self._check(self.L.cs_remesh(self.h), "cs_remesh")
+
+    def remesh_sizing(self, cloth, r):
+        """Face sizing (nf x 4) and node planes (nn x 8) the remeshing step would use now."""
+        nn, nf, _ = self.counts(CS_MESH_CLOTH, cloth)
+        fs = np.zeros((nf, 4), np.float32)
+        pl = np.zeros((nn, 8), np.float32)
+        self._check(self.L.cs_remesh_sizing(self.h, cloth, C.byref(r), fptr(fs), fptr(pl)), "cs_remesh_sizing")
+        return fs, pl
 
     def smoother_omega(self, cloth=0):
         """Damping of the learned smoother of the last assembled system (CS_PRECOND_AMG_LEARNED)."""

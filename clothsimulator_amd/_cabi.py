@@ -146,6 +146,7 @@ class SceneInfo(C.Structure):
         ("n_cloth_faces", C.c_int),
         ("n_motions", C.c_int),
         ("n_moving_obstacles", C.c_int),
+        ("remeshing", C.c_int),
     ]
 
 

@@ -824,6 +824,39 @@ cs_status cs_lpc_train_step_smooth(cs_ctx* C, int cloth_id, float lr, float sol_
     });
 }
 
+cs_status cs_set_remeshing(cs_ctx* C, int cloth_id, const cs_remeshing* r) {
+    if (!C || cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
+    if ((int)C->remeshCfg.size() < C->nCloths) C->remeshCfg.resize(C->nCloths);
+    cs_ctx::RemeshCfg& R = C->remeshCfg[cloth_id];
+    R.has = r != nullptr;
+    if (r)
+        R.r = RemeshParams{r->refine_angle, r->refine_velocity, r->refine_compression, r->rib_stiffening,
+                           r->size_min, r->size_max, r->aspect_min, r->flip_threshold};
+    return CS_OK;
+}
+
+cs_status cs_remesh(cs_ctx* C) {
+    if (!C) return CS_E_ARG;
+    return guard(C, [&]() {
+        remesh_context(C);
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_remesh_sizing(cs_ctx* C, int cloth_id, const cs_remeshing* r, float* face_sizing, float* planes) {
+    if (!C || !r || !face_sizing || cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
+    return guard(C, [&]() {
+        if (!C->finalized) host_finalize(C);
+        const RemeshParams p{r->refine_angle, r->refine_velocity, r->refine_compression, r->rib_stiffening,
+                             r->size_min, r->size_max, r->aspect_min, r->flip_threshold};
+        std::vector<float> fs, pl;
+        remesh_sizing(C, cloth_id, p, fs, planes ? &pl : nullptr);
+        std::copy(fs.begin(), fs.end(), face_sizing);
+        if (planes) std::copy(pl.begin(), pl.end(), planes);
+        return (int)CS_OK;
+    });
+}
+
 cs_status cs_batch_tuning(const cs_ctx* C, int* rows_per_block, int* chunk, int* trials) {
     if (!C || !rows_per_block || !chunk || !trials) return CS_E_ARG;
     *rows_per_block = C->batchTune.rows;

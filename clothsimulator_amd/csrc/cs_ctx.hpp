@@ -21,6 +21,7 @@
 #include "cs_common.hpp"
 #include "cs_motion.hpp"
 #include "cs_prims.hpp"
+#include "cs_remesh.hpp"
 
 namespace cs {
 
@@ -277,4 +278,13 @@ struct cs_ctx {
     uint32_t* hCount = nullptr;  // pinned small readback
     cs::Scratch ws;
     cs::KernelTimer kt;
+    // remeshing: per cloth the "remeshing" block (has = 0: the cloth is never remeshed), per node
+    // the preserve flag (boundary, seam and handle nodes of the loaded mesh; set on the first
+    // remesh), the size of the last flip pass (shared by every cloth, as the reference's static)
+    struct RemeshCfg { bool has = false; cs::RemeshParams r; };
+    std::vector<RemeshCfg> remeshCfg;
+    std::vector<unsigned char> hpres;
+    bool presReady = false;
+    int flipCount = 0;
+    cs::DBuf rmUv, rmPlanes, rmSizing;
 };

@@ -16,6 +16,10 @@ void host_finalize(cs_ctx* C);
 
 // geometry (cs_geometry.hip)
 void launch_face_geometry(cs_ctx* C, int f0, int nf);
+// remeshing (cs_sizing.hip, cs_remesh_ctx.cpp)
+void remesh_sizing(cs_ctx* C, int c, const RemeshParams& r, std::vector<float>& faceSizing,
+                   std::vector<float>* planes);
+void remesh_context(cs_ctx* C);
 void launch_node_geometry(cs_ctx* C, int n0, int nn);
 void launch_update_positions(cs_ctx* C, int n0, int nn);
 void launch_update_velocities(cs_ctx* C, int n0, int nn);

@@ -583,9 +583,23 @@ void rmesh_output(const RMesh& m, int nInputNodes, RMeshOut& o) {
     o.fv.resize(3 * Fc);
     for (size_t i = 0; i < Fc; i++)
         for (int c = 0; c < 3; c++) o.fv[3 * i + c] = vi[m.fv[m.faces[i]][c]];
+    std::vector<int> ei(m.en.size(), -1), fi(m.fv.size(), -1);
+    for (size_t i = 0; i < E; i++) ei[m.edges[i]] = (int)i;
+    for (size_t i = 0; i < Fc; i++) fi[m.faces[i]] = (int)i;
     o.en.resize(2 * E);
-    for (size_t i = 0; i < E; i++)
-        for (int c = 0; c < 2; c++) o.en[2 * i + c] = ni[m.en[m.edges[i]][c]];
+    o.eopp.resize(2 * E);
+    o.eadj.resize(2 * E);
+    for (size_t i = 0; i < E; i++) {
+        const int e = m.edges[i];
+        for (int c = 0; c < 2; c++) {
+            o.en[2 * i + c] = ni[m.en[e][c]];
+            o.eopp[2 * i + c] = m.eopp[e][c] >= 0 ? vi[m.eopp[e][c]] : -1;
+            o.eadj[2 * i + c] = m.eadj[e][c] >= 0 ? fi[m.eadj[e][c]] : -1;
+        }
+    }
+    o.fe.resize(3 * Fc);
+    for (size_t i = 0; i < Fc; i++)
+        for (int c = 0; c < 3; c++) o.fe[3 * i + c] = ei[m.fe[m.faces[i]][c]];
 }
 
 }  // namespace cs

@@ -49,6 +49,7 @@ struct RMeshOut {
     std::vector<float> x, x0, v, uv, sizing;
     std::vector<unsigned char> preserve;
     std::vector<int> vnode, fv, en;
+    std::vector<int> fe, eopp, eadj;  // face edges; per edge side the opposite vertex and the face (-1: none)
     std::vector<int> nodeOrigin;  // per output node: its input node id, or -1 for a new node
 };
 void rmesh_output(const RMesh& m, int nInputNodes, RMeshOut& o);

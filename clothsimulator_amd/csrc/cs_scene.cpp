@@ -8,8 +8,8 @@
 // context and adds the cloths, motions (keyframe lists or a .mot file,
 // src/Simulator.cu:39-79), obstacles (listed, or the "obstacles": "<pattern>"
 // sequence, each moving with the motion of its index, src/Simulator.cu:90-101)
-// and handles in the reference's order.  "remeshing" blocks and the "disable"
-// list are read and ignored (this library never remeshes).
+// and handles in the reference's order; each cloth's "remeshing" block is set on
+// the context unless the "disable" list names "remeshing".
 #include <cmath>
 #include <climits>
 #include <cstring>
@@ -332,7 +332,25 @@ struct Scene {
     std::vector<cs_material_in> materials;
     struct Handles { int cloth, motion; std::vector<int> nodes; };
     std::vector<Handles> handles;
+    std::vector<cs_remeshing> remeshing;  // per cloth: its "remeshing" block (defaults when absent)
 };
+
+// Remeshing ctor (src/Remeshing.cu:3-19)
+cs_remeshing remeshing_in(const cs::json::Value& r) {
+    cs_remeshing o;
+    cs_default_remeshing(&o);
+    o.refine_angle = num(r["refine_angle"], INFINITY, "remeshing.refine_angle");
+    o.refine_velocity = num(r["refine_velocity"], INFINITY, "remeshing.refine_velocity");
+    o.refine_compression = num(r["refine_compression"], INFINITY, "remeshing.refine_compression");
+    o.rib_stiffening = num(r["rib_stiffening"], 1.0f, "remeshing.rib_stiffening");
+    if (!r["size"].is_null()) {
+        o.size_min = (float)r["size"][0].as_number("remeshing.size");
+        o.size_max = (float)r["size"][1].as_number("remeshing.size");
+    }
+    o.aspect_min = num(r["aspect_min"], -INFINITY, "remeshing.aspect_min");
+    o.flip_threshold = num(r["flip_threshold"], 1e-2f, "remeshing.flip_threshold");
+    return o;
+}
 
 Scene read_scene(const std::string& path, const std::string& root, const cs_params* base) {
     Scene S;
@@ -370,7 +388,13 @@ Scene read_scene(const std::string& path, const std::string& root, const cs_para
         Transform T(c["transform"]);
         S.cloths.push_back(load_obj(resolve(root, c["mesh"].as_string("mesh")), T));
         S.materials.push_back(material_in(c["materials"].is_array() ? c["materials"][0] : c["materials"], root));
+        S.remeshing.push_back(remeshing_in(c["remeshing"]));
     }
+    // every cloth remeshes once per frame unless "disable" lists "remeshing" (src/Simulator.cu:858)
+    S.info.remeshing = 1;
+    const cs::json::Value& dis = j["disable"];
+    for (size_t i = 0; i < dis.size(); i++)
+        if (dis[i].is_string() && dis[i].as_string("disable") == "remeshing") S.info.remeshing = 0;
     S.motions = read_motions(j, root);
     const int nm = (int)S.motions.size();
     if (nm > CS_MAX_MOTIONS) throw SceneError(CS_E_CAPACITY, "more than CS_MAX_MOTIONS motions");
@@ -457,6 +481,9 @@ extern "C" cs_status cs_load_scene(const char* config_path, const char* root_dir
         for (const auto& h : S.handles)
             check(cs_add_moving_handles(C, ids[h.cloth], h.nodes.data(), (int)h.nodes.size(), h.motion),
                   "cs_add_moving_handles");
+        if (S.info.remeshing)
+            for (size_t i = 0; i < S.cloths.size(); i++)
+                check(cs_set_remeshing(C, ids[i], &S.remeshing[i]), "cs_set_remeshing");
         *ctx_out = C;
         return CS_OK;
     } catch (const SceneError& e) {

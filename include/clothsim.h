@@ -344,6 +344,8 @@ typedef struct cs_scene_info {
     int n_cloth_nodes, n_cloth_faces;
     int n_motions;           /* "motions": keyframe lists, or the bodies of a "motfile" */
     int n_moving_obstacles;  /* obstacles with a "motion" (every one of an "obstacles": "<pattern>" sequence) */
+    int remeshing;           /* 1 unless "disable" lists "remeshing": then every cloth has its "remeshing"
+                                block set (cs_set_remeshing) and a frame ends with cs_remesh */
 } cs_scene_info;
 
 /* Reads the configuration JSON at config_path; relative mesh and material paths are resolved
@@ -360,7 +362,8 @@ cs_status cs_load_scene(const char* config_path, const char* root_dir, const cs_
 /* Simulator::save's configuration file (src/Simulator.cu:950-957, src/Cloth.cu:865-872): the
  * configuration at config_path with cloth i's "mesh" set to cloth_meshes[i] and its "transform"
  * removed, written as JSON to out_path (comments are not kept).  A later cs_load_scene of out_path
- * resumes from those meshes; handle node ids stay valid (this library never remeshes). */
+ * resumes from those meshes; handle node ids stay valid for cloths that were not remeshed (cs_remesh
+ * renumbers nodes, as the reference's remeshing does). */
 cs_status cs_write_config(const char* config_path, const char* out_path, const char* const* cloth_meshes,
                           int n_cloths);
 /* The transformation of motion `motion_id` of a configuration at `time` (host only: parses the
@@ -401,6 +404,24 @@ cs_status cs_remesh_result_get(const cs_remesh_result* res, float* x, float* x0,
                                float* uv, int* vert_node, int* face_verts, int* edge_nodes, float* vert_sizing,
                                int* node_origin);
 void cs_remesh_result_free(cs_remesh_result* res);
+/* Sets (r != NULL) or clears one cloth's remeshing parameters (the Cloth ctor's Remeshing,
+ * src/Cloth.cu:3-17).  cs_load_scene sets them for every cloth unless the scene disables
+ * remeshing. */
+cs_status cs_set_remeshing(cs_ctx* ctx, int cloth_id, const cs_remeshing* r);
+/* Simulator::remeshingStep (src/Simulator.cu:686-695): every cloth with remeshing parameters is
+ * remeshed at its current state -- sizing on the device from the nearest obstacle planes within
+ * 10 repulsion thicknesses, the curvature, normal, velocity and compression terms; flips, splits
+ * and collapses on the host -- and the context is rebuilt for the new topology (node, face and
+ * edge counts change; handles keep their nodes, under new ids; cs_get_counts / cs_get_topology
+ * give the new meshes).  New nodes take the mean position and velocity of their edge.  The
+ * reference's separation step after remeshing is not run.  A no-op without remeshed cloths. */
+cs_status cs_remesh(cs_ctx* ctx);
+/* The sizing the remeshing step would use for one cloth at the current state, with parameters r:
+ * face_sizing (4 per face, a row-major 2x2 tensor, faces in cs_get_topology order) and, when not
+ * NULL, planes (8 per node: the nearest obstacle point p, 0, the unit normal n, 0; n = 0 where
+ * no obstacle is within 10 repulsion thicknesses).  No counterpart as an entry point in the
+ * reference: the remeshing step computes it internally (src/Cloth.cu:200-265). */
+cs_status cs_remesh_sizing(cs_ctx* ctx, int cloth_id, const cs_remeshing* r, float* face_sizing, float* planes);
 
 #ifdef __cplusplus
 }

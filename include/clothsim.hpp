@@ -4,7 +4,8 @@
 //   Simulator(path)               Simulator::Simulator (src/Simulator.cu:3-135), scene file only
 //   Simulator::physicsStep()      src/Simulator.cu:571-600
 //   Simulator::collisionStep()    src/Simulator.cu:602-684
-//   Simulator::simulateStep()     src/Simulator.cu:838-897 (remeshing and separation disabled)
+//   Simulator::simulateStep()     src/Simulator.cu:838-897 (a frame's last step remeshes unless the
+//                                 scene disables it; the separation step is not run)
 //   Simulator::getCloths() / getObstacles(), Cloth::getMesh(), Obstacle::getMesh()
 //   Mesh::save(path)              OBJ with "v"/"nv" per node, "vt" per uv vertex and "f a/b" faces at
 //                                 precision 20 (src/Mesh.cu:454-507), readable by cs_load_scene
@@ -129,7 +130,13 @@ class Simulator {
     cs_stats simulateStep() {
         cs_stats st = run(cs_step, "cs_step");
         nSteps_++;
-        if (nSteps_ % info_.frame_steps == 0) nFrames_++;
+        if (nSteps_ % info_.frame_steps == 0) {
+            nFrames_++;
+            if (info_.remeshing) {  // Simulator::remeshingStep (src/Simulator.cu:857-871)
+                cs_status s = cs_remesh(ctx_);
+                if (s != CS_OK) throw Error(s, std::string("cs_remesh: ") + cs_last_error(ctx_));
+            }
+        }
         return st;
     }
     // Simulator::finished (src/Simulator.cu:979-981)
@@ -138,7 +145,7 @@ class Simulator {
         return nSteps_ * dt >= info_.end_time || nFrames_ >= info_.end_frame;
     }
     // simulateOffline: save, then steps until finished() with a save after every frame
-    // (src/Simulator.cu:1018-1043; remeshing and separation are not run)
+    // (src/Simulator.cu:1018-1043)
     void simulateOffline(const std::string& directory) {
         save(directory);
         resumeOffline(directory);
