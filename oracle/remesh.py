@@ -109,6 +109,33 @@ def nearest_planes(x, obstacles, thickness):
     return P, N
 
 
+def nearest_plane_candidates(x, obstacles, thickness, rtol=1e-5):
+    """Per node, every plane (p, n) a nearest-point search could return: those of the faces within
+    rtol (relative) of the smallest distance below `thickness`.  Faces that share the closest
+    edge or vertex tie up to rounding, and the point the reference keeps depends on which of them
+    its traversal meets first (with the edge-branch weights, the tied faces give different
+    points)."""
+    x = np.asarray(x, np.float64)
+    out = []
+    for i in range(len(x)):
+        ds = []
+        for y, f in obstacles:
+            y = np.asarray(y, np.float64)
+            for a, b, c in f:
+                d, w = _vf_nearest(x[i], y[a], y[b], y[c])
+                if d < thickness:
+                    ds.append((d, -(w[0] * y[a] + w[1] * y[b] + w[2] * y[c])))
+        c = []
+        if ds:
+            dmin = min(d for d, _ in ds)
+            for d, pt in ds:
+                n = x[i] - pt
+                if d <= dmin * (1 + rtol) and float(np.dot(n, n)) > 1e-8:
+                    c.append((pt, n / np.linalg.norm(n)))
+        out.append(c)
+    return out
+
+
 def _sym_fn(A, fn):
     l, Q = np.linalg.eigh(0.5 * (A + A.T))
     return (Q * fn(l)) @ Q.T
