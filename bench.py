@@ -253,6 +253,42 @@ def summary(r):
             "cg_iters_max": int(np.max(r["iters"])), "failed_steps": r["failed"], "failures": r["failures"]}
 
 
+def remesh_run(workload, frames, steps_per_frame=8):
+    """A remeshing workload (cloths with a "remeshing" block): `frames` frames of `steps_per_frame`
+    steps, each frame ending with the remeshing step (Simulator::simulateStep,
+    src/Simulator.cu:838-897).  Reports the step and remesh wall times and the mesh sizes."""
+    from clothsimulator_amd import CS_MESH_CLOTH, Simulator, scenes
+    spec = scenes.by_name(workload)
+    sim = Simulator(spec)
+    rows, failures = [], []
+    node_steps, t_steps, t_remesh = 0, 0.0, 0.0
+    for f in range(frames):
+        n = sim.counts(CS_MESH_CLOTH, 0)[0]
+        t0 = time.perf_counter()
+        for k in range(steps_per_frame):
+            s, st = sim.step()
+            if s != 0 or not st.pcg_converged:
+                failures.append({"frame": f, "step": k, "status": int(s), "pcg_converged": int(st.pcg_converged)})
+        t1 = time.perf_counter()
+        sim.remesh()
+        t2 = time.perf_counter()
+        sz, ops, rebuild = sim.remesh_times()
+        n2 = sim.counts(CS_MESH_CLOTH, 0)[0]
+        node_steps += n * steps_per_frame
+        t_steps += t1 - t0
+        t_remesh += t2 - t1
+        rows.append({"frame": f, "nodes": n, "nodes_after": n2, "ms_per_step": 1e3 * (t1 - t0) / steps_per_frame,
+                     "ms_remesh": 1e3 * (t2 - t1), "ms_sizing": sz, "ms_operators": ops, "ms_rebuild": rebuild})
+        print("remesh run %s frame %d: %d -> %d nodes, %.2f ms/step, remesh %.1f ms (sizing %.1f, operators %.1f, "
+              "rebuild %.1f)" % (workload, f, n, n2, rows[-1]["ms_per_step"], rows[-1]["ms_remesh"], sz, ops, rebuild),
+              file=sys.stderr, flush=True)
+    return {"metric": "cloth node-steps per second, remeshing included", "workload": spec.name, "frames": frames,
+            "steps_per_frame": steps_per_frame, "value": node_steps / (t_steps + t_remesh),
+            "unit": "node-steps/s", "ms_per_step_mean": 1e3 * t_steps / (frames * steps_per_frame),
+            "ms_per_remesh_mean": 1e3 * t_remesh / frames, "failed_steps": len(failures), "failures": failures,
+            "frames_detail": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +308,9 @@ def main():
     ap.add_argument("--cpu-direct", default="", metavar="WORKLOAD:STEPS,...",
                     help="also time the single-thread direct-solve CPU column on these windows "
                          "(e.g. drape129:20,crumple513:3); hang33 over its 100 steps always runs")
+    ap.add_argument("--remesh-run", default="", metavar="WORKLOAD:FRAMES",
+                    help="only run a remeshing workload (e.g. remesh_drape129:25: 8 steps then a remesh per frame) "
+                         "and print its own JSON line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
     a = ap.parse_args()
@@ -282,6 +321,10 @@ def main():
 
     if a.copy_probe:
         copy_probe(a.copy_probe)
+        return
+    if a.remesh_run:
+        wl, frames = a.remesh_run.split(":")
+        print(json.dumps(remesh_run(wl, int(frames))))
         return
 
     from clothsimulator_amd import scenes

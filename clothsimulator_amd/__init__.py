@@ -30,7 +30,7 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_scene_motion", "cs_get_topology", "cs_write_config", "cs_impact_response",
             "cs_solve_bsr_noalloc", "cs_lpc_train_step_smooth", "cs_default_remeshing", "cs_remesh_mesh",
             "cs_remesh_result_counts", "cs_remesh_result_get", "cs_remesh_result_free", "cs_set_remeshing",
-            "cs_remesh", "cs_remesh_sizing"]
+            "cs_remesh", "cs_remesh_sizing", "cs_remesh_times"]
 
 # trained weights of the learned preconditioner shipped with the package (float32, .npy)
 LPC_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lpc_weights.npy")
@@ -114,6 +114,7 @@ def load_library(path=LIB_PATH):
     L.cs_set_remeshing.argtypes = [vp, C.c_int, RP]
     L.cs_remesh.argtypes = [vp]
     L.cs_remesh_sizing.argtypes = [vp, C.c_int, RP, F, F]
+    L.cs_remesh_times.argtypes = [vp, F, F, F]
     L.cs_remesh_result_free.argtypes = [vp]
     L.cs_remesh_result_free.restype = None
     _lib = L
@@ -238,6 +239,8 @@ class Simulator:
             for cid, c in zip(cids, spec.cloths):
                 if c.handles:
                     self.add_handles(cid, c.handles, getattr(c, "handle_motion", -1))
+                if getattr(c, "remeshing", None) is not None:
+                    self.set_remeshing(cid, c.remeshing)
             for i, o in enumerate(spec.obstacles):
                 oid = self.add_obstacle(o)
                 m = spec.obstacle_motions[i] if getattr(spec, "obstacle_motions", None) else -1
@@ -487,6 +490,12 @@ class Simulator:
     def remesh(self):
         """The remeshing step of every cloth with remeshing parameters (cs_remesh)."""
         self._check(self.L.cs_remesh(self.h), "cs_remesh")
+
+    def remesh_times(self):
+        """(sizing, operators, rebuild) milliseconds of the last remesh (cs_remesh_times)."""
+        t = np.zeros(3, np.float32)
+        self._check(self.L.cs_remesh_times(self.h, fptr(t[0:]), fptr(t[1:]), fptr(t[2:])), "cs_remesh_times")
+        return tuple(float(v) for v in t)
 
     def remesh_sizing(self, cloth, r):
         """Face sizing (nf x 4) and node planes (nn x 8) the remeshing step would use now."""

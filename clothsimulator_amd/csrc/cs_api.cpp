@@ -843,6 +843,14 @@ cs_status cs_remesh(cs_ctx* C) {
     });
 }
 
+cs_status cs_remesh_times(const cs_ctx* C, float* sizing_ms, float* operators_ms, float* rebuild_ms) {
+    if (!C) return CS_E_ARG;
+    if (sizing_ms) *sizing_ms = C->remeshMs[0];
+    if (operators_ms) *operators_ms = C->remeshMs[1];
+    if (rebuild_ms) *rebuild_ms = C->remeshMs[2];
+    return CS_OK;
+}
+
 cs_status cs_remesh_sizing(cs_ctx* C, int cloth_id, const cs_remeshing* r, float* face_sizing, float* planes) {
     if (!C || !r || !face_sizing || cloth_id < 0 || cloth_id >= C->nCloths) return CS_E_ARG;
     return guard(C, [&]() {

@@ -287,4 +287,5 @@ struct cs_ctx {
     bool presReady = false;
     int flipCount = 0;
     cs::DBuf rmUv, rmPlanes, rmSizing;
+    float remeshMs[3] = {0.0f, 0.0f, 0.0f};  // last cs_remesh: sizing (device), operators (host), rebuild
 };

@@ -8,6 +8,7 @@
 // the multigrid hierarchy, the learned factor's pattern, the trees).  Node states come from the
 // device; handles follow their nodes, which are preserved and so never removed.
 #include <algorithm>
+#include <chrono>
 
 #include "cs_kernels.hpp"
 
@@ -74,6 +75,9 @@ void init_preserve(cs_ctx* C) {
 }  // namespace
 
 void remesh_context(cs_ctx* C) {
+    using clk = std::chrono::steady_clock;
+    auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<float, std::milli>(b - a).count(); };
+    float tSizing = 0.0f, tOps = 0.0f;
     if (!C->finalized) host_finalize(C);
     hipStream_t s = C->stream;
     const int N = C->N;
@@ -97,7 +101,10 @@ void remesh_context(cs_ctx* C) {
         if (!M.cloth || m >= (int)C->remeshCfg.size() || !C->remeshCfg[m].has) continue;
         const RemeshParams& r = C->remeshCfg[m].r;
         std::vector<float> fs;
+        const auto t0 = clk::now();
         remesh_sizing(C, m, r, fs, nullptr);
+        const auto t1 = clk::now();
+        tSizing += ms(t0, t1);
         std::vector<float> x0(3L * M.nn);
         for (int i = 0; i < M.nn; i++) {
             x0[3 * i] = x04[M.n0 + i].x; x0[3 * i + 1] = x04[M.n0 + i].y; x0[3 * i + 2] = x04[M.n0 + i].z;
@@ -113,6 +120,7 @@ void remesh_context(cs_ctx* C) {
         C->flipCount = rm.flipCount;
         RMeshOut o;
         rmesh_output(rm, M.nn, o);
+        tOps += ms(t1, clk::now());
         newIndex[m].assign(M.nn, -1);
         for (size_t i = 0; i < o.nodeOrigin.size(); i++)
             if (o.nodeOrigin[i] >= 0) newIndex[m][o.nodeOrigin[i]] = (int)i;
@@ -130,7 +138,11 @@ void remesh_context(cs_ctx* C) {
         loc[m] = std::move(R);
         any = true;
     }
+    C->remeshMs[0] = tSizing;
+    C->remeshMs[1] = tOps;
+    C->remeshMs[2] = 0.0f;
     if (!any) return;
+    const auto tb = clk::now();
     // handles follow their (preserved) nodes: local ids now, global once the meshes are placed
     std::vector<int> handleLocal;
     for (auto& h : C->handles) {
@@ -174,6 +186,7 @@ void remesh_context(cs_ctx* C) {
     C->bamg.ncl = -1;
     C->bamg.selected = false;
     host_finalize(C);
+    C->remeshMs[2] = ms(tb, clk::now());
 }
 
 }  // namespace cs

@@ -124,6 +124,7 @@ class ClothSpec:
     material: object
     handles: List[int] = field(default_factory=list)
     handle_motion: int = -1  # the motion moving the handles (-1: fixed)
+    remeshing: object = None  # a Remeshing struct: the cloth is remeshed by Simulator.remesh()
 
 
 @dataclass
@@ -240,13 +241,28 @@ def instances(count=256, seed=0, side_lo=17, side_hi=129, steps=200, spacing=3.0
     return SceneSpec("batch%d" % count, p, cloths, obstacles, steps)
 
 
+# the reference's conf/hang.json "remeshing" block (Narain et al. 2012, Figure 2(b))
+HANG_REMESHING = dict(refine_angle=0.3, refine_compression=0.005, refine_velocity=0.5, size=(10e-3, 200e-3),
+                      aspect_min=0.2)
+
+
+def drape_remesh(n=65, steps=400):
+    """drape(n) with the remeshing block of the reference's conf/hang.json (a sheet falling on a
+    sphere, remeshed at the end of every 8-step frame): the remeshing workload."""
+    from ._cabi import remeshing_struct
+    spec = drape(n, steps)
+    spec.name = "remesh_drape%d" % n
+    spec.cloths[0].remeshing = remeshing_struct(**HANG_REMESHING)
+    return spec
+
+
 def by_name(name):
     table = {"hang33": lambda: hang(33), "drape129": lambda: drape(129), "crumple513": lambda: crumple(513)}
     if name in table:
         return table[name]()
     # parametrised small variants: hangN, drapeN, crumpleN
     for prefix, fn in (("hang", hang), ("drape", drape), ("crumple", crumple), ("batch", instances),
-                       ("sphere_sweep", sphere_sweep)):
+                       ("sphere_sweep", sphere_sweep), ("remesh_drape", drape_remesh)):
         if name.startswith(prefix) and name[len(prefix):].isdigit():
             return fn(int(name[len(prefix):]))
     raise KeyError(name)

@@ -416,6 +416,10 @@ cs_status cs_set_remeshing(cs_ctx* ctx, int cloth_id, const cs_remeshing* r);
  * give the new meshes).  New nodes take the mean position and velocity of their edge.  The
  * reference's separation step after remeshing is not run.  A no-op without remeshed cloths. */
 cs_status cs_remesh(cs_ctx* ctx);
+/* Wall-clock split of the last cs_remesh (ms): the sizing field on the device (with its
+ * download), the operators on the host, the rebuild of the context (instrumentation; no
+ * counterpart in the reference). */
+cs_status cs_remesh_times(const cs_ctx* ctx, float* sizing_ms, float* operators_ms, float* rebuild_ms);
 /* The sizing the remeshing step would use for one cloth at the current state, with parameters r:
  * face_sizing (4 per face, a row-major 2x2 tensor, faces in cs_get_topology order) and, when not
  * NULL, planes (8 per node: the nearest obstacle point p, 0, the unit normal n, 0; n = 0 where

@@ -87,6 +87,22 @@ def test_scene_info_of_written_scene(tmp_path):
     assert info.end_time == pytest.approx(1.5) and info.end_frame == 2 ** 31 - 1
 
 
+REMESH_BLOCK = ('"remeshing": {"refine_angle": 0.3, "refine_compression": 0.005, "refine_velocity": 0.5, '
+                '"size": [10e-3, 200e-3], "aspect_min": 0.2}')
+
+
+def remeshing_config():
+    return CONFIG.replace('"remeshing": {"refine_angle": 0.3}', REMESH_BLOCK).replace(
+        ',\n    "disable": ["remeshing"]', '')
+
+
+def test_remeshing_flag(tmp_path):
+    """"disable": ["remeshing"] turns the per-frame remeshing off (src/Simulator.cu:858)."""
+    assert cs.scene_info(write_scene(str(tmp_path)), str(tmp_path)).remeshing == 0
+    assert '"disable"' not in remeshing_config()
+    assert cs.scene_info(write_scene(str(tmp_path), remeshing_config()), str(tmp_path)).remeshing == 1
+
+
 @pytest.mark.parametrize("bad, status", [
     ('"obstacles": [{"mesh": "tri.obj", "motion": 0}]', "CS_E_FORMAT.*missing motion"),
     ('"obstacles": "obs_%s.obj"', "CS_E_FORMAT.*%d"),
@@ -133,6 +149,7 @@ def test_reference_configurations_parse():
     assert (loaded["funnel.json"].n_motions, loaded["funnel.json"].n_moving_obstacles) == (1, 1)
     assert (loaded["dress-blue.json"].n_motions, loaded["dress-blue.json"].n_moving_obstacles) == (16, 16)
     assert loaded["sphere.json"].n_moving_obstacles == 1 and loaded["sphere.json"].n_handles == 1
+    assert loaded["hang.json"].remeshing == 1 and loaded["hang32.json"].remeshing == 0
 
 
 def expected_scene():
@@ -220,6 +237,28 @@ def test_cpp_mirror_runs_frames(tmp_path):
     vt, raw = obj_rows(out / "frame2_cloth0.obj", "vt"), obj_rows(tmp_path / "sheet.obj", "v")
     assert len(vt) == 8 and all(np.any(np.all(2 * raw[:, :2] == u, axis=1)) for u in vt)
     assert all("/" in l for l in open(out / "frame2_cloth0.obj") if l.startswith("f "))
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_remeshes_at_frame_ends(tmp_path):
+    """With a full "remeshing" block and no "disable", every frame ends with cs_remesh: the saved
+    frames hold the refined sheet (a valid manifold, the material area kept)."""
+    import subprocess
+    exe = build_host_mirror(str(tmp_path))
+    path = write_scene(str(tmp_path), remeshing_config())
+    out = tmp_path / "frames"
+    out.mkdir()
+    r = subprocess.run([exe, path, str(tmp_path), str(out), "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    n = [len(obj_rows(out / ("frame%d_cloth0.obj" % k), "v")) for k in range(3)]
+    print("remeshed sheet: nodes per saved frame %s" % n)
+    assert n[0] == 6 and n[1] > 6 and n[2] > 6
+    vt = obj_rows(out / "frame2_cloth0.obj", "vt")
+    faces = [[int(t.split("/")[1]) - 1 for t in l.split()[1:]] for l in open(out / "frame2_cloth0.obj")
+             if l.startswith("f ")]
+    area = sum(0.5 * abs(np.cross(vt[b] - vt[a], vt[c] - vt[a])) for a, b, c in faces)
+    assert area == pytest.approx(2.0, rel=1e-5)  # the 1 x 0.5 sheet scaled by 2
+    assert np.all(np.isfinite(obj_rows(out / "frame2_cloth0.obj", "v")))
 
 
 @pytest.mark.gpu
