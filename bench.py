@@ -272,16 +272,18 @@ def remesh_run(workload, frames, steps_per_frame=8):
         t1 = time.perf_counter()
         sim.remesh()
         t2 = time.perf_counter()
-        sz, ops, rebuild = sim.remesh_times()
+        sz, ops, rebuild, sep, sep_solves = sim.remesh_times()
         n2 = sim.counts(CS_MESH_CLOTH, 0)[0]
         node_steps += n * steps_per_frame
         t_steps += t1 - t0
         t_remesh += t2 - t1
         rows.append({"frame": f, "nodes": n, "nodes_after": n2, "ms_per_step": 1e3 * (t1 - t0) / steps_per_frame,
-                     "ms_remesh": 1e3 * (t2 - t1), "ms_sizing": sz, "ms_operators": ops, "ms_rebuild": rebuild})
+                     "ms_remesh": 1e3 * (t2 - t1), "ms_sizing": sz, "ms_operators": ops, "ms_rebuild": rebuild,
+                     "ms_separation": sep, "separation_solves": sep_solves})
         print("remesh run %s frame %d: %d -> %d nodes, %.2f ms/step, remesh %.1f ms (sizing %.1f, operators %.1f, "
-              "rebuild %.1f)" % (workload, f, n, n2, rows[-1]["ms_per_step"], rows[-1]["ms_remesh"], sz, ops, rebuild),
-              file=sys.stderr, flush=True)
+              "rebuild %.1f, separation %.1f with %d solves)" % (workload, f, n, n2, rows[-1]["ms_per_step"],
+                                                                 rows[-1]["ms_remesh"], sz, ops, rebuild, sep,
+                                                                 sep_solves), file=sys.stderr, flush=True)
     return {"metric": "cloth node-steps per second, remeshing included", "workload": spec.name, "frames": frames,
             "steps_per_frame": steps_per_frame, "value": node_steps / (t_steps + t_remesh),
             "unit": "node-steps/s", "ms_per_step_mean": 1e3 * t_steps / (frames * steps_per_frame),

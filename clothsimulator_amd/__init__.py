@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from ._cabi import (CS_E_NOT_CONVERGED, CS_MESH_CLOTH, CS_PRECOND_AMG_LEARNED, CS_PRECOND_LEARNED, CS_MESH_OBSTACLE, CS_OK, IMPACT_DTYPE, LIB_PATH, PROX_DTYPE,
+from ._cabi import (CS_E_COLLISION_FAILED, CS_E_NOT_CONVERGED, CS_MESH_CLOTH, CS_PRECOND_AMG_LEARNED, CS_PRECOND_LEARNED, CS_MESH_OBSTACLE, CS_OK, IMPACT_DTYPE, LIB_PATH, PROX_DTYPE,
                     STATUS_NAMES, ImpactRec, MaterialIn, MeshArrays, MeshIn, Params, ProximityRec, SceneInfo,
                     Remeshing, Stats, Transform, axis_angle, default_params, fptr, iptr, material_struct,
                     remeshing_struct, transform_struct)
@@ -30,7 +30,7 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_scene_motion", "cs_get_topology", "cs_write_config", "cs_impact_response",
             "cs_solve_bsr_noalloc", "cs_lpc_train_step_smooth", "cs_default_remeshing", "cs_remesh_mesh",
             "cs_remesh_result_counts", "cs_remesh_result_get", "cs_remesh_result_free", "cs_set_remeshing",
-            "cs_remesh", "cs_remesh_sizing", "cs_remesh_times"]
+            "cs_remesh", "cs_remesh_sizing", "cs_remesh_times", "cs_separation_step"]
 
 # trained weights of the learned preconditioner shipped with the package (float32, .npy)
 LPC_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lpc_weights.npy")
@@ -114,7 +114,8 @@ def load_library(path=LIB_PATH):
     L.cs_set_remeshing.argtypes = [vp, C.c_int, RP]
     L.cs_remesh.argtypes = [vp]
     L.cs_remesh_sizing.argtypes = [vp, C.c_int, RP, F, F]
-    L.cs_remesh_times.argtypes = [vp, F, F, F]
+    L.cs_remesh_times.argtypes = [vp, F, F, F, F, IP]
+    L.cs_separation_step.argtypes = [vp, IP, F, F]
     L.cs_remesh_result_free.argtypes = [vp]
     L.cs_remesh_result_free.restype = None
     _lib = L
@@ -492,10 +493,23 @@ class Simulator:
         self._check(self.L.cs_remesh(self.h), "cs_remesh")
 
     def remesh_times(self):
-        """(sizing, operators, rebuild) milliseconds of the last remesh (cs_remesh_times)."""
-        t = np.zeros(3, np.float32)
-        self._check(self.L.cs_remesh_times(self.h, fptr(t[0:]), fptr(t[1:]), fptr(t[2:])), "cs_remesh_times")
-        return tuple(float(v) for v in t)
+        """(sizing, operators, rebuild, separation) milliseconds and the separation step's solve count
+        of the last remesh (cs_remesh_times)."""
+        t = np.zeros(4, np.float32)
+        n = C.c_int()
+        self._check(self.L.cs_remesh_times(self.h, fptr(t[0:]), fptr(t[1:]), fptr(t[2:]), fptr(t[3:]), C.byref(n)),
+                    "cs_remesh_times")
+        return tuple(float(v) for v in t) + (n.value,)
+
+    def separation_step(self, backups):
+        """The separation step on the current state; backups: per cloth (uv (F, 3, 2), x (F, 3, 3)),
+        its faces before remeshing.  Returns the status (0, or CS_E_COLLISION_FAILED)."""
+        counts = np.array([len(u) for u, _ in backups], np.int32)
+        uv = np.ascontiguousarray(np.concatenate([np.asarray(u, np.float32).reshape(-1, 6) for u, _ in backups]))
+        xx = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float32).reshape(-1, 9) for _, x in backups]))
+        st = self.L.cs_separation_step(self.h, iptr(counts), fptr(uv), fptr(xx))
+        self._check(st, "cs_separation_step", ok=(CS_OK, CS_E_COLLISION_FAILED))
+        return st
 
     def remesh_sizing(self, cloth, r):
         """Face sizing (nf x 4) and node planes (nn x 8) the remeshing step would use now."""

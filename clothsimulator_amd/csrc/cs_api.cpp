@@ -837,17 +837,41 @@ cs_status cs_set_remeshing(cs_ctx* C, int cloth_id, const cs_remeshing* r) {
 
 cs_status cs_remesh(cs_ctx* C) {
     if (!C) return CS_E_ARG;
+    return guard(C, [&]() { return remesh_context(C); });
+}
+
+cs_status cs_separation_step(cs_ctx* C, const int* faces_per_cloth, const float* backup_uv, const float* backup_x) {
+    if (!C || !faces_per_cloth || !backup_uv || !backup_x) return CS_E_ARG;
+    for (int c = 0; c < C->nCloths; c++)
+        if (faces_per_cloth[c] < 0) return CS_E_ARG;
     return guard(C, [&]() {
-        remesh_context(C);
-        return (int)CS_OK;
+        host_finalize(C);
+        std::vector<std::vector<BackupFace>> backup(C->nCloths);
+        size_t k = 0;
+        for (int c = 0; c < C->nCloths; c++) {
+            backup[c].resize(faces_per_cloth[c]);
+            for (BackupFace& f : backup[c]) {
+                for (int j = 0; j < 3; j++) {
+                    f.u[j][0] = backup_uv[6 * k + 2 * j];
+                    f.u[j][1] = backup_uv[6 * k + 2 * j + 1];
+                    for (int a = 0; a < 3; a++) f.x[j][a] = backup_x[9 * k + 3 * j + a];
+                }
+                k++;
+            }
+        }
+        C->sepIters = 0;
+        return separation_step(C, backup);
     });
 }
 
-cs_status cs_remesh_times(const cs_ctx* C, float* sizing_ms, float* operators_ms, float* rebuild_ms) {
+cs_status cs_remesh_times(const cs_ctx* C, float* sizing_ms, float* operators_ms, float* rebuild_ms,
+                          float* separation_ms, int* separation_solves) {
     if (!C) return CS_E_ARG;
     if (sizing_ms) *sizing_ms = C->remeshMs[0];
     if (operators_ms) *operators_ms = C->remeshMs[1];
     if (rebuild_ms) *rebuild_ms = C->remeshMs[2];
+    if (separation_ms) *separation_ms = C->remeshMs[3];
+    if (separation_solves) *separation_solves = C->sepIters;
     return CS_OK;
 }
 

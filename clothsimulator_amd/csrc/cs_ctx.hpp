@@ -160,6 +160,22 @@ struct LpcNet {
     unsigned version = 0;  // incremented whenever the weights change
 };
 
+// A face as it was before remeshing: material-space and world positions of its corners
+// (BackupFace, src/BackupFace.cuh; Mesh::backupFaces, src/Mesh.cu:116-128).
+struct BackupFace {
+    float u[3][2];
+    float x[3][3];
+};
+
+// One separation constraint: sum_j w_j n . x_j >= thickness over the two faces' six nodes, w the
+// barycentric weights of the first face's point and minus those of the second's.
+struct SepRec {
+    int nodes[6];
+    float w[6];
+    float n[3];
+    float pad;
+};
+
 struct PcgScalars {
     float rz, dq, alpha, beta, rr, bb, tol2bb, pad;
     int iters, done, maxit, pad2;
@@ -287,5 +303,8 @@ struct cs_ctx {
     bool presReady = false;
     int flipCount = 0;
     cs::DBuf rmUv, rmPlanes, rmSizing;
-    float remeshMs[3] = {0.0f, 0.0f, 0.0f};  // last cs_remesh: sizing (device), operators (host), rebuild
+    float remeshMs[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // last cs_remesh: sizing, operators, rebuild, separation
+    // separation step after remeshing (cs_separation.hip)
+    cs::DBuf sepFlag, sepPos, sepRaw, sepRec, sepCons;
+    int sepIters = 0;  // solves of the last separation step
 };

@@ -19,7 +19,9 @@ void launch_face_geometry(cs_ctx* C, int f0, int nf);
 // remeshing (cs_sizing.hip, cs_remesh_ctx.cpp)
 void remesh_sizing(cs_ctx* C, int c, const RemeshParams& r, std::vector<float>& faceSizing,
                    std::vector<float>* planes);
-void remesh_context(cs_ctx* C);
+int remesh_context(cs_ctx* C);  // returns a cs_status (the separation step can fail)
+// separation after remeshing (cs_separation.hip); backup: per cloth, its faces before remeshing
+int separation_step(cs_ctx* C, const std::vector<std::vector<BackupFace>>& backup);
 void launch_node_geometry(cs_ctx* C, int n0, int nn);
 void launch_update_positions(cs_ctx* C, int n0, int nn);
 void launch_update_velocities(cs_ctx* C, int n0, int nn);
@@ -103,5 +105,6 @@ int find_proximities(cs_ctx* C, bool build_trees = true);  // build_trees false:
 int find_impacts(cs_ctx* C, float thickness);  // sorted, unique impacts in C->impSorted (cs_impact_rec)
 int independent_impacts(cs_ctx* C, const cs_impact_rec* dIn, int n, int deform, cs_impact_rec* dOut);
 void collision_response(cs_ctx* C, int nImpacts, int deform, float obstacleMass);
+void separation_response(cs_ctx* C, const SepRec* recs, int n, int deform, float obstacleArea);
 
 }  // namespace cs

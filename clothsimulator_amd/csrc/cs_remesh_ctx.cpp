@@ -18,7 +18,7 @@ namespace {
 
 // One mesh in local ids.
 struct Local {
-    std::vector<float> x, v, uv;
+    std::vector<float> x, x0, v, uv;
     std::vector<unsigned char> pres;
     std::vector<int> vnode, fv, fe, en, eopp, eadj;
 };
@@ -74,7 +74,7 @@ void init_preserve(cs_ctx* C) {
 
 }  // namespace
 
-void remesh_context(cs_ctx* C) {
+int remesh_context(cs_ctx* C) {
     using clk = std::chrono::steady_clock;
     auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<float, std::milli>(b - a).count(); };
     float tSizing = 0.0f, tOps = 0.0f;
@@ -92,12 +92,30 @@ void remesh_context(cs_ctx* C) {
     }
     if (!C->presReady) init_preserve(C);
     const int nm = (int)C->meshes.size();
+    // the frame's faces before remeshing (Simulator::simulateStep, src/Simulator.cu:858-866)
+    std::vector<std::vector<BackupFace>> backup(C->nCloths);
+    for (int c = 0; c < C->nCloths; c++) {
+        const MeshInfo& M = C->meshes[c];
+        backup[c].resize(M.nf);
+        for (int f = 0; f < M.nf; f++)
+            for (int k = 0; k < 3; k++) {
+                const int vt = C->hfv[3 * (M.f0 + f) + k], nd = C->hfn[3 * (M.f0 + f) + k];
+                backup[c][f].u[k][0] = C->huv[2 * vt];
+                backup[c][f].u[k][1] = C->huv[2 * vt + 1];
+                for (int a = 0; a < 3; a++) backup[c][f].x[k][a] = C->hx[3 * nd + a];
+            }
+    }
     std::vector<Local> loc(nm);
     std::vector<std::vector<int>> newIndex(nm);  // remeshed cloths: old local node -> new local node
     bool any = false;
     for (int m = 0; m < nm; m++) {
         const MeshInfo& M = C->meshes[m];
         loc[m] = extract(C, M);
+        for (int i = M.n0; i < M.n0 + M.nn; i++) {
+            loc[m].x0.push_back(x04[i].x);
+            loc[m].x0.push_back(x04[i].y);
+            loc[m].x0.push_back(x04[i].z);
+        }
         if (!M.cloth || m >= (int)C->remeshCfg.size() || !C->remeshCfg[m].has) continue;
         const RemeshParams& r = C->remeshCfg[m].r;
         std::vector<float> fs;
@@ -126,6 +144,7 @@ void remesh_context(cs_ctx* C) {
             if (o.nodeOrigin[i] >= 0) newIndex[m][o.nodeOrigin[i]] = (int)i;
         Local R;
         R.x = std::move(o.x);
+        R.x0 = std::move(o.x0);
         R.v = std::move(o.v);
         R.uv = std::move(o.uv);
         R.pres = std::move(o.preserve);
@@ -140,8 +159,9 @@ void remesh_context(cs_ctx* C) {
     }
     C->remeshMs[0] = tSizing;
     C->remeshMs[1] = tOps;
-    C->remeshMs[2] = 0.0f;
-    if (!any) return;
+    C->remeshMs[2] = C->remeshMs[3] = 0.0f;
+    C->sepIters = 0;
+    if (!any) return CS_OK;
     const auto tb = clk::now();
     // handles follow their (preserved) nodes: local ids now, global once the meshes are placed
     std::vector<int> handleLocal;
@@ -186,7 +206,21 @@ void remesh_context(cs_ctx* C) {
     C->bamg.ncl = -1;
     C->bamg.selected = false;
     host_finalize(C);
-    C->remeshMs[2] = ms(tb, clk::now());
+    // the nodes' previous positions (the step's start), so that the velocities the separation
+    // step recomputes, (x - x0) / dt, are the step's
+    std::vector<float4> x0n(C->N);
+    for (int m = 0; m < nm; m++) {
+        const MeshInfo& M = C->meshes[m];
+        for (int i = 0; i < M.nn; i++)
+            x0n[M.n0 + i] = make_float4(loc[m].x0[3 * i], loc[m].x0[3 * i + 1], loc[m].x0[3 * i + 2], 0.0f);
+    }
+    CS_HIP(hipMemcpyAsync(C->x0.p, x0n.data(), sizeof(float4) * C->N, hipMemcpyHostToDevice, C->stream));
+    CS_HIP(hipStreamSynchronize(C->stream));
+    const auto ts = clk::now();
+    C->remeshMs[2] = ms(tb, ts);
+    const int st = separation_step(C, backup);
+    C->remeshMs[3] = ms(ts, clk::now());
+    return st;
 }
 
 }  // namespace cs

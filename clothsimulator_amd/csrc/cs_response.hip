@@ -21,11 +21,29 @@ namespace {
 
 constexpr int OB = 1024;
 
-__global__ void k_opt_collect(int m, const cs_impact_rec* __restrict__ imps, int deform, int NC, int N,
-                              uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+// The constraints c_i = sum_j w_ij n_i . x_ij - thickness as the two stages store them: impacts
+// (4 nodes, FP64 record fields) and separation constraints (6 nodes: the two faces' points).
+struct ImpView {
+    static constexpr int K = 4;
+    const cs_impact_rec* r;
+    __device__ int node(int i, int j) const { return r[i].nodes[j]; }
+    __device__ float w(int i, int j) const { return (float)r[i].w[j]; }
+    __device__ V3f n(int i) const { return mk((float)r[i].n[0], (float)r[i].n[1], (float)r[i].n[2]); }
+};
+struct SepView {
+    static constexpr int K = 6;
+    const SepRec* r;
+    __device__ int node(int i, int j) const { return r[i].nodes[j]; }
+    __device__ float w(int i, int j) const { return r[i].w[j]; }
+    __device__ V3f n(int i) const { return mk(r[i].n[0], r[i].n[1], r[i].n[2]); }
+};
+
+template <class View>
+__global__ void k_opt_collect(int m, View V, int deform, int NC, int N, uint64_t* __restrict__ keys,
+                              uint32_t* __restrict__ vals) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= m) return;
-    int nd = imps[s >> 2].nodes[s & 3];
+    int nd = V.node(s / View::K, s % View::K);
     keys[s] = (deform == 1 || nd < NC) ? (uint64_t)nd : (uint64_t)N;
     vals[s] = (uint32_t)s;
 }
@@ -65,10 +83,11 @@ __device__ float wg_sum(float v, float* lds) {
     return lds[OB / 64];
 }
 
+template <class View>
 struct OptArgs {
     int nC;
     const uint32_t* nNptr;  // unique node count (scan total)
-    const cs_impact_rec* imps;
+    View V;
     const int* nodes;
     const int* start;
     const int* end;
@@ -76,32 +95,35 @@ struct OptArgs {
     const uint32_t* sortedSlots;
     float4* x;
     const float4* x1;
-    const float* mass;
+    const float* mass;  // per node: the objective's weight (mass for impacts, area for separation)
     int NC;
-    float obstacleMass, thickness;
+    float obstacleMass, thickness;  // obstacle nodes' weight
     float4 *X, *Xn, *G;
     float *lambda, *coef;
 };
 
-__device__ __forceinline__ float node_mass(const OptArgs& A, int u) {
+template <class View>
+__device__ __forceinline__ float node_mass(const OptArgs<View>& A, int u) {
     int nd = A.nodes[u];
     return nd < A.NC ? A.mass[nd] : A.obstacleMass;
 }
 
-__device__ float constraint(const OptArgs& A, const float4* Xp, int i) {
-    const cs_impact_rec& im = A.imps[i];
-    V3f n = mk((float)im.n[0], (float)im.n[1], (float)im.n[2]);
+template <class View>
+__device__ float constraint(const OptArgs<View>& A, const float4* Xp, int i) {
+    const V3f n = A.V.n(i);
     float c = -A.thickness;
-    for (int j = 0; j < 4; j++) {
-        int u = A.slotMap[4 * i + j];
-        V3f p = u > -1 ? ld3(Xp[u]) : ld3(A.x[im.nodes[j]]);
-        c += (float)im.w[j] * dot(n, p);
+    for (int j = 0; j < View::K; j++) {
+        int u = A.slotMap[View::K * i + j];
+        V3f p = u > -1 ? ld3(Xp[u]) : ld3(A.x[A.V.node(i, j)]);
+        c += A.V.w(i, j) * dot(n, p);
     }
     return c;
 }
 
 // value(x) (src/optimization/Optimization.cu:20-31): objective + penalty terms
-__device__ float value(const OptArgs& A, int nN, const float4* Xp, float invMass, float mu, float* lds, bool storeCoef) {
+template <class View>
+__device__ float value(const OptArgs<View>& A, int nN, const float4* Xp, float invMass, float mu, float* lds,
+                       bool storeCoef) {
     float a = 0.0f;
     for (int u = threadIdx.x; u < nN; u += OB) {
         V3f d = ld3(Xp[u]) - ld3(A.x1[A.nodes[u]]);
@@ -117,7 +139,8 @@ __device__ float value(const OptArgs& A, int nN, const float4* Xp, float invMass
     return obj + wg_sum(pen, lds);
 }
 
-__global__ void __launch_bounds__(OB) k_opt_solve(OptArgs A, int* iterOut) {
+template <class View>
+__global__ void __launch_bounds__(OB) k_opt_solve(OptArgs<View> A, int* iterOut) {
     __shared__ float lds[OB / 64 + 1];
     const int nN = (int)*A.nNptr;
     if (nN == 0) return;
@@ -139,13 +162,9 @@ __global__ void __launch_bounds__(OB) k_opt_solve(OptArgs A, int* iterOut) {
             V3f g = (invMass * node_mass(A, u)) * (ld3(A.X[u]) - ld3(A.x1[A.nodes[u]]));
             for (int k = A.start[u]; k < A.end[u]; k++) {
                 const int slot = (int)A.sortedSlots[k];
-                const int i = slot >> 2, j = slot & 3;
+                const int i = slot / View::K, j = slot % View::K;
                 const float coef = A.coef[i];
-                if (coef != 0.0f) {
-                    const cs_impact_rec& ip = A.imps[i];
-                    V3f n = mk((float)ip.n[0], (float)ip.n[1], (float)ip.n[2]);
-                    g += ((mu * coef) * (float)ip.w[j]) * n;
-                }
+                if (coef != 0.0f) g += ((mu * coef) * A.V.w(i, j)) * A.V.n(i);
             }
             A.G[u] = st4(g);
             g2 += norm2(g);
@@ -173,12 +192,11 @@ __global__ void __launch_bounds__(OB) k_opt_solve(OptArgs A, int* iterOut) {
     if (threadIdx.x == 0 && iterOut) *iterOut = iter;
 }
 
-}  // namespace
-
-void collision_response(cs_ctx* C, int nImp, int deform, float obstacleMass) {
+template <class View>
+void opt_response(cs_ctx* C, View V, int nImp, int deform, const float* weight, float obstacleWeight) {
     if (nImp <= 0) return;
     hipStream_t s = C->stream;
-    const int m = 4 * nImp;
+    const int m = View::K * nImp;
     uint64_t* keys = C->optKeys.as<uint64_t>(m);
     uint64_t* ktmp = C->optKeysTmp.as<uint64_t>(m);
     uint32_t* vals = C->optVals.as<uint32_t>(m);
@@ -189,8 +207,8 @@ void collision_response(cs_ctx* C, int nImp, int deform, float obstacleMass) {
     int* start = C->optStart.as<int>(2 * (size_t)m);
     int* end = start + m;
     int* slotMap = C->optSlot.as<int>(m);
-    const cs_impact_rec* imps = C->impAcc.ptr<cs_impact_rec>();
-    hipLaunchKernelGGL(k_opt_collect, dim3(cdiv(m, 256)), dim3(256), 0, s, m, imps, deform, C->NC, C->N, keys, vals);
+    hipLaunchKernelGGL(k_opt_collect<View>, dim3(cdiv(m, 256)), dim3(256), 0, s, m, V, deform, C->NC, C->N, keys,
+                       vals);
     CS_LAUNCH_CHECK();
     int bits = 1;
     while ((1L << bits) <= C->N) bits++;
@@ -202,14 +220,14 @@ void collision_response(cs_ctx* C, int nImp, int deform, float obstacleMass) {
     hipLaunchKernelGGL(k_opt_map, dim3(cdiv(m, 256)), dim3(256), 0, s, m, C->N, keys, vals, flag, pos, nodes, start,
                        end, slotMap);
     CS_LAUNCH_CHECK();
-    OptArgs A;
+    OptArgs<View> A;
     A.nC = nImp;
     A.nNptr = pos + m;
-    A.imps = imps;
+    A.V = V;
     A.nodes = nodes; A.start = start; A.end = end; A.slotMap = slotMap; A.sortedSlots = vals;
-    A.x = C->x.ptr<float4>(); A.x1 = C->x1.ptr<float4>(); A.mass = C->mass.ptr<float>();
+    A.x = C->x.ptr<float4>(); A.x1 = C->x1.ptr<float4>(); A.mass = weight;
     A.NC = C->NC;
-    A.obstacleMass = obstacleMass;
+    A.obstacleMass = obstacleWeight;
     A.thickness = C->P.collision_thickness;
     A.X = C->optX.as<float4>(m);
     A.Xn = C->optNext.as<float4>(m);
@@ -217,8 +235,24 @@ void collision_response(cs_ctx* C, int nImp, int deform, float obstacleMass) {
     A.lambda = C->optLambda.as<float>(nImp);
     A.coef = C->optCoef.as<float>(nImp);
     int* iterOut = C->optScal.as<int>(4);
-    hipLaunchKernelGGL(k_opt_solve, dim3(1), dim3(OB), 0, s, A, iterOut);
+    hipLaunchKernelGGL(k_opt_solve<View>, dim3(1), dim3(OB), 0, s, A, iterOut);
     CS_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void collision_response(cs_ctx* C, int nImp, int deform, float obstacleMass) {
+    ImpView V;
+    V.r = C->impAcc.ptr<cs_impact_rec>();
+    opt_response(C, V, nImp, deform, C->mass.ptr<float>(), obstacleMass);
+}
+
+// SeparationOptimization (src/optimization/SeparationOptimization.cu:3-158): the same solve with
+// per-node area weights (obstacle nodes: obstacleArea) and 6-node constraints.
+void separation_response(cs_ctx* C, const SepRec* recs, int n, int deform, float obstacleArea) {
+    SepView V;
+    V.r = recs;
+    opt_response(C, V, n, deform, C->area.ptr<float>(), obstacleArea);
 }
 
 }  // namespace cs

@@ -413,13 +413,27 @@ cs_status cs_set_remeshing(cs_ctx* ctx, int cloth_id, const cs_remeshing* r);
  * 10 repulsion thicknesses, the curvature, normal, velocity and compression terms; flips, splits
  * and collapses on the host -- and the context is rebuilt for the new topology (node, face and
  * edge counts change; handles keep their nodes, under new ids; cs_get_counts / cs_get_topology
- * give the new meshes).  New nodes take the mean position and velocity of their edge.  The
- * reference's separation step after remeshing is not run.  A no-op without remeshed cloths. */
+ * give the new meshes).  New nodes take the mean position and velocity of their edge.  Then the
+ * separation step (Simulator::separationStep, src/Simulator.cu:745-799): face pairs that cross
+ * after remeshing are pulled apart along the direction that separated their points before it
+ * (an augmented-Lagrangian solve like the impact response's, first with the obstacles fixed,
+ * then deformable), and the cloth velocities become (x - x0) / dt.  CS_E_COLLISION_FAILED when
+ * crossings remain after 100 solves in both passes (the reference prints and exits).  A no-op
+ * without remeshed cloths. */
 cs_status cs_remesh(cs_ctx* ctx);
 /* Wall-clock split of the last cs_remesh (ms): the sizing field on the device (with its
- * download), the operators on the host, the rebuild of the context (instrumentation; no
- * counterpart in the reference). */
-cs_status cs_remesh_times(const cs_ctx* ctx, float* sizing_ms, float* operators_ms, float* rebuild_ms);
+ * download), the operators on the host, the rebuild of the context, the separation step, and the
+ * separation step's solves (instrumentation; no counterpart in the reference).  Any pointer may
+ * be NULL. */
+cs_status cs_remesh_times(const cs_ctx* ctx, float* sizing_ms, float* operators_ms, float* rebuild_ms,
+                          float* separation_ms, int* separation_solves);
+/* The separation step alone on the current state (the second half of cs_remesh), with the
+ * frame's faces before remeshing given per cloth: faces_per_cloth[c] faces of cloth c, each
+ * with its corners' material-space coordinates (backup_uv, 6 floats) and world positions
+ * (backup_x, 9 floats), concatenated over the cloths in order (Mesh::backupFaces,
+ * src/Mesh.cu:116-128).  For tests and callers that edit meshes themselves; same status as
+ * cs_remesh. */
+cs_status cs_separation_step(cs_ctx* ctx, const int* faces_per_cloth, const float* backup_uv, const float* backup_x);
 /* The sizing the remeshing step would use for one cloth at the current state, with parameters r:
  * face_sizing (4 per face, a row-major 2x2 tensor, faces in cs_get_topology order) and, when not
  * NULL, planes (8 per node: the nearest obstacle point p, 0, the unit normal n, 0; n = 0 where
