@@ -197,11 +197,13 @@ __global__ void __launch_bounds__(PB) k_pcg_setup(int nb, const float* __restric
 // t = A d_prev; then the stop test on r.r of the current residual, beta, and
 // q = s + beta t, d = z + beta d_prev (stored to dcur for the block's rows).
 // The grid covers every row in one pass (no second round for a few blocks).
-// STREAM: the matrix is read with non-temporal loads.  Between two SpMVs of a multigrid or learned
-// iteration the cycle's other kernels sweep more data than the 256 MB last-level cache holds, so
-// keeping the matrix there is lost anyway and streaming it past leaves the gathered vectors more
-// room (measured 5.86 against 5.41 TB/s on the 513^2 crumple); under block Jacobi only the
-// update runs in between and the cached matrix is reused, so there it is read normally.
+// STREAM: the matrix is read with non-temporal loads.  Between two SpMVs of an iteration with the
+// learned-smoother multigrid cycle, its four factor sweeps and the level-0 residual move more data
+// than the 256 MB last-level cache keeps for the matrix, and streaming the SpMV's matrix reads past
+// it leaves the gathered vectors more room (5.85 against 5.41 TB/s, 39.5 against 39.8 ms per step
+// on the 513^2 crumple).  With block Jacobi, the learned factor alone or the plain multigrid the
+// cached matrix is reused by the next SpMV and streaming measured slower (90.7 / 70.9 / 33.5 ms
+// against 91.3 / 66.6 / 32.6), so those read it normally.
 template <bool FIRST, bool STREAM>
 __global__ void __launch_bounds__(PB) k_pcg_spmv(int it, int nn, const int* __restrict__ sliceOff,
                                                  const int* __restrict__ sliceW, const int* __restrict__ cols,
@@ -1058,7 +1060,7 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
                 if (sample) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
                 const int tok = sample ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
                 float4* dcur = dgen[it & 1];
-                const bool stream = amgl || amg || learned;
+                const bool stream = amgl;
                 auto spmv = it == first ? (stream ? k_pcg_spmv<true, true> : k_pcg_spmv<true, false>)
                                         : (stream ? k_pcg_spmv<false, true> : k_pcg_spmv<false, false>);
                 hipLaunchKernelGGL(spmv, dim3(nb), dim3(PB), 0, s, it, nn, S.sliceOff.ptr<int>(), S.sliceW.ptr<int>(),
