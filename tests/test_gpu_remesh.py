@@ -169,3 +169,41 @@ def test_hang_remeshed_every_frame_keeps_its_handles():
     print("hang9 remeshed each frame: node counts %s; corner drift %.3g m (without remeshing %.3g m)" % (
         counts, dr, dp))
     assert counts[-1] != 81  # the sheet was remeshed
+
+
+def test_remeshing_one_cloth_leaves_the_others_alone():
+    """Two hanging sheets far apart, only the first remeshed: the second's nodes, faces and handles
+    move to new global ids (the first cloth's node count changed) but its state, topology and the
+    steps after it are bit-identical to the run without remeshing."""
+    def two_sheets():
+        a, b = scenes.hang(9), scenes.hang(9)
+        mb = b.cloths[0].mesh
+        b.cloths[0].mesh = MeshArrays(mb.x + np.array([5.0, 0.0, 0.0], np.float32), mb.faces)
+        a.cloths.append(b.cloths[0])
+        sx, sf = scenes.icosphere(2, 0.2)
+        a.obstacles.append(MeshArrays(sx + np.array([2.5, 0.5, -3.0]), sf))  # away from both sheets
+        return a
+    spec = two_sheets()
+    sims = [Simulator(spec), Simulator(two_sheets())]
+    sims[0].set_remeshing(0, remeshing_struct(**BLOCK))
+    for frame in range(4):  # until the first sheet's mesh has changed
+        for s in sims:
+            for _ in range(8):
+                assert s.step()[0] == 0
+        sims[0].remesh()
+        if sims[0].counts(CS_MESH_CLOTH, 0) != sims[1].counts(CS_MESH_CLOTH, 0):
+            break
+    assert sims[0].counts(CS_MESH_CLOTH, 0) != sims[1].counts(CS_MESH_CLOTH, 0)
+    for k in range(3):
+        a = [s.state(CS_MESH_CLOTH, 1) for s in sims]
+        for u, w in zip(*a):
+            assert np.array_equal(u, w), k
+        for u, w in zip(sims[0].topology(CS_MESH_CLOTH, 1), sims[1].topology(CS_MESH_CLOTH, 1)):
+            assert np.array_equal(u, w)
+        assert np.array_equal(sims[0].state(CS_MESH_OBSTACLE, 0)[0], sims[1].state(CS_MESH_OBSTACLE, 0)[0])
+        for s in sims:
+            assert s.step()[0] == 0
+    # the second sheet's handled corners still hold
+    x = sims[0].state(CS_MESH_CLOTH, 1)[0]
+    corners = spec.cloths[1].mesh.x[spec.cloths[1].handles]
+    assert np.abs(x[spec.cloths[1].handles] - corners).max() <= 2e-3
