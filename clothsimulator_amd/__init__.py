@@ -504,6 +504,8 @@ class Simulator:
     def separation_step(self, backups):
         """The separation step on the current state; backups: per cloth (uv (F, 3, 2), x (F, 3, 3)),
         its faces before remeshing.  Returns the status (0, or CS_E_COLLISION_FAILED)."""
+        if len(backups) != self.n_cloths:
+            raise ValueError("separation_step needs one backup per cloth (%d, got %d)" % (self.n_cloths, len(backups)))
         counts = np.array([len(u) for u, _ in backups], np.int32)
         uv = np.ascontiguousarray(np.concatenate([np.asarray(u, np.float32).reshape(-1, 6) for u, _ in backups]))
         xx = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float32).reshape(-1, 9) for _, x in backups]))
