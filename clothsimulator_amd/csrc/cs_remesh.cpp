@@ -1,9 +1,8 @@
 // cs_remesh.cpp — flips, splits and collapses of the remeshing step on the host.
 //
-// Each pass evaluates its predicate for every edge, takes a maximal independent set by the
-// minimum-index rule of the reference's device path (faces claimed in rounds: an edge is chosen
-// when it holds the smallest index on every face it would change;
-// src/RemeshingHelper.cu:249-338, 560-647, driven by src/Cloth.cu:286-349, 382-432, 567-646),
+// Each pass evaluates its predicate for every edge, takes the maximal independent set of the
+// minimum-index rule of the reference's device path (src/RemeshingHelper.cu:249-338, 560-647,
+// driven by src/Cloth.cu:286-349, 382-432, 567-646; see independent()),
 // applies the operators of the chosen edges (src/Operator.cu:7-217) and updates the element lists
 // as Mesh::apply does (src/Mesh.cu:138-165): removed entries leave in order, new ones are
 // appended in operator order.  The operators of one set touch disjoint faces, so applying them
@@ -174,43 +173,35 @@ bool should_collapse(const RMesh& m, int e, int side, const Rings& g, const Reme
     return true;
 }
 
-// Maximal independent set by the minimum-index rule; facesOf(i, out) lists candidate i's faces.
-template <class FacesOf>
-std::vector<char> independent(size_t n, size_t nFacePool, FacesOf facesOf) {
+// Maximal independent set by the minimum-index rule over candidates whose faces are listed in CSR
+// form (faces of candidate i: cf[cp[i] .. cp[i+1])).  The reference's device rounds (every open
+// candidate writes its index into its faces' minimum; one holding the minimum on all of its faces
+// is chosen and claims them; candidates touching a claimed face drop out) end with exactly the
+// candidates that no smaller chosen candidate shares a face with: the smallest open candidate is
+// always chosen, and a candidate waits only on smaller ones.  That set is the greedy pass in index
+// order, taken here in one sweep.
+std::vector<char> independent(const std::vector<int>& cp, const std::vector<int>& cf, size_t nFacePool) {
+    const size_t n = cp.size() - 1;
     std::vector<char> removed(nFacePool, 0), sel(n, 0);
-    std::vector<int> mi(nFacePool, 0);
-    std::vector<int> fs;
-    size_t unsel = n;
-    for (;;) {
-        const size_t num = unsel;
-        for (size_t i = 0; i < n; i++) {
-            facesOf(i, fs);
-            bool free = true;
-            for (int f : fs) free = free && !removed[f];
-            if (free)
-                for (int f : fs) mi[f] = (int)n;
-        }
-        for (size_t i = 0; i < n; i++) {
-            facesOf(i, fs);
-            bool free = true;
-            for (int f : fs) free = free && !removed[f];
-            if (free)
-                for (int f : fs) mi[f] = std::min(mi[f], (int)i);
-        }
-        for (size_t i = 0; i < n; i++) {
-            facesOf(i, fs);
-            bool ok = true;
-            for (int f : fs) ok = ok && !removed[f] && mi[f] == (int)i;
-            if (ok) {
-                sel[i] = 1;
-                for (int f : fs) removed[f] = 1;
-            }
-        }
-        unsel = 0;
-        for (char s : sel) unsel += !s;
-        if (!(num > unsel)) break;
+    for (size_t i = 0; i < n; i++) {
+        bool free = true;
+        for (int k = cp[i]; k < cp[i + 1] && free; k++) free = !removed[cf[k]];
+        if (!free) continue;
+        sel[i] = 1;
+        for (int k = cp[i]; k < cp[i + 1]; k++) removed[cf[k]] = 1;
     }
     return sel;
+}
+
+// The faces an edge operator changes: the edge's one or two faces.
+void edge_faces_csr(const RMesh& m, const std::vector<int>& edges, std::vector<int>& cp, std::vector<int>& cf) {
+    cp.assign(1, 0);
+    cf.clear();
+    for (int e : edges) {
+        for (int s = 0; s < 2; s++)
+            if (m.eadj[e][s] >= 0) cf.push_back(m.eadj[e][s]);
+        cp.push_back((int)cf.size());
+    }
 }
 
 void remove_from(std::vector<int>& list, const std::vector<int>& rem, size_t pool) {
@@ -225,18 +216,14 @@ void remove_from(std::vector<int>& list, const std::vector<int>& rem, size_t poo
 
 void append(std::vector<int>& list, const std::vector<int>& add) { list.insert(list.end(), add.begin(), add.end()); }
 
-void adj_faces(const RMesh& m, int e, std::vector<int>& out) {
-    out.clear();
-    for (int s = 0; s < 2; s++)
-        if (m.eadj[e][s] >= 0) out.push_back(m.eadj[e][s]);
-}
-
 // ---------------------------------------------------------------- flip (src/RemeshingHelper.cu:320-375)
 bool flip_some(RMesh& m, const RemeshParams& r) {
     std::vector<int> cand;
     for (int e : m.edges)
         if (should_flip(m, e, r)) cand.push_back(e);
-    std::vector<char> sel = independent(cand.size(), m.fv.size(), [&](size_t i, std::vector<int>& o) { adj_faces(m, cand[i], o); });
+    std::vector<int> cp, cf;
+    edge_faces_csr(m, cand, cp, cf);
+    std::vector<char> sel = independent(cp, cf, m.fv.size());
     std::vector<int> ind;
     for (size_t i = 0; i < cand.size(); i++)
         if (sel[i]) ind.push_back(cand[i]);
@@ -288,7 +275,10 @@ bool split_some(RMesh& m, const RemeshParams& r) {
         if (mt > 1.0f) cs.push_back({mt, e});
     }
     std::stable_sort(cs.begin(), cs.end(), [](const C& a, const C& b) { return a.metric > b.metric; });
-    std::vector<char> sel = independent(cs.size(), m.fv.size(), [&](size_t i, std::vector<int>& o) { adj_faces(m, cs[i].e, o); });
+    std::vector<int> ce, cp, cf;
+    for (const C& c : cs) ce.push_back(c.e);
+    edge_faces_csr(m, ce, cp, cf);
+    std::vector<char> sel = independent(cp, cf, m.fv.size());
     std::vector<int> ind;
     for (size_t i = 0; i < cs.size(); i++)
         if (sel[i]) ind.push_back(cs[i].e);
@@ -374,10 +364,13 @@ bool collapse_some(RMesh& m, const RemeshParams& r) {
         if (should_collapse(m, e, 0, g, r)) cand.push_back({e, 0});
         else if (should_collapse(m, e, 1, g, r)) cand.push_back({e, 1});
     }
-    std::vector<char> sel = independent(cand.size(), m.fv.size(), [&](size_t i, std::vector<int>& o) {
-        const int n0 = m.en[cand[i][0]][cand[i][1]];
-        o.assign(g.fidx.begin() + g.fptr[n0], g.fidx.begin() + g.fptr[n0 + 1]);
-    });
+    std::vector<int> cp(1, 0), cf;  // the faces around the node that goes
+    for (const auto& c : cand) {
+        const int n0 = m.en[c[0]][c[1]];
+        cf.insert(cf.end(), g.fidx.begin() + g.fptr[n0], g.fidx.begin() + g.fptr[n0 + 1]);
+        cp.push_back((int)cf.size());
+    }
+    std::vector<char> sel = independent(cp, cf, m.fv.size());
     std::vector<std::array<int, 2>> ind;
     for (size_t i = 0; i < cand.size(); i++)
         if (sel[i]) ind.push_back(cand[i]);

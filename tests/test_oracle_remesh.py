@@ -340,3 +340,19 @@ def test_host_operators_reject_malformed_input():
     bad[0, 0] = len(uv)
     with pytest.raises(ClothError):
         remesh_mesh(rr, x, x, x, np.zeros(len(x)), uv, vnode, bad, en, iso(len(fv), 0.1))
+
+
+def test_minimum_index_rounds_equal_the_greedy_pass():
+    """The reference's device rounds (min index per face, chosen when minimal on all its faces)
+    choose exactly what a greedy pass in index order chooses; the library takes the greedy pass."""
+    rng = np.random.default_rng(5)
+    for trial in range(200):
+        n, nf = int(rng.integers(1, 60)), int(rng.integers(1, 40))
+        faces = [sorted(set(rng.integers(0, nf, int(rng.integers(1, 7))).tolist())) for _ in range(n)]
+        rounds = R.Mesh.independent(list(range(n)), lambda c: faces[c])
+        taken, greedy = set(), []
+        for c in range(n):
+            if not taken & set(faces[c]):
+                greedy.append(c)
+                taken |= set(faces[c])
+        assert rounds == greedy, trial
