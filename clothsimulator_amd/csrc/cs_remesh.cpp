@@ -509,8 +509,13 @@ bool rmesh_build(RMesh& m, int nn, const float* x, const float* x0, const float*
             const long long k = std::min(p, q) * nn + std::max(p, q);
             auto it = std::lower_bound(key.begin(), key.end(), std::make_pair(k, -1));
             if (it == key.end() || it->first != k) return false;
-            m.fe[f][i] = it->second;
-            init_edge(m, it->second, m.fv[f][(i + 2) % 3], f);
+            const int e = it->second;
+            m.fe[f][i] = e;
+            // a manifold mesh fills each edge side once: a second face on the same side (a third
+            // face on the edge, or two faces with the same orientation) is malformed input
+            const int side = m.vnode[m.fv[f][i]] == m.en[e][0] ? 0 : 1;
+            if (m.eadj[e][side] >= 0) return false;
+            init_edge(m, e, m.fv[f][(i + 2) % 3], f);
         }
     for (int i = 0; i < nn; i++) m.nodes.push_back(i);
     for (int k = 0; k < nv; k++) m.verts.push_back(k);

@@ -131,7 +131,8 @@ int remesh_context(cs_ctx* C) {
         RMesh rm;
         if (!rmesh_build(rm, M.nn, L.x.data(), x0.data(), L.v.data(), L.pres.data(), M.nv, L.uv.data(),
                          L.vnode.data(), M.nf, L.fv.data(), M.ne, L.en.data()))
-            throw StatusError(CS_E_FORMAT, "remeshing: a face edge is missing from the edge list");
+            throw StatusError(CS_E_FORMAT, "remeshing: the cloth's topology is not a consistently oriented "
+                                           "manifold (an edge side with two faces)");
         rm.flipCount = C->flipCount;
         rmesh_set_face_sizing(rm, fs.data());
         rmesh_remesh(rm, r);

@@ -389,8 +389,9 @@ void cs_default_remeshing(cs_remeshing* r);
  * distinct (min, max) node pairs ascending).  flip_count carries the size of the last flip pass
  * between calls (the reference keeps it in a function-level static, src/Cloth.cu:321); start it
  * at 0.  Host only: needs no device.  The result is read with cs_remesh_result_* and freed with
- * cs_remesh_result_free.  CS_E_FORMAT when a face edge is missing from edge_nodes or an index
- * is out of range. */
+ * cs_remesh_result_free.  CS_E_FORMAT when a face edge is missing from edge_nodes, an edge side
+ * has two faces (a non-manifold edge, or neighbours with opposite orientations) or an index is
+ * out of range. */
 typedef struct cs_remesh_result cs_remesh_result;
 cs_status cs_remesh_mesh(const cs_remeshing* r, int n_nodes, const float* x, const float* x0, const float* v,
                          const unsigned char* preserve, int n_verts, const float* uv, const int* vert_node,

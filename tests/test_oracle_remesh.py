@@ -340,6 +340,10 @@ def test_host_operators_reject_malformed_input():
     bad[0, 0] = len(uv)
     with pytest.raises(ClothError):
         remesh_mesh(rr, x, x, x, np.zeros(len(x)), uv, vnode, bad, en, iso(len(fv), 0.1))
+    flipped = fv.copy()
+    flipped[0] = flipped[0][::-1]  # one face against its neighbours' orientation
+    with pytest.raises(ClothError):
+        remesh_mesh(rr, x, x, x, np.zeros(len(x)), uv, vnode, flipped, en, iso(len(fv), 0.1))
 
 
 def test_minimum_index_rounds_equal_the_greedy_pass():
