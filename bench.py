@@ -157,7 +157,7 @@ def timed_steps(steps, span):
     return sorted(set(int(i * span // steps) for i in range(steps)))
 
 
-def run(workload, precond, steps, warmup, timed):
+def run(workload, precond, steps, warmup, timed, extra_flags=0):
     """Steps `workload` from its initial state with one preconditioner; returns the timings."""
     from clothsimulator_amd import CS_MESH_CLOTH, CS_MESH_OBSTACLE, Simulator, scenes
     from clothsimulator_amd._cabi import (CS_FLAG_TIME_KERNELS, CS_KERNEL_EVENT_PAIR, CS_KERNEL_PCG_SPMV,
@@ -167,6 +167,7 @@ def run(workload, precond, steps, warmup, timed):
     spec = scenes.by_name(workload)
     if timed:
         spec.params.flags |= CS_FLAG_TIME_KERNELS
+    spec.params.flags |= extra_flags
     spec.params.precond = {"amg": CS_PRECOND_AMG, "jacobi": CS_PRECOND_BLOCK_JACOBI, "learned": CS_PRECOND_LEARNED,
                            "amg-learned": CS_PRECOND_AMG_LEARNED}[precond]
     sim = Simulator(spec)   # the learned preconditioner loads the packaged trained weights
@@ -307,6 +308,9 @@ def main():
     ap.add_argument("--sequential-sample", type=int, default=16, metavar="K",
                     help="batch workloads: also run K of the instances one at a time, each on its own context, "
                          "and report the sequential throughput they imply")
+    ap.add_argument("--persistent", action="store_true",
+                    help="batch workloads: also run block Jacobi with each poll's iterations in one cooperative "
+                         "launch (CS_FLAG_PERSISTENT_BATCH)")
     ap.add_argument("--cpu-direct", default="", metavar="WORKLOAD:STEPS,...",
                     help="also time the single-thread direct-solve CPU column on these windows "
                          "(e.g. drape129:20,crumple513:3); hang33 over its 100 steps always runs")
@@ -394,6 +398,10 @@ def main():
     }
     for p, r in others.items():
         out[PRECONDS[p]] = summary(r)
+    if a.persistent and a.workload.startswith("batch"):
+        from clothsimulator_amd._cabi import CS_FLAG_PERSISTENT_BATCH
+        out["block_jacobi_persistent"] = summary(run(a.workload, "jacobi", steps, a.warmup, False,
+                                                     CS_FLAG_PERSISTENT_BATCH))
     if a.workload.startswith("batch") and a.sequential_sample > 0:
         out["sequential"] = sequential_sample(spec, a.precond, steps, a.sequential_sample, ms["value"])
         out["config"]["instances"] = len(spec.cloths)

@@ -37,6 +37,7 @@ CS_FLAG_TIME_KERNELS = 4
 CS_FLAG_INSTANCES = 8
 CS_FLAG_FIXED_TUNING = 16
 CS_FLAG_GRAPH_CYCLES = 32
+CS_FLAG_PERSISTENT_BATCH = 64
 CS_LBVH_CCD = 1
 CS_LBVH_REFIT = 2
 CS_KERNEL_PCG_SPMV = 0

@@ -24,6 +24,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "cs_kernels.hpp"
 #include "cs_math.hpp"
 
@@ -633,12 +635,10 @@ __global__ void __launch_bounds__(BP) k_bpcg_setup(const BCloth* __restrict__ cl
 // t = A d_prev first, then the stop test from the r.r partials (loaded at entry),
 // beta, q = s + beta t, d = z + beta d_prev and the partial d.q.  A stopped
 // cloth's partials no longer change, so its blocks keep deciding to stop.
+// The body of one (cloth, block) m; lds holds 3 * BP / 64 floats.  Its early returns are uniform
+// over the workgroup (per-cloth values and block sums).
 template <int BP, bool FIRST>
-__global__ void __launch_bounds__(BP) k_bpcg_spmv(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
-                                                  int it) {
-    __shared__ float lds[3 * BP / 64];
-    const int2 m = batch_block(bmap);
-    if (m.x < 0) return;
+__device__ __forceinline__ void bpcg_spmv_block(const BCloth* __restrict__ cl, int2 m, int it, float* lds) {
     const BClothG c = gview(cl[m.x]);
     const int lb = m.y;
     const int p = it & 1;
@@ -740,14 +740,19 @@ __global__ void __launch_bounds__(BP) k_bpcg_spmv(const BCloth* __restrict__ cl,
     if (threadIdx.x == 0) c.part[P_DQ * c.nb + lb] = pdq;
 }
 
+template <int BP, bool FIRST>
+__global__ void __launch_bounds__(BP) k_bpcg_spmv(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
+                                                  int it) {
+    __shared__ float lds[3 * BP / 64];
+    const int2 m = batch_block(bmap);
+    if (m.x < 0) return;
+    bpcg_spmv_block<BP, FIRST>(cl, m, it, lds);
+}
+
 // alpha, x += alpha d, r -= alpha q, z = Minv r, partials r.z and r.r (the row
 // loads are issued before the partials are reduced)
 template <int BP>
-__global__ void __launch_bounds__(BP) k_bpcg_update(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
-                                                    int it) {
-    __shared__ float lds[2 * BP / 64];
-    const int2 m = batch_block(bmap);
-    if (m.x < 0) return;
+__device__ __forceinline__ void bpcg_update_block(const BCloth* __restrict__ cl, int2 m, int it, float* lds) {
     const BClothG c = gview(cl[m.x]);
     const int lb = m.y;
     const int p = it & 1;
@@ -787,6 +792,51 @@ __global__ void __launch_bounds__(BP) k_bpcg_update(const BCloth* __restrict__ c
     if (threadIdx.x == 0) {
         if (!c.amg) c.part[(P_RZ0 + (1 - p)) * c.nb + lb] = acc[0];
         c.part[P_RR * c.nb + lb] = acc[1];
+    }
+}
+
+template <int BP>
+__global__ void __launch_bounds__(BP) k_bpcg_update(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
+                                                    int it) {
+    __shared__ float lds[2 * BP / 64];
+    const int2 m = batch_block(bmap);
+    if (m.x < 0) return;
+    bpcg_update_block<BP>(cl, m, it, lds);
+}
+
+// Iterations [it0, it0 + n) of the block-Jacobi solve of every cloth of the map in one
+// cooperative launch (a grid no larger than the workgroups the device holds at once): each
+// workgroup walks its share of the map, the two phases of an iteration separated by grid
+// barriers.  Workgroups are dealt to the XCDs round-robin; XCD x's workgroups take the x-th
+// eighth of the map (nmap and gridDim.x are multiples of 8), the rows logical_block gives
+// that XCD in a full grid.  Every workgroup runs all n iterations and meets every barrier; the
+// per-block bodies are the launched kernels', so the results are theirs bit for bit.
+template <int BP>
+__global__ void __launch_bounds__(BP) k_bpcg_persist(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
+                                                     int nmap, int it0, int n) {
+    __shared__ float lds[3 * BP / 64];
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const int per = gridDim.x >> 3, xcd = blockIdx.x & 7, span = nmap >> 3;
+    const int2* __restrict__ mine = bmap + xcd * span;
+    for (int k = 0; k < n; k++) {
+        const int it = it0 + k;
+        for (int L = blockIdx.x >> 3; L < span; L += per) {
+            const int2 mm = mine[L];
+            const int2 m = make_int2(__builtin_amdgcn_readfirstlane(mm.x), __builtin_amdgcn_readfirstlane(mm.y));
+            if (m.x < 0) continue;
+            if (it == 0) bpcg_spmv_block<BP, true>(cl, m, it, lds);
+            else bpcg_spmv_block<BP, false>(cl, m, it, lds);
+        }
+        __threadfence();
+        grid.sync();
+        for (int L = blockIdx.x >> 3; L < span; L += per) {
+            const int2 mm = mine[L];
+            const int2 m = make_int2(__builtin_amdgcn_readfirstlane(mm.x), __builtin_amdgcn_readfirstlane(mm.y));
+            if (m.x < 0) continue;
+            bpcg_update_block<BP>(cl, m, it, lds);
+        }
+        __threadfence();
+        grid.sync();
     }
 }
 
@@ -1150,6 +1200,20 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
 // The batched solve with BP rows per block and `chunk` iterations per host poll.  amg: the
 // cloths with a usable hierarchy run one multigrid cycle per iteration (level 0 in these
 // kernels, the coarser levels of all cloths in bamg_cycle's launches).
+// Workgroups of k_bpcg_persist<BP> the device holds at once, a multiple of 8 (the grid of a
+// cooperative launch may not exceed it).
+template <int BP>
+static int persist_grid(cs_ctx* C) {
+    int coop = 0, cus = 0, perCU = 0;
+    CS_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, C->device));
+    if (!coop) throw StatusError(CS_E_HIP, "CS_FLAG_PERSISTENT_BATCH: the device has no cooperative launch");
+    CS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C->device));
+    CS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_bpcg_persist<BP>, BP, 0));
+    const int g = (perCU * cus) & ~7;
+    if (g < 8) throw StatusError(CS_E_HIP, "CS_FLAG_PERSISTENT_BATCH: too few resident workgroups");
+    return g;
+}
+
 template <int BP>
 static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* converged, bool amg) {
     hipStream_t s = C->stream;
@@ -1217,6 +1281,9 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
         }
     }
     const bool anyAmg = std::find(useAmg.begin(), useAmg.end(), 1) != useAmg.end();
+    // block Jacobi only: the multigrid cycle's launches stay launches
+    const bool persist = (C->P.flags & CS_FLAG_PERSISTENT_BATCH) && !anyAmg;
+    const int persistMax = persist ? persist_grid<BP>(C) : 0;
     // one multigrid cycle for the cloths of the map (their r and xs written): z and the partial
     // r.z into `slot`
     auto cycle = [&](const int2* map, int nblk, int liveOnly, int slot) {
@@ -1256,7 +1323,16 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
     for (int restarts = 0;; restarts++) {
         const int passStart = it;
         while (!all && it - passStart <= maxit + 2 * chunk) {
-            for (int k = 0; k < chunk; k++, it++) {
+            if (persist) {  // the poll's iterations in one cooperative launch
+                int nmap = NB, it0 = it, n = chunk;
+                const BCloth* ctab = dtab;
+                const int2* cmap = dmap;
+                void* args[] = {(void*)&ctab, (void*)&cmap, (void*)&nmap, (void*)&it0, (void*)&n};
+                CS_HIP(hipLaunchCooperativeKernel((const void*)k_bpcg_persist<BP>, dim3(std::min(NB, persistMax)),
+                                                  dim3(BP), args, 0, s));
+                it += chunk;
+            }
+            for (int k = 0; k < chunk && !persist; k++, it++) {
                 // sampled: every 8th batch SpMV while every cloth is still iterating (the launch
                 // then covers the whole batch), each after an empty event pair that measures the
                 // markers' own latency

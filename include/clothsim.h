@@ -68,8 +68,12 @@ enum {
                                  (no counterpart) */
     CS_FLAG_FIXED_TUNING = 16, /* batched solves keep their default launch shape instead of autotuning it
                                  (bit-reproducible runs; no counterpart) */
-    CS_FLAG_GRAPH_CYCLES = 32 /* multigrid cycles replayed from HIP graphs captured once per solve instead of
+    CS_FLAG_GRAPH_CYCLES = 32, /* multigrid cycles replayed from HIP graphs captured once per solve instead of
                                  launched kernel by kernel (the same kernels in the same order; no counterpart) */
+    CS_FLAG_PERSISTENT_BATCH = 64 /* the batched block-Jacobi solve (CS_FLAG_INSTANCES) runs the iterations between
+                                 two host polls as one cooperative launch, a grid barrier between the two phases
+                                 of an iteration, instead of two launches per iteration (the same results bit
+                                 for bit; no counterpart) */
 };
 
 /* kernel ids for cs_kernel_time.  CS_KERNEL_EVENT_PAIR times an empty event pair recorded on

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from clothsimulator_amd import CS_MESH_CLOTH, Simulator, scenes
-from clothsimulator_amd._cabi import CS_FLAG_FIXED_TUNING, CS_FLAG_INSTANCES
+from clothsimulator_amd._cabi import CS_FLAG_FIXED_TUNING, CS_FLAG_INSTANCES, CS_FLAG_PERSISTENT_BATCH
 
 pytestmark = pytest.mark.gpu
 
@@ -65,6 +65,32 @@ def test_batched_contacts_match_separate_runs():
         assert np.abs(xb - xs).max() <= 1e-5, (k, np.abs(xb - xs).max())
         assert np.abs(vb - vs).max() <= 2e-3, (k, np.abs(vb - vs).max())
     assert contacts_batch == contacts_single, (contacts_batch, contacts_single)
+
+
+@pytest.mark.parametrize("count,side_lo,side_hi,steps", [(3, 17, 25, 8), (256, 33, 65, 3)])
+def test_persistent_batched_solve_equals_launched(count, side_lo, side_hi, steps):
+    """CS_FLAG_PERSISTENT_BATCH: the iterations between two polls as one cooperative launch with
+    grid barriers give the launched kernels' steps bit for bit (the same per-block bodies in the
+    same order), on instances starting 4 mm above their spheres.  256 instances of sides 33..65
+    make about 2,500 (cloth, 256-row block) pairs, more than the roughly 1,500 workgroups the
+    device holds at once, so each workgroup walks several."""
+    states = []
+    for flags in (CS_FLAG_PERSISTENT_BATCH, 0):
+        spec = scenes.instances(count=count, seed=5, side_lo=side_lo, side_hi=side_hi, gap=0.004)
+        spec.params.flags |= CS_FLAG_FIXED_TUNING | flags
+        sim = Simulator(spec)
+        its = []
+        for _ in range(steps):
+            s, st = sim.step()
+            assert s == 0, s
+            its.append(st.pcg_iters)
+        print("flags %d: iterations %s" % (flags, its), flush=True)
+        states.append(([sim.state(CS_MESH_CLOTH, k) for k in range(count)], its))
+    (a, ia), (b, ib) = states
+    assert ia == ib
+    for k in range(count):
+        for u, v in zip(a[k], b[k]):
+            assert np.array_equal(u, v), k
 
 
 def test_instance_pairs_stay_inside_their_instance():
