@@ -631,13 +631,19 @@ __global__ void __launch_bounds__(BP) k_bpcg_setup(const BCloth* __restrict__ cl
     }
 }
 
+// A matrix load, non-temporal under STREAM (a batch whose matrix cannot stay in the
+// last-level cache between iterations leaves that cache to the gathered vectors).
+template <bool STREAM, class T> __device__ __forceinline__ T ldm(const G1<T>* p) {
+    return STREAM ? __builtin_nontemporal_load(p) : *p;
+}
+
 // Iteration `it` of every unfinished cloth, as k_pcg_spmv: the sweep s = A z,
 // t = A d_prev first, then the stop test from the r.r partials (loaded at entry),
 // beta, q = s + beta t, d = z + beta d_prev and the partial d.q.  A stopped
 // cloth's partials no longer change, so its blocks keep deciding to stop.
 // The body of one (cloth, block) m; lds holds 3 * BP / 64 floats.  Its early returns are uniform
 // over the workgroup (per-cloth values and block sums).
-template <int BP, bool FIRST>
+template <int BP, bool FIRST, bool STREAM>
 __device__ __forceinline__ void bpcg_spmv_block(const BCloth* __restrict__ cl, int2 m, int it, float* lds) {
     const BClothG c = gview(cl[m.x]);
     const int lb = m.y;
@@ -661,13 +667,13 @@ __device__ __forceinline__ void bpcg_spmv_block(const BCloth* __restrict__ cl, i
         // streams from HBM).  The sums still run slot by slot.
         int k = 0;
         for (; k + 1 < W; k += 2) {
-            const int j0 = cp[k * SLICE], j1 = cp[(k + 1) * SLICE];
+            const int j0 = ldm<STREAM>(cp + k * SLICE), j1 = ldm<STREAM>(cp + (k + 1) * SLICE);
             const G1<float>* ap = vp + (size_t)k * 9 * SLICE;
             float a0[9], a1[9];
 #pragma unroll
             for (int q = 0; q < 9; q++) {
-                a0[q] = ap[q * SLICE];
-                a1[q] = ap[(9 + q) * SLICE];
+                a0[q] = ldm<STREAM>(ap + q * SLICE);
+                a1[q] = ldm<STREAM>(ap + (9 + q) * SLICE);
             }
             const f4n z0 = c.z[j0], z1 = c.z[j1];
             f4n e0, e1;
@@ -681,11 +687,11 @@ __device__ __forceinline__ void bpcg_spmv_block(const BCloth* __restrict__ cl, i
             if (!FIRST) acc3(t0, t1, t2, a1, v3(e1));
         }
         if (k < W) {
-            const int j = cp[k * SLICE];
+            const int j = ldm<STREAM>(cp + k * SLICE);
             const G1<float>* ap = vp + (size_t)k * 9 * SLICE;
             float a[9];
 #pragma unroll
-            for (int q = 0; q < 9; q++) a[q] = ap[q * SLICE];
+            for (int q = 0; q < 9; q++) a[q] = ldm<STREAM>(ap + q * SLICE);
             acc3(s0, s1, s2, a, v3(c.z[j]));
             if (!FIRST) acc3(t0, t1, t2, a, v3(dprev[j]));
         }
@@ -740,13 +746,13 @@ __device__ __forceinline__ void bpcg_spmv_block(const BCloth* __restrict__ cl, i
     if (threadIdx.x == 0) c.part[P_DQ * c.nb + lb] = pdq;
 }
 
-template <int BP, bool FIRST>
+template <int BP, bool FIRST, bool STREAM>
 __global__ void __launch_bounds__(BP) k_bpcg_spmv(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
                                                   int it) {
     __shared__ float lds[3 * BP / 64];
     const int2 m = batch_block(bmap);
     if (m.x < 0) return;
-    bpcg_spmv_block<BP, FIRST>(cl, m, it, lds);
+    bpcg_spmv_block<BP, FIRST, STREAM>(cl, m, it, lds);
 }
 
 // alpha, x += alpha d, r -= alpha q, z = Minv r, partials r.z and r.r (the row
@@ -824,8 +830,8 @@ __global__ void __launch_bounds__(BP) k_bpcg_persist(const BCloth* __restrict__ 
             const int2 mm = mine[L];
             const int2 m = make_int2(__builtin_amdgcn_readfirstlane(mm.x), __builtin_amdgcn_readfirstlane(mm.y));
             if (m.x < 0) continue;
-            if (it == 0) bpcg_spmv_block<BP, true>(cl, m, it, lds);
-            else bpcg_spmv_block<BP, false>(cl, m, it, lds);
+            if (it == 0) bpcg_spmv_block<BP, true, false>(cl, m, it, lds);
+            else bpcg_spmv_block<BP, false, false>(cl, m, it, lds);
         }
         __threadfence();
         grid.sync();
@@ -896,7 +902,7 @@ __global__ void __launch_bounds__(BP) k_bpcg_true(const BCloth* __restrict__ cl,
 
 // Level 0 of the multigrid cycle of every listed cloth that uses one: t = r - A xs (sliced
 // ELL, then the contact blocks, in slot order).  liveOnly: cloths already stopped are skipped.
-template <int BP>
+template <int BP, bool STREAM>
 __global__ void __launch_bounds__(BP) k_bpcg_resid0(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
                                                     int liveOnly) {
     const int2 m = batch_block(bmap);
@@ -910,10 +916,10 @@ __global__ void __launch_bounds__(BP) k_bpcg_resid0(const BCloth* __restrict__ c
     const size_t off = (size_t)c.sliceOff[sl];
     float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
     for (int k = 0; k < W; k++) {
-        const int j = c.cols[(off + k) * SLICE + lane];
+        const int j = ldm<STREAM>(c.cols + (off + k) * SLICE + lane);
         float a[9];
 #pragma unroll
-        for (int q = 0; q < 9; q++) a[q] = c.vals[(off + k) * 9 * SLICE + q * SLICE + lane];
+        for (int q = 0; q < 9; q++) a[q] = ldm<STREAM>(c.vals + (off + k) * 9 * SLICE + q * SLICE + lane);
         acc3(y0, y1, y2, a, ld3v(c.xs, j));
     }
     for (int kc = c.cRowPtr[i], ce = c.cRowPtr[i + 1]; kc < ce; kc++) {
@@ -928,7 +934,7 @@ __global__ void __launch_bounds__(BP) k_bpcg_resid0(const BCloth* __restrict__ c
 // Level 0 post-smoothing from the stored residual t = r - A xs: with e = level 1's result,
 // r - A (xs + P e) = t - (A P) e (A P = the static A P plus the contact blocks gathered through
 // the aggregates), z = xs + P e + w Minv (t - (A P) e); the partial of r.z into `slot`.
-template <int BP>
+template <int BP, bool STREAM>
 __global__ void __launch_bounds__(BP) k_bpcg_post0(const BCloth* __restrict__ cl, const int2* __restrict__ bmap,
                                                    int liveOnly, int slot) {
     __shared__ float lds[BP / 64];
@@ -947,8 +953,8 @@ __global__ void __launch_bounds__(BP) k_bpcg_post0(const BCloth* __restrict__ cl
         for (int k = 0; k < WA; k++) {
             float a[9];
 #pragma unroll
-            for (int q = 0; q < 9; q++) a[q] = c.apVals[(aoff + k) * 9 * SLICE + q * SLICE + lane];
-            acc3(y0, y1, y2, a, ld3v(c.e1, c.apCols[(aoff + k) * SLICE + lane]));
+            for (int q = 0; q < 9; q++) a[q] = ldm<STREAM>(c.apVals + (aoff + k) * 9 * SLICE + q * SLICE + lane);
+            acc3(y0, y1, y2, a, ld3v(c.e1, ldm<STREAM>(c.apCols + (aoff + k) * SLICE + lane)));
         }
         for (int kc = c.cRowPtr[i], ce = c.cRowPtr[i + 1]; kc < ce; kc++) {
             float a[9];
@@ -1200,6 +1206,10 @@ int pcg_solve(cs_ctx* C, int c, float* relres, int* converged) {
 // The batched solve with BP rows per block and `chunk` iterations per host poll.  amg: the
 // cloths with a usable hierarchy run one multigrid cycle per iteration (level 0 in these
 // kernels, the coarser levels of all cloths in bamg_cycle's launches).
+// Batches whose static matrices exceed this read them with non-temporal loads in the SpMV: half
+// the 256 MB last-level cache, which also holds the vectors and the other kernels' data.
+constexpr size_t kStreamBytes = 128u << 20;
+
 // Workgroups of k_bpcg_persist<BP> the device holds at once, a multiple of 8 (the grid of a
 // cooperative launch may not exceed it).
 template <int BP>
@@ -1284,14 +1294,18 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
     // block Jacobi only: the multigrid cycle's launches stay launches
     const bool persist = (C->P.flags & CS_FLAG_PERSISTENT_BATCH) && !anyAmg;
     const int persistMax = persist ? persist_grid<BP>(C) : 0;
+    size_t matBytes = 0;  // the static matrices of all cloths: 36 B of values + 4 B of column per slot
+    for (int c = 0; c < ncl; c++) matBytes += (size_t)C->sys[c].totalSlots * SLICE * 40;
+    const bool streamA = matBytes > kStreamBytes;
     // one multigrid cycle for the cloths of the map (their r and xs written): z and the partial
     // r.z into `slot`
     auto cycle = [&](const int2* map, int nblk, int liveOnly, int slot) {
         if (!anyAmg) return;
-        hipLaunchKernelGGL(k_bpcg_resid0<BP>, dim3(nblk), dim3(BP), 0, s, (const BCloth*)C->bTab.p, map, liveOnly);
+        hipLaunchKernelGGL((streamA ? k_bpcg_resid0<BP, true> : k_bpcg_resid0<BP, false>), dim3(nblk), dim3(BP), 0, s,
+                           (const BCloth*)C->bTab.p, map, liveOnly);
         bamg_cycle(C);
-        hipLaunchKernelGGL(k_bpcg_post0<BP>, dim3(nblk), dim3(BP), 0, s, (const BCloth*)C->bTab.p, map, liveOnly,
-                           slot);
+        hipLaunchKernelGGL((streamA ? k_bpcg_post0<BP, true> : k_bpcg_post0<BP, false>), dim3(nblk), dim3(BP), 0, s,
+                           (const BCloth*)C->bTab.p, map, liveOnly, slot);
     };
     // (cloth, block) of every logical block of the unfinished cloths, padded to a multiple of 8
     std::vector<int2> bmap;
@@ -1339,10 +1353,9 @@ static void solve_batched(cs_ctx* C, int chunk, int* iters, float* relres, int* 
                 const bool sample = it % 8 == 1 && NB == NBfull;
                 if (sample) ktimer_end(C, ktimer_begin(C, CS_KERNEL_EVENT_PAIR));
                 const int tok = sample ? ktimer_begin(C, CS_KERNEL_PCG_SPMV) : -1;
-                if (it == 0)
-                    hipLaunchKernelGGL((k_bpcg_spmv<BP, true>), dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
-                else
-                    hipLaunchKernelGGL((k_bpcg_spmv<BP, false>), dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
+                auto spmv = it == 0 ? (streamA ? k_bpcg_spmv<BP, true, true> : k_bpcg_spmv<BP, true, false>)
+                                    : (streamA ? k_bpcg_spmv<BP, false, true> : k_bpcg_spmv<BP, false, false>);
+                hipLaunchKernelGGL(spmv, dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
                 ktimer_end(C, tok);
                 hipLaunchKernelGGL(k_bpcg_update<BP>, dim3(NB), dim3(BP), 0, s, dtab, dmap, it);
                 cycle(dmap, NB, 1, P_RZ0 + (1 - (it & 1)));

@@ -73,7 +73,9 @@ def test_persistent_batched_solve_equals_launched(count, side_lo, side_hi, steps
     grid barriers give the launched kernels' steps bit for bit (the same per-block bodies in the
     same order), on instances starting 4 mm above their spheres.  256 instances of sides 33..65
     make about 2,500 (cloth, 256-row block) pairs, more than the roughly 1,500 workgroups the
-    device holds at once, so each workgroup walks several."""
+    device holds at once, so each workgroup walks several; their static matrices (over 300 MB) also
+    put the launched SpMV on non-temporal loads (above 128 MB), the persistent one on cached
+    ones."""
     states = []
     for flags in (CS_FLAG_PERSISTENT_BATCH, 0):
         spec = scenes.instances(count=count, seed=5, side_lo=side_lo, side_hi=side_hi, gap=0.004)
