@@ -30,7 +30,8 @@ EXPORTED = ["cs_version", "cs_default_params", "cs_create", "cs_destroy", "cs_la
             "cs_scene_motion", "cs_get_topology", "cs_write_config", "cs_impact_response",
             "cs_solve_bsr_noalloc", "cs_lpc_train_step_smooth", "cs_default_remeshing", "cs_remesh_mesh",
             "cs_remesh_result_counts", "cs_remesh_result_get", "cs_remesh_result_free", "cs_set_remeshing",
-            "cs_remesh", "cs_remesh_sizing", "cs_remesh_times", "cs_separation_step"]
+            "cs_remesh", "cs_remesh_sizing", "cs_remesh_times", "cs_separation_step", "cs_frame_begin",
+            "cs_frame_end"]
 
 # trained weights of the learned preconditioner shipped with the package (float32, .npy)
 LPC_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lpc_weights.npy")
@@ -68,6 +69,8 @@ def load_library(path=LIB_PATH):
     F = C.POINTER(C.c_float)
     L.cs_get_state.argtypes = [vp, C.c_int, C.c_int, F, F, F]
     L.cs_set_state.argtypes = [vp, C.c_int, C.c_int, F, F, F]
+    L.cs_frame_begin.argtypes = [vp, C.c_int]
+    L.cs_frame_end.argtypes = [vp, C.c_int, C.c_int, F, F, C.POINTER(C.c_int)]
     L.cs_get_geometry.argtypes = [vp, C.c_int, C.c_int, F, F, F]
     L.cs_material_table.argtypes = [C.POINTER(MaterialIn), F, F, F]
     L.cs_lbvh_pairs.argtypes = [vp, C.c_int, C.c_float, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
@@ -372,6 +375,20 @@ class Simulator:
         x, x0, v = (np.zeros((nn, 3), dtype=np.float32) for _ in range(3))
         self._check(self.L.cs_get_state(self.h, kind, idx, fptr(x), fptr(x0), fptr(v)), "cs_get_state")
         return x, x0, v
+
+    def frame_begin(self, slot):
+        """Snapshot every cloth's positions and velocities into download slot 0 or 1 and start
+        their download; returns at once (cs_frame_begin)."""
+        self._check(self.L.cs_frame_begin(self.h, slot), "cs_frame_begin")
+
+    def frame_end(self, slot, cloth=0):
+        """(x, v) of `cloth` in slot's snapshot, after waiting for its download (cs_frame_end)."""
+        nul = C.POINTER(C.c_float)()
+        n = C.c_int()
+        self._check(self.L.cs_frame_end(self.h, slot, cloth, nul, nul, C.byref(n)), "cs_frame_end")
+        x, v = np.zeros((n.value, 3), dtype=np.float32), np.zeros((n.value, 3), dtype=np.float32)
+        self._check(self.L.cs_frame_end(self.h, slot, cloth, fptr(x), fptr(v), C.byref(n)), "cs_frame_end")
+        return x, v
 
     def set_state(self, kind, idx, x=None, x0=None, v=None):
         conv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)

@@ -301,12 +301,19 @@ void cs_destroy(cs_ctx* C) {
     if (C->hBatch) (void)hipHostFree(C->hBatch);
     if (C->hCount) (void)hipHostFree(C->hCount);
     for (hipEvent_t e : C->kt.events) (void)hipEventDestroy(e);
+    if (C->copy) (void)hipStreamSynchronize(C->copy);
+    for (auto& F : C->frame) {
+        if (F.done) (void)hipEventDestroy(F.done);
+        if (F.host) (void)hipHostFree(F.host);
+    }
+    if (C->evSnap) (void)hipEventDestroy(C->evSnap);
     cs_ctx* c = C;
     // device buffers are released by the DBuf destructors
-    hipStream_t s = c->stream, s2 = c->side;
+    hipStream_t s = c->stream, s2 = c->side, s3 = c->copy;
     delete c;
     if (s) (void)hipStreamDestroy(s);
     if (s2) (void)hipStreamDestroy(s2);
+    if (s3) (void)hipStreamDestroy(s3);
 }
 
 const char* cs_last_error(const cs_ctx* C) { return C ? C->err.c_str() : "null context"; }
@@ -524,6 +531,80 @@ cs_status cs_get_state(cs_ctx* C, int kind, int id, float* x, float* x0, float* 
         }
         return (int)CS_OK;
     });
+}
+
+cs_status cs_frame_begin(cs_ctx* C, int slot) {
+    if (!C || slot < 0 || slot > 1) return CS_E_ARG;
+    return guard(C, [&]() {
+        host_finalize(C);
+        auto& F = C->frame[slot];
+        if (!C->copy) {
+            CS_HIP(hipStreamCreateWithFlags(&C->copy, hipStreamNonBlocking));
+            CS_HIP(hipEventCreateWithFlags(&C->evSnap, hipEventDisableTiming));
+        }
+        if (!F.done) CS_HIP(hipEventCreateWithFlags(&F.done, hipEventDisableTiming));
+        int total = 0;
+        std::vector<int> n0, nn;
+        for (int c = 0; c < C->nCloths; c++) {
+            const MeshInfo* M = mesh_of(C, CS_MESH_CLOTH, c);
+            n0.push_back(M->n0);
+            nn.push_back(M->nn);
+            total = std::max(total, M->n0 + M->nn);
+        }
+        if (F.begun) {
+            // the slot's previous download still reads its snapshot and writes its host copy
+            if (F.cap < (size_t)total) CS_HIP(hipEventSynchronize(F.done));
+            else CS_HIP(hipStreamWaitEvent(C->stream, F.done, 0));
+        }
+        if (F.cap < (size_t)total) {  // both buffers grow together (device buffers never shrink)
+            if (F.host) CS_HIP(hipHostFree(F.host));
+            F.host = nullptr;
+            F.cap = 0;
+            CS_HIP(hipHostMalloc(reinterpret_cast<void**>(&F.host), sizeof(float4) * 2 * (size_t)total,
+                                 hipHostMallocDefault));
+            F.cap = total;
+        }
+        F.begun = false;
+        if (total > 0) {
+            float4* st = F.stage.as<float4>(2 * (size_t)total);
+            CS_HIP(hipMemcpyAsync(st, C->x.p, sizeof(float4) * total, hipMemcpyDeviceToDevice, C->stream));
+            CS_HIP(hipMemcpyAsync(st + total, C->v.p, sizeof(float4) * total, hipMemcpyDeviceToDevice, C->stream));
+            CS_HIP(hipEventRecord(C->evSnap, C->stream));
+            CS_HIP(hipStreamWaitEvent(C->copy, C->evSnap, 0));
+            CS_HIP(hipMemcpyAsync(F.host, st, sizeof(float4) * 2 * (size_t)total, hipMemcpyDeviceToHost, C->copy));
+        }
+        CS_HIP(hipEventRecord(F.done, C->copy));
+        F.total = total;
+        F.n0 = n0;
+        F.nn = nn;
+        F.begun = true;
+        return (int)CS_OK;
+    });
+}
+
+cs_status cs_frame_end(cs_ctx* C, int slot, int cloth, float* x, float* v, int* n_nodes) {
+    if (!C || slot < 0 || slot > 1) return CS_E_ARG;
+    auto& F = C->frame[slot];
+    if (!F.begun) return CS_E_STATE;
+    if (cloth < 0 || cloth >= (int)F.nn.size()) return CS_E_ARG;
+    // no guard(): this call may run on another host thread while the context steps, and touches
+    // only the slot's event and pinned copy
+    if (hipEventSynchronize(F.done) != hipSuccess) return CS_E_HIP;
+    const int n = F.nn[cloth];
+    const float4* hx = F.host + F.n0[cloth];
+    const float4* hv = F.host + F.total + F.n0[cloth];
+    for (int i = 0; i < n && x; i++) {
+        x[3 * i] = hx[i].x;
+        x[3 * i + 1] = hx[i].y;
+        x[3 * i + 2] = hx[i].z;
+    }
+    for (int i = 0; i < n && v; i++) {
+        v[3 * i] = hv[i].x;
+        v[3 * i + 1] = hv[i].y;
+        v[3 * i + 2] = hv[i].z;
+    }
+    if (n_nodes) *n_nodes = n;
+    return CS_OK;
 }
 
 cs_status cs_set_state(cs_ctx* C, int kind, int id, const float* x, const float* x0, const float* v) {

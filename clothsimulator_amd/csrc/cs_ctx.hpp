@@ -257,6 +257,20 @@ struct cs_ctx {
     } batchTune;
     void* hBatch = nullptr;  // pinned readback of per-cloth data (scalars, contact bounds)
     size_t hBatchCap = 0;    // bytes
+    // asynchronous frame download (cs_frame_begin / cs_frame_end): two slots, each a device
+    // snapshot of every cloth node's x then v (float4) and its pinned host copy, downloaded on
+    // the copy stream
+    struct FrameSlot {
+        cs::DBuf stage;
+        float4* host = nullptr;
+        size_t cap = 0;              // nodes the pinned copy holds
+        hipEvent_t done = nullptr;   // recorded on the copy stream after the download
+        bool begun = false;
+        int total = 0;               // cloth nodes in the snapshot
+        std::vector<int> n0, nn;     // per cloth at the snapshot
+    } frame[2];
+    hipStream_t copy = nullptr;
+    hipEvent_t evSnap = nullptr;
     // batched multigrid of the independent cloths (cs_amg.hip): every cloth keeps its own
     // hierarchy; one launch covers level l of every cloth that has one
     struct BatchAmg {

@@ -235,6 +235,19 @@ cs_status cs_set_state(cs_ctx* ctx, int kind, int id, const float* x, const floa
  * (3 nf) and face uv-vertex ids (3 nf), both mesh-local, and the uv vertices (2 n_verts).  NULL
  * buffers query n_verts only.  Host data: needs no device. */
 cs_status cs_get_topology(cs_ctx* ctx, int kind, int id, int* face_nodes, int* face_verts, float* uv, int* n_verts);
+/* Asynchronous frame download, double-buffered (no counterpart: Simulator::save copies every
+ * cloth's nodes synchronously, src/Simulator.cu:950-957 through src/Cloth.cu:865-887 and
+ * src/Mesh.cu:454-507).  cs_frame_begin snapshots the positions and velocities of every cloth
+ * node into device buffer `slot` (0 or 1) on the context's stream, starts their download into
+ * pinned host memory on a copy stream, and returns without waiting: the steps that follow run
+ * while the download is in flight.  Beginning a slot again first waits (on the device) for its
+ * previous download.  cs_frame_end waits for slot's download and copies cloth `cloth`'s nodes
+ * of the snapshot (3 floats per node into x and v, either may be NULL; n_nodes receives the
+ * count, which a remesh after cs_frame_begin does not change).  cs_frame_end touches only the
+ * slot, so it may run on another host thread while the context steps, unless that slot is being
+ * begun at the same time.  CS_E_STATE: the slot was never begun. */
+cs_status cs_frame_begin(cs_ctx* ctx, int slot);
+cs_status cs_frame_end(cs_ctx* ctx, int slot, int cloth, float* x, float* v, int* n_nodes);
 /* node normal (3n), area (n), mass (n) as maintained by Mesh::updateNodeGeometries (src/Mesh.cu:182-214) */
 cs_status cs_get_geometry(cs_ctx* ctx, int kind, int id, float* normals, float* area, float* mass);
 

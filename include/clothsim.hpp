@@ -4,8 +4,8 @@
 //   Simulator(path)               Simulator::Simulator (src/Simulator.cu:3-135), scene file only
 //   Simulator::physicsStep()      src/Simulator.cu:571-600
 //   Simulator::collisionStep()    src/Simulator.cu:602-684
-//   Simulator::simulateStep()     src/Simulator.cu:838-897 (a frame's last step remeshes unless the
-//                                 scene disables it; the separation step is not run)
+//   Simulator::simulateStep()     src/Simulator.cu:838-897 (a frame's last step remeshes, separation
+//                                 step included, unless the scene disables remeshing)
 //   Simulator::getCloths() / getObstacles(), Cloth::getMesh(), Obstacle::getMesh()
 //   Mesh::save(path)              OBJ with "v"/"nv" per node, "vt" per uv vertex and "f a/b" faces at
 //                                 precision 20 (src/Mesh.cu:454-507), readable by cs_load_scene
@@ -13,10 +13,13 @@
 //   Simulator::simulateOffline    src/Simulator.cu:1018-1043 (a save per frame until finished())
 //   Simulator(Resume, directory)  Resume mode: directory/config.json, nFrames = lastFrame(),
 //   Simulator::resumeOffline      nSteps = nFrames * frameSteps (src/Simulator.cu:1045-1083)
+//   Simulator::saveOverlapped     save() with the frame's nodes snapshotted on the device and its files
+//                                 written on a writer thread while the steps go on (no counterpart)
 // Errors throw clothsim::Error (the reference prints and exit(1)s instead).
 #pragma once
 #include <cstdio>
 #include <fstream>
+#include <future>
 #include <iomanip>
 #include <memory>
 #include <stdexcept>
@@ -64,13 +67,19 @@ class MeshHandle {
   public:
     MeshHandle(cs_ctx* ctx, int kind, int id) : ctx_(ctx), kind_(kind), id_(id) {}
     Mesh getMesh() const {
+        Mesh m = topology();
+        const size_t nn = 3 * (size_t)nodes();
+        m.x.resize(nn);
+        m.x0.resize(nn);
+        m.v.resize(nn);
+        check(cs_get_state(ctx_, kind_, id_, m.x.data(), m.x0.data(), m.v.data()), "cs_get_state");
+        return m;
+    }
+    // the faces and material-space vertices alone (host data)
+    Mesh topology() const {
         int nn = 0, nf = 0, ne = 0;
         check(cs_get_counts(ctx_, kind_, id_, &nn, &nf, &ne), "cs_get_counts");
         Mesh m;
-        m.x.resize(3 * (size_t)nn);
-        m.x0.resize(3 * (size_t)nn);
-        m.v.resize(3 * (size_t)nn);
-        check(cs_get_state(ctx_, kind_, id_, m.x.data(), m.x0.data(), m.v.data()), "cs_get_state");
         int nv = 0;
         check(cs_get_topology(ctx_, kind_, id_, nullptr, nullptr, nullptr, &nv), "cs_get_topology");
         m.faces.resize(3 * (size_t)nf);
@@ -79,6 +88,11 @@ class MeshHandle {
         check(cs_get_topology(ctx_, kind_, id_, m.faces.data(), m.faceVerts.data(), m.uv.data(), &nv),
               "cs_get_topology");
         return m;
+    }
+    int nodes() const {
+        int nn = 0, nf = 0, ne = 0;
+        check(cs_get_counts(ctx_, kind_, id_, &nn, &nf, &ne), "cs_get_counts");
+        return nn;
     }
     void setState(const Mesh& m) {
         check(cs_set_state(ctx_, kind_, id_, m.x.data(), m.x0.empty() ? nullptr : m.x0.data(),
@@ -121,7 +135,13 @@ class Simulator {
         cs_status s = cs_set_step_count(ctx_, nSteps_);
         if (s != CS_OK) throw Error(s, std::string("cs_set_step_count: ") + cs_last_error(ctx_));
     }
-    ~Simulator() { cs_destroy(ctx_); }
+    ~Simulator() {
+        try {
+            flush();
+        } catch (...) {
+        }
+        cs_destroy(ctx_);
+    }
     Simulator(const Simulator&) = delete;
     Simulator& operator=(const Simulator&) = delete;
 
@@ -150,28 +170,70 @@ class Simulator {
         save(directory);
         resumeOffline(directory);
     }
-    // resumeOffline's loop (src/Simulator.cu:1073-1083), for a context made with Resume
-    void resumeOffline(const std::string& directory) {
+    // resumeOffline's loop (src/Simulator.cu:1073-1083), for a context made with Resume; with
+    // `overlapped`, each frame is written by saveOverlapped while the next one steps
+    void resumeOffline(const std::string& directory, bool overlapped = false) {
         while (!finished()) {
             simulateStep();
-            if (nSteps_ % info_.frame_steps == 0) save(directory);
+            if (nSteps_ % info_.frame_steps != 0) continue;
+            if (overlapped) saveOverlapped(directory);
+            else save(directory);
         }
+        flush();
     }
     // Simulator::save (src/Simulator.cu:950-957, src/Cloth.cu:865-887): every cloth as
     // <directory>/frame<nFrames>_cloth<i>.obj and <directory>/config.json naming those meshes.
-    void save(const std::string& directory) const {
+    void save(const std::string& directory) {
+        flush();  // an overlapped frame's config.json first
         std::vector<std::string> paths;
         for (size_t c = 0; c < cloths_.size(); c++) {
-            paths.push_back(directory + "/frame" + std::to_string(nFrames_) + "_cloth" + std::to_string(c) + ".obj");
+            paths.push_back(framePath(directory, nFrames_, (int)c));
             cloths_[c].getMesh().save(paths.back());
         }
-        std::vector<const char*> p;
-        for (const std::string& s : paths) p.push_back(s.c_str());
-        const std::string cfg = directory + "/config.json";
-        const std::string tmp = cfg + ".tmp";
-        cs_status s = cs_write_config(path_.c_str(), tmp.c_str(), p.data(), (int)p.size());
-        if (s != CS_OK) throw Error(s, std::string("cs_write_config: ") + cs_scene_error());
-        if (std::rename(tmp.c_str(), cfg.c_str()) != 0) throw Error(CS_E_ARG, "cannot write " + cfg);
+        writeConfig(directory, paths);
+    }
+    // save() overlapped with the steps that follow: every cloth's nodes are snapshotted on the
+    // device into one of two download slots (cs_frame_begin) and the frame's files are written on
+    // a writer thread, its config.json after the previous frame's.  The same files as save();
+    // flush() (also run by resumeOffline and the destructor) waits for the pending writes and
+    // rethrows their errors.
+    void saveOverlapped(const std::string& directory) {
+        const int slot = slot_;
+        slot_ ^= 1;
+        if (pending_[slot].valid()) pending_[slot].get();  // its writer has read the slot
+        cs_status s = cs_frame_begin(ctx_, slot);
+        if (s != CS_OK) throw Error(s, std::string("cs_frame_begin: ") + cs_last_error(ctx_));
+        std::vector<Mesh> meshes;
+        for (const Cloth& c : cloths_) meshes.push_back(c.topology());
+        std::shared_future<void> prev = last_;
+        const int frame = nFrames_;
+        last_ = pending_[slot] =
+            std::async(std::launch::async, [this, directory, slot, frame, prev, meshes]() mutable {
+                std::vector<std::string> paths;
+                for (size_t c = 0; c < meshes.size(); c++) {
+                    int n = 0;
+                    cs_status st = cs_frame_end(ctx_, slot, (int)c, nullptr, nullptr, &n);
+                    if (st == CS_OK) {
+                        meshes[c].x.resize(3 * (size_t)n);
+                        meshes[c].v.resize(3 * (size_t)n);
+                        st = cs_frame_end(ctx_, slot, (int)c, meshes[c].x.data(), meshes[c].v.data(), &n);
+                    }
+                    if (st != CS_OK) throw Error(st, "cs_frame_end failed");
+                    paths.push_back(framePath(directory, frame, (int)c));
+                    meshes[c].save(paths.back());
+                }
+                if (prev.valid()) prev.get();
+                writeConfig(directory, paths);
+            }).share();
+    }
+    void flush() {
+        for (auto& f : pending_)
+            if (f.valid()) {
+                std::shared_future<void> g = f;
+                f = std::shared_future<void>();
+                g.get();
+            }
+        last_ = std::shared_future<void>();
     }
     // Simulator::lastFrame (src/Simulator.cu:962-977): the last frame whose every cloth file exists
     static int lastFrame(const std::string& directory, int nCloths) {
@@ -194,6 +256,19 @@ class Simulator {
     cs_ctx* handle() { return ctx_; }
 
   private:
+    static std::string framePath(const std::string& directory, int frame, int cloth) {
+        return directory + "/frame" + std::to_string(frame) + "_cloth" + std::to_string(cloth) + ".obj";
+    }
+    // config.json naming the frame's cloth files, written to a temporary and renamed
+    void writeConfig(const std::string& directory, const std::vector<std::string>& paths) const {
+        std::vector<const char*> p;
+        for (const std::string& q : paths) p.push_back(q.c_str());
+        const std::string cfg = directory + "/config.json";
+        const std::string tmp = cfg + ".tmp";
+        cs_status s = cs_write_config(path_.c_str(), tmp.c_str(), p.data(), (int)p.size());
+        if (s != CS_OK) throw Error(s, std::string("cs_write_config: ") + cs_scene_error());
+        if (std::rename(tmp.c_str(), cfg.c_str()) != 0) throw Error(CS_E_ARG, "cannot write " + cfg);
+    }
     void load(const cs_params* base) {
         cs_status s = cs_load_scene(path_.c_str(), root_.c_str(), base, &ctx_, &info_);
         if (s != CS_OK) throw Error(s, std::string("cs_load_scene: ") + cs_scene_error());
@@ -212,6 +287,9 @@ class Simulator {
     std::vector<Cloth> cloths_;
     std::vector<Obstacle> obstacles_;
     int nSteps_ = 0, nFrames_ = 0;
+    int slot_ = 0;                          // download slot of the next saveOverlapped
+    std::shared_future<void> pending_[2];   // the writer of each slot's last frame
+    std::shared_future<void> last_;         // the latest writer
 };
 
 }  // namespace clothsim

@@ -187,7 +187,7 @@ def build_host_mirror(out_dir):
     exe = os.path.join(out_dir, "host_mirror")
     subprocess.check_call(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(root, "include"),
                            os.path.join(root, "tests", "cpp", "host_mirror.cpp"), "-o", exe, "-L", lib_dir,
-                           "-lclothsim", "-Wl,-rpath," + lib_dir])
+                           "-lclothsim", "-Wl,-rpath," + lib_dir, "-pthread"])
     return exe
 
 
@@ -259,6 +259,31 @@ def test_cpp_mirror_remeshes_at_frame_ends(tmp_path):
     area = sum(0.5 * abs(np.cross(vt[b] - vt[a], vt[c] - vt[a])) for a, b, c in faces)
     assert area == pytest.approx(2.0, rel=1e-5)  # the 1 x 0.5 sheet scaled by 2
     assert np.all(np.isfinite(obj_rows(out / "frame2_cloth0.obj", "v")))
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_overlapped_saves_equal_direct_ones(tmp_path):
+    """Simulator::saveOverlapped (device snapshot, files written on a writer thread while the
+    next frame steps) writes byte for byte the files of Simulator::save, here on the remeshing
+    sheet, whose node count changes at every frame end."""
+    import subprocess
+    exe = build_host_mirror(str(tmp_path))
+    path = write_scene(str(tmp_path), remeshing_config())
+    outs = []
+    for mode in ("direct", "overlapped"):
+        out = tmp_path / mode
+        out.mkdir()
+        r = subprocess.run([exe, path, str(tmp_path), str(out), "3", mode], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append(out)
+    names = sorted(os.listdir(outs[0]))
+    assert names == sorted(os.listdir(outs[1])) and len(names) == 5
+    for n in names:
+        a, b = open(outs[0] / n).read(), open(outs[1] / n).read()
+        if n == "config.json":
+            a, b = a.replace(str(outs[0]), "OUT"), b.replace(str(outs[1]), "OUT")
+        assert a == b, n
 
 
 @pytest.mark.gpu
