@@ -17,10 +17,10 @@
 //                                 written on a writer thread while the steps go on (no counterpart)
 // Errors throw clothsim::Error (the reference prints and exit(1)s instead).
 #pragma once
+#include <charconv>
 #include <cstdio>
 #include <fstream>
 #include <future>
-#include <iomanip>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -42,20 +42,45 @@ struct Mesh {
     std::vector<int> faceVerts;   // 3 uv-vertex ids per face (mesh-local)
     std::vector<float> uv;        // 2 per uv vertex (material space)
     int nodes() const { return (int)(x.size() / 3); }
+    // Numbers are formatted by std::to_chars, general format at precision 20: the text of a
+    // stream at setprecision(20) (printf's %.20g of the float), without the stream's cost.
     void save(const std::string& path) const {
-        std::ofstream out(path);
-        if (!out) throw Error(CS_E_ARG, "cannot write " + path);
-        out << std::setprecision(20);
+        std::string o;
+        o.reserve(nodes() * 150 + faces.size() * 12 + uv.size() * 25);
+        auto num = [&o](float f) {
+            char b[64];
+            o.append(b, std::to_chars(b, b + sizeof b, (double)f, std::chars_format::general, 20).ptr);
+        };
+        auto idx = [&o](int k) {
+            char b[16];
+            o.append(b, std::to_chars(b, b + sizeof b, k).ptr);
+        };
+        auto row = [&](const char* tag, const float* p, int n) {
+            o += tag;
+            for (int c = 0; c < n; c++) {
+                o += ' ';
+                num(p[c]);
+            }
+            o += '\n';
+        };
         for (int i = 0; i < nodes(); i++) {
-            out << "v " << x[3 * i] << " " << x[3 * i + 1] << " " << x[3 * i + 2] << "\n";
-            out << "nv " << v[3 * i] << " " << v[3 * i + 1] << " " << v[3 * i + 2] << "\n";
+            row("v", &x[3 * (size_t)i], 3);
+            row("nv", &v[3 * (size_t)i], 3);
         }
-        for (size_t k = 0; k + 1 < uv.size(); k += 2) out << "vt " << uv[k] << " " << uv[k + 1] << "\n";
+        for (size_t k = 0; k + 1 < uv.size(); k += 2) row("vt", &uv[k], 2);
         for (size_t f = 0; f + 2 < faces.size(); f += 3) {
-            out << "f";
-            for (int c = 0; c < 3; c++) out << " " << faces[f + c] + 1 << "/" << faceVerts[f + c] + 1;
-            out << "\n";
+            o += 'f';
+            for (int c = 0; c < 3; c++) {
+                o += ' ';
+                idx(faces[f + c] + 1);
+                o += '/';
+                idx(faceVerts[f + c] + 1);
+            }
+            o += '\n';
         }
+        std::ofstream out(path, std::ios::binary);
+        if (!out) throw Error(CS_E_ARG, "cannot write " + path);
+        out.write(o.data(), (std::streamsize)o.size());
         if (!out) throw Error(CS_E_ARG, "cannot write " + path);
     }
 };

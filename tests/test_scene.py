@@ -191,6 +191,33 @@ def build_host_mirror(out_dir):
     return exe
 
 
+def test_mesh_save_text_is_the_references(tmp_path):
+    """Mesh::save writes v / nv / vt / f rows as the reference's stream at precision 20 does
+    (src/Mesh.cu:454-507): printf's %.20g of each float (here from Python's exact formatter) and
+    1-based "node/vertex" face indices, including zeros, tiny, huge and inexact values."""
+    import subprocess
+    vals = np.array([0.0, -0.0, 1e-30, 3.4e38, 0.1, -2.5, 1.0 / 3.0, 1e-45, 123456.789], dtype=np.float32)
+    src = tmp_path / "save.cpp"
+    src.write_text('#include "clothsim.hpp"\nint main(int, char** argv) {\n  clothsim::Mesh m;\n'
+                   '  m.x = {%s};\n  m.v = {%s};\n  m.uv = {%s};\n  m.faces = {0, 1, 2};\n'
+                   '  m.faceVerts = {2, 0, 1};\n  m.save(argv[1]);\n}\n' % (
+                       ", ".join("%.9ef" % v for v in vals), ", ".join("%.9ef" % v for v in vals[::-1]),
+                       ", ".join("%.9ef" % v for v in vals[:6])))
+    exe = tmp_path / "save"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lib_dir = os.path.join(root, "clothsimulator_amd")
+    subprocess.check_call(["g++", "-std=c++17", "-O2", "-I", os.path.join(root, "include"), str(src), "-o", str(exe),
+                           "-L", lib_dir, "-lclothsim", "-Wl,-rpath," + lib_dir])
+    out = tmp_path / "m.obj"
+    subprocess.check_call([str(exe), str(out)])
+    g = lambda a: " ".join("%.20g" % float(t) for t in a)
+    want = []
+    for i in range(3):
+        want += ["v " + g(vals[3 * i:3 * i + 3]), "nv " + g(vals[::-1][3 * i:3 * i + 3])]
+    want += ["vt " + g(vals[2 * k:2 * k + 2]) for k in range(3)] + ["f 1/3 2/1 3/2"]
+    assert out.read_text().splitlines() == want
+
+
 def test_write_config_round_trip(tmp_path):
     """cs_write_config keeps the configuration, points cloth 0 at a new mesh and drops its
     transform; the written file loads to the same scene."""
