@@ -753,6 +753,7 @@ __global__ void __launch_bounds__(AB) k_amg_post0ap(int nn, Level0 A, const int*
 
 // Level 0 with the learned smoother, before its sweeps: the residual after the coarse
 // correction, res = t - A_static P e - C P e (into t), and x' = x + P e (into x).
+template <bool STREAM>
 __global__ void __launch_bounds__(AB) k_amg_post0ap_res(int nn, Level0 A, const int* __restrict__ apSliceOff,
                                                         const int* __restrict__ apSliceW,
                                                         const int* __restrict__ apCols,
@@ -766,11 +767,15 @@ __global__ void __launch_bounds__(AB) k_amg_post0ap_res(int nn, Level0 A, const 
     const size_t aoff = (size_t)apSliceOff[s];
     float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
     for (int m = 0; m < WA; m++) {
-        const V3f ej = ld3v(e, apCols[(aoff + m) * SLICE + lane]);
-        const float* a = apVals + (aoff + m) * 9 * SLICE + lane;
-        y0 += a[0 * SLICE] * ej.x + a[1 * SLICE] * ej.y + a[2 * SLICE] * ej.z;
-        y1 += a[3 * SLICE] * ej.x + a[4 * SLICE] * ej.y + a[5 * SLICE] * ej.z;
-        y2 += a[6 * SLICE] * ej.x + a[7 * SLICE] * ej.y + a[8 * SLICE] * ej.z;
+        const int* cp = apCols + (aoff + m) * SLICE + lane;
+        const V3f ej = ld3v(e, STREAM ? __builtin_nontemporal_load(cp) : *cp);
+        const float* ap = apVals + (aoff + m) * 9 * SLICE + lane;
+        float a[9];
+#pragma unroll
+        for (int c = 0; c < 9; c++) a[c] = STREAM ? __builtin_nontemporal_load(ap + c * SLICE) : ap[c * SLICE];
+        y0 += a[0] * ej.x + a[1] * ej.y + a[2] * ej.z;
+        y1 += a[3] * ej.x + a[4] * ej.y + a[5] * ej.z;
+        y2 += a[6] * ej.x + a[7] * ej.y + a[8] * ej.z;
     }
     for (int k = A.cRowPtr[i], ce = A.cRowPtr[i + 1]; k < ce; k++) {
         const V3f ej = ld3v(e, agg[A.cCol[k]]);
@@ -1094,7 +1099,8 @@ void amg_apply(cs_ctx* C, int c, const float* r, float4* z, float* partRZ, int n
         const AmgLevel& N = G.L[l + 1];
         const float* e = (l + 1 == G.nl - 1) ? N.x.ptr<float>() : N.t.ptr<float>();
         if (l == 0 && learned) {
-            hipLaunchKernelGGL(k_amg_post0ap_res, dim3(cdiv(L.n, AB)), dim3(AB), 0, s, L.n, A0,
+            hipLaunchKernelGGL((learned ? k_amg_post0ap_res<true> : k_amg_post0ap_res<false>), dim3(cdiv(L.n, AB)),
+                               dim3(AB), 0, s, L.n, A0,
                                G.apSliceOff.ptr<int>(), G.apSliceW.ptr<int>(), G.apCols.ptr<int>(),
                                G.apVals.ptr<float>(), L.agg.ptr<int>(), e, L.x.ptr<float>(), L.t.ptr<float>());
             lpc_smooth_post(C, c, L.t.ptr<float>(), L.x.ptr<float>(), r, z, partRZ, nb);
