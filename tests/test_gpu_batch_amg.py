@@ -59,6 +59,20 @@ def test_batched_multigrid_matches_separate_multigrid_runs():
     assert np.all(np.abs(its_b - its_s) <= np.maximum(3, 0.1 * its_s)), (its_b, its_s)
 
 
+def test_streamed_batched_multigrid_matches_separate_runs():
+    """24 instances of sides 113..129: about 183 MB of static blocks (40 B each, about 13 per
+    node), above the 128 MB from which the batched SpMV, level-0 residual and post-smoothing read
+    their matrices with non-temporal loads; every instance still equals its separate multigrid
+    run to 1e-5 m."""
+    spec = _batch(24, 5, 113, 129)
+    mb = sum(c.mesh.n_nodes for c in spec.cloths) * 13 * 40 / 2 ** 20
+    assert mb > 160, mb
+    worst, its_b, its_s, res = _compare(spec, 3)
+    assert res <= 1e-5
+    assert worst <= 1e-5
+    assert np.all(np.abs(its_b - its_s) <= np.maximum(3, 0.1 * its_s)), (its_b, its_s)
+
+
 def test_batched_multigrid_with_contacts_matches_separate_runs():
     """Cloths resting on their spheres: level 1 of each cloth adds its own contact diagonal
     blocks (the batched contact assembly's per-cloth views).  In contact, a difference at the
